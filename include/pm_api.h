@@ -1,0 +1,174 @@
+/* pm_api.h -- C ABI of the MI355X particle-mesh (PM) gravity step.
+ *
+ * One periodic box on one GPU.  Every pointer argument that names mesh or
+ * particle data is a DEVICE pointer (hipMalloc'd, coarse-grained); sizes are
+ * element counts; `stream` is a hipStream_t passed as void* (NULL = the
+ * default stream).  Functions return 0 (PM_OK) or a negative PM_ERR_* code
+ * and leave a message for pm_last_error() (thread-local).  Calls are
+ * asynchronous with respect to the host unless stated otherwise.
+ *
+ * Conventions shared with the reference (yukiknk/Particle-Mesh):
+ *   positions in cell units with node i at coordinate i, cell index by
+ *   truncation (utils.cpp:79-88); meshes row-major [x][y][z], z fastest;
+ *   density contrast delta = mass * CIC - 1 with mass = Ng^3/Np
+ *   (main.cpp:23, utils.cpp:38-68); the centred difference
+ *   (phi[i-1]-phi[i+1])/2 as the force (utils.cpp:115-137).
+ *
+ * Two groups of entry points:
+ *   1. reference-layout operators, FP64, one per reference routine, taking
+ *      the reference's buffers and sizes (the drop-in for src/utils,
+ *      src/decomp/slab and src/fft);
+ *   2. the fused one-box step on a plan (periodic indexing instead of ghost
+ *      planes and halos, FP32 or FP64 mesh, FP64 positions/velocities).
+ */
+#ifndef PM_API_H
+#define PM_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+#define PM_OK 0
+#define PM_ERR_ARG (-1)      /* bad size, null pointer, unsupported mode */
+#define PM_ERR_HIP (-2)      /* a HIP runtime call or kernel launch failed */
+#define PM_ERR_NOMEM (-3)    /* device allocation failed */
+#define PM_ERR_RANGE (-4)    /* particles outside the box (reported by a check) */
+
+/* Precision of the mesh / FFT / force path. */
+#define PM_FP32 0
+#define PM_FP64 1
+
+/* Green's functions (see pm_plan_create). */
+#define PM_GREEN_REF_KH 0    /* -3 Omega0 / (4 (3 - sum cos 2 pi n_i/N)): fftw/fft.cpp:50-57 */
+#define PM_GREEN_CONT_K2 1   /* -N^2 / (pi |n|^2): -4 pi G rho_bar / k^2, unit box, G = M = 1 */
+
+/* Deposit variants. */
+#define PM_DEPOSIT_ATOMIC 0  /* one thread per particle, 8 global atomic adds */
+
+/* Last error message of the calling thread ("" if none). */
+const char* pm_last_error(void);
+/* Library version string. */
+const char* pm_version(void);
+
+/* ---- 1. reference-layout operators (FP64) -------------------------------
+ * The sizes mirror the reference arguments: nx = nx_loc + 1 etc. are the
+ * cube sides with the + ghost plane; x0/y0/z0 the cube origin (0 for one
+ * box); nx1 = nx + 2 etc. the sides of the -1/+2 halo cube. */
+
+/* init_buf (utils.cpp:38-68) for one grid: -1 on [0,nx-1)x[0,ny-1)x[0,nz-1),
+ * 0 on the + ghost planes.  The per-thread grids of the reference do not
+ * exist here (the deposit uses atomics), so `nthreads` is not an argument. */
+int pm_init_buf(double* buf, int nx, int ny, int nz, void* stream);
+
+/* deposit_particles + cic_deposit + reduce_private_to_shared
+ * (utils.cpp:70-113, utils_inline.h:25-49): adds mass * CIC weights of the
+ * np particles to buf (nx*ny*nz).  A particle whose cell does not lie in
+ * [x0, x0+nx-1) x ... is skipped and counted; pm_range_errors() reports the
+ * count. */
+int pm_deposit_particles(const double* posx, const double* posy, const double* posz, int64_t np,
+                         double* buf, int nx, int ny, int nz, int x0, int y0, int z0,
+                         double mass, void* stream);
+
+/* TransposeSlab + ReorderSlab at one box (transpose_slab.cpp:3-39,
+ * reorder_slab.cpp:3-68): fold the + ghost planes of the (ng+1)^3 cube
+ * periodically into the ng x ng x stride slab (stride >= ng; padding
+ * columns untouched).  Deterministic gather, no atomics. */
+int pm_reorder_slab(const double* cube, double* slab, int ng, int stride, void* stream);
+
+/* ReorderSlabBack + TransposeSlabBack at one box (reorder_slab_back.cpp:3-49,
+ * transpose_slab_back.cpp:3-49): slab -> (ng+3)^3 cube with the -1/+2
+ * periodic halo on every axis. */
+int pm_reorder_slab_back(const double* slab, double* halo, int ng, int stride, void* stream);
+
+/* centered_derivatives (utils.cpp:115-137): in is nx1*ny1*nz1, the outputs
+ * (nx1-2)*(ny1-2)*(nz1-2); d_d?[i] = (in[i-1] - in[i+1]) * 0.5. */
+int pm_centered_derivatives(const double* in, double* d_dx, double* d_dy, double* d_dz, int nx1,
+                            int ny1, int nz1, void* stream);
+
+/* update_particles + trilerp1 + function_a (utils.cpp:141-170,
+ * utils_inline.h:51-70): p += trilerp(d_d?) * f(a) * Da with Da = 0.001.
+ * The gradient meshes are (nx)*ny*nz with ny, nz the cube sides. Particles
+ * outside the cube are skipped and counted as in pm_deposit_particles. */
+int pm_update_particles(const double* posx, const double* posy, const double* posz,
+                        const double* d_dx, const double* d_dy, const double* d_dz, double* px,
+                        double* py, double* pz, double a, int64_t np, int nx, int ny, int nz,
+                        int x0, int y0, int z0, void* stream);
+
+/* Number of particles skipped by range checks since the last call
+ * (synchronises the device; resets the counter). */
+int64_t pm_range_errors(void);
+
+/* f(a) = 1/sqrt(Om/a + Ok + OL a^2) (utils_inline.h:51-53), host-side. */
+double pm_function_a(double a);
+
+/* The FFT class (fftw/fft.h:10-48, ffte1/fft.h:12-42) at one box.  The real
+ * slab is dense (stride = ng, the ffte1 layout); the half spectrum lives in
+ * the handle, layout [kz][ky][kx], kz in [0, ng/2].  As in ffte1, backward()
+ * is normalised by 1/ng^3 and apply_green() carries no 1/ng^3
+ * (ffte1/fft.cpp:28-35, pzdfft3d.f:90). */
+typedef struct pm_fft pm_fft_t;
+int pm_fft_create(pm_fft_t** out, int ng, int precision, void* stream);
+int pm_fft_destroy(pm_fft_t* fft);
+/* slab (ng^3, mesh precision) -> spectrum */
+int pm_fft_forward(pm_fft_t* fft, const void* slab);
+/* spectrum *= G(k) * (1/a) with the KH Green's function of the reference */
+int pm_fft_apply_green(pm_fft_t* fft, double a);
+/* spectrum -> slab, times 1/ng^3 */
+int pm_fft_backward(pm_fft_t* fft, void* slab);
+/* Device pointer of the spectrum ((ng/2+1)*ng*ng complex, interleaved). */
+void* pm_fft_spectrum(pm_fft_t* fft);
+
+/* ---- 2. fused one-box step --------------------------------------------- */
+typedef struct pm_plan pm_plan_t;
+
+/* Create a plan for an ng^3 periodic mesh (ng a power of two in [8, 1024])
+ * and np particles.  green_mode selects the Green's function; green_scale
+ * multiplies it (1/a for the reference's apply_green(a); 1 for CONT_K2).
+ * The plan owns its meshes and FFT work space (about 4.2 ng^3 reals). */
+int pm_plan_create(pm_plan_t** out, int ng, int64_t np, int precision, int green_mode,
+                   double green_scale, void* stream);
+int pm_plan_destroy(pm_plan_t* plan);
+int pm_plan_set_stream(pm_plan_t* plan, void* stream);
+/* Device pointers of the plan's density (after pm_deposit: mass*CIC, without
+ * the -1) and potential (after pm_solve) meshes, ng^3 in the mesh precision. */
+void* pm_plan_density(pm_plan_t* plan);
+void* pm_plan_potential(pm_plan_t* plan);
+
+/* Zero the density mesh and deposit mass*CIC of the particles (periodic). */
+int pm_deposit(pm_plan_t* plan, const double* posx, const double* posy, const double* posz,
+               int variant);
+/* potential = IFFT[G * scale / ng^3 * FFT[density - 1]] (5 passes). */
+int pm_solve(pm_plan_t* plan);
+/* v += g(x) * kick; x += v * drift wrapped into [0, ng), with g the
+ * trilinear interpolation of the centred difference of the potential. */
+int pm_interp_kick_drift(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx,
+                         double* vy, double* vz, double kick, double drift);
+/* Per-particle forces only (g into gx, gy, gz; FP64). */
+int pm_interp_force(pm_plan_t* plan, const double* posx, const double* posy, const double* posz,
+                    double* gx, double* gy, double* gz);
+/* One force evaluation + kick + drift: deposit, solve, interp_kick_drift. */
+int pm_step(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
+            double* vz, double kick, double drift, int variant);
+/* nsteps kick-drift-kick steps of size dt with merged kicks (nsteps + 1
+ * force evaluations: kick dt/2 first, dt between, dt/2 last). */
+int pm_run_kdk(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
+               double* vz, int nsteps, double dt, int variant);
+
+/* Phase timing (hipEvents on the plan's stream).  When enabled, each entry
+ * point records events around its kernels; pm_plan_phase_ms synchronises
+ * and returns the accumulated milliseconds per phase
+ * (0 zero+deposit, 1 z fwd, 2 y fwd, 3 x+green, 4 y inv, 5 z inv, 6 interp)
+ * and the number of samples; pm_plan_reset_timing clears them. */
+#define PM_NPHASES 7
+int pm_plan_enable_timing(pm_plan_t* plan, int enable);
+int pm_plan_phase_ms(pm_plan_t* plan, double* ms, int64_t* counts);
+int pm_plan_reset_timing(pm_plan_t* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PM_API_H */

@@ -1,0 +1,355 @@
+// C ABI glue: error state, FFT tables, the FFT handle and the fused-step plan.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pm_internal.h"
+
+namespace pm {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+int check_launch(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(PM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return PM_OK;
+}
+
+static int hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return PM_OK;
+    return fail(e == hipErrorOutOfMemory ? PM_ERR_NOMEM : PM_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// One range-error counter per device, allocated on first use.
+static unsigned long long* g_range[64] = {};
+
+unsigned long long* range_counter() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (!g_range[dev]) {
+        void* p = nullptr;
+        if (hipMalloc(&p, sizeof(unsigned long long)) != hipSuccess) return nullptr;
+        if (hipMemset(p, 0, sizeof(unsigned long long)) != hipSuccess) return nullptr;
+        g_range[dev] = (unsigned long long*)p;
+    }
+    return g_range[dev];
+}
+
+int tables_create(FftTables* t, int ng, int prec, hipStream_t st) {
+    (void)st;
+    t->ng = ng;
+    t->prec = prec;
+    const int M = ng / 2;
+    const size_t es = prec == PM_FP32 ? sizeof(float2) : sizeof(double2);
+    std::vector<double> twm(2 * (size_t)M), twn(2 * (size_t)ng), ct(ng);
+    const double two_pi = 2.0 * M_PI;
+    for (int m = 0; m < M; ++m) {
+        twm[2 * m] = std::cos(two_pi * m / M);
+        twm[2 * m + 1] = -std::sin(two_pi * m / M);
+    }
+    for (int m = 0; m < ng; ++m) {
+        twn[2 * m] = std::cos(two_pi * m / ng);
+        twn[2 * m + 1] = -std::sin(two_pi * m / ng);
+        ct[m] = std::cos(two_pi / ng * m);  // same expression as ffte1/fft.cpp:29,33
+    }
+    int rc = hip_ok(hipMalloc(&t->twM, es * M), "tables: hipMalloc");
+    if (rc) return rc;
+    rc = hip_ok(hipMalloc(&t->twN, es * ng), "tables: hipMalloc");
+    if (rc) return rc;
+    rc = hip_ok(hipMalloc((void**)&t->cos_tab, sizeof(double) * ng), "tables: hipMalloc");
+    if (rc) return rc;
+    if (prec == PM_FP32) {
+        std::vector<float> fm(twm.begin(), twm.end()), fn(twn.begin(), twn.end());
+        rc = hip_ok(hipMemcpy(t->twM, fm.data(), es * M, hipMemcpyHostToDevice), "tables: copy");
+        if (!rc) rc = hip_ok(hipMemcpy(t->twN, fn.data(), es * ng, hipMemcpyHostToDevice), "tables: copy");
+    } else {
+        rc = hip_ok(hipMemcpy(t->twM, twm.data(), es * M, hipMemcpyHostToDevice), "tables: copy");
+        if (!rc) rc = hip_ok(hipMemcpy(t->twN, twn.data(), es * ng, hipMemcpyHostToDevice), "tables: copy");
+    }
+    if (!rc) rc = hip_ok(hipMemcpy(t->cos_tab, ct.data(), sizeof(double) * ng, hipMemcpyHostToDevice), "tables: copy");
+    return rc;
+}
+
+void tables_destroy(FftTables* t) {
+    if (t->twM) (void)hipFree(t->twM);
+    if (t->twN) (void)hipFree(t->twN);
+    if (t->cos_tab) (void)hipFree(t->cos_tab);
+    t->twM = t->twN = nullptr;
+    t->cos_tab = nullptr;
+}
+
+static size_t spec_elems(int ng) { return (size_t)(ng / 2 + 1) * ng * ng; }
+
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" const char* pm_last_error(void) { return g_error.c_str(); }
+extern "C" const char* pm_version(void) { return "pm-mi355x 0.1 (gfx950)"; }
+
+extern "C" int64_t pm_range_errors(void) {
+    unsigned long long* c = range_counter();
+    if (!c) return fail(PM_ERR_HIP, "pm_range_errors: no counter");
+    unsigned long long h = 0;
+    if (hip_ok(hipDeviceSynchronize(), "pm_range_errors: sync")) return PM_ERR_HIP;
+    if (hip_ok(hipMemcpy(&h, c, sizeof(h), hipMemcpyDeviceToHost), "pm_range_errors: copy")) return PM_ERR_HIP;
+    if (hip_ok(hipMemset(c, 0, sizeof(h)), "pm_range_errors: reset")) return PM_ERR_HIP;
+    return (int64_t)h;
+}
+
+extern "C" double pm_function_a(double a) {
+    // utils_inline.h:13-16,51-53
+    const double om = 0.3, ol = 0.7, ok = 1.0 - om - ol;
+    return 1.0 / std::sqrt(om / a + ok + ol * a * a);
+}
+
+// ------------------------------------------------------------------ FFT API
+extern "C" int pm_fft_create(pm_fft_t** out, int ng, int precision, void* stream) {
+    if (!out) return fail(PM_ERR_ARG, "pm_fft_create: null handle");
+    *out = nullptr;
+    if (!valid_ng(ng)) return fail(PM_ERR_ARG, "pm_fft_create: ng must be a power of two in [8, 1024]");
+    if (precision != PM_FP32 && precision != PM_FP64) return fail(PM_ERR_ARG, "pm_fft_create: bad precision");
+    pm_fft* f = new pm_fft();
+    f->ng = ng;
+    f->prec = precision;
+    f->stream = (hipStream_t)stream;
+    const size_t cs = precision == PM_FP32 ? sizeof(float2) : sizeof(double2);
+    int rc = tables_create(&f->tab, ng, precision, f->stream);
+    if (!rc) rc = hip_ok(hipMalloc(&f->specA, cs * spec_elems(ng)), "pm_fft_create: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc(&f->specB, cs * spec_elems(ng)), "pm_fft_create: hipMalloc");
+    if (rc) {
+        pm_fft_destroy(f);
+        return rc;
+    }
+    *out = f;
+    return PM_OK;
+}
+
+extern "C" int pm_fft_destroy(pm_fft_t* f) {
+    if (!f) return PM_OK;
+    tables_destroy(&f->tab);
+    if (f->specA) (void)hipFree(f->specA);
+    if (f->specB) (void)hipFree(f->specB);
+    delete f;
+    return PM_OK;
+}
+
+extern "C" int pm_fft_forward(pm_fft_t* f, const void* slab) {
+    if (!f || !slab) return fail(PM_ERR_ARG, "pm_fft_forward: null argument");
+    const GreenArgs ga{PM_GREEN_REF_KH, 1.0, f->tab.cos_tab};
+    fft_pass(f->prec, f->ng, kPassZFwd, f->stream, slab, f->specA, f->tab.twM, f->tab.twN, ga, 1.0, false);
+    fft_pass(f->prec, f->ng, kPassYFwd, f->stream, f->specA, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false);
+    fft_pass(f->prec, f->ng, kPassXFwd, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false);
+    return check_launch("pm_fft_forward");
+}
+
+extern "C" int pm_fft_apply_green(pm_fft_t* f, double a) {
+    if (!f || !(a > 0.0)) return fail(PM_ERR_ARG, "pm_fft_apply_green: bad argument");
+    const GreenArgs ga{PM_GREEN_REF_KH, 1.0 / a, f->tab.cos_tab};
+    fft_pass(f->prec, f->ng, kPassGreen, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false);
+    return check_launch("pm_fft_apply_green");
+}
+
+extern "C" int pm_fft_backward(pm_fft_t* f, void* slab) {
+    if (!f || !slab) return fail(PM_ERR_ARG, "pm_fft_backward: null argument");
+    const GreenArgs ga{PM_GREEN_REF_KH, 1.0, f->tab.cos_tab};
+    const double n3 = (double)f->ng * f->ng * f->ng;
+    fft_pass(f->prec, f->ng, kPassXInv, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false);
+    fft_pass(f->prec, f->ng, kPassYInv, f->stream, f->specB, f->specA, f->tab.twM, f->tab.twN, ga, 1.0, false);
+    fft_pass(f->prec, f->ng, kPassZInv, f->stream, f->specA, slab, f->tab.twM, f->tab.twN, ga, 1.0 / n3, false);
+    return check_launch("pm_fft_backward");
+}
+
+extern "C" void* pm_fft_spectrum(pm_fft_t* f) { return f ? f->specB : nullptr; }
+
+// ------------------------------------------------------------------ plan API
+extern "C" int pm_plan_destroy(pm_plan_t* p) {
+    if (!p) return PM_OK;
+    tables_destroy(&p->tab);
+    void* bufs[4] = {p->rho, p->phi, p->specA, p->specB};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto& e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete p;
+    return PM_OK;
+}
+
+extern "C" int pm_plan_create(pm_plan_t** out, int ng, int64_t np, int precision, int green_mode,
+                              double green_scale, void* stream) {
+    if (!out) return fail(PM_ERR_ARG, "pm_plan_create: null handle");
+    *out = nullptr;
+    if (!valid_ng(ng)) return fail(PM_ERR_ARG, "pm_plan_create: ng must be a power of two in [8, 1024]");
+    if (np <= 0) return fail(PM_ERR_ARG, "pm_plan_create: np must be positive");
+    if (precision != PM_FP32 && precision != PM_FP64) return fail(PM_ERR_ARG, "pm_plan_create: bad precision");
+    if (green_mode != PM_GREEN_REF_KH && green_mode != PM_GREEN_CONT_K2)
+        return fail(PM_ERR_ARG, "pm_plan_create: bad green mode");
+    if (!std::isfinite(green_scale)) return fail(PM_ERR_ARG, "pm_plan_create: bad green scale");
+    pm_plan* p = new pm_plan();
+    p->ng = ng;
+    p->np = np;
+    p->prec = precision;
+    p->green_mode = green_mode;
+    p->green_scale = green_scale;
+    p->stream = (hipStream_t)stream;
+    const size_t rs = precision == PM_FP32 ? sizeof(float) : sizeof(double);
+    const size_t nc = (size_t)ng * ng * ng;
+    int rc = tables_create(&p->tab, ng, precision, p->stream);
+    if (!rc) rc = hip_ok(hipMalloc(&p->rho, rs * nc), "pm_plan_create: hipMalloc rho");
+    if (!rc) rc = hip_ok(hipMalloc(&p->phi, rs * nc), "pm_plan_create: hipMalloc phi");
+    if (!rc) rc = hip_ok(hipMalloc(&p->specA, 2 * rs * spec_elems(ng)), "pm_plan_create: hipMalloc spec");
+    if (!rc) rc = hip_ok(hipMalloc(&p->specB, 2 * rs * spec_elems(ng)), "pm_plan_create: hipMalloc spec");
+    if (!rc) rc = launch_fill(precision, p->phi, nc, 0.0, p->stream);
+    if (rc) {
+        pm_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return PM_OK;
+}
+
+extern "C" int pm_plan_set_stream(pm_plan_t* p, void* stream) {
+    if (!p) return fail(PM_ERR_ARG, "pm_plan_set_stream: null plan");
+    p->stream = (hipStream_t)stream;
+    return PM_OK;
+}
+
+extern "C" void* pm_plan_density(pm_plan_t* p) { return p ? p->rho : nullptr; }
+extern "C" void* pm_plan_potential(pm_plan_t* p) { return p ? p->phi : nullptr; }
+
+static inline void mark(pm_plan* p, int i) {
+    if (p->timing) (void)hipEventRecord(p->ev[i], p->stream);
+}
+
+static void accumulate(pm_plan* p, int first, int last) {
+    if (!p->timing) return;
+    (void)hipEventSynchronize(p->ev[last]);
+    for (int i = first; i < last; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]) == hipSuccess) {
+            p->phase_ms[i] += ms;
+            p->phase_n[i] += 1;
+        }
+    }
+}
+
+static int deposit_impl(pm_plan* p, const double* x, const double* y, const double* z, int variant) {
+    if (variant != PM_DEPOSIT_ATOMIC) return fail(PM_ERR_ARG, "pm_deposit: unknown variant");
+    const size_t nc = (size_t)p->ng * p->ng * p->ng;
+    int rc = launch_fill(p->prec, p->rho, nc, 0.0, p->stream);
+    if (rc) return rc;
+    const double mass = (double)nc / (double)p->np;
+    return launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, range_counter(), p->stream);
+}
+
+static int solve_impl(pm_plan* p) {
+    const int N = p->ng;
+    const GreenArgs ga{p->green_mode, p->green_scale / ((double)N * N * N), p->tab.cos_tab};
+    const void* tm = p->tab.twM;
+    const void* tn = p->tab.twN;
+    fft_pass(p->prec, N, kPassZFwd, p->stream, p->rho, p->specA, tm, tn, ga, 1.0, true);
+    mark(p, 2);
+    fft_pass(p->prec, N, kPassYFwd, p->stream, p->specA, p->specB, tm, tn, ga, 1.0, false);
+    mark(p, 3);
+    fft_pass(p->prec, N, kPassXFused, p->stream, nullptr, p->specB, tm, tn, ga, 1.0, false);
+    mark(p, 4);
+    fft_pass(p->prec, N, kPassYInv, p->stream, p->specB, p->specA, tm, tn, ga, 1.0, false);
+    mark(p, 5);
+    fft_pass(p->prec, N, kPassZInv, p->stream, p->specA, p->phi, tm, tn, ga, 1.0, false);
+    mark(p, 6);
+    return check_launch("pm_solve");
+}
+
+extern "C" int pm_deposit(pm_plan_t* p, const double* x, const double* y, const double* z, int variant) {
+    if (!p || !x || !y || !z) return fail(PM_ERR_ARG, "pm_deposit: null argument");
+    return deposit_impl(p, x, y, z, variant);
+}
+
+extern "C" int pm_solve(pm_plan_t* p) {
+    if (!p) return fail(PM_ERR_ARG, "pm_solve: null plan");
+    const bool t = p->timing;
+    p->timing = false;
+    const int rc = solve_impl(p);
+    p->timing = t;
+    return rc;
+}
+
+extern "C" int pm_interp_kick_drift(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy,
+                                    double* vz, double kick, double drift) {
+    if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_interp_kick_drift: null argument");
+    return launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 0, range_counter(),
+                         p->stream);
+}
+
+extern "C" int pm_interp_force(pm_plan_t* p, const double* x, const double* y, const double* z, double* gx,
+                               double* gy, double* gz) {
+    if (!p || !x || !y || !z || !gx || !gy || !gz) return fail(PM_ERR_ARG, "pm_interp_force: null argument");
+    return launch_interp(p->prec, (double*)x, (double*)y, (double*)z, gx, gy, gz, p->np, p->ng, p->phi, 0.0, 0.0,
+                         1, range_counter(), p->stream);
+}
+
+extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                       double kick, double drift, int variant) {
+    if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_step: null argument");
+    mark(p, 0);
+    int rc = deposit_impl(p, x, y, z, variant);
+    if (rc) return rc;
+    mark(p, 1);
+    rc = solve_impl(p);
+    if (rc) return rc;
+    rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 0, range_counter(),
+                       p->stream);
+    if (rc) return rc;
+    mark(p, 7);
+    accumulate(p, 0, 7);
+    return PM_OK;
+}
+
+extern "C" int pm_run_kdk(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                          int nsteps, double dt, int variant) {
+    if (!p || nsteps < 0) return fail(PM_ERR_ARG, "pm_run_kdk: bad arguments");
+    for (int s = 0; s <= nsteps; ++s) {
+        const double kick = (s == 0 || s == nsteps) ? 0.5 * dt : dt;
+        const double drift = s == nsteps ? 0.0 : dt;
+        const int rc = pm_step(p, x, y, z, vx, vy, vz, kick, drift, variant);
+        if (rc) return rc;
+    }
+    return PM_OK;
+}
+
+extern "C" int pm_plan_enable_timing(pm_plan_t* p, int enable) {
+    if (!p) return fail(PM_ERR_ARG, "pm_plan_enable_timing: null plan");
+    if (enable && !p->ev[0]) {
+        for (auto& e : p->ev)
+            if (hip_ok(hipEventCreate(&e), "pm_plan_enable_timing: event")) return PM_ERR_HIP;
+    }
+    p->timing = enable != 0;
+    return PM_OK;
+}
+
+extern "C" int pm_plan_phase_ms(pm_plan_t* p, double* ms, int64_t* counts) {
+    if (!p || !ms) return fail(PM_ERR_ARG, "pm_plan_phase_ms: null argument");
+    for (int i = 0; i < PM_NPHASES; ++i) {
+        ms[i] = p->phase_ms[i];
+        if (counts) counts[i] = p->phase_n[i];
+    }
+    return PM_OK;
+}
+
+extern "C" int pm_plan_reset_timing(pm_plan_t* p) {
+    if (!p) return fail(PM_ERR_ARG, "pm_plan_reset_timing: null plan");
+    for (int i = 0; i < PM_NPHASES; ++i) {
+        p->phase_ms[i] = 0.0;
+        p->phase_n[i] = 0;
+    }
+    return PM_OK;
+}

@@ -1,0 +1,43 @@
+// Shared device helpers for the PM kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace pm {
+
+constexpr int kXcds = 8;  // MI355X: 256 CUs in 8 XCDs, one L2 each
+
+// Workgroups are dispatched round-robin over the XCDs (block b -> XCD b % 8).
+// Remap so that each XCD walks one contiguous range of logical blocks, which
+// keeps neighbouring particles / mesh lines in the same L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
+    const uint32_t q = nb / kXcds, r = nb % kXcds;
+    const uint32_t xcd = bid % kXcds, idx = bid / kXcds;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+template <typename R> struct vec2;
+template <> struct vec2<float> { using type = float2; };
+template <> struct vec2<double> { using type = double2; };
+template <typename R> using cplx_t = typename vec2<R>::type;
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {
+    return C{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename C> __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
+// multiply by -i (DIR=-1) or +i (DIR=+1)
+template <int DIR, typename C> __device__ __forceinline__ C mul_i(C a) {
+    return DIR < 0 ? C{a.y, -a.x} : C{-a.y, a.x};
+}
+
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+
+}  // namespace pm

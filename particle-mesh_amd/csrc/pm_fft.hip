@@ -1,0 +1,406 @@
+// Batched Stockham FFT passes for the 3-D real-to-complex Poisson solve.
+//
+// Layouts (N = mesh side, M = N/2, all row-major, last index fastest):
+//   real mesh   R[x][y][z]            N^3 reals
+//   spectrum A  C1[kz][x][y]          (M+1) * N * N complex   (after the z pass)
+//   spectrum B  C2[kz][ky][kx]        (M+1) * N * N complex   (after the y pass)
+// Every pass reads or writes whole lines on one side and 128-byte segments
+// of T consecutive lines on the other, staging a tile of T lines in LDS:
+//   zfwd : R  -> C1  real FFT along z (M-point complex FFT + split)
+//   yfwd : C1 -> C2  complex FFT along y
+//   x    : C2 -> C2  FFT along x, Green's multiply, inverse FFT along x
+//   yinv : C2 -> C1
+//   zinv : C1 -> R   real inverse FFT along z, times a normalisation
+// The reference computes the same transform with FFTW-MPI (fftw/fft.cpp:46-47,
+// 96-97) or FFTE (pdzfft3d.f:77-265, pzdfft3d.f:76-301), both unnormalised
+// with e^{-i} forward; the Green's function follows fftw/fft.cpp:50-57 and
+// ffte1/fft.cpp:28-35.
+#include "pm_internal.h"
+
+namespace pm {
+
+constexpr int kFftThreads = 256;
+
+// LDS line pitch: one complex of padding per line so that the same index of
+// consecutive lines falls in different banks.
+__host__ __device__ constexpr int lds_pitch(int L) { return L + 1; }
+
+template <int R, int DIR, typename C> struct Dft;
+
+template <int DIR, typename C> struct Dft<2, DIR, C> {
+    static __device__ __forceinline__ void run(C* u) {
+        const C a = u[0], b = u[1];
+        u[0] = cadd(a, b);
+        u[1] = csub(a, b);
+    }
+};
+
+template <int DIR, typename C> struct Dft<4, DIR, C> {
+    static __device__ __forceinline__ void run(C* u) {
+        const C s02 = cadd(u[0], u[2]), d02 = csub(u[0], u[2]);
+        const C s13 = cadd(u[1], u[3]), d13 = mul_i<DIR>(csub(u[1], u[3]));
+        u[0] = cadd(s02, s13);
+        u[2] = csub(s02, s13);
+        u[1] = cadd(d02, d13);
+        u[3] = csub(d02, d13);
+    }
+};
+
+template <int DIR, typename C> struct Dft<8, DIR, C> {
+    static __device__ __forceinline__ void run(C* u) {
+        using Rl = decltype(u[0].x);
+        C e[4] = {u[0], u[2], u[4], u[6]};
+        C o[4] = {u[1], u[3], u[5], u[7]};
+        Dft<4, DIR, C>::run(e);
+        Dft<4, DIR, C>::run(o);
+        const Rl c = Rl(0.70710678118654752440);
+        // W8^1 = c + i*DIR*c, W8^2 = DIR*i, W8^3 = -c + i*DIR*c
+        const C o1 = C{c * (o[1].x - Rl(DIR) * o[1].y), c * (o[1].y + Rl(DIR) * o[1].x)};
+        const C o2 = mul_i<DIR>(o[2]);
+        const C o3 = C{-c * (o[3].x + Rl(DIR) * o[3].y), c * (Rl(DIR) * o[3].x - o[3].y)};
+        u[0] = cadd(e[0], o[0]);
+        u[4] = csub(e[0], o[0]);
+        u[1] = cadd(e[1], o1);
+        u[5] = csub(e[1], o1);
+        u[2] = cadd(e[2], o2);
+        u[6] = csub(e[2], o2);
+        u[3] = cadd(e[3], o3);
+        u[7] = csub(e[3], o3);
+    }
+};
+
+// One radix-R Stockham autosort stage over T lines of length L held in LDS
+// (line pitch lds_pitch(L)).  p is the length of the sub-transforms already
+// done; tw[m] = exp(-2 pi i m / L).
+template <typename C, int L, int T, int R, int DIR>
+__device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict__ tw) {
+    constexpr int Q = L / R;
+    constexpr int NBF = T * Q;
+    constexpr int PER = (NBF + kFftThreads - 1) / kFftThreads;
+    constexpr int LP = lds_pitch(L);
+    C u[PER][R];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int g = threadIdx.x + q * kFftThreads;
+        if (NBF % kFftThreads == 0 || g < NBF) {
+            const int line = g / Q, b = g % Q;
+            const C* row = s + line * LP;
+#pragma unroll
+            for (int r = 0; r < R; ++r) u[q][r] = row[b + r * Q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int g = threadIdx.x + q * kFftThreads;
+        if (NBF % kFftThreads == 0 || g < NBF) {
+            const int line = g / Q, b = g % Q;
+            const int k = b & (p - 1);
+            if (p > 1) {
+                const int step = L / (R * p);
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const C w = tw[r * k * step];
+                    u[q][r] = DIR < 0 ? cmul(u[q][r], w) : cmulc(u[q][r], w);
+                }
+            }
+            Dft<R, DIR, C>::run(u[q]);
+            const int j = (b - k) * R + k;
+            C* row = s + line * LP;
+#pragma unroll
+            for (int r = 0; r < R; ++r) row[j + r * p] = u[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full length-L FFT of T lines in LDS: radix-8 stages, then one radix-4 or
+// radix-2 stage for the remaining factor.
+template <typename C, int L, int T, int DIR>
+__device__ __forceinline__ void fft_lines(C* s, const C* __restrict__ tw) {
+    constexpr int lg = ilog2(L);
+    int p = 1;
+    if constexpr (lg >= 3) {
+#pragma unroll
+        for (int st = 0; st < lg / 3; ++st) {
+            stockham_stage<C, L, T, 8, DIR>(s, p, tw);
+            p *= 8;
+        }
+    }
+    if constexpr (lg % 3 == 2) stockham_stage<C, L, T, 4, DIR>(s, p, tw);
+    if constexpr (lg % 3 == 1) stockham_stage<C, L, T, 2, DIR>(s, p, tw);
+}
+
+// Lines per workgroup: 128-byte global segments (16 float2 / 8 double2),
+// halved at N >= 1024 to keep the LDS tile at 64 KB.
+template <typename Rl> __host__ __device__ constexpr int tile_lines(int N) {
+    const int base = sizeof(Rl) == 4 ? 16 : 8;
+    const int t = N >= 1024 ? base / 2 : base;
+    return t < N ? t : N;
+}
+
+// ---------------------------------------------------------------- z forward
+template <typename Rl, int N, bool SUB1>
+__global__ __launch_bounds__(kFftThreads) void k_zfwd(const Rl* __restrict__ rho,
+                                                      cplx_t<Rl>* __restrict__ out,
+                                                      const cplx_t<Rl>* __restrict__ twM,
+                                                      const cplx_t<Rl>* __restrict__ twN) {
+    using C = cplx_t<Rl>;
+    constexpr int M = N / 2, T = tile_lines<Rl>(N), LP = lds_pitch(M), YT = N / T;
+    __shared__ C s[T * LP];
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int x = tile / YT, y0 = (tile % YT) * T;
+    const C* src = reinterpret_cast<const C*>(rho + ((size_t)x * N + y0) * N);
+    for (int g = threadIdx.x; g < T * M; g += kFftThreads) {
+        C v = src[g];
+        if (SUB1) {
+            v.x -= Rl(1);
+            v.y -= Rl(1);
+        }
+        s[(g / M) * LP + (g % M)] = v;
+    }
+    __syncthreads();
+    fft_lines<C, M, T, -1>(s, twM);
+    // X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2, Z[M] = Z[0]
+    for (int g = threadIdx.x; g < T * (M + 1); g += kFftThreads) {
+        const int kz = g / T, line = g % T;
+        const C* row = s + line * LP;
+        const C zk = row[kz == M ? 0 : kz];
+        const C zm = row[kz == 0 ? 0 : M - kz];
+        const C e = C{Rl(0.5) * (zk.x + zm.x), Rl(0.5) * (zk.y - zm.y)};
+        const C d = C{Rl(0.5) * (zk.x - zm.x), Rl(0.5) * (zk.y + zm.y)};
+        const C wd = cmul(twN[kz], d);
+        out[((size_t)kz * N + x) * N + y0 + line] = C{e.x + wd.y, e.y - wd.x};
+    }
+}
+
+// ---------------------------------------------------------------- y forward
+template <typename Rl, int N>
+__global__ __launch_bounds__(kFftThreads) void k_yfwd(const cplx_t<Rl>* __restrict__ in,
+                                                      cplx_t<Rl>* __restrict__ out,
+                                                      const cplx_t<Rl>* __restrict__ twN) {
+    using C = cplx_t<Rl>;
+    constexpr int T = tile_lines<Rl>(N), LP = lds_pitch(N), XT = N / T;
+    __shared__ C s[T * LP];
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int kz = tile / XT, x0 = (tile % XT) * T;
+    const C* src = in + ((size_t)kz * N + x0) * N;
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) s[(g / N) * LP + (g % N)] = src[g];
+    __syncthreads();
+    fft_lines<C, N, T, -1>(s, twN);
+    C* dst = out + (size_t)kz * N * N + x0;
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) {
+        const int ky = g / T, line = g % T;
+        dst[(size_t)ky * N + line] = s[line * LP + ky];
+    }
+}
+
+// Green's function times 'scale' at integer wavevector (i, j, k), i, j in
+// [0, N), k in [0, N/2]; cos_tab[n] = cos(2 pi n / N).
+__device__ __forceinline__ double green_value(int mode, int N, int i, int j, int k,
+                                              const double* __restrict__ cos_tab) {
+    if ((i | j | k) == 0) return 0.0;
+    if (mode == PM_GREEN_REF_KH) {
+        // ffte1/fft.cpp:33 (fftw/fft.cpp:55 carries the 1/N^3 separately)
+        return -3.0 * 1.0 / (4.0 * (3.0 - cos_tab[i] - cos_tab[j] - cos_tab[k]));
+    }
+    const int si = i <= N / 2 ? i : i - N;
+    const int sj = j <= N / 2 ? j : j - N;
+    const int sk = k <= N / 2 ? k : k - N;
+    const double n2 = (double)si * si + (double)sj * sj + (double)sk * sk;
+    return -((double)N * N) / (3.14159265358979323846 * n2);
+}
+
+// ------------------------------------------------- x: forward/Green/inverse
+template <typename Rl, int N, int MODE>
+__global__ __launch_bounds__(kFftThreads) void k_x(cplx_t<Rl>* __restrict__ data,
+                                                   const cplx_t<Rl>* __restrict__ twN,
+                                                   GreenArgs ga) {
+    using C = cplx_t<Rl>;
+    constexpr int T = tile_lines<Rl>(N), LP = lds_pitch(N), YT = N / T;
+    __shared__ C s[T * LP];
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int kz = tile / YT, ky0 = (tile % YT) * T;
+    C* base = data + ((size_t)kz * N + ky0) * N;
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) s[(g / N) * LP + (g % N)] = base[g];
+    __syncthreads();
+    if constexpr (MODE & kXForward) fft_lines<C, N, T, -1>(s, twN);
+    if constexpr (MODE & kXGreen) {
+        for (int g = threadIdx.x; g < T * N; g += kFftThreads) {
+            const int line = g / N, kx = g % N;
+            const Rl gv = (Rl)(green_value(ga.mode, N, kx, ky0 + line, kz, ga.cos_tab) * ga.scale);
+            C& v = s[line * LP + kx];
+            v.x *= gv;
+            v.y *= gv;
+        }
+        __syncthreads();
+    }
+    if constexpr (MODE & kXInverse) fft_lines<C, N, T, +1>(s, twN);
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) base[g] = s[(g / N) * LP + (g % N)];
+}
+
+// ---------------------------------------------------------------- y inverse
+template <typename Rl, int N>
+__global__ __launch_bounds__(kFftThreads) void k_yinv(const cplx_t<Rl>* __restrict__ in,
+                                                      cplx_t<Rl>* __restrict__ out,
+                                                      const cplx_t<Rl>* __restrict__ twN) {
+    using C = cplx_t<Rl>;
+    constexpr int T = tile_lines<Rl>(N), LP = lds_pitch(N), XT = N / T;
+    __shared__ C s[T * LP];
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int kz = tile / XT, x0 = (tile % XT) * T;
+    const C* src = in + (size_t)kz * N * N + x0;
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) {
+        const int ky = g / T, line = g % T;
+        s[line * LP + ky] = src[(size_t)ky * N + line];
+    }
+    __syncthreads();
+    fft_lines<C, N, T, +1>(s, twN);
+    C* dst = out + ((size_t)kz * N + x0) * N;
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) dst[g] = s[(g / N) * LP + (g % N)];
+}
+
+// ---------------------------------------------------------------- z inverse
+template <typename Rl, int N>
+__global__ __launch_bounds__(kFftThreads) void k_zinv(const cplx_t<Rl>* __restrict__ in,
+                                                      Rl* __restrict__ out,
+                                                      const cplx_t<Rl>* __restrict__ twM,
+                                                      const cplx_t<Rl>* __restrict__ twN, Rl norm) {
+    using C = cplx_t<Rl>;
+    constexpr int M = N / 2, T = tile_lines<Rl>(N), LP = lds_pitch(M), YT = N / T;
+    constexpr int NPRE = T * M;
+    constexpr int PER = (NPRE + kFftThreads - 1) / kFftThreads;
+    __shared__ C s[T * LP];
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int x = tile / YT, y0 = (tile % YT) * T;
+    for (int g = threadIdx.x; g < T * (M + 1); g += kFftThreads) {
+        const int kz = g / T, line = g % T;
+        s[line * LP + kz] = in[((size_t)kz * N + x) * N + y0 + line];
+    }
+    __syncthreads();
+    // Z[k] = (X[k] + conj X[M-k]) + i W^{-k} (X[k] - conj X[M-k]), k in [0, M);
+    // the imaginary parts of X[0] and X[M] are dropped (Hermitian input).
+    C z[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int g = threadIdx.x + q * kFftThreads;
+        if (NPRE % kFftThreads == 0 || g < NPRE) {
+            const int line = g / M, k = g % M;
+            const C* row = s + line * LP;
+            C xk = row[k], xm = row[M - k];
+            if (k == 0) {
+                xk.y = Rl(0);
+                xm.y = Rl(0);
+            }
+            const C a = C{xk.x + xm.x, xk.y - xm.y};
+            const C b = cmulc(C{xk.x - xm.x, xk.y + xm.y}, twN[k]);
+            z[q] = C{a.x - b.y, a.y + b.x};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int g = threadIdx.x + q * kFftThreads;
+        if (NPRE % kFftThreads == 0 || g < NPRE) s[(g / M) * LP + (g % M)] = z[q];
+    }
+    __syncthreads();
+    fft_lines<C, M, T, +1>(s, twM);
+    C* dst = reinterpret_cast<C*>(out + ((size_t)x * N + y0) * N);
+    for (int g = threadIdx.x; g < T * M; g += kFftThreads) {
+        const C v = s[(g / M) * LP + (g % M)];
+        dst[g] = C{v.x * norm, v.y * norm};
+    }
+}
+
+// ---------------------------------------------------------- apply_green
+template <typename Rl>
+__global__ __launch_bounds__(256) void k_apply_green(cplx_t<Rl>* __restrict__ spec, int N,
+                                                     GreenArgs ga) {
+    const size_t total = (size_t)(N / 2 + 1) * N * N;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(e % N);
+        const int ky = (int)((e / N) % N);
+        const int kz = (int)(e / ((size_t)N * N));
+        const Rl gv = (Rl)(green_value(ga.mode, N, kx, ky, kz, ga.cos_tab) * ga.scale);
+        cplx_t<Rl> v = spec[e];
+        v.x *= gv;
+        v.y *= gv;
+        spec[e] = v;
+    }
+}
+
+// ================================================================ launchers
+template <typename Rl, int N>
+static void launch_pass(int pass, hipStream_t st, const void* in, void* out, const void* twM,
+                        const void* twN, const GreenArgs& ga, double norm, bool sub1) {
+    using C = cplx_t<Rl>;
+    constexpr int T = tile_lines<Rl>(N);
+    constexpr int M = N / 2;
+    const dim3 blk(kFftThreads);
+    const unsigned zt = (unsigned)((size_t)N * N / T);            // z passes: (x, y-tile)
+    const unsigned yt = (unsigned)((size_t)(M + 1) * N / T);      // y / x passes
+    switch (pass) {
+        case kPassZFwd:
+            if (sub1)
+                hipLaunchKernelGGL((k_zfwd<Rl, N, true>), dim3(zt), blk, 0, st, (const Rl*)in, (C*)out,
+                                   (const C*)twM, (const C*)twN);
+            else
+                hipLaunchKernelGGL((k_zfwd<Rl, N, false>), dim3(zt), blk, 0, st, (const Rl*)in, (C*)out,
+                                   (const C*)twM, (const C*)twN);
+            break;
+        case kPassYFwd:
+            hipLaunchKernelGGL((k_yfwd<Rl, N>), dim3(yt), blk, 0, st, (const C*)in, (C*)out, (const C*)twN);
+            break;
+        case kPassXFused:
+            hipLaunchKernelGGL((k_x<Rl, N, kXForward | kXGreen | kXInverse>), dim3(yt), blk, 0, st, (C*)out,
+                               (const C*)twN, ga);
+            break;
+        case kPassXFwd:
+            hipLaunchKernelGGL((k_x<Rl, N, kXForward>), dim3(yt), blk, 0, st, (C*)out, (const C*)twN, ga);
+            break;
+        case kPassXInv:
+            hipLaunchKernelGGL((k_x<Rl, N, kXInverse>), dim3(yt), blk, 0, st, (C*)out, (const C*)twN, ga);
+            break;
+        case kPassYInv:
+            hipLaunchKernelGGL((k_yinv<Rl, N>), dim3(yt), blk, 0, st, (const C*)in, (C*)out, (const C*)twN);
+            break;
+        case kPassZInv:
+            hipLaunchKernelGGL((k_zinv<Rl, N>), dim3(zt), blk, 0, st, (const C*)in, (Rl*)out, (const C*)twM,
+                               (const C*)twN, (Rl)norm);
+            break;
+        case kPassGreen: {
+            const size_t total = (size_t)(M + 1) * N * N;
+            const unsigned nb = (unsigned)((total + 255) / 256 < 65536 * 4 ? (total + 255) / 256 : 65536 * 4);
+            hipLaunchKernelGGL((k_apply_green<Rl>), dim3(nb), dim3(256), 0, st, (C*)out, N, ga);
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+template <typename Rl>
+static int dispatch_pass(int N, int pass, hipStream_t st, const void* in, void* out, const void* twM,
+                         const void* twN, const GreenArgs& ga, double norm, bool sub1) {
+    switch (N) {
+        case 8: launch_pass<Rl, 8>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 16: launch_pass<Rl, 16>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 32: launch_pass<Rl, 32>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 64: launch_pass<Rl, 64>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 128: launch_pass<Rl, 128>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 256: launch_pass<Rl, 256>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 512: launch_pass<Rl, 512>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 1024: launch_pass<Rl, 1024>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        default: return -1;
+    }
+    return 0;
+}
+
+int fft_pass(int prec, int N, int pass, hipStream_t st, const void* in, void* out, const void* twM,
+             const void* twN, const GreenArgs& ga, double norm, bool sub1) {
+    return prec == PM_FP32 ? dispatch_pass<float>(N, pass, st, in, out, twM, twN, ga, norm, sub1)
+                           : dispatch_pass<double>(N, pass, st, in, out, twM, twN, ga, norm, sub1);
+}
+
+}  // namespace pm

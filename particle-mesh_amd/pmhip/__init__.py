@@ -1,0 +1,264 @@
+"""ctypes binding of libpmhip.so (the C ABI declared in include/pm_api.h).
+
+Device memory comes from torch (``torch.empty(..., device="cuda")``); the
+library only sees raw device pointers and the current HIP stream.  Every
+call goes through the HIP library: if it is not built, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "lib" / "libpmhip.so"
+
+PM_OK = 0
+PM_ERR_ARG = -1
+PM_ERR_HIP = -2
+PM_ERR_NOMEM = -3
+PM_ERR_RANGE = -4
+
+PM_FP32 = 0
+PM_FP64 = 1
+PM_GREEN_REF_KH = 0
+PM_GREEN_CONT_K2 = 1
+PM_DEPOSIT_ATOMIC = 0
+PM_NPHASES = 7
+
+PHASE_NAMES = ["deposit", "fft_z_fwd", "fft_y_fwd", "fft_x_green", "fft_y_inv", "fft_z_inv", "interp_kick_drift"]
+
+# Reference constants (utils_inline.h:13-18).
+DA = 0.001
+START_A = 0.9
+
+_c_int = ctypes.c_int
+_c_i64 = ctypes.c_int64
+_c_dbl = ctypes.c_double
+_c_vp = ctypes.c_void_p
+_c_vpp = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> (restype, argtypes); mirrors include/pm_api.h
+SIGNATURES = {
+    "pm_last_error": (ctypes.c_char_p, []),
+    "pm_version": (ctypes.c_char_p, []),
+    "pm_init_buf": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_vp]),
+    "pm_deposit_particles": (_c_int, [_c_vp, _c_vp, _c_vp, _c_i64, _c_vp, _c_int, _c_int, _c_int,
+                                      _c_int, _c_int, _c_int, _c_dbl, _c_vp]),
+    "pm_reorder_slab": (_c_int, [_c_vp, _c_vp, _c_int, _c_int, _c_vp]),
+    "pm_reorder_slab_back": (_c_int, [_c_vp, _c_vp, _c_int, _c_int, _c_vp]),
+    "pm_centered_derivatives": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_int, _c_int, _c_vp]),
+    "pm_update_particles": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
+                                     _c_dbl, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                     _c_vp]),
+    "pm_range_errors": (_c_i64, []),
+    "pm_function_a": (_c_dbl, [_c_dbl]),
+    "pm_fft_create": (_c_int, [_c_vpp, _c_int, _c_int, _c_vp]),
+    "pm_fft_destroy": (_c_int, [_c_vp]),
+    "pm_fft_forward": (_c_int, [_c_vp, _c_vp]),
+    "pm_fft_apply_green": (_c_int, [_c_vp, _c_dbl]),
+    "pm_fft_backward": (_c_int, [_c_vp, _c_vp]),
+    "pm_fft_spectrum": (_c_vp, [_c_vp]),
+    "pm_plan_create": (_c_int, [_c_vpp, _c_int, _c_i64, _c_int, _c_int, _c_dbl, _c_vp]),
+    "pm_plan_destroy": (_c_int, [_c_vp]),
+    "pm_plan_set_stream": (_c_int, [_c_vp, _c_vp]),
+    "pm_plan_density": (_c_vp, [_c_vp]),
+    "pm_plan_potential": (_c_vp, [_c_vp]),
+    "pm_deposit": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_int]),
+    "pm_solve": (_c_int, [_c_vp]),
+    "pm_interp_kick_drift": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl]),
+    "pm_interp_force": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "pm_step": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl, _c_int]),
+    "pm_run_kdk": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_dbl, _c_int]),
+    "pm_plan_enable_timing": (_c_int, [_c_vp, _c_int]),
+    "pm_plan_phase_ms": (_c_int, [_c_vp, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_i64)]),
+    "pm_plan_reset_timing": (_c_int, [_c_vp]),
+}
+
+_lib = None
+
+
+class PMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pm error {code}: {msg}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    """Load libpmhip.so once; raise if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    return lib().pm_last_error().decode()
+
+
+def check(rc: int) -> None:
+    if rc != PM_OK:
+        raise PMError(rc, last_error())
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def dptr(t) -> int:
+    """Device pointer of a contiguous CUDA tensor."""
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return t.data_ptr()
+
+
+def current_stream() -> int:
+    return _torch().cuda.current_stream().cuda_stream
+
+
+class _DevView:
+    """Minimal __cuda_array_interface__ exporter for library-owned memory."""
+
+    def __init__(self, ptr: int, shape, typestr: str):
+        self.__cuda_array_interface__ = {
+            "shape": tuple(shape),
+            "typestr": typestr,
+            "data": (int(ptr), False),
+            "version": 3,
+            "strides": None,
+        }
+
+
+def device_view(ptr: int, shape, dtype):
+    """A torch tensor aliasing library-owned device memory (no copy)."""
+    torch = _torch()
+    typestr = {torch.float32: "<f4", torch.float64: "<f8"}[dtype]
+    return torch.as_tensor(_DevView(ptr, shape, typestr), device="cuda")
+
+
+def mesh_dtype(precision: int):
+    torch = _torch()
+    return torch.float32 if precision == PM_FP32 else torch.float64
+
+
+class Plan:
+    """The fused one-box PM step (pm_plan_* of include/pm_api.h)."""
+
+    def __init__(self, ng: int, np_: int, precision: int = PM_FP32, green: int = PM_GREEN_REF_KH,
+                 green_scale: float = 1.0 / START_A, stream: int | None = None):
+        self.ng, self.np, self.precision = ng, np_, precision
+        self._stream = current_stream() if stream is None else stream
+        h = ctypes.c_void_p()
+        check(lib().pm_plan_create(ctypes.byref(h), ng, np_, precision, green, green_scale, self._stream))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().pm_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def density(self):
+        n = self.ng
+        return device_view(lib().pm_plan_density(self.h), (n, n, n), mesh_dtype(self.precision))
+
+    def potential(self):
+        n = self.ng
+        return device_view(lib().pm_plan_potential(self.h), (n, n, n), mesh_dtype(self.precision))
+
+    def deposit(self, pos, variant: int = PM_DEPOSIT_ATOMIC):
+        check(lib().pm_deposit(self.h, dptr(pos[0]), dptr(pos[1]), dptr(pos[2]), variant))
+
+    def solve(self):
+        check(lib().pm_solve(self.h))
+
+    def interp_kick_drift(self, pos, vel, kick: float, drift: float):
+        check(lib().pm_interp_kick_drift(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel),
+                                         kick, drift))
+
+    def interp_force(self, pos, out):
+        check(lib().pm_interp_force(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in out)))
+
+    def step(self, pos, vel, kick: float, drift: float, variant: int = PM_DEPOSIT_ATOMIC):
+        check(lib().pm_step(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel), kick, drift, variant))
+
+    def run_kdk(self, pos, vel, nsteps: int, dt: float, variant: int = PM_DEPOSIT_ATOMIC):
+        check(lib().pm_run_kdk(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel), nsteps, dt, variant))
+
+    def enable_timing(self, on: bool = True):
+        check(lib().pm_plan_enable_timing(self.h, 1 if on else 0))
+
+    def reset_timing(self):
+        check(lib().pm_plan_reset_timing(self.h))
+
+    def phase_ms(self):
+        ms = (ctypes.c_double * PM_NPHASES)()
+        cnt = (ctypes.c_int64 * PM_NPHASES)()
+        check(lib().pm_plan_phase_ms(self.h, ms, cnt))
+        return list(ms), list(cnt)
+
+
+class FFT:
+    """The reference's FFT class at one box (pm_fft_* of include/pm_api.h)."""
+
+    def __init__(self, ng: int, precision: int = PM_FP64, stream: int | None = None):
+        self.ng, self.precision = ng, precision
+        h = ctypes.c_void_p()
+        check(lib().pm_fft_create(ctypes.byref(h), ng, precision, current_stream() if stream is None else stream))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().pm_fft_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, slab):
+        check(lib().pm_fft_forward(self.h, dptr(slab)))
+
+    def apply_green(self, a: float):
+        check(lib().pm_fft_apply_green(self.h, a))
+
+    def backward(self, slab):
+        check(lib().pm_fft_backward(self.h, dptr(slab)))
+
+    def spectrum(self):
+        """Complex view [kz][ky][kx] (kz in [0, ng/2]) of the handle's spectrum."""
+        torch = _torch()
+        n = self.ng
+        v = device_view(lib().pm_fft_spectrum(self.h), ((n // 2 + 1), n, n, 2), mesh_dtype(self.precision))
+        return torch.view_as_complex(v)
+
+
+def range_errors() -> int:
+    n = lib().pm_range_errors()
+    if n < 0:
+        check(int(n))
+    return int(n)

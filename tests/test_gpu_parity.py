@@ -1,0 +1,238 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 oracle and
+the reference's golden fixtures.
+
+Tolerances: FP64 mesh paths agree with the oracle to ~1e-12 relative (only
+summation order and FMA contraction differ); FP32 mesh paths are bounded by
+FP32 rounding of the mesh, FFT and force (relative 1e-5 on the potential,
+1e-4 on per-particle forces relative to the largest force).
+"""
+import numpy as np
+import pytest
+
+import pmhip
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def to_dev(arrs, dev):
+    import torch
+
+    return [torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev) for a in arrs]
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+# -------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("prec,tol", [(pmhip.PM_FP64, 1e-12), (pmhip.PM_FP32, 2e-6)])
+@pytest.mark.parametrize("ng", [8, 16, 32, 64, 128])
+def test_fft_forward_backward_vs_numpy(dev, ng, prec, tol):
+    import torch
+
+    print(f"case fft ng={ng} prec={prec}", flush=True)
+    rng = np.random.default_rng(ng)
+    r = rng.standard_normal((ng, ng, ng))
+    dt = pmhip.mesh_dtype(prec)
+    slab = torch.tensor(r, dtype=dt, device=dev)
+    f = pmhip.FFT(ng, prec)
+    f.forward(slab)
+    torch.cuda.synchronize()
+    spec = host(f.spectrum())  # [kz][ky][kx]
+    ref = np.fft.rfftn(r).transpose(2, 1, 0)
+    assert rel(spec, ref) < tol
+    out = torch.empty_like(slab)
+    f.backward(out)
+    torch.cuda.synchronize()
+    assert rel(host(out), r) < tol
+    f.close()
+
+
+def test_fft_apply_green_matches_oracle(dev):
+    import torch
+
+    ng, a = 32, 0.9
+    rng = np.random.default_rng(3)
+    d = rng.standard_normal((ng, ng, ng))
+    d -= d.mean()
+    slab = torch.tensor(d, device=dev)
+    f = pmhip.FFT(ng, pmhip.PM_FP64)
+    f.forward(slab)
+    f.apply_green(a)
+    out = torch.empty_like(slab)
+    f.backward(out)
+    torch.cuda.synchronize()
+    assert rel(host(out), pyoracle.solve(d, pyoracle.GREEN_REF_KH, 1.0 / a)) < 1e-12
+    f.close()
+
+
+# -------------------------------------------------- reference-layout operators
+def test_reference_layout_step_vs_golden(dev, golden_dir):
+    """The drop-in operators in the reference's order reproduce the
+    reference's own one-kick fixture."""
+    import torch
+
+    lib = pmhip.lib()
+    for name in ("ref_kick_ng16_np4096", "ref_kick_ng16_np1728", "ref_kick_ng32_np4096"):
+        print(f"case reference layout {name}", flush=True)
+        f = np.load(golden_dir / f"{name}.npz")
+        ng, a = int(f["ng"]), float(f["a"])
+        pos = to_dev((f["posx"], f["posy"], f["posz"]), dev)
+        npart = pos[0].numel()
+        n1, n3 = ng + 1, ng + 3
+        cube = torch.empty(n1 ** 3, dtype=torch.float64, device=dev)
+        slab = torch.empty(ng ** 3, dtype=torch.float64, device=dev)
+        halo = torch.empty(n3 ** 3, dtype=torch.float64, device=dev)
+        grads = [torch.empty(n1 ** 3, dtype=torch.float64, device=dev) for _ in range(3)]
+        mom = [torch.zeros(npart, dtype=torch.float64, device=dev) for _ in range(3)]
+        st = pmhip.current_stream()
+        pmhip.check(lib.pm_init_buf(cube.data_ptr(), n1, n1, n1, st))
+        pmhip.check(lib.pm_deposit_particles(*(p.data_ptr() for p in pos), npart, cube.data_ptr(), n1, n1, n1, 0, 0, 0,
+                                             ng ** 3 / npart, st))
+        pmhip.check(lib.pm_reorder_slab(cube.data_ptr(), slab.data_ptr(), ng, ng, st))
+        fft = pmhip.FFT(ng, pmhip.PM_FP64)
+        fft.forward(slab)
+        fft.apply_green(a)
+        fft.backward(slab)
+        pmhip.check(lib.pm_reorder_slab_back(slab.data_ptr(), halo.data_ptr(), ng, ng, st))
+        pmhip.check(lib.pm_centered_derivatives(halo.data_ptr(), *(g.data_ptr() for g in grads), n3, n3, n3, st))
+        pmhip.check(lib.pm_update_particles(*(p.data_ptr() for p in pos), *(g.data_ptr() for g in grads),
+                                            *(m.data_ptr() for m in mom), a, npart, n1, n1, n1, 0, 0, 0, st))
+        torch.cuda.synchronize()
+        assert pmhip.range_errors() == 0
+        if "slab" in f:
+            assert rel(host(slab), f["phi"].ravel()) < 1e-12
+        for m, key in zip(mom, ("px", "py", "pz")):
+            assert rel(host(m), f[key]) < 1e-11
+        fft.close()
+
+
+def test_reference_layout_operators_vs_oracle(dev):
+    import torch
+
+    lib = pmhip.lib()
+    ng, npart = 16, 2500
+    rng = np.random.default_rng(9)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    pos = to_dev(ph, dev)
+    p_ref = [np.zeros(npart) for _ in range(3)]
+    ref = pyoracle.reference_step(ng, ph, p_ref)
+    n1, n3 = ng + 1, ng + 3
+    st = pmhip.current_stream()
+    cube = torch.empty(n1 ** 3, dtype=torch.float64, device=dev)
+    pmhip.check(lib.pm_init_buf(cube.data_ptr(), n1, n1, n1, st))
+    pmhip.check(lib.pm_deposit_particles(*(p.data_ptr() for p in pos), npart, cube.data_ptr(), n1, n1, n1, 0, 0, 0,
+                                         ng ** 3 / npart, st))
+    torch.cuda.synchronize()
+    assert rel(host(cube), ref["cube"]) < 1e-13
+    slab = torch.full((ng * ng * (ng + 2),), 7.0, dtype=torch.float64, device=dev)  # padded stride
+    pmhip.check(lib.pm_reorder_slab(cube.data_ptr(), slab.data_ptr(), ng, ng + 2, st))
+    torch.cuda.synchronize()
+    s = host(slab).reshape(ng, ng, ng + 2)
+    assert rel(s[:, :, :ng].ravel(), ref["slab"]) < 1e-13
+    assert np.all(s[:, :, ng:] == 7.0)  # padding untouched
+    phi = torch.tensor(ref["phi"], device=dev)
+    halo = torch.empty(n3 ** 3, dtype=torch.float64, device=dev)
+    pmhip.check(lib.pm_reorder_slab_back(phi.data_ptr(), halo.data_ptr(), ng, ng, st))
+    grads = [torch.empty(n1 ** 3, dtype=torch.float64, device=dev) for _ in range(3)]
+    pmhip.check(lib.pm_centered_derivatives(halo.data_ptr(), *(g.data_ptr() for g in grads), n3, n3, n3, st))
+    torch.cuda.synchronize()
+    assert np.array_equal(host(halo), ref["halo"])
+    for g, k in zip(grads, ("dx", "dy", "dz")):
+        assert np.array_equal(host(g), ref[k])
+
+
+def test_range_errors_counted(dev):
+    import torch
+
+    lib = pmhip.lib()
+    pmhip.range_errors()
+    ng = 8
+    n1 = ng + 1
+    pos = to_dev([np.array([1.0, 8.0, -0.5, 3.0]), np.array([1.0, 1.0, 1.0, np.nan]), np.array([1.0] * 4)], dev)
+    cube = torch.zeros(n1 ** 3, dtype=torch.float64, device=dev)
+    pmhip.check(lib.pm_deposit_particles(*(p.data_ptr() for p in pos), 4, cube.data_ptr(), n1, n1, n1, 0, 0, 0, 1.0,
+                                         pmhip.current_stream()))
+    assert pmhip.range_errors() == 3
+    assert abs(float(cube.sum()) - 1.0) < 1e-15
+
+
+# ----------------------------------------------------------- fused plan path
+@pytest.mark.parametrize("ng,npart", [(16, 4096), (32, 20000), (64, 100000)])
+def test_plan_deposit_solve_force_fp64(dev, ng, npart):
+    import torch
+
+    print(f"case plan fp64 ng={ng} np={npart}", flush=True)
+    rng = np.random.default_rng(ng + 1)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    pos = to_dev(ph, dev)
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_REF_KH, 1.0 / 0.9) as plan:
+        plan.deposit(pos)
+        plan.solve()
+        g = [torch.empty(npart, dtype=torch.float64, device=dev) for _ in range(3)]
+        plan.interp_force(pos, g)
+        torch.cuda.synchronize()
+        delta_ref = pyoracle.deposit_periodic(ph, ng)
+        assert rel(host(plan.density()) - 1.0, delta_ref) < 1e-12
+        phi_ref = pyoracle.solve(delta_ref, pyoracle.GREEN_REF_KH, 1.0 / 0.9)
+        assert rel(host(plan.potential()), phi_ref) < 1e-11
+        g_ref = pyoracle.interp_force(ph, phi_ref)
+        for a, b in zip(g, g_ref):
+            assert rel(host(a), b) < 1e-10
+
+
+@pytest.mark.parametrize("ng,npart", [(16, 4096), (64, 100000)])
+def test_plan_fp32_within_fp32_bounds(dev, ng, npart):
+    import torch
+
+    print(f"case plan fp32 ng={ng} np={npart}", flush=True)
+    rng = np.random.default_rng(ng + 2)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    pos = to_dev(ph, dev)
+    with pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.deposit(pos)
+        plan.solve()
+        g = [torch.empty(npart, dtype=torch.float64, device=dev) for _ in range(3)]
+        plan.interp_force(pos, g)
+        torch.cuda.synchronize()
+        delta_ref = pyoracle.deposit_periodic(ph, ng)
+        rho = host(plan.density()).astype(np.float64)
+        assert np.max(np.abs(rho - 1.0 - delta_ref)) < 2e-6 * max(1.0, np.abs(delta_ref).max())
+        phi_ref = pyoracle.solve(delta_ref, pyoracle.GREEN_CONT_K2, 1.0)
+        assert rel(host(plan.potential()), phi_ref) < 1e-5
+        g_ref = pyoracle.interp_force(ph, phi_ref)
+        gmax = max(np.abs(v).max() for v in g_ref)
+        for a, b in zip(g, g_ref):
+            assert np.max(np.abs(host(a) - b)) < 1e-4 * gmax
+
+
+def test_kdk_config1_fp64_matches_oracle(dev):
+    """64^3 particles from the reference's own random stream on a 64^3 mesh,
+    10 KDK steps, dt = 0.01, continuous -1/k^2 Green's function (the
+    reference's CPU-runnable case, in unit-box units)."""
+    import torch
+
+    ng, nrow, nsteps, dt = 64, 64, 10, 0.01
+    npart = nrow ** 3
+    ph = list(pyoracle.init_particles(npart, ng))
+    vh = [np.zeros(npart) for _ in range(3)]
+    pos = to_dev(ph, dev)
+    vel = to_dev(vh, dev)
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.run_kdk(pos, vel, nsteps, dt)
+        torch.cuda.synchronize()
+    assert pmhip.range_errors() == 0
+    pyoracle.run_kdk(ph, vh, ng, nsteps, dt, pyoracle.GREEN_CONT_K2, 1.0)
+    for a, b in zip(pos, ph):
+        d = host(a) - b
+        d -= ng * np.round(d / ng)  # periodic distance
+        assert np.max(np.abs(d)) < 1e-8
+    vmax = max(np.abs(v).max() for v in vh)
+    for a, b in zip(vel, vh):
+        assert np.max(np.abs(host(a) - b)) < 1e-8 * vmax

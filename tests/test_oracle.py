@@ -1,0 +1,236 @@
+"""Pins of the FP64 oracle (oracle/pm_oracle.c) -- CPU only.
+
+Each oracle function is checked against something it does not restate:
+the reference's own outputs (golden fixtures written by the reference's
+unmodified code, and the live reference build where present), libstdc++'s
+mt19937_64 and the C++ standard's known answer, numpy's FFT, and closed
+forms / conservation laws of the PM scheme.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ------------------------------------------------------------ random stream
+def test_mt19937_64_matches_libstdcpp(golden_dir):
+    g = np.load(golden_dir / "std_mt19937_64.npz")
+    for s, first, last in zip(g["seeds"], g["first16"], g["out10000"]):
+        raw = pyoracle.mt_raw(int(s), 10000)
+        assert np.array_equal(raw[:16], first)
+        assert raw[9999] == last
+    # C++ standard [rand.predef]: 10000th output of default-seeded mt19937_64
+    assert int(pyoracle.mt_raw(5489, 10000)[9999]) == 9981545732273789042
+
+
+@pytest.mark.parametrize("name", ["ref_kick_ng16_np4096", "ref_kick_ng16_np1728", "ref_kick_ng32_np4096"])
+def test_init_particles_bitexact(golden_dir, name):
+    f = np.load(golden_dir / f"{name}.npz")
+    ng = int(f["ng"])
+    pos = pyoracle.init_particles(f["posx"].size, ng)
+    for a, key in zip(pos, ("posx", "posy", "posz")):
+        assert np.array_equal(a, f[key])
+
+
+# ------------------------------------------------------- reference fixtures
+@pytest.mark.parametrize("name", ["ref_kick_ng16_np4096", "ref_kick_ng16_np1728", "ref_kick_ng32_np4096"])
+def test_reference_kick_fixture(golden_dir, name):
+    f = np.load(golden_dir / f"{name}.npz")
+    ng = int(f["ng"])
+    pos = (f["posx"], f["posy"], f["posz"])
+    p = [np.zeros(pos[0].size) for _ in range(3)]
+    out = pyoracle.reference_step(ng, pos, p, a=float(f["a"]))
+    if "slab" in f:
+        assert rel(out["slab"], f["slab"].ravel()) < 1e-14
+        assert rel(out["phi"], f["phi"].ravel()) < 1e-13
+    for a, key in zip(p, ("px", "py", "pz")):
+        assert rel(a, f[key]) < 1e-12
+
+
+@pytest.mark.skipif(not pyoracle.ref_available(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("ng,npart,seed", [(16, 3000, 1), (32, 5000, 2), (8, 700, 3)])
+def test_oracle_vs_live_reference(ng, npart, seed):
+    rng = np.random.default_rng(seed)
+    pos = tuple(rng.uniform(0, ng, npart) for _ in range(3))
+    ref = pyoracle.ref_step(ng, pos)
+    p = [np.zeros(npart) for _ in range(3)]
+    out = pyoracle.reference_step(ng, pos, p)
+    for k in ("cube", "slab", "phi", "halo", "dx", "dy", "dz"):
+        assert rel(out[k], ref[k]) < 1e-13, k
+    for a, k in zip(p, ("px", "py", "pz")):
+        assert rel(a, ref[k]) < 1e-12, k
+
+
+# ---------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("ng", [2, 4, 8, 16, 32])
+def test_fft_matches_numpy(ng):
+    rng = np.random.default_rng(ng)
+    r = rng.standard_normal((ng, ng, ng))
+    spec = pyoracle.fft_r2c(r)
+    ref = np.fft.rfftn(r)
+    assert rel(spec, ref) < 1e-13
+    back = pyoracle.fft_c2r(ref)
+    assert rel(back, np.fft.irfftn(ref, s=r.shape, axes=(0, 1, 2)) * ng ** 3) < 1e-13
+    assert rel(back / ng ** 3, r) < 1e-13
+
+
+def test_fft_delta_and_parseval():
+    ng = 16
+    d = np.zeros((ng, ng, ng))
+    d[0, 0, 0] = 1.0
+    assert np.allclose(pyoracle.fft_r2c(d), 1.0, atol=1e-15)
+    r = np.random.default_rng(0).standard_normal((ng, ng, ng))
+    s = np.fft.fftn(r)  # full spectrum for Parseval
+    half = pyoracle.fft_r2c(r)
+    assert np.allclose(half, s[:, :, : ng // 2 + 1], atol=1e-11)
+    assert abs(np.sum(np.abs(s) ** 2) / ng ** 3 - np.sum(r ** 2)) < 1e-9 * np.sum(r ** 2)
+
+
+def test_fft_rejects_non_power_of_two():
+    x = np.zeros(27)
+    out = np.zeros(2 * 3 * 3 * 2)
+    assert pyoracle.orc().orc_fft_r2c(3, x.ctypes.data, out.ctypes.data) == -1
+
+
+# -------------------------------------------------------------------- Green
+def test_green_ref_kh_closed_form():
+    ng = 16
+    for (i, j, k) in [(1, 0, 0), (0, 3, 5), (7, 8, 8), (15, 1, 2)]:
+        s2 = sum(np.sin(np.pi * n / ng) ** 2 for n in (i, j, k))
+        # (3 Omega0 / 2) times the inverse of the 7-point Laplacian symbol -4 sum sin^2
+        exact = 1.5 * (-1.0 / (4 * s2))
+        assert abs(pyoracle.green(pyoracle.GREEN_REF_KH, ng, i, j, k) - exact) < 1e-13 * abs(exact)
+    assert pyoracle.green(pyoracle.GREEN_REF_KH, ng, 0, 0, 0) == 0.0
+    assert pyoracle.green(pyoracle.GREEN_CONT_K2, ng, 0, 0, 0) == 0.0
+    # continuous: -4 pi / k^2 with k = 2 pi n, times ng^2 (cell units), n signed
+    assert abs(pyoracle.green(pyoracle.GREEN_CONT_K2, ng, 15, 0, 2) - (-4 * np.pi / (4 * np.pi ** 2 * 5)) * ng ** 2) < 1e-12
+
+
+@pytest.mark.parametrize("mode", [pyoracle.GREEN_REF_KH, pyoracle.GREEN_CONT_K2])
+def test_solve_plane_wave(mode):
+    ng = 16
+    n = (2, 3, 1)
+    x = np.arange(ng)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    th = 2 * np.pi * (n[0] * X + n[1] * Y + n[2] * Z) / ng
+    delta = np.cos(th)
+    a = 0.9
+    phi = pyoracle.solve(delta, mode, 1.0 / a)
+    g = pyoracle.green(mode, ng, *n)
+    assert rel(phi, g / a * np.cos(th)) < 1e-12
+
+
+# ------------------------------------------------------------------ deposit
+def test_deposit_conserves_mass_and_matches_fold():
+    ng, npart = 16, 3000
+    rng = np.random.default_rng(5)
+    pos = tuple(rng.uniform(0, ng, npart) for _ in range(3))
+    d = pyoracle.deposit_periodic(pos, ng)
+    assert abs(d.sum()) < 1e-10  # sum of mass * weights = ng^3, minus ng^3
+    cube = np.zeros((ng + 1) ** 3)
+    pyoracle.orc().orc_init_cube(ng, cube.ctypes.data)
+    pyoracle.orc().orc_deposit_cube(npart, *(v.ctypes.data for v in pos), ng ** 3 / npart, ng, cube.ctypes.data)
+    slab = np.zeros(ng ** 3)
+    pyoracle.orc().orc_ghost_fold(ng, cube.ctypes.data, slab.ctypes.data)
+    assert rel(d.ravel(), slab) < 1e-13
+
+
+def test_deposit_node_and_midpoint():
+    ng = 8
+    d = pyoracle.deposit_periodic((np.array([3.0]), np.array([7.5]), np.array([0.25])), ng, mass=1.0)
+    exp = np.full((ng, ng, ng), -1.0)
+    for yi, wy in ((7, 0.5), (0, 0.5)):  # 7.5 straddles the periodic boundary
+        for zi, wz in ((0, 0.75), (1, 0.25)):
+            exp[3, yi, zi] += wy * wz
+    assert np.allclose(d, exp, atol=1e-15)
+
+
+# ------------------------------------------------------------- force / kick
+def _force(pos, ng, mode=pyoracle.GREEN_REF_KH):
+    d = pyoracle.deposit_periodic(pos, ng)
+    phi = pyoracle.solve(d, mode, 1.0)
+    return pyoracle.interp_force(pos, phi), phi
+
+
+def test_momentum_conservation():
+    ng, npart = 16, 2000
+    rng = np.random.default_rng(7)
+    pos = tuple(rng.uniform(0, ng, npart) for _ in range(3))
+    g, _ = _force(pos, ng)
+    gmax = max(np.abs(v).max() for v in g)
+    for v in g:
+        assert abs(v.sum()) < 1e-12 * gmax * npart
+
+
+def test_single_particle_no_self_force_and_pair_symmetry():
+    ng = 16
+    g, _ = _force((np.array([3.3]), np.array([10.7]), np.array([5.1])), ng)
+    assert max(abs(v[0]) for v in g) < 1e-12
+    pos = (np.array([3.3, 9.1]), np.array([10.7, 6.2]), np.array([5.1, 12.9]))
+    g, _ = _force(pos, ng)
+    for v in g:
+        assert abs(v[0] + v[1]) < 1e-12 * max(abs(v[0]), 1e-30) + 1e-15
+    assert max(abs(v[0]) for v in g) > 1e-4  # they do attract
+
+
+def test_uniform_lattice_zero_force_and_free_streaming():
+    ng = 8
+    idx = np.arange(ng, dtype=np.float64)
+    X, Y, Z = np.meshgrid(idx, idx, idx, indexing="ij")
+    pos = [X.ravel().copy(), Y.ravel().copy(), Z.ravel().copy()]
+    g, phi = _force(tuple(pos), ng)
+    assert np.abs(phi).max() < 1e-13
+    assert max(np.abs(v).max() for v in g) < 1e-13
+    # a uniform velocity keeps the lattice uniform: exact free streaming
+    vel = [np.full(ng ** 3, 0.5), np.zeros(ng ** 3), np.zeros(ng ** 3)]
+    pyoracle.run_kdk(pos, vel, ng, 4, 0.5, pyoracle.GREEN_CONT_K2, 1.0)
+    assert np.allclose(pos[0], (X.ravel() + 1.0) % ng, atol=1e-12)
+    assert np.allclose(vel[0], 0.5, atol=1e-12)
+
+
+def test_centred_difference_of_plane_wave():
+    ng = 16
+    x = np.arange(ng)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    th = 2 * np.pi * 3 / ng
+    phi = np.cos(th * X)
+    # at nodes the interpolated force is the centred difference sin(th x) sin(th)
+    pos = (np.array([5.0, 11.0]), np.array([2.0, 2.0]), np.array([7.0, 0.0]))
+    g = pyoracle.interp_force(pos, phi)
+    assert np.allclose(g[0], np.sin(th * pos[0]) * np.sin(th), atol=1e-14)
+    assert np.allclose(g[1], 0.0, atol=1e-15) and np.allclose(g[2], 0.0, atol=1e-15)
+
+
+def test_interp_is_deposit_transpose():
+    ng, npart = 8, 50
+    rng = np.random.default_rng(11)
+    pos = tuple(rng.uniform(0, ng, npart) for _ in range(3))
+    w = rng.standard_normal(npart)
+    f = rng.standard_normal((ng, ng, ng))
+    # <deposit(x, w), f> via unit-mass deposits of each particle
+    lhs = 0.0
+    for p in range(npart):
+        d = pyoracle.deposit_periodic(tuple(np.array([v[p]]) for v in pos), ng, mass=1.0) + 1.0
+        lhs += w[p] * np.sum(d * f)
+    # sum_p w_p interp(f, x_p): trilinear interpolation through the potential
+    # path, using phi such that the centred difference along x equals f is
+    # not needed -- instead compare against the direct trilinear weights.
+    rhs = 0.0
+    for p in range(npart):
+        x, y, z = (v[p] for v in pos)
+        ix, iy, iz = int(x), int(y), int(z)
+        tx, ty, tz = x - ix, y - iy, z - iz
+        for a, wa in ((0, 1 - tx), (1, tx)):
+            for b, wb in ((0, 1 - ty), (1, ty)):
+                for c, wc in ((0, 1 - tz), (1, tz)):
+                    rhs += w[p] * wa * wb * wc * f[(ix + a) % ng, (iy + b) % ng, (iz + c) % ng]
+    assert abs(lhs - rhs) < 1e-10 * max(abs(rhs), 1.0)
+
+
+def test_function_a():
+    a = 0.9
+    assert abs(pyoracle.function_a(a) - 1.0 / np.sqrt(0.3 / a + 0.7 * a * a)) < 1e-15
