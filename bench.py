@@ -1,0 +1,242 @@
+"""Benchmark of the one-box PM step on one MI355X.
+
+Workload (one GPU's share of the benchmark box): 512^3 particles on a 512^3
+mesh, Zel'dovich lattice ICs with a k^-2 density spectrum scaled to 1.5-cell
+rms displacement (clustered: shell crossing stresses the deposit), growing-
+mode velocities, continuous -1/k^2 Green's function (unit box, G = M = 1),
+KDK steps of dt = 0.005.  A step = one force evaluation (zero + CIC deposit,
+5-pass R2C/Green/C2R FFT solve, interpolation of the centred-difference
+force) + kick + drift, all on the GPU with inputs resident in HBM.
+
+Prints one JSON line: value = FP32-mesh Mparticle-steps/s (FP64 positions
+and velocities); the FP64-mesh path, per-phase times, the roofline of the
+dominant kernel and the CPU baseline (the FP64 oracle on the host cores, on
+a bounded sample of the same ICs) are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "particle-mesh_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ng", type=int, default=512)
+    ap.add_argument("--nrow", type=int, default=512)
+    ap.add_argument("--rms", type=float, default=1.5)
+    ap.add_argument("--dt", type=float, default=0.005)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-mesh leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (PMC byte-counter calibration)")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(ng: int, npart: int, prec: int):
+    """Compulsory HBM bytes per launch of each phase (b = mesh real bytes;
+    positions and velocities FP64).  Spectrum: (ng/2+1) ng^2 complex."""
+    b = 4 if prec == 0 else 8
+    nc = ng ** 3
+    nk = (ng // 2 + 1) * ng * ng
+    return {
+        "deposit": 24 * npart + b * nc,                 # read x; write the mesh once
+        "fft_z_fwd": b * nc + 2 * b * nk,               # real in, half spectrum out
+        "fft_y_fwd": 4 * b * nk,                        # spectrum in + out
+        "fft_x_green": 4 * b * nk,
+        "fft_y_inv": 4 * b * nk,
+        "fft_z_inv": 2 * b * nk + b * nc,
+        "interp_kick_drift": b * nc + 96 * npart,       # read phi; read+write x, v
+    }
+
+
+def make_ics(nrow, ng, rms, seed, dev):
+    import torch
+    import pmhip
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    noise = torch.randn(nrow ** 3, dtype=torch.float64, device=dev, generator=g)
+    vel_factor = math.sqrt(4.0 * math.pi)  # growing mode of a static background, G = rho_bar = 1
+    pos, vel = pmhip.zeldovich_ic(nrow, ng, noise, rms, vel_factor)
+    del noise
+    torch.cuda.synchronize()
+    return pos, vel
+
+
+def run_gpu(prec, pos0, vel0, args, ng, npart):
+    """Warm up, then time exactly args.steps steps; returns (ms_per_step,
+    phase ms per launch)."""
+    import torch
+    import pmhip
+
+    pos = [p.clone() for p in pos0]
+    vel = [v.clone() for v in vel0]
+    plan = pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0)
+    dt = args.dt
+    step = 0
+
+    def one():
+        nonlocal step
+        plan.step(pos, vel, 0.5 * dt if step == 0 else dt, dt)
+        step += 1
+
+    for _ in range(args.warmup):
+        one()
+    torch.cuda.synchronize()
+    plan.reset_timing()
+    plan.enable_timing(args.steps)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        one()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    ms, cnt = plan.phase_ms()
+    nerr = pmhip.range_errors()
+    plan.close()
+    phases = {n: (m / c if c else None) for n, m, c in zip(pmhip.PHASE_NAMES, ms, cnt)}
+    return ev_ms / args.steps, wall / args.steps * 1e3, phases, nerr, (pos, vel)
+
+
+def cpu_baseline(pos_dev, vel_dev, ng, args):
+    """The FP64 oracle (C/OpenMP restatement of the reference step) on the
+    same ICs, args.cpu_steps steps, timed on the host cores."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    pyoracle.set_threads(threads)
+    pos = [p.cpu().numpy().copy() for p in pos_dev]
+    vel = [v.cpu().numpy().copy() for v in vel_dev]
+    npart = pos[0].size
+    mass = ng ** 3 / npart
+    delta = np.zeros((ng, ng, ng))
+    # one untimed pass so that the oracle's work buffers are allocated
+    delta[:] = pyoracle.deposit_periodic(pos, ng, mass)
+    t0 = time.perf_counter()
+    for s in range(args.cpu_steps):
+        delta = pyoracle.deposit_periodic(pos, ng, mass)
+        phi = pyoracle.solve(delta, pyoracle.GREEN_CONT_K2, 1.0)
+        pyoracle.interp_kick_drift(pos, vel, phi, args.dt, args.dt)
+    t = (time.perf_counter() - t0) / args.cpu_steps
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": npart / t / 1e6, "unit": "Mparticle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_steps} step(s) of the FP64 oracle (deposit, FFT solve, interpolation+kick+drift) "
+                      f"on the same {round(npart ** (1 / 3))}^3 ICs, {t:.2f} s/step, {threads} OpenMP threads, {model}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU's box only (got --gpus {args.gpus}); the multi-GPU slab "
+              "decomposition is not built. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import pmhip
+
+    dev = torch.device("cuda:0")
+    ng, nrow = args.ng, args.nrow
+    npart = nrow ** 3
+    log(f"ICs: {nrow}^3 particles, {ng}^3 mesh, rms {args.rms} cells, seed {args.seed}")
+    pos0, vel0 = make_ics(nrow, ng, args.rms, args.seed, dev)
+
+    log("FP32-mesh leg")
+    ms32, wall32, ph32, err32, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart)
+    log(f"FP32: {ms32:.3f} ms/step, phases {ph32}")
+    res64 = None
+    if not args.no_fp64:
+        log("FP64-mesh leg")
+        ms64, wall64, ph64, err64, _ = run_gpu(pmhip.PM_FP64, pos0, vel0, args, ng, npart)
+        log(f"FP64: {ms64:.3f} ms/step, phases {ph64}")
+        res64 = {"value": npart / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "phases_ms": ph64}
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+
+    nbytes = algorithmic_bytes(ng, npart, pmhip.PM_FP32)
+    dom = max((k for k in ph32 if ph32[k]), key=lambda k: ph32[k])
+    achieved = nbytes[dom] / (ph32[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": nbytes[dom], "avg_ms": round(ph32[dom], 4)}
+    step_bytes = sum(nbytes.values())
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline leg")
+        cpu = cpu_baseline(pos0, vel0, ng, args)
+        log(f"CPU: {cpu['value']:.2f} Mparticle-steps/s")
+    value = npart / (ms32 * 1e-3) / 1e6
+    out = {
+        "metric": "Mparticle-steps/s at 512^3 (FP32 vs FP64 PM, 1 GPU)",
+        "value": round(value, 2),
+        "unit": "Mparticle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 mesh/FFT/force, f64 positions/velocities",
+        "data": "synthetic Zel'dovich lattice ICs generated on the GPU",
+        "config": {"workload": f"PM KDK step, {nrow}^3 particles on a {ng}^3 periodic mesh, Zel'dovich k^-2 ICs "
+                               f"{args.rms}-cell rms, dt={args.dt}, CIC atomic deposit, 5-pass FFT, CONT_K2 Green",
+                   "ng": ng, "np": npart, "dt": args.dt, "rms_cells": args.rms},
+        "phases_ms": {k: (round(v, 4) if v else v) for k, v in ph32.items()},
+        "step_algorithmic_GB": round(step_bytes / 1e9, 3),
+        "step_achieved_GBs": round(step_bytes / (ms32 * 1e-3) / 1e9, 1),
+        "wall_ms_per_step": round(wall32, 4),
+        "fp64": ({"value": round(res64["value"], 2), "ms_per_step": round(res64["ms_per_step"], 4),
+                  "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["phases_ms"].items()}}
+                 if res64 else None),
+        "adjoint_forward_ratio": None,
+        "range_errors": err32,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["vs_cpu_baseline"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

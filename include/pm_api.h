@@ -122,6 +122,18 @@ int pm_fft_backward(pm_fft_t* fft, void* slab);
 /* Device pointer of the spectrum ((ng/2+1)*ng*ng complex, interleaved). */
 void* pm_fft_spectrum(pm_fft_t* fft);
 
+/* ---- initial conditions (no counterpart in the reference) -------------- */
+/* Zel'dovich ICs on a displaced nrow^3 particle lattice in an ng^3 mesh
+ * (ng % nrow == 0): q = (i+1/2, j+1/2, k+1/2) * ng/nrow, psi_hat_a =
+ * i k_a / |k|^3 * FFT[noise] (density spectrum ~ k^-2, k = 0 and the Nyquist
+ * k_a removed), x = q + s psi wrapped into [0, ng), v = vel_factor * s psi,
+ * with s set so that the rms of |s psi| is rms_cells.  noise is a white
+ * Gaussian field, nrow^3 FP64, row-major.  Particle p = (i*nrow + j)*nrow + k.
+ * Synchronous; allocates FFT scratch for the call. */
+int pm_ic_zeldovich(int nrow, int ng, const double* noise, double rms_cells, double vel_factor,
+                    double* posx, double* posy, double* posz, double* vx, double* vy, double* vz,
+                    void* stream);
+
 /* ---- 2. fused one-box step --------------------------------------------- */
 typedef struct pm_plan pm_plan_t;
 
@@ -158,13 +170,14 @@ int pm_step(pm_plan_t* plan, double* posx, double* posy, double* posz, double* v
 int pm_run_kdk(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
                double* vz, int nsteps, double dt, int variant);
 
-/* Phase timing (hipEvents on the plan's stream).  When enabled, each entry
- * point records events around its kernels; pm_plan_phase_ms synchronises
+/* Phase timing (hipEvents on the plan's stream).  pm_plan_enable_timing
+ * reserves events for the next max_steps calls of pm_step (0 turns timing
+ * off); recording needs no synchronisation.  pm_plan_phase_ms synchronises
  * and returns the accumulated milliseconds per phase
  * (0 zero+deposit, 1 z fwd, 2 y fwd, 3 x+green, 4 y inv, 5 z inv, 6 interp)
  * and the number of samples; pm_plan_reset_timing clears them. */
 #define PM_NPHASES 7
-int pm_plan_enable_timing(pm_plan_t* plan, int enable);
+int pm_plan_enable_timing(pm_plan_t* plan, int max_steps);
 int pm_plan_phase_ms(pm_plan_t* plan, double* ms, int64_t* counts);
 int pm_plan_reset_timing(pm_plan_t* plan);
 

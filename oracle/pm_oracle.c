@@ -449,6 +449,11 @@ int orc_solve(int ng, int mode, double scale, const double* delta, double* phi) 
 /* ======================================================================
  * Periodic one-box step.
  * ==================================================================== */
+/* Thread-private grids of the periodic deposit, kept between calls as the
+ * reference keeps its real_buf (main.cpp:42). */
+static double* g_priv = NULL;
+static size_t g_priv_len = 0;
+
 void orc_deposit_periodic(int64_t np, const double* posx, const double* posy, const double* posz,
                           double mass, int ng, double* delta) {
     /* init_buf + deposit_particles + reduce_private_to_shared + ghost fold
@@ -457,31 +462,38 @@ void orc_deposit_periodic(int64_t np, const double* posx, const double* posy, co
      * thread as in utils.cpp:74-77. */
     const size_t nc = (size_t)ng * ng * ng;
     const int nt = omp_get_max_threads();
-    double* priv = (double*)calloc(nc * (size_t)(nt > 1 ? nt - 1 : 1), sizeof(double));
-    for (size_t c = 0; c < nc; ++c) delta[c] = 0.0;
+    const size_t need = nc * (size_t)(nt > 1 ? nt - 1 : 0);
+    if (need > g_priv_len) {
+        free(g_priv);
+        g_priv = (double*)malloc(sizeof(double) * need);
+        g_priv_len = g_priv ? need : 0;
+    }
+    double* priv = g_priv;
 #pragma omp parallel num_threads(nt)
     {
         const int tid = omp_get_thread_num();
         double* acc = tid == 0 ? delta : priv + (size_t)(tid - 1) * nc;
+        for (size_t c = 0; c < nc; ++c) acc[c] = 0.0;
+#pragma omp barrier
 #pragma omp for schedule(static)
         for (int64_t p = 0; p < np; ++p) {
             const int ix = (int)posx[p], iy = (int)posy[p], iz = (int)posz[p];
             const double tx = posx[p] - ix, ty = posy[p] - iy, tz = posz[p] - iz;
-            const size_t x0 = (size_t)ix, x1 = (size_t)((ix + 1) % ng);
-            const size_t y0 = (size_t)iy, y1 = (size_t)((iy + 1) % ng);
-            const size_t z0 = (size_t)iz, z1 = (size_t)((iz + 1) % ng);
+            const size_t N = (size_t)ng;
+            const size_t x0 = (size_t)ix * N * N, x1 = (size_t)(ix + 1 == ng ? 0 : ix + 1) * N * N;
+            const size_t y0 = (size_t)iy * N, y1 = (size_t)(iy + 1 == ng ? 0 : iy + 1) * N;
+            const size_t z0 = (size_t)iz, z1 = (size_t)(iz + 1 == ng ? 0 : iz + 1);
             const double sx0 = 1.0 - tx, sx1 = tx, sy0 = 1.0 - ty, sy1 = ty;
             const double sxy00 = sx0 * sy0, sxy10 = sx1 * sy0, sxy01 = sx0 * sy1, sxy11 = sx1 * sy1;
             const double m0 = mass * (1.0 - tz), m1 = mass * tz;
-            const size_t N = (size_t)ng;
-            acc[(x0 * N + y0) * N + z0] += sxy00 * m0;
-            acc[(x1 * N + y0) * N + z0] += sxy10 * m0;
-            acc[(x0 * N + y1) * N + z0] += sxy01 * m0;
-            acc[(x1 * N + y1) * N + z0] += sxy11 * m0;
-            acc[(x0 * N + y0) * N + z1] += sxy00 * m1;
-            acc[(x1 * N + y0) * N + z1] += sxy10 * m1;
-            acc[(x0 * N + y1) * N + z1] += sxy01 * m1;
-            acc[(x1 * N + y1) * N + z1] += sxy11 * m1;
+            acc[x0 + y0 + z0] += sxy00 * m0;
+            acc[x1 + y0 + z0] += sxy10 * m0;
+            acc[x0 + y1 + z0] += sxy01 * m0;
+            acc[x1 + y1 + z0] += sxy11 * m0;
+            acc[x0 + y0 + z1] += sxy00 * m1;
+            acc[x1 + y0 + z1] += sxy10 * m1;
+            acc[x0 + y1 + z1] += sxy01 * m1;
+            acc[x1 + y1 + z1] += sxy11 * m1;
         }
     }
 #pragma omp parallel for schedule(static)
@@ -490,57 +502,74 @@ void orc_deposit_periodic(int64_t np, const double* posx, const double* posy, co
         for (int t = 1; t < nt; ++t) s += priv[(size_t)(t - 1) * nc + c];
         delta[c] = s - 1.0;
     }
-    free(priv);
 }
 
-static inline double grad_node(const double* phi, int ng, int axis, int i, int j, int k) {
-    /* centred difference of utils.cpp:130-132 on the periodic mesh */
-    const size_t N = (size_t)ng;
-    int im = i, ip = i, jm = j, jp = j, km = k, kp = k;
-    if (axis == 0) {
-        im = (i + ng - 1) % ng;
-        ip = (i + 1) % ng;
-    } else if (axis == 1) {
-        jm = (j + ng - 1) % ng;
-        jp = (j + 1) % ng;
-    } else {
-        km = (k + ng - 1) % ng;
-        kp = (k + 1) % ng;
+/* The three centred-difference meshes (utils.cpp:115-137) on the periodic
+ * mesh, kept between calls like the reference's real_buf1..3 (main.cpp:52-54). */
+static double* g_grad = NULL;
+static size_t g_grad_len = 0;
+
+static const double* periodic_gradients(const double* phi, int ng) {
+    const size_t N = (size_t)ng, nc = N * N * N;
+    if (3 * nc > g_grad_len) {
+        free(g_grad);
+        g_grad = (double*)malloc(sizeof(double) * 3 * nc);
+        g_grad_len = g_grad ? 3 * nc : 0;
     }
-    return (phi[((size_t)im * N + jm) * N + km] - phi[((size_t)ip * N + jp) * N + kp]) * 0.5;
+    double* gx = g_grad;
+    double* gy = g_grad + nc;
+    double* gz = g_grad + 2 * nc;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 0; i < ng; ++i)
+        for (int j = 0; j < ng; ++j) {
+            const size_t im = (size_t)((i + ng - 1) % ng), ip = (size_t)((i + 1) % ng);
+            const size_t jm = (size_t)((j + ng - 1) % ng), jp = (size_t)((j + 1) % ng);
+            const double* row = phi + ((size_t)i * N + j) * N;
+            const double* rxm = phi + (im * N + j) * N;
+            const double* rxp = phi + (ip * N + j) * N;
+            const double* rym = phi + ((size_t)i * N + jm) * N;
+            const double* ryp = phi + ((size_t)i * N + jp) * N;
+            const size_t o = ((size_t)i * N + j) * N;
+            for (int k = 0; k < ng; ++k) {
+                const int km = k == 0 ? ng - 1 : k - 1, kp = k + 1 == ng ? 0 : k + 1;
+                gx[o + k] = (rxm[k] - rxp[k]) * 0.5;
+                gy[o + k] = (rym[k] - ryp[k]) * 0.5;
+                gz[o + k] = (row[km] - row[kp]) * 0.5;
+            }
+        }
+    return g_grad;
 }
 
-static inline void force_at(const double* phi, int ng, double x, double y, double z, double* g) {
-    /* trilerp1 (utils_inline.h:55-70) of the three centred differences */
+static inline double trilerp_periodic(const double* f, size_t x0, size_t x1, size_t y0, size_t y1, size_t z0,
+                                      size_t z1, double tx, double ty, double tz) {
+    /* trilerp1 (utils_inline.h:55-70) with periodic node offsets */
+    const double mx = 1.0 - tx, my = 1.0 - ty, mz = 1.0 - tz;
+    const double f00 = f[x0 + y0 + z0] * mx + f[x1 + y0 + z0] * tx;
+    const double f10 = f[x0 + y1 + z0] * mx + f[x1 + y1 + z0] * tx;
+    const double fz0 = f00 * my + f10 * ty;
+    const double g00 = f[x0 + y0 + z1] * mx + f[x1 + y0 + z1] * tx;
+    const double g10 = f[x0 + y1 + z1] * mx + f[x1 + y1 + z1] * tx;
+    const double fz1 = g00 * my + g10 * ty;
+    return fz0 * mz + fz1 * tz;
+}
+
+static inline void force_from_grads(const double* grads, int ng, double x, double y, double z, double* g) {
+    const size_t N = (size_t)ng, nc = N * N * N;
     const int ix = (int)x, iy = (int)y, iz = (int)z;
     const double tx = x - ix, ty = y - iy, tz = z - iz;
-    const double mx = 1.0 - tx, my = 1.0 - ty, mz = 1.0 - tz;
-    const int x1 = (ix + 1) % ng, y1 = (iy + 1) % ng, z1 = (iz + 1) % ng;
-    for (int a = 0; a < 3; ++a) {
-        const double f000 = grad_node(phi, ng, a, ix, iy, iz);
-        const double f100 = grad_node(phi, ng, a, x1, iy, iz);
-        const double f010 = grad_node(phi, ng, a, ix, y1, iz);
-        const double f110 = grad_node(phi, ng, a, x1, y1, iz);
-        const double f001 = grad_node(phi, ng, a, ix, iy, z1);
-        const double f101 = grad_node(phi, ng, a, x1, iy, z1);
-        const double f011 = grad_node(phi, ng, a, ix, y1, z1);
-        const double f111 = grad_node(phi, ng, a, x1, y1, z1);
-        const double f00 = f000 * mx + f100 * tx;
-        const double f10 = f010 * mx + f110 * tx;
-        const double fz0 = f00 * my + f10 * ty;
-        const double g00 = f001 * mx + f101 * tx;
-        const double g10 = f011 * mx + f111 * tx;
-        const double fz1 = g00 * my + g10 * ty;
-        g[a] = fz0 * mz + fz1 * tz;
-    }
+    const size_t x0 = (size_t)ix * N * N, x1 = (size_t)(ix + 1 == ng ? 0 : ix + 1) * N * N;
+    const size_t y0 = (size_t)iy * N, y1 = (size_t)(iy + 1 == ng ? 0 : iy + 1) * N;
+    const size_t z0 = (size_t)iz, z1 = (size_t)(iz + 1 == ng ? 0 : iz + 1);
+    for (int d = 0; d < 3; ++d) g[d] = trilerp_periodic(grads + d * nc, x0, x1, y0, y1, z0, z1, tx, ty, tz);
 }
 
 void orc_interp_force(int64_t np, const double* posx, const double* posy, const double* posz,
                       const double* phi, int ng, double* gx, double* gy, double* gz) {
+    const double* grads = periodic_gradients(phi, ng);
 #pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < np; ++p) {
         double g[3];
-        force_at(phi, ng, posx[p], posy[p], posz[p], g);
+        force_from_grads(grads, ng, posx[p], posy[p], posz[p], g);
         gx[p] = g[0];
         gy[p] = g[1];
         gz[p] = g[2];
@@ -556,13 +585,14 @@ static inline double wrap_pos(double x, double box) {
 void orc_interp_kick_drift(int64_t np, double* posx, double* posy, double* posz, double* vx,
                            double* vy, double* vz, const double* phi, int ng, double kick,
                            double drift) {
-    /* update_particles (utils.cpp:141-170) plus the drift it leaves out
-     * (utils.cpp:145-146 commented) */
+    /* centered_derivatives + update_particles (utils.cpp:115-170) plus the
+     * drift the reference leaves out (utils.cpp:145-146 commented) */
     const double box = (double)ng;
+    const double* grads = periodic_gradients(phi, ng);
 #pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < np; ++p) {
         double g[3];
-        force_at(phi, ng, posx[p], posy[p], posz[p], g);
+        force_from_grads(grads, ng, posx[p], posy[p], posz[p], g);
         vx[p] += g[0] * kick;
         vy[p] += g[1] * kick;
         vz[p] += g[2] * kick;

@@ -219,3 +219,32 @@ def ref_step(ng: int, pos, a: float = 0.9):
     assert rc == 0
     out["px"], out["py"], out["pz"] = p
     return out
+
+
+# ------------------------------------------------------- initial conditions
+def zeldovich_ic(noise: np.ndarray, ng: int, rms_cells: float, vel_factor: float):
+    """numpy restatement of pm_ic_zeldovich (FP64): displaced lattice from a
+    white-noise field on the nrow^3 particle lattice."""
+    nrow = noise.shape[0]
+    w = np.fft.rfftn(noise)
+    f = np.fft.fftfreq(nrow, 1.0 / nrow)
+    kx = f[:, None, None]
+    ky = f[None, :, None]
+    kz = np.arange(nrow // 2 + 1, dtype=np.float64)[None, None, :]
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    psi = []
+    for ka in (kx, ky, kz):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            fac = np.where((k2 > 0) & (np.abs(ka) != nrow // 2), ka / (k2 * np.sqrt(k2)), 0.0)
+        psi.append(np.fft.irfftn(1j * fac * w, s=noise.shape, axes=(0, 1, 2)).ravel())
+    s = rms_cells / np.sqrt(np.mean(psi[0] ** 2 + psi[1] ** 2 + psi[2] ** 2))
+    idx = (np.arange(nrow) + 0.5) * (ng / nrow)
+    Q = np.meshgrid(idx, idx, idx, indexing="ij")
+    pos, vel = [], []
+    for q, p in zip(Q, psi):
+        x = q.ravel() + s * p
+        x = x - ng * np.floor(x / ng)
+        x[x >= ng] = 0.0
+        pos.append(x)
+        vel.append(vel_factor * s * p)
+    return pos, vel

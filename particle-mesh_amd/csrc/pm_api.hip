@@ -226,20 +226,31 @@ extern "C" int pm_plan_set_stream(pm_plan_t* p, void* stream) {
 extern "C" void* pm_plan_density(pm_plan_t* p) { return p ? p->rho : nullptr; }
 extern "C" void* pm_plan_potential(pm_plan_t* p) { return p ? p->phi : nullptr; }
 
+// Record event i of the current timing slot (no-op when timing is off or
+// every slot is used).
 static inline void mark(pm_plan* p, int i) {
-    if (p->timing) (void)hipEventRecord(p->ev[i], p->stream);
+    if (p->timing_used < p->timing_slots)
+        (void)hipEventRecord(p->ev[(size_t)p->timing_used * (PM_NPHASES + 1) + i], p->stream);
 }
 
-static void accumulate(pm_plan* p, int first, int last) {
-    if (!p->timing) return;
-    (void)hipEventSynchronize(p->ev[last]);
-    for (int i = first; i < last; ++i) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]) == hipSuccess) {
-            p->phase_ms[i] += ms;
-            p->phase_n[i] += 1;
+static inline void next_slot(pm_plan* p) {
+    if (p->timing_used < p->timing_slots) ++p->timing_used;
+}
+
+// Sum the elapsed time of every recorded slot into phase_ms (synchronises).
+static void collect(pm_plan* p) {
+    for (int s = 0; s < p->timing_used; ++s) {
+        hipEvent_t* e = &p->ev[(size_t)s * (PM_NPHASES + 1)];
+        (void)hipEventSynchronize(e[PM_NPHASES]);
+        for (int i = 0; i < PM_NPHASES; ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e[i], e[i + 1]) == hipSuccess) {
+                p->phase_ms[i] += ms;
+                p->phase_n[i] += 1;
+            }
         }
     }
+    p->timing_used = 0;
 }
 
 static int deposit_impl(pm_plan* p, const double* x, const double* y, const double* z, int variant) {
@@ -276,10 +287,10 @@ extern "C" int pm_deposit(pm_plan_t* p, const double* x, const double* y, const 
 
 extern "C" int pm_solve(pm_plan_t* p) {
     if (!p) return fail(PM_ERR_ARG, "pm_solve: null plan");
-    const bool t = p->timing;
-    p->timing = false;
+    const int slots = p->timing_slots;
+    p->timing_slots = 0;  // only pm_step records phases
     const int rc = solve_impl(p);
-    p->timing = t;
+    p->timing_slots = slots;
     return rc;
 }
 
@@ -310,7 +321,7 @@ extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx
                        p->stream);
     if (rc) return rc;
     mark(p, 7);
-    accumulate(p, 0, 7);
+    next_slot(p);
     return PM_OK;
 }
 
@@ -326,18 +337,25 @@ extern "C" int pm_run_kdk(pm_plan_t* p, double* x, double* y, double* z, double*
     return PM_OK;
 }
 
-extern "C" int pm_plan_enable_timing(pm_plan_t* p, int enable) {
-    if (!p) return fail(PM_ERR_ARG, "pm_plan_enable_timing: null plan");
-    if (enable && !p->ev[0]) {
-        for (auto& e : p->ev)
-            if (hip_ok(hipEventCreate(&e), "pm_plan_enable_timing: event")) return PM_ERR_HIP;
-    }
-    p->timing = enable != 0;
+extern "C" int pm_plan_enable_timing(pm_plan_t* p, int max_steps) {
+    if (!p || max_steps < 0) return fail(PM_ERR_ARG, "pm_plan_enable_timing: bad argument");
+    collect(p);
+    for (auto& e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    p->ev.assign((size_t)max_steps * (PM_NPHASES + 1), nullptr);
+    for (auto& e : p->ev)
+        if (hip_ok(hipEventCreate(&e), "pm_plan_enable_timing: event")) {
+            p->timing_slots = 0;
+            return PM_ERR_HIP;
+        }
+    p->timing_slots = max_steps;
+    p->timing_used = 0;
     return PM_OK;
 }
 
 extern "C" int pm_plan_phase_ms(pm_plan_t* p, double* ms, int64_t* counts) {
     if (!p || !ms) return fail(PM_ERR_ARG, "pm_plan_phase_ms: null argument");
+    collect(p);
     for (int i = 0; i < PM_NPHASES; ++i) {
         ms[i] = p->phase_ms[i];
         if (counts) counts[i] = p->phase_n[i];
@@ -347,6 +365,7 @@ extern "C" int pm_plan_phase_ms(pm_plan_t* p, double* ms, int64_t* counts) {
 
 extern "C" int pm_plan_reset_timing(pm_plan_t* p) {
     if (!p) return fail(PM_ERR_ARG, "pm_plan_reset_timing: null plan");
+    collect(p);
     for (int i = 0; i < PM_NPHASES; ++i) {
         p->phase_ms[i] = 0.0;
         p->phase_n[i] = 0;

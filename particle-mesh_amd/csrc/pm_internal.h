@@ -2,6 +2,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "pm_api.h"
 #include "pm_common.h"
@@ -75,8 +76,12 @@ struct pm_plan {
     void* specA = nullptr;   // (ng/2+1) ng^2 complex
     void* specB = nullptr;
     pm::FftTables tab;
-    bool timing = false;
-    hipEvent_t ev[PM_NPHASES + 1] = {};
+    // Phase timing: one set of PM_NPHASES+1 events per recorded step, read
+    // back (and summed) only by pm_plan_phase_ms, so that recording needs no
+    // synchronisation inside a timed loop.
+    int timing_slots = 0;
+    int timing_used = 0;
+    std::vector<hipEvent_t> ev;
     double phase_ms[PM_NPHASES] = {};
     int64_t phase_n[PM_NPHASES] = {};
 };

@@ -59,6 +59,8 @@ SIGNATURES = {
     "pm_fft_apply_green": (_c_int, [_c_vp, _c_dbl]),
     "pm_fft_backward": (_c_int, [_c_vp, _c_vp]),
     "pm_fft_spectrum": (_c_vp, [_c_vp]),
+    "pm_ic_zeldovich": (_c_int, [_c_int, _c_int, _c_vp, _c_dbl, _c_dbl, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
+                                 _c_vp]),
     "pm_plan_create": (_c_int, [_c_vpp, _c_int, _c_i64, _c_int, _c_int, _c_dbl, _c_vp]),
     "pm_plan_destroy": (_c_int, [_c_vp]),
     "pm_plan_set_stream": (_c_int, [_c_vp, _c_vp]),
@@ -207,8 +209,9 @@ class Plan:
     def run_kdk(self, pos, vel, nsteps: int, dt: float, variant: int = PM_DEPOSIT_ATOMIC):
         check(lib().pm_run_kdk(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel), nsteps, dt, variant))
 
-    def enable_timing(self, on: bool = True):
-        check(lib().pm_plan_enable_timing(self.h, 1 if on else 0))
+    def enable_timing(self, max_steps: int):
+        """Record phase events for the next max_steps steps (0 = off)."""
+        check(lib().pm_plan_enable_timing(self.h, max_steps))
 
     def reset_timing(self):
         check(lib().pm_plan_reset_timing(self.h))
@@ -255,6 +258,18 @@ class FFT:
         n = self.ng
         v = device_view(lib().pm_fft_spectrum(self.h), ((n // 2 + 1), n, n, 2), mesh_dtype(self.precision))
         return torch.view_as_complex(v)
+
+
+def zeldovich_ic(nrow: int, ng: int, noise, rms_cells: float, vel_factor: float, stream: int | None = None):
+    """Zel'dovich lattice ICs on the GPU (pm_ic_zeldovich); noise is an
+    nrow^3 float64 device tensor.  Returns (pos, vel) lists of device tensors."""
+    torch = _torch()
+    n = nrow ** 3
+    pos = [torch.empty(n, dtype=torch.float64, device=noise.device) for _ in range(3)]
+    vel = [torch.empty(n, dtype=torch.float64, device=noise.device) for _ in range(3)]
+    check(lib().pm_ic_zeldovich(nrow, ng, dptr(noise), rms_cells, vel_factor, *(dptr(t) for t in pos),
+                                *(dptr(t) for t in vel), current_stream() if stream is None else stream))
+    return pos, vel
 
 
 def range_errors() -> int:

@@ -236,3 +236,21 @@ def test_kdk_config1_fp64_matches_oracle(dev):
     vmax = max(np.abs(v).max() for v in vh)
     for a, b in zip(vel, vh):
         assert np.max(np.abs(host(a) - b)) < 1e-8 * vmax
+
+
+@pytest.mark.parametrize("nrow,ng", [(16, 16), (32, 64)])
+def test_zeldovich_ic_vs_numpy(dev, nrow, ng):
+    import torch
+
+    print(f"case zeldovich nrow={nrow} ng={ng}", flush=True)
+    noise = np.random.default_rng(nrow).standard_normal((nrow, nrow, nrow))
+    pos, vel = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise.ravel(), device=dev), 0.7, 2.0)
+    ref_pos, ref_vel = pyoracle.zeldovich_ic(noise, ng, 0.7, 2.0)
+    for a, b in zip(pos, ref_pos):
+        d = host(a) - b
+        d -= ng * np.round(d / ng)
+        assert np.max(np.abs(d)) < 1e-10
+    for a, b in zip(vel, ref_vel):
+        assert np.max(np.abs(host(a) - b)) < 1e-10
+    disp = np.sqrt(np.mean(sum(host(v) ** 2 for v in vel))) / 2.0
+    assert abs(disp - 0.7) < 1e-9
