@@ -47,7 +47,9 @@ extern "C" {
 #define PM_GREEN_CONT_K2 1   /* -N^2 / (pi |n|^2): -4 pi G rho_bar / k^2, unit box, G = M = 1 */
 
 /* Deposit variants. */
-#define PM_DEPOSIT_ATOMIC 0  /* one thread per particle, 8 global atomic adds */
+#define PM_DEPOSIT_ATOMIC 0  /* one thread per particle, 8 global float atomic adds */
+#define PM_DEPOSIT_SORTED 1  /* particles binned by 16^3-cell tile, exact 64-bit fixed-point
+                                accumulation in LDS, deterministic (no global float atomics) */
 
 /* Last error message of the calling thread ("" if none). */
 const char* pm_last_error(void);

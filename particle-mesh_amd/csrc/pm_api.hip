@@ -175,7 +175,8 @@ extern "C" void* pm_fft_spectrum(pm_fft_t* f) { return f ? f->specB : nullptr; }
 extern "C" int pm_plan_destroy(pm_plan_t* p) {
     if (!p) return PM_OK;
     tables_destroy(&p->tab);
-    void* bufs[4] = {p->rho, p->phi, p->specA, p->specB};
+    void* bufs[10] = {p->rho,        p->phi,         p->specA,        p->specB,         p->sorted.cnt,
+                      p->sorted.off, p->sorted.key, p->sorted.slot, p->sorted.idx, p->sorted.shell};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& e : p->ev)
@@ -253,12 +254,40 @@ static void collect(pm_plan* p) {
     p->timing_used = 0;
 }
 
+static int ensure_sorted(pm_plan* p) {
+    SortedScratch& s = p->sorted;
+    if (s.idx) return PM_OK;
+    const int ts = sorted_tile(p->ng);
+    const size_t nt = (size_t)(p->ng / ts);
+    const size_t ntiles = nt * nt * nt;
+    const size_t rs = p->prec == PM_FP32 ? sizeof(float) : sizeof(double);
+    int rc = hip_ok(hipMalloc((void**)&s.cnt, 4 * ntiles), "sorted deposit: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc((void**)&s.off, 4 * (ntiles + 1)), "sorted deposit: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc((void**)&s.key, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc((void**)&s.slot, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc(&s.shell, rs * sorted_shell_elems(p->ng)), "sorted deposit: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc((void**)&s.idx, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
+    if (rc) {
+        void* bufs[6] = {s.cnt, s.off, s.key, s.slot, s.shell, s.idx};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        s = SortedScratch();
+    }
+    return rc;
+}
+
 static int deposit_impl(pm_plan* p, const double* x, const double* y, const double* z, int variant) {
-    if (variant != PM_DEPOSIT_ATOMIC) return fail(PM_ERR_ARG, "pm_deposit: unknown variant");
     const size_t nc = (size_t)p->ng * p->ng * p->ng;
+    const double mass = (double)nc / (double)p->np;
+    if (variant == PM_DEPOSIT_SORTED) {
+        int rc = ensure_sorted(p);
+        if (rc) return rc;
+        return launch_deposit_sorted(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
+                                     p->stream);
+    }
+    if (variant != PM_DEPOSIT_ATOMIC) return fail(PM_ERR_ARG, "pm_deposit: unknown variant");
     int rc = launch_fill(p->prec, p->rho, nc, 0.0, p->stream);
     if (rc) return rc;
-    const double mass = (double)nc / (double)p->np;
     return launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, range_counter(), p->stream);
 }
 

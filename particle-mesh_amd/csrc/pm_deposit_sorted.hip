@@ -1,0 +1,270 @@
+// Deterministic CIC deposit: particles binned by mesh tile, each tile's
+// mass accumulated exactly in 64-bit fixed point in LDS.
+//
+// deposit_particles (utils.cpp:70-102) gives every thread a private grid
+// and sums the grids (utils.cpp:104-113); the GPU equivalent of a private
+// grid is a workgroup's LDS tile:
+//   1. bin_count   : tile of each particle's cell, slot within the tile
+//                    (wave-aggregated atomics: one per distinct tile per wave)
+//   2. scan        : exclusive prefix sum of the tile counts
+//   3. bin_scatter : particle indices in tile order
+//   4. tiles       : one workgroup per tile of TS^3 cells; the CIC weights of
+//                    its particles are added in 2^-32 fixed point to a
+//                    (TS+1)^3 LDS node tile (integer adds: exact, order-free),
+//                    the TS^3 owned nodes are written to the mesh and the
+//                    upper shell (nodes TS on some axis, owned by the next
+//                    tiles) to a per-tile shell buffer
+//   5. fold        : each tile adds the shells of its lower neighbours to its
+//                    lower-face nodes in a fixed order -- the periodic ghost
+//                    fold of reorder_slab.cpp:43-67 at tile granularity.
+// Every mesh node is written by exactly one workgroup: no global float
+// atomics, and the result does not depend on the particle order.
+#include "pm_internal.h"
+
+namespace pm {
+
+constexpr uint32_t kInvalidKey = 0xFFFFFFFFu;
+
+__device__ __forceinline__ bool in_box_s(double x, double box) { return x >= 0.0 && x < box; }
+
+// 1. tile id and slot of every particle.
+__global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px, const double* __restrict__ py,
+                                                   const double* __restrict__ pz, int64_t np, int ng, int ts,
+                                                   uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
+                                                   uint32_t* __restrict__ slot,
+                                                   unsigned long long* __restrict__ err) {
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int lane = __lane_id();
+    bool pending = false;
+    uint32_t k = kInvalidKey;
+    if (i < np) {
+        const double x = px[i], y = py[i], z = pz[i];
+        const double box = (double)ng;
+        if (in_box_s(x, box) && in_box_s(y, box) && in_box_s(z, box)) {
+            const int nt = ng / ts;
+            const int tx = (int)x / ts, ty = (int)y / ts, tz = (int)z / ts;
+            k = (uint32_t)((tx * nt + ty) * nt + tz);
+            pending = true;
+        } else {
+            atomicAdd(err, 1ULL);
+        }
+    }
+    uint32_t s = 0;
+    // wave-uniform loop: one atomic per distinct key in the wave
+    while (true) {
+        const unsigned long long todo = __ballot(pending);
+        if (todo == 0ULL) break;
+        const int leader = __ffsll((long long)todo) - 1;
+        const uint32_t lk = (uint32_t)__shfl((int)k, leader);
+        const bool mine = pending && k == lk;
+        const unsigned long long m = __ballot(mine);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[lk], (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, leader);
+        if (mine) {
+            s = base + (uint32_t)__popcll(m & ((1ULL << lane) - 1ULL));
+            pending = false;
+        }
+    }
+    if (i < np) {
+        key[i] = k;
+        slot[i] = s;
+    }
+}
+
+// 2. exclusive scan of n counts into off[0..n] (one workgroup of 1024).
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, int n) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += cnt[i];
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // Hillis-Steele inclusive scan
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t == 0 ? 0u : part[t - 1];
+    for (int i = b; i < e; ++i) {
+        off[i] = run;
+        run += cnt[i];
+    }
+    if (t == 1023) off[n] = part[1023];
+}
+
+// 3. particle indices in tile order.
+__global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
+                                                     const uint32_t* __restrict__ off, int64_t np,
+                                                     uint32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (i >= np) return;
+    const uint32_t k = key[i];
+    if (k == kInvalidKey) return;
+    idx[off[k] + slot[i]] = (uint32_t)i;
+}
+
+template <int TS> __host__ __device__ constexpr int shell_size() {
+    return (TS + 1) * (TS + 1) * (TS + 1) - TS * TS * TS;
+}
+
+// Shell index of a node with local coordinates (a, b, c), max == TS:
+// face x (a == TS): (TS+1)^2 entries, face y (b == TS, a < TS): TS (TS+1),
+// face z (c == TS, a, b < TS): TS^2.
+template <int TS> __device__ __forceinline__ int shell_index(int a, int b, int c) {
+    constexpr int E = TS + 1;
+    if (a == TS) return b * E + c;
+    if (b == TS) return E * E + a * E + c;
+    return E * E + TS * E + a * TS + b;
+}
+
+// 4. one workgroup per tile.
+template <typename Rl, int TS>
+__global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict__ px, const double* __restrict__ py,
+                                                       const double* __restrict__ pz,
+                                                       const uint32_t* __restrict__ idx,
+                                                       const uint32_t* __restrict__ off, int ng, double mass,
+                                                       Rl* __restrict__ rho, Rl* __restrict__ shell) {
+    constexpr int E = TS + 1, E3 = E * E * E, SH = shell_size<TS>();
+    __shared__ unsigned long long acc[E3];
+    const int nt = ng / TS;
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
+              tz = (int)(t % (uint32_t)nt);
+    const int ox = tx * TS, oy = ty * TS, oz = tz * TS;
+    for (int i = threadIdx.x; i < E3; i += 256) acc[i] = 0ULL;
+    __syncthreads();
+    const uint32_t beg = off[t], end = off[t + 1];
+    const double fix = 4294967296.0 * mass;  // 2^32 * m
+    for (uint32_t q = beg + threadIdx.x; q < end; q += 256) {
+        const uint32_t p = idx[q];
+        const double x = px[p], y = py[p], z = pz[p];
+        const int ix = (int)x, iy = (int)y, iz = (int)z;
+        const double fx = x - ix, fy = y - iy, fz = z - iz;
+        // cic_deposit weights (utils_inline.h:25-49)
+        const double sx0 = 1.0 - fx, sy0 = 1.0 - fy;
+        const double sxy00 = sx0 * sy0, sxy10 = fx * sy0, sxy01 = sx0 * fy, sxy11 = fx * fy;
+        const double m0 = fix * (1.0 - fz), m1 = fix * fz;
+        const int a = ix - ox, b = iy - oy, c = iz - oz;
+        unsigned long long* n0 = acc + (a * E + b) * E + c;
+        atomicAdd(n0, (unsigned long long)__double2ll_rn(sxy00 * m0));
+        atomicAdd(n0 + E * E, (unsigned long long)__double2ll_rn(sxy10 * m0));
+        atomicAdd(n0 + E, (unsigned long long)__double2ll_rn(sxy01 * m0));
+        atomicAdd(n0 + E * E + E, (unsigned long long)__double2ll_rn(sxy11 * m0));
+        atomicAdd(n0 + 1, (unsigned long long)__double2ll_rn(sxy00 * m1));
+        atomicAdd(n0 + E * E + 1, (unsigned long long)__double2ll_rn(sxy10 * m1));
+        atomicAdd(n0 + E + 1, (unsigned long long)__double2ll_rn(sxy01 * m1));
+        atomicAdd(n0 + E * E + E + 1, (unsigned long long)__double2ll_rn(sxy11 * m1));
+    }
+    __syncthreads();
+    const double unfix = 1.0 / 4294967296.0;
+    const size_t N = (size_t)ng;
+    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
+        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
+        rho[((size_t)(ox + a) * N + (oy + b)) * N + (oz + c)] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
+    }
+    Rl* sh = shell + (size_t)t * SH;
+    for (int i = threadIdx.x; i < SH; i += 256) {
+        int a, b, c;
+        if (i < E * E) {
+            a = TS;
+            b = i / E;
+            c = i % E;
+        } else if (i < E * E + TS * E) {
+            const int j = i - E * E;
+            a = j / E;
+            b = TS;
+            c = j % E;
+        } else {
+            const int j = i - E * E - TS * E;
+            a = j / TS;
+            b = j % TS;
+            c = TS;
+        }
+        sh[i] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
+    }
+}
+
+// 5. add the lower neighbours' shells to each tile's lower-face nodes.
+template <typename Rl, int TS>
+__global__ __launch_bounds__(256) void k_fold_shells(const Rl* __restrict__ shell, int ng, Rl* __restrict__ rho) {
+    constexpr int SH = shell_size<TS>();
+    const int nt = ng / TS;
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
+              tz = (int)(t % (uint32_t)nt);
+    const size_t N = (size_t)ng;
+    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
+        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
+        const int on = (a == 0) | ((b == 0) << 1) | ((c == 0) << 2);
+        if (!on) continue;
+        const size_t node = ((size_t)(tx * TS + a) * N + (ty * TS + b)) * N + (tz * TS + c);
+        Rl v = rho[node];
+        for (int d = 1; d < 8; ++d) {
+            if ((d & on) != d) continue;
+            const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
+            const int sx = tx - dx < 0 ? nt - 1 : tx - dx;
+            const int sy = ty - dy < 0 ? nt - 1 : ty - dy;
+            const int sz = tz - dz < 0 ? nt - 1 : tz - dz;
+            const size_t st = ((size_t)sx * nt + sy) * nt + sz;
+            v += shell[st * SH + shell_index<TS>(a + TS * dx, b + TS * dy, c + TS * dz)];
+        }
+        rho[node] = v;
+    }
+}
+
+int sorted_tile(int ng) { return ng >= 16 ? 16 : 8; }
+
+size_t sorted_shell_elems(int ng) {
+    const int ts = sorted_tile(ng);
+    const size_t nt = (size_t)(ng / ts);
+    return nt * nt * nt * (size_t)(ts == 16 ? shell_size<16>() : shell_size<8>());
+}
+
+int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
+                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st) {
+    const int ts = sorted_tile(ng);
+    const int nt = ng / ts;
+    const uint32_t ntiles = (uint32_t)nt * nt * nt;
+    if (np > 0xFFFFFFF0LL) return fail(PM_ERR_ARG, "sorted deposit: more than 2^32 particles");
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(std::min<uint32_t>((ntiles + 255) / 256, 8192)), dim3(256), 0, st,
+                       s.cnt, (size_t)ntiles, 0u);
+    const unsigned nb = (unsigned)((np + 255) / 256);
+    if (np > 0) {
+        hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, px, py, pz, np, ng, ts, s.cnt, s.key, s.slot,
+                           err);
+    }
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, (int)ntiles);
+    if (np > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, s.key, s.slot, s.off, np, s.idx);
+    if (prec == PM_FP32) {
+        if (ts == 16) {
+            hipLaunchKernelGGL((k_deposit_tiles<float, 16>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx, s.off,
+                               ng, mass, (float*)rho, (float*)s.shell);
+            hipLaunchKernelGGL((k_fold_shells<float, 16>), dim3(ntiles), dim3(256), 0, st, (const float*)s.shell, ng,
+                               (float*)rho);
+        } else {
+            hipLaunchKernelGGL((k_deposit_tiles<float, 8>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx, s.off,
+                               ng, mass, (float*)rho, (float*)s.shell);
+            hipLaunchKernelGGL((k_fold_shells<float, 8>), dim3(ntiles), dim3(256), 0, st, (const float*)s.shell, ng,
+                               (float*)rho);
+        }
+    } else {
+        if (ts == 16) {
+            hipLaunchKernelGGL((k_deposit_tiles<double, 16>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx,
+                               s.off, ng, mass, (double*)rho, (double*)s.shell);
+            hipLaunchKernelGGL((k_fold_shells<double, 16>), dim3(ntiles), dim3(256), 0, st, (const double*)s.shell,
+                               ng, (double*)rho);
+        } else {
+            hipLaunchKernelGGL((k_deposit_tiles<double, 8>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx,
+                               s.off, ng, mass, (double*)rho, (double*)s.shell);
+            hipLaunchKernelGGL((k_fold_shells<double, 8>), dim3(ntiles), dim3(256), 0, st, (const double*)s.shell,
+                               ng, (double*)rho);
+        }
+    }
+    return check_launch("deposit_sorted");
+}
+
+}  // namespace pm

@@ -44,6 +44,28 @@ void tables_destroy(FftTables* t);
 int fft_pass(int prec, int N, int pass, hipStream_t st, const void* in, void* out, const void* twM,
              const void* twN, const GreenArgs& ga, double norm, bool sub1);
 
+// Grid-stride fill (used instead of hipMemsetAsync so that the step can be
+// captured in a graph).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T* __restrict__ p, size_t n, T v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Work space of the sorted deposit (pm_deposit_sorted.hip).
+struct SortedScratch {
+    uint32_t* cnt = nullptr;   // ntiles
+    uint32_t* off = nullptr;   // ntiles + 1
+    uint32_t* key = nullptr;   // np
+    uint32_t* slot = nullptr;  // np
+    uint32_t* idx = nullptr;   // np
+    void* shell = nullptr;     // ntiles * shell_size reals
+};
+int sorted_tile(int ng);
+size_t sorted_shell_elems(int ng);
+int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
+                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st);
+
 // Particle kernels (pm_particles.hip).
 int launch_fill(int prec, void* p, size_t n, double v, hipStream_t st);
 int launch_deposit_atomic(int prec, const double* px, const double* py, const double* pz,
@@ -76,6 +98,7 @@ struct pm_plan {
     void* specA = nullptr;   // (ng/2+1) ng^2 complex
     void* specB = nullptr;
     pm::FftTables tab;
+    pm::SortedScratch sorted;  // allocated on the first sorted deposit
     // Phase timing: one set of PM_NPHASES+1 events per recorded step, read
     // back (and summed) only by pm_plan_phase_ms, so that recording needs no
     // synchronisation inside a timed loop.

@@ -7,12 +7,6 @@ namespace pm {
 
 constexpr int kPartThreads = 256;
 
-template <typename Rl>
-__global__ __launch_bounds__(256) void k_fill(Rl* __restrict__ p, size_t n, Rl v) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
-
 int launch_fill(int prec, void* p, size_t n, double v, hipStream_t st) {
     const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
     if (prec == PM_FP32)

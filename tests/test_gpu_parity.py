@@ -254,3 +254,84 @@ def test_zeldovich_ic_vs_numpy(dev, nrow, ng):
         assert np.max(np.abs(host(a) - b)) < 1e-10
     disp = np.sqrt(np.mean(sum(host(v) ** 2 for v in vel))) / 2.0
     assert abs(disp - 0.7) < 1e-9
+
+
+# --------------------------------------------------------- sorted deposit
+def _plan_density(dev, ng, ph, prec, variant):
+    import torch
+
+    pos = to_dev(ph, dev)
+    with pmhip.Plan(ng, ph[0].size, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.deposit(pos, variant)
+        torch.cuda.synchronize()
+        return host(plan.density()).astype(np.float64)
+
+
+@pytest.mark.parametrize("ng,npart", [(8, 3000), (16, 4096), (32, 30000), (64, 250000)])
+def test_sorted_deposit_fp64_vs_oracle(dev, ng, npart):
+    print(f"case sorted deposit fp64 ng={ng} np={npart}", flush=True)
+    rng = np.random.default_rng(100 + ng)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    rho = _plan_density(dev, ng, ph, pmhip.PM_FP64, pmhip.PM_DEPOSIT_SORTED)
+    ref = pyoracle.deposit_periodic(ph, ng) + 1.0
+    # contributions are rounded to 2^-32 mass units before the exact sum
+    assert np.max(np.abs(rho - ref)) < 1e-8
+    assert pmhip.range_errors() == 0
+
+
+def test_sorted_deposit_deterministic_and_order_free(dev):
+    ng, npart = 32, 50000
+    rng = np.random.default_rng(7)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    a = _plan_density(dev, ng, ph, pmhip.PM_FP32, pmhip.PM_DEPOSIT_SORTED)
+    b = _plan_density(dev, ng, ph, pmhip.PM_FP32, pmhip.PM_DEPOSIT_SORTED)
+    perm = rng.permutation(npart)
+    c = _plan_density(dev, ng, [v[perm] for v in ph], pmhip.PM_FP32, pmhip.PM_DEPOSIT_SORTED)
+    assert np.array_equal(a, b)
+    assert np.array_equal(a, c)  # fixed-point sums do not depend on the particle order
+    atomic = _plan_density(dev, ng, ph, pmhip.PM_FP32, pmhip.PM_DEPOSIT_ATOMIC)
+    assert np.max(np.abs(a - atomic)) < 1e-5 * max(1.0, np.abs(a).max())
+
+
+def test_sorted_deposit_clustered_and_edges(dev):
+    """Dense cells (many particles in one cell), particles on tile faces and
+    at the periodic box edge, empty tiles."""
+    ng = 32
+    rng = np.random.default_rng(3)
+    n1 = 20000
+    clump = [np.full(n1, 15.999) + rng.uniform(0, 1e-3, n1) for _ in range(3)]
+    edge = [np.array([0.0, 31.999999, 16.0, 15.5, 31.0]), np.array([0.0, 31.999999, 16.0, 0.5, 15.9999999]),
+            np.array([0.0, 31.999999, 16.0, 31.5, 7.0])]
+    ph = [np.concatenate([c, e]) for c, e in zip(clump, edge)]
+    rho = _plan_density(dev, ng, ph, pmhip.PM_FP64, pmhip.PM_DEPOSIT_SORTED)
+    ref = pyoracle.deposit_periodic(ph, ng, mass=ng ** 3 / ph[0].size) + 1.0
+    assert np.max(np.abs(rho - ref)) < 1e-8 * max(1.0, np.abs(ref).max())
+    assert abs(rho.sum() - ng ** 3) < 1e-6
+
+
+def test_sorted_deposit_skips_out_of_box(dev):
+    ng = 16
+    pmhip.range_errors()
+    ph = [np.array([1.5, 16.0, -1e-9, 3.0, np.nan]), np.array([1.5] * 5), np.array([1.5] * 5)]
+    rho = _plan_density(dev, ng, ph, pmhip.PM_FP64, pmhip.PM_DEPOSIT_SORTED)
+    assert pmhip.range_errors() == 3
+    # the two in-box particles carry mass ng^3/5 each
+    assert abs(rho.sum() - 2 * ng ** 3 / 5) < 1e-6
+
+
+@pytest.mark.parametrize("variant", [pmhip.PM_DEPOSIT_ATOMIC, pmhip.PM_DEPOSIT_SORTED])
+def test_kdk_fp64_variants_match_oracle(dev, variant):
+    import torch
+
+    ng, npart, nsteps, dt = 32, 32 ** 3, 5, 0.01
+    ph = list(pyoracle.init_particles(npart, ng))
+    vh = [np.zeros(npart) for _ in range(3)]
+    pos, vel = to_dev(ph, dev), to_dev(vh, dev)
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.run_kdk(pos, vel, nsteps, dt, variant)
+        torch.cuda.synchronize()
+    pyoracle.run_kdk(ph, vh, ng, nsteps, dt, pyoracle.GREEN_CONT_K2, 1.0)
+    for a, b in zip(pos, ph):
+        d = host(a) - b
+        d -= ng * np.round(d / ng)
+        assert np.max(np.abs(d)) < 1e-7
