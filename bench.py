@@ -39,8 +39,11 @@ def parse():
     ap.add_argument("--rms", type=float, default=1.5)
     ap.add_argument("--dt", type=float, default=0.005)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--deposit", choices=["sorted", "atomic"], default="sorted")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-mesh leg")
+    ap.add_argument("--no-alt-deposit", action="store_true", help="skip the leg with the other deposit variant")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -83,7 +86,7 @@ def make_ics(nrow, ng, rms, seed, dev):
     return pos, vel
 
 
-def run_gpu(prec, pos0, vel0, args, ng, npart):
+def run_gpu(prec, pos0, vel0, args, ng, npart, variant):
     """Warm up, then time exactly args.steps steps; returns (ms_per_step,
     phase ms per launch)."""
     import torch
@@ -97,7 +100,7 @@ def run_gpu(prec, pos0, vel0, args, ng, npart):
 
     def one():
         nonlocal step
-        plan.step(pos, vel, 0.5 * dt if step == 0 else dt, dt)
+        plan.step(pos, vel, 0.5 * dt if step == 0 else dt, dt, variant)
         step += 1
 
     for _ in range(args.warmup):
@@ -121,6 +124,48 @@ def run_gpu(prec, pos0, vel0, args, ng, npart):
     plan.close()
     phases = {n: (m / c if c else None) for n, m, c in zip(pmhip.PHASE_NAMES, ms, cnt)}
     return ev_ms / args.steps, wall / args.steps * 1e3, phases, nerr, (pos, vel)
+
+
+def run_adjoint_leg(pos0, vel0, args, ng, npart, variant):
+    """pm_run_kdk over args.steps steps, then the full reverse sweep
+    pm_run_adjoint (reverse-time recomputation, no stored trajectory) with a
+    constant position cotangent; returns the time ratio and the reversal
+    drift of the recovered initial positions / velocities."""
+    import torch
+    import pmhip
+
+    pos = [p.clone() for p in pos0]
+    vel = [v.clone() for v in vel0]
+    plan = pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0)
+    n = args.steps
+    plan.run_kdk(pos, vel, 1, args.dt, variant)  # warm-up (allocates the binning)
+    pos = [p.clone() for p in pos0]
+    vel = [v.clone() for v in vel0]
+    xb = [torch.full_like(p, 1e-3) for p in pos]
+    vb = [torch.zeros_like(p) for p in pos]
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    plan.run_kdk(pos, vel, n, args.dt, variant)
+    ev[1].record(stream)
+    plan.run_adjoint(pos, vel, xb, vb, n, args.dt, variant)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    t_f, t_a = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    dx = []
+    for a, b in zip(pos, pos0):
+        d = a - b
+        d = d - ng * torch.round(d / ng)
+        dx.append(d)
+    d2 = sum((d * d) for d in dx)
+    vmax = max(float(v.abs().max()) for v in vel0)
+    dv = max(float((a - b).abs().max()) for a, b in zip(vel, vel0))
+    plan.close()
+    return {"steps": n, "force_evaluations": n + 1, "forward_ms": round(t_f, 3), "adjoint_ms": round(t_a, 3),
+            "ratio": round(t_a / t_f, 3), "reversal_drift_rms_cells": float(torch.sqrt(d2.mean())),
+            "reversal_drift_max_cells": float(torch.sqrt(d2.max())), "reversal_dv_max_rel": dv / vmax,
+            "mode": "reverse-time leapfrog recomputation, FP32 mesh, FP64 positions"}
 
 
 def cpu_baseline(pos_dev, vel_dev, ng, args):
@@ -174,22 +219,39 @@ def main():
     log(f"ICs: {nrow}^3 particles, {ng}^3 mesh, rms {args.rms} cells, seed {args.seed}")
     pos0, vel0 = make_ics(nrow, ng, args.rms, args.seed, dev)
 
-    log("FP32-mesh leg")
-    ms32, wall32, ph32, err32, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart)
+    variant = pmhip.PM_DEPOSIT_SORTED if args.deposit == "sorted" else pmhip.PM_DEPOSIT_ATOMIC
+    alt_variant = pmhip.PM_DEPOSIT_ATOMIC if args.deposit == "sorted" else pmhip.PM_DEPOSIT_SORTED
+    alt_name = "atomic" if args.deposit == "sorted" else "sorted"
+    log(f"FP32-mesh leg ({args.deposit} deposit)")
+    ms32, wall32, ph32, err32, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart, variant)
     log(f"FP32: {ms32:.3f} ms/step, phases {ph32}")
+    alt = None
+    if not args.no_alt_deposit:
+        log(f"FP32-mesh leg ({alt_name} deposit)")
+        ms_a, _, ph_a, _, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart, alt_variant)
+        alt = {"deposit": alt_name, "value": round(npart / (ms_a * 1e-3) / 1e6, 2), "ms_per_step": round(ms_a, 4),
+               "deposit_ms": round(ph_a["deposit"], 4)}
+        log(f"FP32 {alt_name}: {ms_a:.3f} ms/step")
     res64 = None
     if not args.no_fp64:
         log("FP64-mesh leg")
-        ms64, wall64, ph64, err64, _ = run_gpu(pmhip.PM_FP64, pos0, vel0, args, ng, npart)
+        ms64, wall64, ph64, err64, _ = run_gpu(pmhip.PM_FP64, pos0, vel0, args, ng, npart, variant)
         log(f"FP64: {ms64:.3f} ms/step, phases {ph64}")
         res64 = {"value": npart / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "phases_ms": ph64}
 
+    adj = None
+    if not args.no_adjoint:
+        log("adjoint leg")
+        adj = run_adjoint_leg(pos0, vel0, args, ng, npart, variant)
+        log(f"adjoint: {adj}")
+
     if args.copy_calibration:
+        # a plain streaming copy of 1 GiB (read 1 GiB, write 1 GiB) for PMC byte-counter scaling
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
         a.fill_(1.0)
         for _ in range(3):
-            b.copy_(a)
+            torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
 
     nbytes = algorithmic_bytes(ng, npart, pmhip.PM_FP32)
@@ -219,7 +281,8 @@ def main():
         "dtype": "f32 mesh/FFT/force, f64 positions/velocities",
         "data": "synthetic Zel'dovich lattice ICs generated on the GPU",
         "config": {"workload": f"PM KDK step, {nrow}^3 particles on a {ng}^3 periodic mesh, Zel'dovich k^-2 ICs "
-                               f"{args.rms}-cell rms, dt={args.dt}, CIC atomic deposit, 5-pass FFT, CONT_K2 Green",
+                               f"{args.rms}-cell rms, dt={args.dt}, {args.deposit} CIC deposit, 5-pass FFT, "
+                               f"CONT_K2 Green",
                    "ng": ng, "np": npart, "dt": args.dt, "rms_cells": args.rms},
         "phases_ms": {k: (round(v, 4) if v else v) for k, v in ph32.items()},
         "step_algorithmic_GB": round(step_bytes / 1e9, 3),
@@ -228,7 +291,9 @@ def main():
         "fp64": ({"value": round(res64["value"], 2), "ms_per_step": round(res64["ms_per_step"], 4),
                   "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["phases_ms"].items()}}
                  if res64 else None),
-        "adjoint_forward_ratio": None,
+        "alt_deposit": alt,
+        "adjoint_forward_ratio": adj["ratio"] if adj else None,
+        "adjoint": adj,
         "range_errors": err32,
         "roofline": roofline,
         "cpu_baseline": cpu,

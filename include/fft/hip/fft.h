@@ -1,0 +1,36 @@
+// The FFT class of the reference (fftw/fft.h:10-48, ffte1/fft.h:12-42) for
+// one GPU owning the whole box, over the C ABI pm_fft_* (include/pm_api.h).
+// buf is the caller-owned DEVICE slab of Ng^3 doubles (row stride Ng, the
+// ffte1 layout); forward() and backward() work in place on it; the half
+// spectrum lives in the handle.  As in ffte1, backward() is normalised by
+// 1/Ng^3 and apply_green() multiplies by the reference's KH Green's function
+// times 1/a (ffte1/fft.cpp:28-35,46-52).  color() is always 1 and there is
+// no communicator: one rank holds the only slab.
+#pragma once
+#include <cstddef>
+
+struct pm_fft;
+
+class FFT {
+public:
+    FFT(int n, double* buf);
+    ~FFT();
+    FFT(const FFT&) = delete;
+    FFT& operator=(const FFT&) = delete;
+
+    int color() const;
+
+    ptrdiff_t local_alloc() const;
+    ptrdiff_t local_n0() const;
+    ptrdiff_t local_0_start() const;
+
+    void forward();
+    void backward();
+
+    void apply_green(const double a) const;
+
+private:
+    int n_;
+    double* real_;
+    pm_fft* h_;
+};

@@ -75,16 +75,23 @@ int pm_deposit_particles(const double* posx, const double* posy, const double* p
                          double* buf, int nx, int ny, int nz, int x0, int y0, int z0,
                          double mass, void* stream);
 
-/* TransposeSlab + ReorderSlab at one box (transpose_slab.cpp:3-39,
- * reorder_slab.cpp:3-68): fold the + ghost planes of the (ng+1)^3 cube
- * periodically into the ng x ng x stride slab (stride >= ng; padding
- * columns untouched).  Deterministic gather, no atomics. */
+/* TransposeSlab at one box (transpose_slab.cpp:3-39): the rank's cube goes
+ * to the only slab owner, itself -- a device copy of the (ng+1)^3 cube. */
+int pm_transpose_slab(const double* cube, double* recv, int ng, void* stream);
+
+/* ReorderSlab at one box (reorder_slab.cpp:3-68): fold the + ghost planes of
+ * the (ng+1)^3 cube periodically into the ng x ng x stride slab (stride >=
+ * ng; padding columns untouched).  Deterministic gather, no atomics. */
 int pm_reorder_slab(const double* cube, double* slab, int ng, int stride, void* stream);
 
-/* ReorderSlabBack + TransposeSlabBack at one box (reorder_slab_back.cpp:3-49,
- * transpose_slab_back.cpp:3-49): slab -> (ng+3)^3 cube with the -1/+2
- * periodic halo on every axis. */
-int pm_reorder_slab_back(const double* slab, double* halo, int ng, int stride, void* stream);
+/* ReorderSlabBack at one box (reorder_slab_back.cpp:3-49): slab ->
+ * ng x (ng+3) x (ng+3) block with the -1/+2 periodic halo on y and z. */
+int pm_reorder_slab_back(const double* slab, double* out, int ng, int stride, void* stream);
+
+/* TransposeSlabBack at one box (transpose_slab_back.cpp:3-49): the x halo
+ * (plane -1 from plane ng-1, planes ng, ng+1 from planes 0, 1) around the
+ * ng planes of pm_reorder_slab_back's block -> the (ng+3)^3 halo cube. */
+int pm_transpose_slab_back(const double* in, double* halo, int ng, void* stream);
 
 /* centered_derivatives (utils.cpp:115-137): in is nx1*ny1*nz1, the outputs
  * (nx1-2)*(ny1-2)*(nz1-2); d_d?[i] = (in[i-1] - in[i+1]) * 0.5. */
@@ -171,6 +178,45 @@ int pm_step(pm_plan_t* plan, double* posx, double* posy, double* posz, double* v
  * force evaluations: kick dt/2 first, dt between, dt/2 last). */
 int pm_run_kdk(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
                double* vz, int nsteps, double dt, int variant);
+
+/* ---- 3. adjoint (no counterpart in the reference: it has no gradients) ---
+ * Reverse mode of the fused step.  With F(x) the per-particle force of
+ * pm_interp_force, u a per-particle cotangent and m = ng^3/np:
+ *   J_F^T u = sum_a u_a grad_x interp(g_a)  +  m grad_x interp(lambda),
+ *   lambda  = S sum_a D_a^T U_a,  U_a = interp^T(u_a),
+ * S the (symmetric) Green's-function solve and D_a the centred difference
+ * (D_a^T = -D_a).  The adjoint entry points allocate their work meshes
+ * (3 + 1 meshes and the binning) on first use. */
+
+/* Transposed CIC (adjoint of the deposit w.r.t. positions):
+ * xbar += m * grad_x sum_c W(x, c) mesh_bar[c]; mesh_bar is ng^3 in the
+ * plan's precision.  The gradient of a density loss goes through this. */
+int pm_adjoint_deposit(pm_plan_t* plan, const double* posx, const double* posy, const double* posz,
+                       const void* mesh_bar, double* xbar, double* ybar, double* zbar);
+/* Interpolation transpose: U_a = sum_p W(x_p, .) u_p^a into the plan's
+ * adjoint meshes 0..2 (LDS tiles over the tile binning of x). */
+int pm_interp_transpose(pm_plan_t* plan, const double* posx, const double* posy, const double* posz,
+                        const double* ux, const double* uy, const double* uz);
+/* Adjoint FFT solve: lambda = S sum_a D_a^T U_a into adjoint mesh 3. */
+int pm_adjoint_solve(pm_plan_t* plan);
+/* Device pointer of adjoint mesh `which` (0..2: U_a, 3: lambda), or NULL. */
+void* pm_plan_adjoint_mesh(pm_plan_t* plan, int which);
+/* xbar += J_F(x)^T u (recomputes the potential at x). */
+int pm_adjoint_force(pm_plan_t* plan, const double* posx, const double* posy, const double* posz,
+                     const double* ux, const double* uy, const double* uz, double* xbar, double* ybar,
+                     double* zbar, int variant);
+/* One reverse step of pm_step(kick, drift) without a stored trajectory:
+ * x -= drift v; vbar += drift xbar; v -= kick F(x); xbar += kick J_F^T vbar.
+ * On entry (x, v) is the state after the forward step and (xbar, vbar) the
+ * cotangent of it; on exit both refer to the state before the step. */
+int pm_adjoint_step(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
+                    double* vz, double* xbar, double* ybar, double* zbar, double* vxbar, double* vybar,
+                    double* vzbar, double kick, double drift, int variant);
+/* Reverse sweep of pm_run_kdk(nsteps, dt): reverse-time leapfrog
+ * recomputation, O(1) memory in the number of steps. */
+int pm_run_adjoint(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
+                   double* vz, double* xbar, double* ybar, double* zbar, double* vxbar, double* vybar,
+                   double* vzbar, int nsteps, double dt, int variant);
 
 /* Phase timing (hipEvents on the plan's stream).  pm_plan_enable_timing
  * reserves events for the next max_steps calls of pm_step (0 turns timing

@@ -1,0 +1,385 @@
+// Adjoint (reverse-mode) kernels of the PM step.  The reference has no
+// gradients; these differentiate the forward step of this library:
+//
+//   rho = m * sum_p W(x_p, .)            CIC deposit (utils.cpp:70-102)
+//   phi = S (rho - 1)                     FFT solve, S symmetric (G real, even)
+//   g_a = D_a phi                         centred difference (utils.cpp:130-132)
+//   F_p = sum_c W(x_p, c) g[c]            trilinear interpolation (utils_inline.h:55-70)
+//
+// For a per-particle cotangent u, the vector-Jacobian product is
+//   (J_F^T u)_q = sum_a u_q^a grad_x [sum_c W(x_q, c) g_a[c]]        (term A)
+//               + m grad_x [sum_c W(x_q, c) lambda[c]]               (term B)
+//   lambda = S mu,  mu = sum_a D_a^T U_a,  U_a = sum_p W(x_p, .) u_p^a,
+//   (D_a^T U)[c] = (U[c + e_a] - U[c - e_a]) / 2.
+// The CIC weights are piecewise linear: grad_x W(x, corner) takes the +-1
+// slope along the axis and the linear weights along the other two.
+#include "pm_internal.h"
+
+namespace pm {
+
+__device__ __forceinline__ bool in_box_a(double x, double box) { return x >= 0.0 && x < box; }
+
+// ---------------------------------------------------------------- helpers
+template <typename Rl> struct Corner {
+    int x[2], y[2], z[2];  // periodic node indices
+    Rl tx, ty, tz;         // fractional offsets
+};
+
+template <typename Rl>
+__device__ __forceinline__ void corner_setup(double x, double y, double z, int N, int (&xa)[4], int (&ya)[4],
+                                             int (&za)[4], Rl& tx, Rl& ty, Rl& tz) {
+    const int ix = (int)x, iy = (int)y, iz = (int)z;
+    tx = (Rl)(x - ix);
+    ty = (Rl)(y - iy);
+    tz = (Rl)(z - iz);
+    xa[0] = ix == 0 ? N - 1 : ix - 1;
+    xa[1] = ix;
+    xa[2] = ix + 1 == N ? 0 : ix + 1;
+    xa[3] = ix + 2 >= N ? ix + 2 - N : ix + 2;
+    ya[0] = iy == 0 ? N - 1 : iy - 1;
+    ya[1] = iy;
+    ya[2] = iy + 1 == N ? 0 : iy + 1;
+    ya[3] = iy + 2 >= N ? iy + 2 - N : iy + 2;
+    za[0] = iz == 0 ? N - 1 : iz - 1;
+    za[1] = iz;
+    za[2] = iz + 1 == N ? 0 : iz + 1;
+    za[3] = iz + 2 >= N ? iz + 2 - N : iz + 2;
+}
+
+// value and gradient (d/dx, d/dy, d/dz) of the trilinear interpolant
+template <typename Rl>
+__device__ __forceinline__ void tri_val_grad(const Rl (&f)[2][2][2], Rl tx, Rl ty, Rl tz, Rl& val, Rl (&gr)[3]) {
+    const Rl mx = Rl(1) - tx, my = Rl(1) - ty, mz = Rl(1) - tz;
+    // trilerp1 order (utils_inline.h:55-70) for the value
+    const Rl f00 = f[0][0][0] * mx + f[1][0][0] * tx;
+    const Rl f10 = f[0][1][0] * mx + f[1][1][0] * tx;
+    const Rl fz0 = f00 * my + f10 * ty;
+    const Rl g00 = f[0][0][1] * mx + f[1][0][1] * tx;
+    const Rl g10 = f[0][1][1] * mx + f[1][1][1] * tx;
+    const Rl fz1 = g00 * my + g10 * ty;
+    val = fz0 * mz + fz1 * tz;
+    // d/dx: slope +-1 along x
+    const Rl dx00 = f[1][0][0] - f[0][0][0], dx10 = f[1][1][0] - f[0][1][0];
+    const Rl dx01 = f[1][0][1] - f[0][0][1], dx11 = f[1][1][1] - f[0][1][1];
+    gr[0] = (dx00 * my + dx10 * ty) * mz + (dx01 * my + dx11 * ty) * tz;
+    // d/dy
+    gr[1] = (f10 - f00) * mz + (g10 - g00) * tz;
+    // d/dz
+    gr[2] = fz1 - fz0;
+}
+
+// centred differences of phi at the 8 corners (as in k_interp)
+template <typename Rl>
+__device__ __forceinline__ void corner_gradients(const Rl* __restrict__ phi, int N, const int (&xa)[4],
+                                                 const int (&ya)[4], const int (&za)[4], Rl (&gx)[2][2][2],
+                                                 Rl (&gy)[2][2][2], Rl (&gz)[2][2][2]) {
+    auto P = [&](int a, int b, int c) -> Rl { return phi[((size_t)xa[a] * N + ya[b]) * N + za[c]]; };
+    const Rl h = Rl(0.5);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                gx[a][b][c] = (P(a, b + 1, c + 1) - P(a + 2, b + 1, c + 1)) * h;
+                gy[a][b][c] = (P(a + 1, b, c + 1) - P(a + 1, b + 2, c + 1)) * h;
+                gz[a][b][c] = (P(a + 1, b + 1, c) - P(a + 1, b + 1, c + 2)) * h;
+            }
+}
+
+// ------------------------------------------------- adjoint of the deposit
+// xbar_q += scale * grad_x sum_c W(x_q, c) mbar[c]   (transposed CIC)
+template <typename Rl>
+__global__ __launch_bounds__(256) void k_adj_deposit(const double* __restrict__ px, const double* __restrict__ py,
+                                                     const double* __restrict__ pz, int64_t np, int N,
+                                                     const Rl* __restrict__ mbar, double scale,
+                                                     double* __restrict__ bx, double* __restrict__ by,
+                                                     double* __restrict__ bz, unsigned long long* __restrict__ err) {
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (i >= np) return;
+    const double x = px[i], y = py[i], z = pz[i];
+    const double box = (double)N;
+    if (!(in_box_a(x, box) && in_box_a(y, box) && in_box_a(z, box))) {
+        atomicAdd(err, 1ULL);
+        return;
+    }
+    int xa[4], ya[4], za[4];
+    Rl tx, ty, tz;
+    corner_setup<Rl>(x, y, z, N, xa, ya, za, tx, ty, tz);
+    Rl f[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) f[a][b][c] = mbar[((size_t)xa[a + 1] * N + ya[b + 1]) * N + za[c + 1]];
+    Rl v, gr[3];
+    tri_val_grad(f, tx, ty, tz, v, gr);
+    bx[i] += scale * (double)gr[0];
+    by[i] += scale * (double)gr[1];
+    bz[i] += scale * (double)gr[2];
+}
+
+// ------------------------------------------- interpolation transpose
+// U_a[c] += W(x_p, c) u_p^a for a = 0..2, one workgroup per tile of the
+// sorted binning (LDS float atomics, then owned nodes + shells as in the
+// forward deposit).
+template <typename Rl, int TS>
+__global__ __launch_bounds__(256) void k_deposit_vec3_tiles(const double* __restrict__ px,
+                                                            const double* __restrict__ py,
+                                                            const double* __restrict__ pz,
+                                                            const double* __restrict__ ux,
+                                                            const double* __restrict__ uy,
+                                                            const double* __restrict__ uz,
+                                                            const uint32_t* __restrict__ idx,
+                                                            const uint32_t* __restrict__ off, int ng,
+                                                            Rl* __restrict__ U0, Rl* __restrict__ U1,
+                                                            Rl* __restrict__ U2, Rl* __restrict__ shell) {
+    constexpr int E = TS + 1, E3 = E * E * E, SH = (TS + 1) * (TS + 1) * (TS + 1) - TS * TS * TS;
+    __shared__ Rl acc[3][E3];
+    const int nt = ng / TS;
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
+              tz = (int)(t % (uint32_t)nt);
+    const int ox = tx * TS, oy = ty * TS, oz = tz * TS;
+    for (int i = threadIdx.x; i < 3 * E3; i += 256) (&acc[0][0])[i] = Rl(0);
+    __syncthreads();
+    const uint32_t beg = off[t], end = off[t + 1];
+    for (uint32_t q = beg + threadIdx.x; q < end; q += 256) {
+        const uint32_t p = idx[q];
+        const double x = px[p], y = py[p], z = pz[p];
+        const int ix = (int)x, iy = (int)y, iz = (int)z;
+        const double fx = x - ix, fy = y - iy, fz = z - iz;
+        const double w[8] = {(1 - fx) * (1 - fy) * (1 - fz), (1 - fx) * (1 - fy) * fz, (1 - fx) * fy * (1 - fz),
+                             (1 - fx) * fy * fz,             fx * (1 - fy) * (1 - fz), fx * (1 - fy) * fz,
+                             fx * fy * (1 - fz),             fx * fy * fz};
+        const double u[3] = {ux[p], uy[p], uz[p]};
+        const int n0 = ((ix - ox) * E + (iy - oy)) * E + (iz - oz);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int n = n0 + ((k >> 2) & 1) * E * E + ((k >> 1) & 1) * E + (k & 1);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) atomicAdd(&acc[a][n], (Rl)(w[k] * u[a]));
+        }
+    }
+    __syncthreads();
+    const size_t N = (size_t)ng;
+    Rl* out[3] = {U0, U1, U2};
+    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
+        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
+        const size_t node = ((size_t)(ox + a) * N + (oy + b)) * N + (oz + c);
+        const int n = (a * E + b) * E + c;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[d][node] = acc[d][n];
+    }
+    const size_t ntiles = (size_t)nt * nt * nt;
+    for (int i = threadIdx.x; i < SH; i += 256) {
+        int a, b, c;
+        if (i < E * E) {
+            a = TS;
+            b = i / E;
+            c = i % E;
+        } else if (i < E * E + TS * E) {
+            const int j = i - E * E;
+            a = j / E;
+            b = TS;
+            c = j % E;
+        } else {
+            const int j = i - E * E - TS * E;
+            a = j / TS;
+            b = j % TS;
+            c = TS;
+        }
+        const int n = (a * E + b) * E + c;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) shell[((size_t)d * ntiles + t) * SH + i] = acc[d][n];
+    }
+}
+
+// ------------------------------------------------- mu = sum_a D_a^T U_a
+template <typename Rl>
+__global__ __launch_bounds__(256) void k_div3(const Rl* __restrict__ U0, const Rl* __restrict__ U1,
+                                              const Rl* __restrict__ U2, int N, Rl* __restrict__ mu) {
+    const size_t total = (size_t)N * N * N;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % N);
+        const int j = (int)((e / N) % N);
+        const int i = (int)(e / ((size_t)N * N));
+        const size_t sx = (size_t)N * N, sy = (size_t)N;
+        const size_t ip = i + 1 == N ? e - (size_t)i * sx : e + sx;
+        const size_t im = i == 0 ? e + (size_t)(N - 1) * sx : e - sx;
+        const size_t jp = j + 1 == N ? e - (size_t)j * sy : e + sy;
+        const size_t jm = j == 0 ? e + (size_t)(N - 1) * sy : e - sy;
+        const size_t kp = k + 1 == N ? e - (size_t)k : e + 1;
+        const size_t km = k == 0 ? e + (size_t)(N - 1) : e - 1;
+        mu[e] = ((U0[ip] - U0[im]) + (U1[jp] - U1[jm]) + (U2[kp] - U2[km])) * Rl(0.5);
+    }
+}
+
+// -------------------------------------------------- reverse-step gather
+// v -= kick * F(x); xbar += kick * (term A + term B); MODE 1 skips the
+// velocity update (pure VJP).
+template <typename Rl, int MODE>
+__global__ __launch_bounds__(256) void k_adj_gather(const double* __restrict__ px, const double* __restrict__ py,
+                                                    const double* __restrict__ pz, double* __restrict__ vx,
+                                                    double* __restrict__ vy, double* __restrict__ vz,
+                                                    const double* __restrict__ ux, const double* __restrict__ uy,
+                                                    const double* __restrict__ uz, double* __restrict__ bx,
+                                                    double* __restrict__ by, double* __restrict__ bz, int64_t np,
+                                                    int N, const Rl* __restrict__ phi, const Rl* __restrict__ lam,
+                                                    double mass, double kick, double vjp_scale,
+                                                    unsigned long long* __restrict__ err) {
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (i >= np) return;
+    const double x = px[i], y = py[i], z = pz[i];
+    const double box = (double)N;
+    if (!(in_box_a(x, box) && in_box_a(y, box) && in_box_a(z, box))) {
+        atomicAdd(err, 1ULL);
+        return;
+    }
+    int xa[4], ya[4], za[4];
+    Rl tx, ty, tz;
+    corner_setup<Rl>(x, y, z, N, xa, ya, za, tx, ty, tz);
+    Rl gx[2][2][2], gy[2][2][2], gz[2][2][2], lf[2][2][2];
+    corner_gradients(phi, N, xa, ya, za, gx, gy, gz);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) lf[a][b][c] = lam[((size_t)xa[a + 1] * N + ya[b + 1]) * N + za[c + 1]];
+    Rl F[3], dG[3][3], dL[3], lv;
+    tri_val_grad(gx, tx, ty, tz, F[0], dG[0]);
+    tri_val_grad(gy, tx, ty, tz, F[1], dG[1]);
+    tri_val_grad(gz, tx, ty, tz, F[2], dG[2]);
+    tri_val_grad(lf, tx, ty, tz, lv, dL);
+    const double u[3] = {ux[i], uy[i], uz[i]};
+    double gb[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+        gb[b] = u[0] * (double)dG[0][b] + u[1] * (double)dG[1][b] + u[2] * (double)dG[2][b] + mass * (double)dL[b];
+    bx[i] += vjp_scale * gb[0];
+    by[i] += vjp_scale * gb[1];
+    bz[i] += vjp_scale * gb[2];
+    if constexpr (MODE == 0) {
+        vx[i] -= kick * (double)F[0];
+        vy[i] -= kick * (double)F[1];
+        vz[i] -= kick * (double)F[2];
+    }
+}
+
+// x -= drift * v (wrapped); u = vbar + drift * xbar (in place on vbar)
+__global__ __launch_bounds__(256) void k_undrift(double* __restrict__ px, double* __restrict__ py,
+                                                 double* __restrict__ pz, const double* __restrict__ vx,
+                                                 const double* __restrict__ vy, const double* __restrict__ vz,
+                                                 const double* __restrict__ bx, const double* __restrict__ by,
+                                                 const double* __restrict__ bz, double* __restrict__ ubx,
+                                                 double* __restrict__ uby, double* __restrict__ ubz, int64_t np,
+                                                 double box, double drift) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= np) return;
+    auto wrap = [&](double v) {
+        if (v < 0.0) v += box;
+        if (v >= box) v -= box;
+        return v;
+    };
+    px[i] = wrap(px[i] - drift * vx[i]);
+    py[i] = wrap(py[i] - drift * vy[i]);
+    pz[i] = wrap(pz[i] - drift * vz[i]);
+    ubx[i] += drift * bx[i];
+    uby[i] += drift * by[i];
+    ubz[i] += drift * bz[i];
+}
+
+// ================================================================ launchers
+int launch_adj_deposit(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
+                       const void* mbar, double scale, double* bx, double* by, double* bz, unsigned long long* err,
+                       hipStream_t st) {
+    if (np <= 0) return PM_OK;
+    const unsigned nb = (unsigned)((np + 255) / 256);
+    if (prec == PM_FP32)
+        hipLaunchKernelGGL(k_adj_deposit<float>, dim3(nb), dim3(256), 0, st, px, py, pz, np, ng, (const float*)mbar,
+                           scale, bx, by, bz, err);
+    else
+        hipLaunchKernelGGL(k_adj_deposit<double>, dim3(nb), dim3(256), 0, st, px, py, pz, np, ng,
+                           (const double*)mbar, scale, bx, by, bz, err);
+    return check_launch("adjoint_deposit");
+}
+
+int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
+                        const double* uy, const double* uz, int ng, const SortedScratch& s, void* const U[3],
+                        void* shell3, hipStream_t st) {
+    const int ts = sorted_tile(ng);
+    const int nt = ng / ts;
+    const uint32_t ntiles = (uint32_t)nt * nt * nt;
+    const size_t sh = sorted_shell_elems(ng);
+    const size_t rs = prec == PM_FP32 ? sizeof(float) : sizeof(double);
+#define PM_VEC3(RL, TSV)                                                                                          \
+    hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles), dim3(256), 0, st, px, py, pz, ux, uy, uz,     \
+                       s.idx, s.off, ng, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3);                            \
+    for (int d = 0; d < 3; ++d)                                                                                     \
+        launch_fold_shells(prec, ts, ng, (const char*)shell3 + d * sh * rs, U[d], st);
+    if (prec == PM_FP32) {
+        if (ts == 16) {
+            PM_VEC3(float, 16)
+        } else {
+            PM_VEC3(float, 8)
+        }
+    } else {
+        if (ts == 16) {
+            PM_VEC3(double, 16)
+        } else {
+            PM_VEC3(double, 8)
+        }
+    }
+#undef PM_VEC3
+    return check_launch("deposit_vec3");
+}
+
+int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st) {
+    const size_t n = (size_t)ng * ng * ng;
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    if (prec == PM_FP32)
+        hipLaunchKernelGGL(k_div3<float>, dim3(nb), dim3(256), 0, st, (const float*)U[0], (const float*)U[1],
+                           (const float*)U[2], ng, (float*)mu);
+    else
+        hipLaunchKernelGGL(k_div3<double>, dim3(nb), dim3(256), 0, st, (const double*)U[0], (const double*)U[1],
+                           (const double*)U[2], ng, (double*)mu);
+    return check_launch("div3");
+}
+
+int launch_adj_gather(int prec, int mode, const double* px, const double* py, const double* pz, double* vx,
+                      double* vy, double* vz, const double* ux, const double* uy, const double* uz, double* bx,
+                      double* by, double* bz, int64_t np, int ng, const void* phi, const void* lam, double mass,
+                      double kick, double vjp_scale, unsigned long long* err, hipStream_t st) {
+    if (np <= 0) return PM_OK;
+    const unsigned nb = (unsigned)((np + 255) / 256);
+#define PM_GATHER(RL, M)                                                                                          \
+    hipLaunchKernelGGL((k_adj_gather<RL, M>), dim3(nb), dim3(256), 0, st, px, py, pz, vx, vy, vz, ux, uy, uz, bx, \
+                       by, bz, np, ng, (const RL*)phi, (const RL*)lam, mass, kick, vjp_scale, err)
+    if (prec == PM_FP32) {
+        if (mode == 0)
+            PM_GATHER(float, 0);
+        else
+            PM_GATHER(float, 1);
+    } else {
+        if (mode == 0)
+            PM_GATHER(double, 0);
+        else
+            PM_GATHER(double, 1);
+    }
+#undef PM_GATHER
+    return check_launch("adjoint_gather");
+}
+
+int launch_undrift(double* px, double* py, double* pz, const double* vx, const double* vy, const double* vz,
+                   const double* bx, const double* by, const double* bz, double* ubx, double* uby, double* ubz,
+                   int64_t np, int ng, double drift, hipStream_t st) {
+    if (np <= 0) return PM_OK;
+    const unsigned nb = (unsigned)((np + 255) / 256);
+    hipLaunchKernelGGL(k_undrift, dim3(nb), dim3(256), 0, st, px, py, pz, vx, vy, vz, bx, by, bz, ubx, uby, ubz, np,
+                       (double)ng, drift);
+    return check_launch("undrift");
+}
+
+}  // namespace pm

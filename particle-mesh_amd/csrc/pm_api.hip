@@ -179,6 +179,9 @@ extern "C" int pm_plan_destroy(pm_plan_t* p) {
                       p->sorted.off, p->sorted.key, p->sorted.slot, p->sorted.idx, p->sorted.shell};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    void* abufs[5] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell};
+    for (void* b : abufs)
+        if (b) (void)hipFree(b);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
     delete p;
@@ -291,12 +294,13 @@ static int deposit_impl(pm_plan* p, const double* x, const double* y, const doub
     return launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, range_counter(), p->stream);
 }
 
-static int solve_impl(pm_plan* p) {
+// potential = IFFT[G * scale / N^3 * FFT[in - sub1]], in -> out (5 passes)
+static int solve_into(pm_plan* p, const void* in, void* out, bool sub1) {
     const int N = p->ng;
     const GreenArgs ga{p->green_mode, p->green_scale / ((double)N * N * N), p->tab.cos_tab};
     const void* tm = p->tab.twM;
     const void* tn = p->tab.twN;
-    fft_pass(p->prec, N, kPassZFwd, p->stream, p->rho, p->specA, tm, tn, ga, 1.0, true);
+    fft_pass(p->prec, N, kPassZFwd, p->stream, in, p->specA, tm, tn, ga, 1.0, sub1);
     mark(p, 2);
     fft_pass(p->prec, N, kPassYFwd, p->stream, p->specA, p->specB, tm, tn, ga, 1.0, false);
     mark(p, 3);
@@ -304,10 +308,12 @@ static int solve_impl(pm_plan* p) {
     mark(p, 4);
     fft_pass(p->prec, N, kPassYInv, p->stream, p->specB, p->specA, tm, tn, ga, 1.0, false);
     mark(p, 5);
-    fft_pass(p->prec, N, kPassZInv, p->stream, p->specA, p->phi, tm, tn, ga, 1.0, false);
+    fft_pass(p->prec, N, kPassZInv, p->stream, p->specA, out, tm, tn, ga, 1.0, false);
     mark(p, 6);
     return check_launch("pm_solve");
 }
+
+static int solve_impl(pm_plan* p) { return solve_into(p, p->rho, p->phi, true); }
 
 extern "C" int pm_deposit(pm_plan_t* p, const double* x, const double* y, const double* z, int variant) {
     if (!p || !x || !y || !z) return fail(PM_ERR_ARG, "pm_deposit: null argument");
@@ -398,6 +404,144 @@ extern "C" int pm_plan_reset_timing(pm_plan_t* p) {
     for (int i = 0; i < PM_NPHASES; ++i) {
         p->phase_ms[i] = 0.0;
         p->phase_n[i] = 0;
+    }
+    return PM_OK;
+}
+
+// ------------------------------------------------------------------ adjoint
+static int ensure_adjoint(pm_plan* p) {
+    if (p->adjLam) return PM_OK;
+    const size_t rs = p->prec == PM_FP32 ? sizeof(float) : sizeof(double);
+    const size_t nc = (size_t)p->ng * p->ng * p->ng;
+    int rc = ensure_sorted(p);
+    for (int a = 0; a < 3 && !rc; ++a) rc = hip_ok(hipMalloc(&p->adjU[a], rs * nc), "adjoint: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc(&p->adjShell, 3 * rs * sorted_shell_elems(p->ng)), "adjoint: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc(&p->adjLam, rs * nc), "adjoint: hipMalloc");
+    if (rc) {
+        void* bufs[5] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjShell, p->adjLam};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        p->adjU[0] = p->adjU[1] = p->adjU[2] = p->adjShell = p->adjLam = nullptr;
+    }
+    return rc;
+}
+
+// rho and phi at x with the given deposit variant, leaving the tile binning
+// of x in p->sorted (the interpolation transpose needs it).
+static int forward_field(pm_plan* p, const double* x, const double* y, const double* z, int variant) {
+    const size_t nc = (size_t)p->ng * p->ng * p->ng;
+    const double mass = (double)nc / (double)p->np;
+    int rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
+    if (rc) return rc;
+    if (variant == PM_DEPOSIT_SORTED) {
+        rc = launch_deposit_binned(p->prec, x, y, z, p->ng, mass, p->rho, p->sorted, p->stream);
+    } else if (variant == PM_DEPOSIT_ATOMIC) {
+        rc = launch_fill(p->prec, p->rho, nc, 0.0, p->stream);
+        if (!rc) rc = launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, range_counter(), p->stream);
+    } else {
+        rc = fail(PM_ERR_ARG, "adjoint: unknown deposit variant");
+    }
+    if (rc) return rc;
+    return solve_into(p, p->rho, p->phi, true);
+}
+
+// lambda = S sum_a D_a^T U_a with U_a the interpolation transpose of u at x
+// (binning of x already in p->sorted); mu is staged in p->rho.
+static int adjoint_field(pm_plan* p, const double* x, const double* y, const double* z, const double* ux,
+                         const double* uy, const double* uz) {
+    int rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->ng, p->sorted, p->adjU, p->adjShell, p->stream);
+    if (!rc) rc = launch_div3(p->prec, p->adjU, p->ng, p->rho, p->stream);
+    if (!rc) rc = solve_into(p, p->rho, p->adjLam, false);
+    return rc;
+}
+
+extern "C" int pm_adjoint_deposit(pm_plan_t* p, const double* x, const double* y, const double* z,
+                                  const void* mesh_bar, double* xbar, double* ybar, double* zbar) {
+    if (!p || !x || !y || !z || !mesh_bar || !xbar || !ybar || !zbar)
+        return fail(PM_ERR_ARG, "pm_adjoint_deposit: null argument");
+    const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
+    return launch_adj_deposit(p->prec, x, y, z, p->np, p->ng, mesh_bar, mass, xbar, ybar, zbar, range_counter(),
+                              p->stream);
+}
+
+extern "C" int pm_interp_transpose(pm_plan_t* p, const double* x, const double* y, const double* z,
+                                   const double* ux, const double* uy, const double* uz) {
+    if (!p || !x || !y || !z || !ux || !uy || !uz) return fail(PM_ERR_ARG, "pm_interp_transpose: null argument");
+    int rc = ensure_adjoint(p);
+    if (!rc) rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
+    if (!rc)
+        rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->ng, p->sorted, p->adjU, p->adjShell, p->stream);
+    return rc;
+}
+
+extern "C" int pm_adjoint_solve(pm_plan_t* p) {
+    if (!p) return fail(PM_ERR_ARG, "pm_adjoint_solve: null plan");
+    int rc = ensure_adjoint(p);
+    if (rc) return rc;
+    const int slots = p->timing_slots;
+    p->timing_slots = 0;
+    rc = launch_div3(p->prec, p->adjU, p->ng, p->rho, p->stream);
+    if (!rc) rc = solve_into(p, p->rho, p->adjLam, false);
+    p->timing_slots = slots;
+    return rc;
+}
+
+extern "C" void* pm_plan_adjoint_mesh(pm_plan_t* p, int which) {
+    if (!p || which < 0 || which > 3) return nullptr;
+    return which < 3 ? p->adjU[which] : p->adjLam;
+}
+
+extern "C" int pm_adjoint_force(pm_plan_t* p, const double* x, const double* y, const double* z, const double* ux,
+                                const double* uy, const double* uz, double* xbar, double* ybar, double* zbar,
+                                int variant) {
+    if (!p || !x || !y || !z || !ux || !uy || !uz || !xbar || !ybar || !zbar)
+        return fail(PM_ERR_ARG, "pm_adjoint_force: null argument");
+    int rc = ensure_adjoint(p);
+    if (rc) return rc;
+    const int slots = p->timing_slots;
+    p->timing_slots = 0;
+    const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
+    rc = forward_field(p, x, y, z, variant);
+    if (!rc) rc = adjoint_field(p, x, y, z, ux, uy, uz);
+    if (!rc)
+        rc = launch_adj_gather(p->prec, 1, x, y, z, nullptr, nullptr, nullptr, ux, uy, uz, xbar, ybar, zbar, p->np,
+                               p->ng, p->phi, p->adjLam, mass, 0.0, 1.0, range_counter(), p->stream);
+    p->timing_slots = slots;
+    return rc;
+}
+
+extern "C" int pm_adjoint_step(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                               double* xbar, double* ybar, double* zbar, double* vxbar, double* vybar,
+                               double* vzbar, double kick, double drift, int variant) {
+    if (!p || !x || !y || !z || !vx || !vy || !vz || !xbar || !ybar || !zbar || !vxbar || !vybar || !vzbar)
+        return fail(PM_ERR_ARG, "pm_adjoint_step: null argument");
+    int rc = ensure_adjoint(p);
+    if (rc) return rc;
+    const int slots = p->timing_slots;
+    p->timing_slots = 0;
+    const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
+    // undo the drift; the velocity cotangent becomes u = vbar + drift * xbar
+    rc = launch_undrift(x, y, z, vx, vy, vz, xbar, ybar, zbar, vxbar, vybar, vzbar, p->np, p->ng, drift, p->stream);
+    if (!rc) rc = forward_field(p, x, y, z, variant);
+    if (!rc) rc = adjoint_field(p, x, y, z, vxbar, vybar, vzbar);
+    // undo the kick and add kick * J_F^T u to xbar
+    if (!rc)
+        rc = launch_adj_gather(p->prec, 0, x, y, z, vx, vy, vz, vxbar, vybar, vzbar, xbar, ybar, zbar, p->np, p->ng,
+                               p->phi, p->adjLam, mass, kick, kick, range_counter(), p->stream);
+    p->timing_slots = slots;
+    return rc;
+}
+
+extern "C" int pm_run_adjoint(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                              double* xbar, double* ybar, double* zbar, double* vxbar, double* vybar, double* vzbar,
+                              int nsteps, double dt, int variant) {
+    if (!p || nsteps < 0) return fail(PM_ERR_ARG, "pm_run_adjoint: bad arguments");
+    for (int s = nsteps; s >= 0; --s) {
+        const double kick = (s == 0 || s == nsteps) ? 0.5 * dt : dt;
+        const double drift = s == nsteps ? 0.0 : dt;
+        const int rc =
+            pm_adjoint_step(p, x, y, z, vx, vy, vz, xbar, ybar, zbar, vxbar, vybar, vzbar, kick, drift, variant);
+        if (rc) return rc;
     }
     return PM_OK;
 }

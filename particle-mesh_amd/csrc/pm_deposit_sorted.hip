@@ -27,48 +27,73 @@ constexpr uint32_t kInvalidKey = 0xFFFFFFFFu;
 
 __device__ __forceinline__ bool in_box_s(double x, double box) { return x >= 0.0 && x < box; }
 
-// 1. tile id and slot of every particle.
+// 1. tile id and slot of every particle.  A workgroup takes a chunk of
+// kBinChunk particles, counts them per tile in an LDS hash table (open
+// addressing), and issues one global atomic per distinct tile of the chunk,
+// all in parallel; a particle's slot is that atomic's result plus its rank
+// within the chunk.  Spatially coherent chunks touch few tiles.
+constexpr int kBinPer = 8;                    // particles per thread
+constexpr int kBinChunk = 256 * kBinPer;      // particles per workgroup
+constexpr int kBinHash = 2 * kBinChunk;       // hash slots (load factor <= 1/2)
+
+__device__ __forceinline__ uint32_t bin_hash(uint32_t k) { return (k * 2654435761u) >> 20; }
+
 __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px, const double* __restrict__ py,
                                                    const double* __restrict__ pz, int64_t np, int ng, int ts,
                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
                                                    uint32_t* __restrict__ slot,
                                                    unsigned long long* __restrict__ err) {
-    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    const int lane = __lane_id();
-    bool pending = false;
-    uint32_t k = kInvalidKey;
-    if (i < np) {
-        const double x = px[i], y = py[i], z = pz[i];
-        const double box = (double)ng;
-        if (in_box_s(x, box) && in_box_s(y, box) && in_box_s(z, box)) {
-            const int nt = ng / ts;
-            const int tx = (int)x / ts, ty = (int)y / ts, tz = (int)z / ts;
-            k = (uint32_t)((tx * nt + ty) * nt + tz);
-            pending = true;
-        } else {
-            atomicAdd(err, 1ULL);
+    static_assert((kBinHash & (kBinHash - 1)) == 0, "hash size must be a power of two");
+    __shared__ uint32_t hkey[kBinHash];
+    __shared__ uint32_t hcnt[kBinHash];
+    for (int h = threadIdx.x; h < kBinHash; h += 256) {
+        hkey[h] = kInvalidKey;
+        hcnt[h] = 0u;
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kBinChunk;
+    const int nt = ng / ts;
+    const double box = (double)ng;
+    uint32_t k[kBinPer], h[kBinPer], rank[kBinPer];
+#pragma unroll
+    for (int j = 0; j < kBinPer; ++j) {
+        const int64_t i = base + threadIdx.x + 256 * j;
+        k[j] = kInvalidKey;
+        h[j] = 0u;
+        rank[j] = 0u;
+        if (i < np) {
+            const double x = px[i], y = py[i], z = pz[i];
+            if (in_box_s(x, box) && in_box_s(y, box) && in_box_s(z, box)) {
+                k[j] = (uint32_t)((((int)x / ts) * nt + (int)y / ts) * nt + (int)z / ts);
+            } else {
+                atomicAdd(err, 1ULL);
+            }
+        }
+        if (k[j] != kInvalidKey) {
+            uint32_t s = bin_hash(k[j]) & (kBinHash - 1);
+            while (true) {  // at most kBinChunk distinct keys: a free slot always exists
+                const uint32_t old = atomicCAS(&hkey[s], kInvalidKey, k[j]);
+                if (old == kInvalidKey || old == k[j]) break;
+                s = (s + 1) & (kBinHash - 1);
+            }
+            h[j] = s;
+            rank[j] = atomicAdd(&hcnt[s], 1u);
         }
     }
-    uint32_t s = 0;
-    // wave-uniform loop: one atomic per distinct key in the wave
-    while (true) {
-        const unsigned long long todo = __ballot(pending);
-        if (todo == 0ULL) break;
-        const int leader = __ffsll((long long)todo) - 1;
-        const uint32_t lk = (uint32_t)__shfl((int)k, leader);
-        const bool mine = pending && k == lk;
-        const unsigned long long m = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&cnt[lk], (uint32_t)__popcll(m));
-        base = (uint32_t)__shfl((int)base, leader);
-        if (mine) {
-            s = base + (uint32_t)__popcll(m & ((1ULL << lane) - 1ULL));
-            pending = false;
-        }
+    __syncthreads();
+    // one global atomic per distinct tile; hcnt becomes the chunk's base slot
+    for (int s = threadIdx.x; s < kBinHash; s += 256) {
+        const uint32_t kk = hkey[s];
+        if (kk != kInvalidKey) hcnt[s] = atomicAdd(&cnt[kk], hcnt[s]);
     }
-    if (i < np) {
-        key[i] = k;
-        slot[i] = s;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBinPer; ++j) {
+        const int64_t i = base + threadIdx.x + 256 * j;
+        if (i < np) {
+            key[i] = k[j];
+            slot[i] = k[j] == kInvalidKey ? 0u : hcnt[h[j]] + rank[j];
+        }
     }
 }
 
@@ -197,10 +222,27 @@ __global__ __launch_bounds__(256) void k_fold_shells(const Rl* __restrict__ shel
     const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
               tz = (int)(t % (uint32_t)nt);
     const size_t N = (size_t)ng;
-    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
-        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
+    // lower-face nodes: a == 0 (TS^2), then b == 0 < a (TS (TS-1)), then
+    // c == 0 < a, b ((TS-1)^2)
+    constexpr int NF = TS * TS + TS * (TS - 1) + (TS - 1) * (TS - 1);
+    for (int i = threadIdx.x; i < NF; i += 256) {
+        int a, b, c;
+        if (i < TS * TS) {
+            a = 0;
+            b = i / TS;
+            c = i % TS;
+        } else if (i < TS * TS + TS * (TS - 1)) {
+            const int j = i - TS * TS;
+            a = 1 + j / TS;
+            b = 0;
+            c = j % TS;
+        } else {
+            const int j = i - TS * TS - TS * (TS - 1);
+            a = 1 + j / (TS - 1);
+            b = 1 + j % (TS - 1);
+            c = 0;
+        }
         const int on = (a == 0) | ((b == 0) << 1) | ((c == 0) << 2);
-        if (!on) continue;
         const size_t node = ((size_t)(tx * TS + a) * N + (ty * TS + b)) * N + (tz * TS + c);
         Rl v = rho[node];
         for (int d = 1; d < 8; ++d) {
@@ -224,47 +266,78 @@ size_t sorted_shell_elems(int ng) {
     return nt * nt * nt * (size_t)(ts == 16 ? shell_size<16>() : shell_size<8>());
 }
 
-int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
-                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st) {
+int launch_bin(const double* px, const double* py, const double* pz, int64_t np, int ng, SortedScratch& s,
+               unsigned long long* err, hipStream_t st) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
     if (np > 0xFFFFFFF0LL) return fail(PM_ERR_ARG, "sorted deposit: more than 2^32 particles");
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(std::min<uint32_t>((ntiles + 255) / 256, 8192)), dim3(256), 0, st,
                        s.cnt, (size_t)ntiles, 0u);
-    const unsigned nb = (unsigned)((np + 255) / 256);
     if (np > 0) {
-        hipLaunchKernelGGL(k_bin_count, dim3(nb), dim3(256), 0, st, px, py, pz, np, ng, ts, s.cnt, s.key, s.slot,
+        const unsigned nchunk = (unsigned)((np + kBinChunk - 1) / kBinChunk);
+        hipLaunchKernelGGL(k_bin_count, dim3(nchunk), dim3(256), 0, st, px, py, pz, np, ng, ts, s.cnt, s.key, s.slot,
                            err);
     }
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, (int)ntiles);
-    if (np > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, s.key, s.slot, s.off, np, s.idx);
+    if (np > 0) {
+        const unsigned nb = (unsigned)((np + 255) / 256);
+        hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, s.key, s.slot, s.off, np, s.idx);
+    }
+    return check_launch("bin");
+}
+
+int launch_fold_shells(int prec, int ts, int ng, const void* shell, void* rho, hipStream_t st) {
+    const int nt = ng / ts;
+    const uint32_t ntiles = (uint32_t)nt * nt * nt;
     if (prec == PM_FP32) {
-        if (ts == 16) {
+        if (ts == 16)
+            hipLaunchKernelGGL((k_fold_shells<float, 16>), dim3(ntiles), dim3(256), 0, st, (const float*)shell, ng,
+                               (float*)rho);
+        else
+            hipLaunchKernelGGL((k_fold_shells<float, 8>), dim3(ntiles), dim3(256), 0, st, (const float*)shell, ng,
+                               (float*)rho);
+    } else {
+        if (ts == 16)
+            hipLaunchKernelGGL((k_fold_shells<double, 16>), dim3(ntiles), dim3(256), 0, st, (const double*)shell, ng,
+                               (double*)rho);
+        else
+            hipLaunchKernelGGL((k_fold_shells<double, 8>), dim3(ntiles), dim3(256), 0, st, (const double*)shell, ng,
+                               (double*)rho);
+    }
+    return check_launch("fold_shells");
+}
+
+int launch_deposit_binned(int prec, const double* px, const double* py, const double* pz, int ng, double mass,
+                          void* rho, SortedScratch& s, hipStream_t st) {
+    const int ts = sorted_tile(ng);
+    const int nt = ng / ts;
+    const uint32_t ntiles = (uint32_t)nt * nt * nt;
+    if (prec == PM_FP32) {
+        if (ts == 16)
             hipLaunchKernelGGL((k_deposit_tiles<float, 16>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx, s.off,
                                ng, mass, (float*)rho, (float*)s.shell);
-            hipLaunchKernelGGL((k_fold_shells<float, 16>), dim3(ntiles), dim3(256), 0, st, (const float*)s.shell, ng,
-                               (float*)rho);
-        } else {
+        else
             hipLaunchKernelGGL((k_deposit_tiles<float, 8>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx, s.off,
                                ng, mass, (float*)rho, (float*)s.shell);
-            hipLaunchKernelGGL((k_fold_shells<float, 8>), dim3(ntiles), dim3(256), 0, st, (const float*)s.shell, ng,
-                               (float*)rho);
-        }
     } else {
-        if (ts == 16) {
+        if (ts == 16)
             hipLaunchKernelGGL((k_deposit_tiles<double, 16>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx,
                                s.off, ng, mass, (double*)rho, (double*)s.shell);
-            hipLaunchKernelGGL((k_fold_shells<double, 16>), dim3(ntiles), dim3(256), 0, st, (const double*)s.shell,
-                               ng, (double*)rho);
-        } else {
+        else
             hipLaunchKernelGGL((k_deposit_tiles<double, 8>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx,
                                s.off, ng, mass, (double*)rho, (double*)s.shell);
-            hipLaunchKernelGGL((k_fold_shells<double, 8>), dim3(ntiles), dim3(256), 0, st, (const double*)s.shell,
-                               ng, (double*)rho);
-        }
     }
-    return check_launch("deposit_sorted");
+    int rc = check_launch("deposit_tiles");
+    if (rc) return rc;
+    return launch_fold_shells(prec, ts, ng, s.shell, rho, st);
+}
+
+int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
+                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st) {
+    int rc = launch_bin(px, py, pz, np, ng, s, err, st);
+    if (rc) return rc;
+    return launch_deposit_binned(prec, px, py, pz, ng, mass, rho, s, st);
 }
 
 }  // namespace pm

@@ -65,6 +65,28 @@ int sorted_tile(int ng);
 size_t sorted_shell_elems(int ng);
 int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
                           double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st);
+// binning only (count, scan, scatter) and the per-tile deposit of a binning
+int launch_bin(const double* px, const double* py, const double* pz, int64_t np, int ng, SortedScratch& s,
+               unsigned long long* err, hipStream_t st);
+int launch_deposit_binned(int prec, const double* px, const double* py, const double* pz, int ng, double mass,
+                          void* rho, SortedScratch& s, hipStream_t st);
+int launch_fold_shells(int prec, int ts, int ng, const void* shell, void* rho, hipStream_t st);
+
+// Adjoint kernels (pm_adjoint.hip).
+int launch_adj_deposit(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
+                       const void* mbar, double scale, double* bx, double* by, double* bz, unsigned long long* err,
+                       hipStream_t st);
+int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
+                        const double* uy, const double* uz, int ng, const SortedScratch& s, void* const U[3],
+                        void* shell3, hipStream_t st);
+int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st);
+int launch_adj_gather(int prec, int mode, const double* px, const double* py, const double* pz, double* vx,
+                      double* vy, double* vz, const double* ux, const double* uy, const double* uz, double* bx,
+                      double* by, double* bz, int64_t np, int ng, const void* phi, const void* lam, double mass,
+                      double kick, double vjp_scale, unsigned long long* err, hipStream_t st);
+int launch_undrift(double* px, double* py, double* pz, const double* vx, const double* vy, const double* vz,
+                   const double* bx, const double* by, const double* bz, double* ubx, double* uby, double* ubz,
+                   int64_t np, int ng, double drift, hipStream_t st);
 
 // Particle kernels (pm_particles.hip).
 int launch_fill(int prec, void* p, size_t n, double v, hipStream_t st);
@@ -99,6 +121,9 @@ struct pm_plan {
     void* specB = nullptr;
     pm::FftTables tab;
     pm::SortedScratch sorted;  // allocated on the first sorted deposit
+    void* adjU[3] = {};        // adjoint meshes U_a (interpolation transpose)
+    void* adjLam = nullptr;    // lambda = S D^T U
+    void* adjShell = nullptr;  // 3 shell buffers of the vec3 deposit
     // Phase timing: one set of PM_NPHASES+1 events per recorded step, read
     // back (and summed) only by pm_plan_phase_ms, so that recording needs no
     // synchronisation inside a timed loop.

@@ -70,18 +70,32 @@ __global__ __launch_bounds__(256) void k_reorder_slab(const double* __restrict__
     }
 }
 
-// ReorderSlabBack + TransposeSlabBack at one box (reorder_slab_back.cpp:11-21,
-// transpose_slab_back.cpp:22-28): -1/+2 periodic halo on every axis.
+// ReorderSlabBack at one box (reorder_slab_back.cpp:11-21): slab ->
+// ng x (ng+3) x (ng+3) with the -1/+2 periodic halo on y and z.
 __global__ __launch_bounds__(256) void k_reorder_slab_back(const double* __restrict__ slab,
-                                                           double* __restrict__ halo, int ng, int stride) {
+                                                           double* __restrict__ out, int ng, int stride) {
     const int m = ng + 3;
-    const size_t total = (size_t)m * m * m;
+    const size_t total = (size_t)ng * m * m;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
         const int k = (int)(e % m);
         const int j = (int)((e / m) % m);
         const int i = (int)(e / ((size_t)m * m));
-        const int si = (i - 1 + ng) % ng, sj = (j - 1 + ng) % ng, sk = (k - 1 + ng) % ng;
-        halo[e] = slab[((size_t)si * ng + sj) * stride + sk];
+        const int sj = (j - 1 + ng) % ng, sk = (k - 1 + ng) % ng;
+        out[e] = slab[((size_t)i * ng + sj) * stride + sk];
+    }
+}
+
+// TransposeSlabBack at one box (transpose_slab_back.cpp:13-28): the x halo,
+// plane -1 <- ng-1 and planes ng, ng+1 <- 0, 1, around the ng planes.
+__global__ __launch_bounds__(256) void k_transpose_slab_back(const double* __restrict__ in,
+                                                             double* __restrict__ halo, int ng) {
+    const int m = ng + 3;
+    const size_t plane = (size_t)m * m;
+    const size_t total = plane * m;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e / plane);
+        const int si = (i - 1 + ng) % ng;
+        halo[e] = in[(size_t)si * plane + e % plane];
     }
 }
 
@@ -174,12 +188,27 @@ extern "C" int pm_reorder_slab(const double* cube, double* slab, int ng, int str
     return check_launch("pm_reorder_slab");
 }
 
-extern "C" int pm_reorder_slab_back(const double* slab, double* halo, int ng, int stride, void* stream) {
-    if (!slab || !halo || ng < 2 || stride < ng) return fail(PM_ERR_ARG, "pm_reorder_slab_back: bad arguments");
-    const size_t n = (size_t)(ng + 3) * (ng + 3) * (ng + 3);
-    hipLaunchKernelGGL(k_reorder_slab_back, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, slab, halo, ng,
+extern "C" int pm_transpose_slab(const double* cube, double* recv, int ng, void* stream) {
+    if (!cube || !recv || ng < 2) return fail(PM_ERR_ARG, "pm_transpose_slab: bad arguments");
+    const size_t bytes = sizeof(double) * (size_t)(ng + 1) * (ng + 1) * (ng + 1);
+    const hipError_t e = hipMemcpyAsync(recv, cube, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(PM_ERR_HIP, std::string("pm_transpose_slab: ") + hipGetErrorString(e));
+    return PM_OK;
+}
+
+extern "C" int pm_reorder_slab_back(const double* slab, double* out, int ng, int stride, void* stream) {
+    if (!slab || !out || ng < 2 || stride < ng) return fail(PM_ERR_ARG, "pm_reorder_slab_back: bad arguments");
+    const size_t n = (size_t)ng * (ng + 3) * (ng + 3);
+    hipLaunchKernelGGL(k_reorder_slab_back, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, slab, out, ng,
                        stride);
     return check_launch("pm_reorder_slab_back");
+}
+
+extern "C" int pm_transpose_slab_back(const double* in, double* halo, int ng, void* stream) {
+    if (!in || !halo || ng < 2) return fail(PM_ERR_ARG, "pm_transpose_slab_back: bad arguments");
+    const size_t n = (size_t)(ng + 3) * (ng + 3) * (ng + 3);
+    hipLaunchKernelGGL(k_transpose_slab_back, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, in, halo, ng);
+    return check_launch("pm_transpose_slab_back");
 }
 
 extern "C" int pm_centered_derivatives(const double* in, double* d_dx, double* d_dy, double* d_dz, int nx1,

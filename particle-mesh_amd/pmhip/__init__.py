@@ -48,6 +48,8 @@ SIGNATURES = {
                                       _c_int, _c_int, _c_int, _c_dbl, _c_vp]),
     "pm_reorder_slab": (_c_int, [_c_vp, _c_vp, _c_int, _c_int, _c_vp]),
     "pm_reorder_slab_back": (_c_int, [_c_vp, _c_vp, _c_int, _c_int, _c_vp]),
+    "pm_transpose_slab": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp]),
+    "pm_transpose_slab_back": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp]),
     "pm_centered_derivatives": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_int, _c_int, _c_vp]),
     "pm_update_particles": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
                                      _c_dbl, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
@@ -73,6 +75,13 @@ SIGNATURES = {
     "pm_interp_force": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "pm_step": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl, _c_int]),
     "pm_run_kdk": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_dbl, _c_int]),
+    "pm_adjoint_deposit": (_c_int, [_c_vp] * 8),
+    "pm_interp_transpose": (_c_int, [_c_vp] * 7),
+    "pm_adjoint_solve": (_c_int, [_c_vp]),
+    "pm_plan_adjoint_mesh": (_c_vp, [_c_vp, _c_int]),
+    "pm_adjoint_force": (_c_int, [_c_vp] * 10 + [_c_int]),
+    "pm_adjoint_step": (_c_int, [_c_vp] * 13 + [_c_dbl, _c_dbl, _c_int]),
+    "pm_run_adjoint": (_c_int, [_c_vp] * 13 + [_c_int, _c_dbl, _c_int]),
     "pm_plan_enable_timing": (_c_int, [_c_vp, _c_int]),
     "pm_plan_phase_ms": (_c_int, [_c_vp, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_i64)]),
     "pm_plan_reset_timing": (_c_int, [_c_vp]),
@@ -209,6 +218,36 @@ class Plan:
 
     def run_kdk(self, pos, vel, nsteps: int, dt: float, variant: int = PM_DEPOSIT_ATOMIC):
         check(lib().pm_run_kdk(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel), nsteps, dt, variant))
+
+    # ---------------------------------------------------------- adjoint
+    def adjoint_deposit(self, pos, mesh_bar, xbar):
+        """xbar += m grad_x sum_c W(x, c) mesh_bar[c] (transposed CIC)."""
+        check(lib().pm_adjoint_deposit(self.h, *(dptr(t) for t in pos), dptr(mesh_bar), *(dptr(t) for t in xbar)))
+
+    def interp_transpose(self, pos, u):
+        check(lib().pm_interp_transpose(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in u)))
+
+    def adjoint_solve(self):
+        check(lib().pm_adjoint_solve(self.h))
+
+    def adjoint_mesh(self, which: int):
+        n = self.ng
+        ptr = lib().pm_plan_adjoint_mesh(self.h, which)
+        if not ptr:
+            raise PMError(PM_ERR_ARG, "adjoint meshes not allocated")
+        return device_view(ptr, (n, n, n), mesh_dtype(self.precision))
+
+    def adjoint_force(self, pos, u, xbar, variant: int = PM_DEPOSIT_SORTED):
+        check(lib().pm_adjoint_force(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in u),
+                                     *(dptr(t) for t in xbar), variant))
+
+    def adjoint_step(self, pos, vel, xbar, vbar, kick: float, drift: float, variant: int = PM_DEPOSIT_SORTED):
+        check(lib().pm_adjoint_step(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel),
+                                    *(dptr(t) for t in xbar), *(dptr(t) for t in vbar), kick, drift, variant))
+
+    def run_adjoint(self, pos, vel, xbar, vbar, nsteps: int, dt: float, variant: int = PM_DEPOSIT_SORTED):
+        check(lib().pm_run_adjoint(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel),
+                                   *(dptr(t) for t in xbar), *(dptr(t) for t in vbar), nsteps, dt, variant))
 
     def enable_timing(self, max_steps: int):
         """Record phase events for the next max_steps steps (0 = off)."""

@@ -1,0 +1,233 @@
+"""GPU tests of the hand-written adjoint kernels (through the C ABI).
+
+Pins: a numpy scatter restatement of the interpolation transpose, the
+oracle's solve for the adjoint solve, central finite differences of the
+oracle's force and deposit for the vector-Jacobian products, exact reversal
+of a step, and torch float64 autograd through an independent restatement of
+the forward step (tests/torch_pm.py) as the FP64 reference gradient.
+Tolerances are stated per test (FP64 mesh unless noted).
+"""
+import numpy as np
+import pytest
+
+import pmhip
+import pmhip.autograd as pag
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(arrs, dev):
+    import torch
+
+    return [torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev) for a in arrs]
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def cic_scatter(pos, w, ng):
+    """numpy np.add.at CIC of per-particle weights w on the periodic mesh."""
+    out = np.zeros((ng, ng, ng))
+    i = [np.floor(p).astype(np.int64) for p in pos]
+    t = [p - ii for p, ii in zip(pos, i)]
+    for a in (0, 1):
+        for b in (0, 1):
+            for c in (0, 1):
+                wt = (t[0] if a else 1 - t[0]) * (t[1] if b else 1 - t[1]) * (t[2] if c else 1 - t[2])
+                np.add.at(out, ((i[0] + a) % ng, (i[1] + b) % ng, (i[2] + c) % ng), wt * w)
+    return out
+
+
+@pytest.mark.parametrize("prec,tol", [(pmhip.PM_FP64, 1e-12), (pmhip.PM_FP32, 2e-6)])
+def test_interp_transpose_vs_numpy(dev, prec, tol):
+    import torch
+
+    ng, npart = 32, 20000
+    rng = np.random.default_rng(1)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    uh = [rng.standard_normal(npart) for _ in range(3)]
+    with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.interp_transpose(to_dev(ph, dev), to_dev(uh, dev))
+        torch.cuda.synchronize()
+        for a in range(3):
+            ref = cic_scatter(ph, uh[a], ng)
+            got = host(plan.adjoint_mesh(a)).astype(np.float64)
+            assert np.max(np.abs(got - ref)) < tol * np.abs(ref).max()
+
+
+def test_adjoint_solve_vs_oracle(dev):
+    import torch
+
+    ng, npart = 32, 8000
+    rng = np.random.default_rng(2)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    uh = [rng.standard_normal(npart) for _ in range(3)]
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.interp_transpose(to_dev(ph, dev), to_dev(uh, dev))
+        plan.adjoint_solve()
+        torch.cuda.synchronize()
+        U = [cic_scatter(ph, uh[a], ng) for a in range(3)]
+        # (D_a^T U)[c] = (U[c + e_a] - U[c - e_a]) / 2
+        mu = sum((np.roll(U[a], -1, a) - np.roll(U[a], 1, a)) * 0.5 for a in range(3))
+        lam = pyoracle.solve(mu, pyoracle.GREEN_CONT_K2, 1.0)
+        assert np.max(np.abs(host(plan.adjoint_mesh(3)) - lam)) < 1e-11 * np.abs(lam).max()
+
+
+def test_adjoint_deposit_vs_finite_differences(dev):
+    import torch
+
+    ng, npart = 16, 300
+    rng = np.random.default_rng(3)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    mbar = rng.standard_normal((ng, ng, ng))
+    xbar = to_dev([np.zeros(npart)] * 3, dev)
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.adjoint_deposit(to_dev(ph, dev), torch.tensor(mbar, device=dev), xbar)
+        torch.cuda.synchronize()
+    e = [rng.standard_normal(npart) for _ in range(3)]
+    eps = 1e-7
+    lp = np.sum(pyoracle.deposit_periodic([p + eps * d for p, d in zip(ph, e)], ng) * mbar)
+    lm = np.sum(pyoracle.deposit_periodic([p - eps * d for p, d in zip(ph, e)], ng) * mbar)
+    fd = (lp - lm) / (2 * eps)
+    an = sum(np.dot(host(b), d) for b, d in zip(xbar, e))
+    assert abs(fd - an) < 1e-6 * max(1.0, abs(an))
+
+
+@pytest.mark.parametrize("variant", [pmhip.PM_DEPOSIT_SORTED, pmhip.PM_DEPOSIT_ATOMIC])
+def test_adjoint_force_vs_finite_differences(dev, variant):
+    """<u, J_F e> by central differences of the oracle's force equals
+    <J_F^T u, e> from the adjoint kernels."""
+    import torch
+
+    ng, npart = 16, 500
+    rng = np.random.default_rng(4)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    uh = [rng.standard_normal(npart) for _ in range(3)]
+    xbar = to_dev([np.zeros(npart)] * 3, dev)
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.adjoint_force(to_dev(ph, dev), to_dev(uh, dev), xbar, variant)
+        torch.cuda.synchronize()
+
+    def F(pos):
+        d = pyoracle.deposit_periodic(pos, ng)
+        return pyoracle.interp_force(pos, pyoracle.solve(d, pyoracle.GREEN_CONT_K2, 1.0))
+
+    for trial in range(3):
+        e = [rng.standard_normal(npart) for _ in range(3)]
+        eps = 1e-6
+        fp = F([p + eps * d for p, d in zip(ph, e)])
+        fm = F([p - eps * d for p, d in zip(ph, e)])
+        fd = sum(np.dot(u, (a - b) / (2 * eps)) for u, a, b in zip(uh, fp, fm))
+        an = sum(np.dot(host(b), d) for b, d in zip(xbar, e))
+        assert abs(fd - an) < 1e-5 * max(1.0, abs(an)), (trial, fd, an)
+
+
+def test_adjoint_step_reverses_forward_step(dev):
+    import torch
+
+    ng, npart = 32, 32 ** 3
+    ph = list(pyoracle.init_particles(npart, ng))
+    pos = to_dev(ph, dev)
+    vel = to_dev([np.zeros(npart)] * 3, dev)
+    xb = to_dev([np.zeros(npart)] * 3, dev)
+    vb = to_dev([np.zeros(npart)] * 3, dev)
+    kick, drift = 0.01, 0.01
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.step(pos, vel, kick, drift, pmhip.PM_DEPOSIT_SORTED)
+        plan.adjoint_step(pos, vel, xb, vb, kick, drift, pmhip.PM_DEPOSIT_SORTED)
+        torch.cuda.synchronize()
+    for a, b in zip(pos, ph):
+        d = host(a) - b
+        d -= ng * np.round(d / ng)
+        assert np.max(np.abs(d)) < 1e-11
+    assert max(np.abs(host(v)).max() for v in vel) < 1e-11
+    assert max(np.abs(host(v)).max() for v in xb + vb) == 0.0  # zero cotangent stays zero
+
+
+def _density_loss_setup(dev, ng, nrow, seed):
+    import torch
+
+    rng = np.random.default_rng(seed)
+    q = (np.arange(nrow) + 0.5) * (ng / nrow)
+    Q = np.stack([a.ravel() for a in np.meshgrid(q, q, q, indexing="ij")])
+    psi = 0.4 * rng.standard_normal(Q.shape)
+    target = np.abs(rng.standard_normal((ng, ng, ng))) + 0.5
+    return torch.tensor(Q, device=dev), torch.tensor(psi, device=dev), torch.tensor(target, device=dev)
+
+
+@pytest.mark.parametrize("variant,checkpoint", [(pmhip.PM_DEPOSIT_SORTED, False), (pmhip.PM_DEPOSIT_ATOMIC, False),
+                                                (pmhip.PM_DEPOSIT_ATOMIC, True)])
+def test_gradient_through_kdk_vs_torch_autograd_and_fd(dev, variant, checkpoint):
+    """L = ||rho(x_n) - target||^2 after n KDK steps, gradient w.r.t. the
+    initial displacement psi (x0 = q + psi, v0 = alpha psi): adjoint kernels
+    vs torch float64 autograd of tests/torch_pm.py vs central differences.
+    Central differences use the FP64 atomic deposit only: the sorted
+    deposit rounds each contribution to 2^-32 mass units, which a 1e-6
+    position step resolves only to ~1e-4 relative."""
+    import torch
+    import torch_pm
+
+    ng, nrow, nsteps, dt, alpha = 16, 16, 4, 0.01, 1.5
+    npart = nrow ** 3
+    Q, psi0, target = _density_loss_setup(dev, ng, nrow, 5)
+    plan = pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0)
+
+    def loss_gpu(psi):
+        x0 = torch.remainder(Q + psi, float(ng))
+        x, v = pag.evolve(x0, alpha * psi, plan, nsteps, dt, variant=variant, checkpoint=checkpoint)
+        rho = pag.density(x, plan, variant)
+        return ((rho - target) ** 2).sum()
+
+    psi = psi0.clone().requires_grad_(True)
+    L = loss_gpu(psi)
+    (g_adj,) = torch.autograd.grad(L, psi)
+
+    # torch float64 autograd reference (independent forward restatement)
+    green = torch_pm.green_cont_k2(ng, dev)
+    psi_r = psi0.clone().requires_grad_(True)
+    x0 = torch.remainder(Q + psi_r, float(ng))
+    x, v = torch_pm.run_kdk(x0, alpha * psi_r, ng, nsteps, dt, 1.0, green)
+    Lr = ((torch_pm.deposit(x, ng, 1.0) - target) ** 2).sum()
+    (g_ref,) = torch.autograd.grad(Lr, psi_r)
+    assert abs(float(L.detach()) - float(Lr.detach())) < 1e-10 * abs(float(Lr.detach()))
+    err = float((g_adj - g_ref).abs().max() / g_ref.abs().max())
+    assert err < 1e-9, err
+    if variant != pmhip.PM_DEPOSIT_ATOMIC:
+        plan.close()
+        return
+
+    # central differences on 16 random coordinates
+    rng = np.random.default_rng(6)
+    idx = [(int(rng.integers(3)), int(rng.integers(npart))) for _ in range(16)]
+    eps = 1e-6
+    with torch.no_grad():
+        for (a, p) in idx:
+            dp = psi0.clone()
+            dp[a, p] += eps
+            dm = psi0.clone()
+            dm[a, p] -= eps
+            fd = (float(loss_gpu(dp)) - float(loss_gpu(dm))) / (2 * eps)
+            an = float(g_adj[a, p])
+            assert abs(fd - an) < 1e-5 * max(1.0, abs(an)) + 1e-6 * float(g_adj.abs().max()), (a, p, fd, an)
+    plan.close()
+
+
+def test_fp32_adjoint_close_to_fp64(dev):
+    import torch
+
+    ng, nrow, nsteps, dt = 32, 32, 5, 0.01
+    npart = nrow ** 3
+    Q, psi0, target = _density_loss_setup(dev, ng, nrow, 9)
+    grads = []
+    for prec in (pmhip.PM_FP64, pmhip.PM_FP32):
+        plan = pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0)
+        psi = psi0.clone().requires_grad_(True)
+        x, v = pag.evolve(torch.remainder(Q + psi, float(ng)), 1.5 * psi, plan, nsteps, dt)
+        L = ((pag.density(x, plan).double() - target) ** 2).sum()
+        (g,) = torch.autograd.grad(L, psi)
+        grads.append(g)
+        plan.close()
+    err = float((grads[0] - grads[1]).abs().max() / grads[0].abs().max())
+    assert err < 1e-3, err

@@ -87,7 +87,9 @@ def test_reference_layout_step_vs_golden(dev, golden_dir):
         npart = pos[0].numel()
         n1, n3 = ng + 1, ng + 3
         cube = torch.empty(n1 ** 3, dtype=torch.float64, device=dev)
+        recv = torch.empty(n1 ** 3, dtype=torch.float64, device=dev)
         slab = torch.empty(ng ** 3, dtype=torch.float64, device=dev)
+        yz = torch.empty(ng * n3 * n3, dtype=torch.float64, device=dev)
         halo = torch.empty(n3 ** 3, dtype=torch.float64, device=dev)
         grads = [torch.empty(n1 ** 3, dtype=torch.float64, device=dev) for _ in range(3)]
         mom = [torch.zeros(npart, dtype=torch.float64, device=dev) for _ in range(3)]
@@ -95,12 +97,14 @@ def test_reference_layout_step_vs_golden(dev, golden_dir):
         pmhip.check(lib.pm_init_buf(cube.data_ptr(), n1, n1, n1, st))
         pmhip.check(lib.pm_deposit_particles(*(p.data_ptr() for p in pos), npart, cube.data_ptr(), n1, n1, n1, 0, 0, 0,
                                              ng ** 3 / npart, st))
-        pmhip.check(lib.pm_reorder_slab(cube.data_ptr(), slab.data_ptr(), ng, ng, st))
+        pmhip.check(lib.pm_transpose_slab(cube.data_ptr(), recv.data_ptr(), ng, st))
+        pmhip.check(lib.pm_reorder_slab(recv.data_ptr(), slab.data_ptr(), ng, ng, st))
         fft = pmhip.FFT(ng, pmhip.PM_FP64)
         fft.forward(slab)
         fft.apply_green(a)
         fft.backward(slab)
-        pmhip.check(lib.pm_reorder_slab_back(slab.data_ptr(), halo.data_ptr(), ng, ng, st))
+        pmhip.check(lib.pm_reorder_slab_back(slab.data_ptr(), yz.data_ptr(), ng, ng, st))
+        pmhip.check(lib.pm_transpose_slab_back(yz.data_ptr(), halo.data_ptr(), ng, st))
         pmhip.check(lib.pm_centered_derivatives(halo.data_ptr(), *(g.data_ptr() for g in grads), n3, n3, n3, st))
         pmhip.check(lib.pm_update_particles(*(p.data_ptr() for p in pos), *(g.data_ptr() for g in grads),
                                             *(m.data_ptr() for m in mom), a, npart, n1, n1, n1, 0, 0, 0, st))
@@ -139,7 +143,12 @@ def test_reference_layout_operators_vs_oracle(dev):
     assert np.all(s[:, :, ng:] == 7.0)  # padding untouched
     phi = torch.tensor(ref["phi"], device=dev)
     halo = torch.empty(n3 ** 3, dtype=torch.float64, device=dev)
-    pmhip.check(lib.pm_reorder_slab_back(phi.data_ptr(), halo.data_ptr(), ng, ng, st))
+    yz = torch.empty(ng * n3 * n3, dtype=torch.float64, device=dev)
+    pmhip.check(lib.pm_reorder_slab_back(phi.data_ptr(), yz.data_ptr(), ng, ng, st))
+    torch.cuda.synchronize()
+    # the y/z halo block alone is what reorder_slab_back.cpp:11-21 builds
+    assert np.array_equal(host(yz), ref["halo"].reshape(n3, n3, n3)[1:ng + 1].ravel())
+    pmhip.check(lib.pm_transpose_slab_back(yz.data_ptr(), halo.data_ptr(), ng, st))
     grads = [torch.empty(n1 ** 3, dtype=torch.float64, device=dev) for _ in range(3)]
     pmhip.check(lib.pm_centered_derivatives(halo.data_ptr(), *(g.data_ptr() for g in grads), n3, n3, n3, st))
     torch.cuda.synchronize()
