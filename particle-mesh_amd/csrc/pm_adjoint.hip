@@ -145,21 +145,40 @@ __global__ __launch_bounds__(256) void k_deposit_vec3_tiles(const double* __rest
     for (int i = threadIdx.x; i < 3 * E3; i += 256) (&acc[0][0])[i] = Rl(0);
     __syncthreads();
     const uint32_t beg = off[t], end = off[t + 1];
-    for (uint32_t q = beg + threadIdx.x; q < end; q += 256) {
-        const uint32_t p = idx[q];
-        const double x = px[p], y = py[p], z = pz[p];
-        const int ix = (int)x, iy = (int)y, iz = (int)z;
-        const double fx = x - ix, fy = y - iy, fz = z - iz;
-        const double w[8] = {(1 - fx) * (1 - fy) * (1 - fz), (1 - fx) * (1 - fy) * fz, (1 - fx) * fy * (1 - fz),
-                             (1 - fx) * fy * fz,             fx * (1 - fy) * (1 - fz), fx * (1 - fy) * fz,
-                             fx * fy * (1 - fz),             fx * fy * fz};
-        const double u[3] = {ux[p], uy[p], uz[p]};
-        const int n0 = ((ix - ox) * E + (iy - oy)) * E + (iz - oz);
+    constexpr int B = 4;  // particles in flight per thread
+    for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += 256 * B) {
+        uint32_t p[B];
+        double x[B], y[B], z[B], u[B][3];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int n = n0 + ((k >> 2) & 1) * E * E + ((k >> 1) & 1) * E + (k & 1);
+        for (int j = 0; j < B; ++j) {
+            const uint32_t q = q0 + 256 * j;
+            p[j] = q < end ? idx[q] : 0xFFFFFFFFu;
+        }
 #pragma unroll
-            for (int a = 0; a < 3; ++a) atomicAdd(&acc[a][n], (Rl)(w[k] * u[a]));
+        for (int j = 0; j < B; ++j) {
+            if (p[j] == 0xFFFFFFFFu) continue;
+            x[j] = px[p[j]];
+            y[j] = py[p[j]];
+            z[j] = pz[p[j]];
+            u[j][0] = ux[p[j]];
+            u[j][1] = uy[p[j]];
+            u[j][2] = uz[p[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            if (p[j] == 0xFFFFFFFFu) continue;
+            const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
+            const double fx = x[j] - ix, fy = y[j] - iy, fz = z[j] - iz;
+            const double w[8] = {(1 - fx) * (1 - fy) * (1 - fz), (1 - fx) * (1 - fy) * fz, (1 - fx) * fy * (1 - fz),
+                                 (1 - fx) * fy * fz,             fx * (1 - fy) * (1 - fz), fx * (1 - fy) * fz,
+                                 fx * fy * (1 - fz),             fx * fy * fz};
+            const int n0 = ((ix - ox) * E + (iy - oy)) * E + (iz - oz);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int n = n0 + ((k >> 2) & 1) * E * E + ((k >> 1) & 1) * E + (k & 1);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) atomicAdd(&acc[a][n], (Rl)(w[k] * u[j][a]));
+            }
         }
     }
     __syncthreads();

@@ -51,34 +51,68 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
         hcnt[h] = 0u;
     }
     __syncthreads();
-    const int64_t base = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kBinChunk;
+    // each thread takes kBinPer consecutive particles: neighbouring particles
+    // usually share a tile, so runs of equal keys cost one hash insert
+    const int64_t base = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kBinChunk + (int64_t)threadIdx.x * kBinPer;
     const int nt = ng / ts;
     const double box = (double)ng;
     uint32_t k[kBinPer], h[kBinPer], rank[kBinPer];
+    double x[kBinPer], y[kBinPer], z[kBinPer];
+    const bool aligned = (((uintptr_t)px | (uintptr_t)py | (uintptr_t)pz) & 15) == 0;
+    if (aligned && base + kBinPer <= np) {
+        const double2* vx = reinterpret_cast<const double2*>(px + base);
+        const double2* vy = reinterpret_cast<const double2*>(py + base);
+        const double2* vz = reinterpret_cast<const double2*>(pz + base);
+#pragma unroll
+        for (int j = 0; j < kBinPer / 2; ++j) {
+            const double2 a = vx[j], b = vy[j], c = vz[j];
+            x[2 * j] = a.x;
+            x[2 * j + 1] = a.y;
+            y[2 * j] = b.x;
+            y[2 * j + 1] = b.y;
+            z[2 * j] = c.x;
+            z[2 * j + 1] = c.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kBinPer; ++j) {
+            const bool in = base + j < np;
+            x[j] = in ? px[base + j] : -1.0;
+            y[j] = in ? py[base + j] : -1.0;
+            z[j] = in ? pz[base + j] : -1.0;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < kBinPer; ++j) {
-        const int64_t i = base + threadIdx.x + 256 * j;
-        k[j] = kInvalidKey;
         h[j] = 0u;
         rank[j] = 0u;
-        if (i < np) {
-            const double x = px[i], y = py[i], z = pz[i];
-            if (in_box_s(x, box) && in_box_s(y, box) && in_box_s(z, box)) {
-                k[j] = (uint32_t)((((int)x / ts) * nt + (int)y / ts) * nt + (int)z / ts);
-            } else {
-                atomicAdd(err, 1ULL);
-            }
+        if (in_box_s(x[j], box) && in_box_s(y[j], box) && in_box_s(z[j], box)) {
+            k[j] = (uint32_t)((((int)x[j] / ts) * nt + (int)y[j] / ts) * nt + (int)z[j] / ts);
+        } else {
+            k[j] = kInvalidKey;
+            if (base + j < np) atomicAdd(err, 1ULL);
         }
-        if (k[j] != kInvalidKey) {
-            uint32_t s = bin_hash(k[j]) & (kBinHash - 1);
+    }
+    int start = 0;
+#pragma unroll
+    for (int j = 1; j <= kBinPer; ++j) {
+        if (j < kBinPer && k[j] == k[start]) continue;
+        if (k[start] != kInvalidKey) {
+            uint32_t s = bin_hash(k[start]) & (kBinHash - 1);
             while (true) {  // at most kBinChunk distinct keys: a free slot always exists
-                const uint32_t old = atomicCAS(&hkey[s], kInvalidKey, k[j]);
-                if (old == kInvalidKey || old == k[j]) break;
+                const uint32_t old = atomicCAS(&hkey[s], kInvalidKey, k[start]);
+                if (old == kInvalidKey || old == k[start]) break;
                 s = (s + 1) & (kBinHash - 1);
             }
-            h[j] = s;
-            rank[j] = atomicAdd(&hcnt[s], 1u);
+            const uint32_t r = atomicAdd(&hcnt[s], (uint32_t)(j - start));
+#pragma unroll
+            for (int m = 0; m < kBinPer; ++m)
+                if (m >= start && m < j) {
+                    h[m] = s;
+                    rank[m] = r + (uint32_t)(m - start);
+                }
         }
+        start = j;
     }
     __syncthreads();
     // one global atomic per distinct tile; hcnt becomes the chunk's base slot
@@ -89,7 +123,7 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kBinPer; ++j) {
-        const int64_t i = base + threadIdx.x + 256 * j;
+        const int64_t i = base + j;
         if (i < np) {
             key[i] = k[j];
             slot[i] = k[j] == kInvalidKey ? 0u : hcnt[h[j]] + rank[j];
@@ -132,6 +166,8 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t* __restrict_
     idx[off[k] + slot[i]] = (uint32_t)i;
 }
 
+constexpr int kDepBatch = 4;  // particles in flight per thread in the tile kernels
+
 template <int TS> __host__ __device__ constexpr int shell_size() {
     return (TS + 1) * (TS + 1) * (TS + 1) - TS * TS * TS;
 }
@@ -164,25 +200,44 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict_
     __syncthreads();
     const uint32_t beg = off[t], end = off[t + 1];
     const double fix = 4294967296.0 * mass;  // 2^32 * m
-    for (uint32_t q = beg + threadIdx.x; q < end; q += 256) {
-        const uint32_t p = idx[q];
-        const double x = px[p], y = py[p], z = pz[p];
-        const int ix = (int)x, iy = (int)y, iz = (int)z;
-        const double fx = x - ix, fy = y - iy, fz = z - iz;
-        // cic_deposit weights (utils_inline.h:25-49)
-        const double sx0 = 1.0 - fx, sy0 = 1.0 - fy;
-        const double sxy00 = sx0 * sy0, sxy10 = fx * sy0, sxy01 = sx0 * fy, sxy11 = fx * fy;
-        const double m0 = fix * (1.0 - fz), m1 = fix * fz;
-        const int a = ix - ox, b = iy - oy, c = iz - oz;
-        unsigned long long* n0 = acc + (a * E + b) * E + c;
-        atomicAdd(n0, (unsigned long long)__double2ll_rn(sxy00 * m0));
-        atomicAdd(n0 + E * E, (unsigned long long)__double2ll_rn(sxy10 * m0));
-        atomicAdd(n0 + E, (unsigned long long)__double2ll_rn(sxy01 * m0));
-        atomicAdd(n0 + E * E + E, (unsigned long long)__double2ll_rn(sxy11 * m0));
-        atomicAdd(n0 + 1, (unsigned long long)__double2ll_rn(sxy00 * m1));
-        atomicAdd(n0 + E * E + 1, (unsigned long long)__double2ll_rn(sxy10 * m1));
-        atomicAdd(n0 + E + 1, (unsigned long long)__double2ll_rn(sxy01 * m1));
-        atomicAdd(n0 + E * E + E + 1, (unsigned long long)__double2ll_rn(sxy11 * m1));
+    // kDepBatch particles per thread per pass: their index and position loads
+    // are all in flight before the first LDS atomic
+    for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += 256 * kDepBatch) {
+        uint32_t p[kDepBatch];
+        double x[kDepBatch], y[kDepBatch], z[kDepBatch];
+#pragma unroll
+        for (int j = 0; j < kDepBatch; ++j) {
+            const uint32_t q = q0 + 256 * j;
+            p[j] = q < end ? idx[q] : kInvalidKey;
+        }
+#pragma unroll
+        for (int j = 0; j < kDepBatch; ++j) {
+            if (p[j] != kInvalidKey) {
+                x[j] = px[p[j]];
+                y[j] = py[p[j]];
+                z[j] = pz[p[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kDepBatch; ++j) {
+            if (p[j] == kInvalidKey) continue;
+            const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
+            const double fx = x[j] - ix, fy = y[j] - iy, fz = z[j] - iz;
+            // cic_deposit weights (utils_inline.h:25-49)
+            const double sx0 = 1.0 - fx, sy0 = 1.0 - fy;
+            const double sxy00 = sx0 * sy0, sxy10 = fx * sy0, sxy01 = sx0 * fy, sxy11 = fx * fy;
+            const double m0 = fix * (1.0 - fz), m1 = fix * fz;
+            const int a = ix - ox, b = iy - oy, c = iz - oz;
+            unsigned long long* n0 = acc + (a * E + b) * E + c;
+            atomicAdd(n0, (unsigned long long)__double2ll_rn(sxy00 * m0));
+            atomicAdd(n0 + E * E, (unsigned long long)__double2ll_rn(sxy10 * m0));
+            atomicAdd(n0 + E, (unsigned long long)__double2ll_rn(sxy01 * m0));
+            atomicAdd(n0 + E * E + E, (unsigned long long)__double2ll_rn(sxy11 * m0));
+            atomicAdd(n0 + 1, (unsigned long long)__double2ll_rn(sxy00 * m1));
+            atomicAdd(n0 + E * E + 1, (unsigned long long)__double2ll_rn(sxy10 * m1));
+            atomicAdd(n0 + E + 1, (unsigned long long)__double2ll_rn(sxy01 * m1));
+            atomicAdd(n0 + E * E + E + 1, (unsigned long long)__double2ll_rn(sxy11 * m1));
+        }
     }
     __syncthreads();
     const double unfix = 1.0 / 4294967296.0;
