@@ -100,7 +100,10 @@ def run_gpu(prec, pos0, vel0, args, ng, npart, variant):
 
     def one():
         nonlocal step
-        plan.step(pos, vel, 0.5 * dt if step == 0 else dt, dt, variant)
+        # after the first step the positions are the ones this plan wrote:
+        # let the deposit bin from the keys the interpolation left
+        plan.step(pos, vel, 0.5 * dt if step == 0 else dt, dt,
+                  variant | (pmhip.PM_STEP_REUSE_BINS if step > 0 else 0))
         step += 1
 
     for _ in range(args.warmup):

@@ -50,6 +50,12 @@ extern "C" {
 #define PM_DEPOSIT_ATOMIC 0  /* one thread per particle, 8 global float atomic adds */
 #define PM_DEPOSIT_SORTED 1  /* particles binned by 16^3-cell tile, exact 64-bit fixed-point
                                 accumulation in LDS, deterministic (no global float atomics) */
+/* Flag for pm_step's variant: the positions are exactly those written by the
+ * previous pm_step of this plan (same arrays, untouched since), so the
+ * binning can start from the tile keys that step's interpolation left
+ * behind instead of re-reading the positions.  Only honoured with
+ * PM_DEPOSIT_SORTED; pm_run_kdk sets it itself after its first step. */
+#define PM_STEP_REUSE_BINS 0x100
 
 /* Last error message of the calling thread ("" if none). */
 const char* pm_last_error(void);

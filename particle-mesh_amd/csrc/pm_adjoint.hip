@@ -122,36 +122,48 @@ __global__ __launch_bounds__(256) void k_adj_deposit(const double* __restrict__ 
 
 // ------------------------------------------- interpolation transpose
 // U_a[c] += W(x_p, c) u_p^a for a = 0..2, one workgroup per tile of the
-// sorted binning (LDS float atomics, then owned nodes + shells as in the
-// forward deposit).
+// sorted binning.  The cotangents have arbitrary sign and scale, so each
+// component is accumulated in signed 64-bit fixed point with a power-of-two
+// scale 2^40 / 2^ceil(log2 max|u_a|) taken from a global max (k_absmax3):
+// integer LDS adds are exact and order-free (LDS float atomics serialise on
+// this part).  Owned nodes and shells are then written as in the forward
+// deposit.
+constexpr int kVec3Threads = 1024;
+
+__device__ __forceinline__ double vec3_scale(unsigned long long maxbits) {
+    const double m = __longlong_as_double((long long)maxbits);
+    if (!(m > 0.0) || !isfinite(m)) return 1.0;
+    int e = 0;
+    frexp(m, &e);  // m < 2^e
+    return ldexp(1.0, 40 - e);
+}
+
 template <typename Rl, int TS>
-__global__ __launch_bounds__(256) void k_deposit_vec3_tiles(const double* __restrict__ px,
-                                                            const double* __restrict__ py,
-                                                            const double* __restrict__ pz,
-                                                            const double* __restrict__ ux,
-                                                            const double* __restrict__ uy,
-                                                            const double* __restrict__ uz,
-                                                            const uint32_t* __restrict__ idx,
-                                                            const uint32_t* __restrict__ off, int ng,
-                                                            Rl* __restrict__ U0, Rl* __restrict__ U1,
-                                                            Rl* __restrict__ U2, Rl* __restrict__ shell) {
+__global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const double* __restrict__ ux, const double* __restrict__ uy, const double* __restrict__ uz,
+    const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off, int ng,
+    const unsigned long long* __restrict__ umax, Rl* __restrict__ U0, Rl* __restrict__ U1, Rl* __restrict__ U2,
+    Rl* __restrict__ shell) {
     constexpr int E = TS + 1, E3 = E * E * E, SH = (TS + 1) * (TS + 1) * (TS + 1) - TS * TS * TS;
-    __shared__ Rl acc[3][E3];
+    constexpr int NT = kVec3Threads;
+    __shared__ unsigned long long acc[3][E3];
     const int nt = ng / TS;
     const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
               tz = (int)(t % (uint32_t)nt);
     const int ox = tx * TS, oy = ty * TS, oz = tz * TS;
-    for (int i = threadIdx.x; i < 3 * E3; i += 256) (&acc[0][0])[i] = Rl(0);
+    const double sc[3] = {vec3_scale(umax[0]), vec3_scale(umax[1]), vec3_scale(umax[2])};
+    for (int i = threadIdx.x; i < 3 * E3; i += NT) (&acc[0][0])[i] = 0ULL;
     __syncthreads();
     const uint32_t beg = off[t], end = off[t + 1];
-    constexpr int B = 4;  // particles in flight per thread
-    for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += 256 * B) {
+    constexpr int B = 2;  // particles in flight per thread
+    for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += NT * B) {
         uint32_t p[B];
         double x[B], y[B], z[B], u[B][3];
 #pragma unroll
         for (int j = 0; j < B; ++j) {
-            const uint32_t q = q0 + 256 * j;
+            const uint32_t q = q0 + NT * j;
             p[j] = q < end ? idx[q] : 0xFFFFFFFFu;
         }
 #pragma unroll
@@ -160,9 +172,9 @@ __global__ __launch_bounds__(256) void k_deposit_vec3_tiles(const double* __rest
             x[j] = px[p[j]];
             y[j] = py[p[j]];
             z[j] = pz[p[j]];
-            u[j][0] = ux[p[j]];
-            u[j][1] = uy[p[j]];
-            u[j][2] = uz[p[j]];
+            u[j][0] = ux[p[j]] * sc[0];
+            u[j][1] = uy[p[j]] * sc[1];
+            u[j][2] = uz[p[j]] * sc[2];
         }
 #pragma unroll
         for (int j = 0; j < B; ++j) {
@@ -177,22 +189,24 @@ __global__ __launch_bounds__(256) void k_deposit_vec3_tiles(const double* __rest
             for (int k = 0; k < 8; ++k) {
                 const int n = n0 + ((k >> 2) & 1) * E * E + ((k >> 1) & 1) * E + (k & 1);
 #pragma unroll
-                for (int a = 0; a < 3; ++a) atomicAdd(&acc[a][n], (Rl)(w[k] * u[j][a]));
+                for (int a = 0; a < 3; ++a)
+                    atomicAdd(&acc[a][n], (unsigned long long)__double2ll_rn(w[k] * u[j][a]));
             }
         }
     }
     __syncthreads();
+    const double inv[3] = {1.0 / sc[0], 1.0 / sc[1], 1.0 / sc[2]};
     const size_t N = (size_t)ng;
     Rl* out[3] = {U0, U1, U2};
-    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
+    for (int i = threadIdx.x; i < TS * TS * TS; i += NT) {
         const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
         const size_t node = ((size_t)(ox + a) * N + (oy + b)) * N + (oz + c);
         const int n = (a * E + b) * E + c;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) out[d][node] = acc[d][n];
+        for (int d = 0; d < 3; ++d) out[d][node] = (Rl)((double)(long long)acc[d][n] * inv[d]);
     }
     const size_t ntiles = (size_t)nt * nt * nt;
-    for (int i = threadIdx.x; i < SH; i += 256) {
+    for (int i = threadIdx.x; i < SH; i += NT) {
         int a, b, c;
         if (i < E * E) {
             a = TS;
@@ -211,7 +225,26 @@ __global__ __launch_bounds__(256) void k_deposit_vec3_tiles(const double* __rest
         }
         const int n = (a * E + b) * E + c;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) shell[((size_t)d * ntiles + t) * SH + i] = acc[d][n];
+        for (int d = 0; d < 3; ++d)
+            shell[((size_t)d * ntiles + t) * SH + i] = (Rl)((double)(long long)acc[d][n] * inv[d]);
+    }
+}
+
+// max |u_a| over all particles (bit patterns of non-negative doubles order
+// like the values, so an integer atomic max works)
+__global__ __launch_bounds__(256) void k_absmax3(const double* __restrict__ ux, const double* __restrict__ uy,
+                                                 const double* __restrict__ uz, int64_t np,
+                                                 unsigned long long* __restrict__ umax) {
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < np; i += (int64_t)gridDim.x * 256) {
+        m[0] = fmax(m[0], fabs(ux[i]));
+        m[1] = fmax(m[1], fabs(uy[i]));
+        m[2] = fmax(m[2], fabs(uz[i]));
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        for (int o = 32; o > 0; o >>= 1) m[a] = fmax(m[a], __shfl_xor(m[a], o));
+        if (__lane_id() == 0) atomicMax(&umax[a], (unsigned long long)__double_as_longlong(m[a]));
     }
 }
 
@@ -326,17 +359,22 @@ int launch_adj_deposit(int prec, const double* px, const double* py, const doubl
 }
 
 int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
-                        const double* uy, const double* uz, int ng, const SortedScratch& s, void* const U[3],
-                        void* shell3, hipStream_t st) {
+                        const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s,
+                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
     const size_t sh = sorted_shell_elems(ng);
     const size_t rs = prec == PM_FP32 ? sizeof(float) : sizeof(double);
-#define PM_VEC3(RL, TSV)                                                                                          \
-    hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles), dim3(256), 0, st, px, py, pz, ux, uy, uz,     \
-                       s.idx, s.off, ng, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3);                            \
-    for (int d = 0; d < 3; ++d)                                                                                     \
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, st, umax, (size_t)3, 0ULL);
+    if (np > 0) {
+        const unsigned nb = (unsigned)std::min<int64_t>((np + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_absmax3, dim3(nb), dim3(256), 0, st, ux, uy, uz, np, umax);
+    }
+#define PM_VEC3(RL, TSV)                                                                                            \
+    hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles), dim3(kVec3Threads), 0, st, px, py, pz, ux, uy, \
+                       uz, s.idx, s.off, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3);                    \
+    for (int d = 0; d < 3; ++d)                                                                                       \
         launch_fold_shells(prec, ts, ng, (const char*)shell3 + d * sh * rs, U[d], st);
     if (prec == PM_FP32) {
         if (ts == 16) {

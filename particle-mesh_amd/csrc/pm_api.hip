@@ -175,11 +175,11 @@ extern "C" void* pm_fft_spectrum(pm_fft_t* f) { return f ? f->specB : nullptr; }
 extern "C" int pm_plan_destroy(pm_plan_t* p) {
     if (!p) return PM_OK;
     tables_destroy(&p->tab);
-    void* bufs[10] = {p->rho,        p->phi,         p->specA,        p->specB,         p->sorted.cnt,
-                      p->sorted.off, p->sorted.key, p->sorted.slot, p->sorted.idx, p->sorted.shell};
+    void* bufs[11] = {p->rho,        p->phi,         p->specA,        p->specB,       p->sorted.cnt,  p->sorted.off,
+                      p->sorted.key, p->sorted.slot, p->sorted.idx, p->sorted.cur, p->sorted.shell};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    void* abufs[5] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell};
+    void* abufs[6] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell, p->adjMax};
     for (void* b : abufs)
         if (b) (void)hipFree(b);
     for (auto& e : p->ev)
@@ -269,9 +269,10 @@ static int ensure_sorted(pm_plan* p) {
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.key, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.slot, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc(&s.shell, rs * sorted_shell_elems(p->ng)), "sorted deposit: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc((void**)&s.cur, 4 * ntiles), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.idx, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
     if (rc) {
-        void* bufs[6] = {s.cnt, s.off, s.key, s.slot, s.shell, s.idx};
+        void* bufs[7] = {s.cnt, s.off, s.key, s.slot, s.shell, s.cur, s.idx};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         s = SortedScratch();
@@ -346,14 +347,46 @@ extern "C" int pm_interp_force(pm_plan_t* p, const double* x, const double* y, c
 extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
                        double kick, double drift, int variant) {
     if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_step: null argument");
+    const bool reuse = (variant & PM_STEP_REUSE_BINS) != 0;
+    variant &= ~PM_STEP_REUSE_BINS;
+    if (variant != PM_DEPOSIT_SORTED && variant != PM_DEPOSIT_ATOMIC) return fail(PM_ERR_ARG, "pm_step: bad variant");
     mark(p, 0);
-    int rc = deposit_impl(p, x, y, z, variant);
+    int rc;
+    if (variant == PM_DEPOSIT_SORTED) {
+        // binning of x: from the keys the previous step's interpolation left
+        // (caller's guarantee that x is unchanged since), else from scratch
+        rc = ensure_sorted(p);
+        if (!rc) {
+            const bool keys = reuse && p->keys_valid && p->keys_x == x;
+            rc = keys ? launch_bin_from_keys(p->np, p->ng, p->sorted, p->stream)
+                      : launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
+        }
+        const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
+        if (!rc) rc = launch_deposit_binned(p->prec, x, y, z, p->ng, mass, p->rho, p->sorted, p->stream);
+    } else {
+        rc = deposit_impl(p, x, y, z, variant);
+    }
+    p->keys_valid = false;
     if (rc) return rc;
     mark(p, 1);
     rc = solve_impl(p);
     if (rc) return rc;
-    rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 0, range_counter(),
-                       p->stream);
+    if (variant == PM_DEPOSIT_SORTED) {
+        // FP32: particles in memory order (a coherent order after the first
+        // steps keeps the gathers in L2); FP64: tile order with phi in LDS,
+        // which halves the FP64 gather traffic.  Both leave the next binning.
+        if (p->prec == PM_FP32)
+            rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 2, range_counter(),
+                               p->stream, sorted_tile(p->ng), p->sorted.cnt, p->sorted.key);
+        else
+            rc = launch_interp_tiles(p->prec, x, y, z, vx, vy, vz, p->ng, p->phi, kick, drift, p->sorted, 1,
+                                     range_counter(), p->stream);
+        p->keys_valid = rc == PM_OK;
+        p->keys_x = x;
+    } else {
+        rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 0, range_counter(),
+                           p->stream);
+    }
     if (rc) return rc;
     mark(p, 7);
     next_slot(p);
@@ -363,10 +396,11 @@ extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx
 extern "C" int pm_run_kdk(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
                           int nsteps, double dt, int variant) {
     if (!p || nsteps < 0) return fail(PM_ERR_ARG, "pm_run_kdk: bad arguments");
+    variant &= ~PM_STEP_REUSE_BINS;
     for (int s = 0; s <= nsteps; ++s) {
         const double kick = (s == 0 || s == nsteps) ? 0.5 * dt : dt;
         const double drift = s == nsteps ? 0.0 : dt;
-        const int rc = pm_step(p, x, y, z, vx, vy, vz, kick, drift, variant);
+        const int rc = pm_step(p, x, y, z, vx, vy, vz, kick, drift, s == 0 ? variant : variant | PM_STEP_REUSE_BINS);
         if (rc) return rc;
     }
     return PM_OK;
@@ -417,11 +451,14 @@ static int ensure_adjoint(pm_plan* p) {
     for (int a = 0; a < 3 && !rc; ++a) rc = hip_ok(hipMalloc(&p->adjU[a], rs * nc), "adjoint: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc(&p->adjShell, 3 * rs * sorted_shell_elems(p->ng)), "adjoint: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc(&p->adjLam, rs * nc), "adjoint: hipMalloc");
+    if (!rc) rc = hip_ok(hipMalloc((void**)&p->adjMax, 3 * sizeof(unsigned long long)), "adjoint: hipMalloc");
     if (rc) {
         void* bufs[5] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjShell, p->adjLam};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
+        if (p->adjMax) (void)hipFree(p->adjMax);
         p->adjU[0] = p->adjU[1] = p->adjU[2] = p->adjShell = p->adjLam = nullptr;
+        p->adjMax = nullptr;
     }
     return rc;
 }
@@ -449,7 +486,8 @@ static int forward_field(pm_plan* p, const double* x, const double* y, const dou
 // (binning of x already in p->sorted); mu is staged in p->rho.
 static int adjoint_field(pm_plan* p, const double* x, const double* y, const double* z, const double* ux,
                          const double* uy, const double* uz) {
-    int rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->ng, p->sorted, p->adjU, p->adjShell, p->stream);
+    int rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
+                                 p->adjMax, p->stream);
     if (!rc) rc = launch_div3(p->prec, p->adjU, p->ng, p->rho, p->stream);
     if (!rc) rc = solve_into(p, p->rho, p->adjLam, false);
     return rc;
@@ -470,7 +508,8 @@ extern "C" int pm_interp_transpose(pm_plan_t* p, const double* x, const double* 
     int rc = ensure_adjoint(p);
     if (!rc) rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
     if (!rc)
-        rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->ng, p->sorted, p->adjU, p->adjShell, p->stream);
+        rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
+                                 p->adjMax, p->stream);
     return rc;
 }
 

@@ -21,9 +21,12 @@ namespace pm {
 
 constexpr int kFftThreads = 256;
 
-// LDS line pitch: one complex of padding per line so that the same index of
+// LDS layout of a line: element i sits at lpos(i) = i + i/8, so that the
+// stride-8 and stride-64 writes of the Stockham stages spread over the
+// banks; the line pitch adds one more complex so that the same element of
 // consecutive lines falls in different banks.
-__host__ __device__ constexpr int lds_pitch(int L) { return L + 1; }
+__host__ __device__ constexpr int lds_pitch(int L) { return L + (L >> 3) + 1; }
+__device__ __forceinline__ int lpos(int i) { return i + (i >> 3); }
 
 template <int R, int DIR, typename C> struct Dft;
 
@@ -86,7 +89,7 @@ __device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict_
             const int line = g / Q, b = g % Q;
             const C* row = s + line * LP;
 #pragma unroll
-            for (int r = 0; r < R; ++r) u[q][r] = row[b + r * Q];
+            for (int r = 0; r < R; ++r) u[q][r] = row[lpos(b + r * Q)];
         }
     }
     __syncthreads();
@@ -108,7 +111,7 @@ __device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict_
             const int j = (b - k) * R + k;
             C* row = s + line * LP;
 #pragma unroll
-            for (int r = 0; r < R; ++r) row[j + r * p] = u[q][r];
+            for (int r = 0; r < R; ++r) row[lpos(j + r * p)] = u[q][r];
         }
     }
     __syncthreads();
@@ -157,7 +160,7 @@ __global__ __launch_bounds__(kFftThreads) void k_zfwd(const Rl* __restrict__ rho
             v.x -= Rl(1);
             v.y -= Rl(1);
         }
-        s[(g / M) * LP + (g % M)] = v;
+        s[(g / M) * LP + lpos(g % M)] = v;
     }
     __syncthreads();
     fft_lines<C, M, T, -1>(s, twM);
@@ -165,8 +168,8 @@ __global__ __launch_bounds__(kFftThreads) void k_zfwd(const Rl* __restrict__ rho
     for (int g = threadIdx.x; g < T * (M + 1); g += kFftThreads) {
         const int kz = g / T, line = g % T;
         const C* row = s + line * LP;
-        const C zk = row[kz == M ? 0 : kz];
-        const C zm = row[kz == 0 ? 0 : M - kz];
+        const C zk = row[lpos(kz == M ? 0 : kz)];
+        const C zm = row[lpos(kz == 0 ? 0 : M - kz)];
         const C e = C{Rl(0.5) * (zk.x + zm.x), Rl(0.5) * (zk.y - zm.y)};
         const C d = C{Rl(0.5) * (zk.x - zm.x), Rl(0.5) * (zk.y + zm.y)};
         const C wd = cmul(twN[kz], d);
@@ -185,13 +188,13 @@ __global__ __launch_bounds__(kFftThreads) void k_yfwd(const cplx_t<Rl>* __restri
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / XT, x0 = (tile % XT) * T;
     const C* src = in + ((size_t)kz * N + x0) * N;
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) s[(g / N) * LP + (g % N)] = src[g];
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) s[(g / N) * LP + lpos(g % N)] = src[g];
     __syncthreads();
     fft_lines<C, N, T, -1>(s, twN);
     C* dst = out + (size_t)kz * N * N + x0;
     for (int g = threadIdx.x; g < T * N; g += kFftThreads) {
         const int ky = g / T, line = g % T;
-        dst[(size_t)ky * N + line] = s[line * LP + ky];
+        dst[(size_t)ky * N + line] = s[line * LP + lpos(ky)];
     }
 }
 
@@ -222,21 +225,21 @@ __global__ __launch_bounds__(kFftThreads) void k_x(cplx_t<Rl>* __restrict__ data
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / YT, ky0 = (tile % YT) * T;
     C* base = data + ((size_t)kz * N + ky0) * N;
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) s[(g / N) * LP + (g % N)] = base[g];
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) s[(g / N) * LP + lpos(g % N)] = base[g];
     __syncthreads();
     if constexpr (MODE & kXForward) fft_lines<C, N, T, -1>(s, twN);
     if constexpr (MODE & kXGreen) {
         for (int g = threadIdx.x; g < T * N; g += kFftThreads) {
             const int line = g / N, kx = g % N;
             const Rl gv = (Rl)(green_value(ga.mode, N, kx, ky0 + line, kz, ga.cos_tab) * ga.scale);
-            C& v = s[line * LP + kx];
+            C& v = s[line * LP + lpos(kx)];
             v.x *= gv;
             v.y *= gv;
         }
         __syncthreads();
     }
     if constexpr (MODE & kXInverse) fft_lines<C, N, T, +1>(s, twN);
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) base[g] = s[(g / N) * LP + (g % N)];
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) base[g] = s[(g / N) * LP + lpos(g % N)];
 }
 
 // ---------------------------------------------------------------- y inverse
@@ -252,12 +255,12 @@ __global__ __launch_bounds__(kFftThreads) void k_yinv(const cplx_t<Rl>* __restri
     const C* src = in + (size_t)kz * N * N + x0;
     for (int g = threadIdx.x; g < T * N; g += kFftThreads) {
         const int ky = g / T, line = g % T;
-        s[line * LP + ky] = src[(size_t)ky * N + line];
+        s[line * LP + lpos(ky)] = src[(size_t)ky * N + line];
     }
     __syncthreads();
     fft_lines<C, N, T, +1>(s, twN);
     C* dst = out + ((size_t)kz * N + x0) * N;
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) dst[g] = s[(g / N) * LP + (g % N)];
+    for (int g = threadIdx.x; g < T * N; g += kFftThreads) dst[g] = s[(g / N) * LP + lpos(g % N)];
 }
 
 // ---------------------------------------------------------------- z inverse
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(kFftThreads) void k_zinv(const cplx_t<Rl>* __restri
     const int x = tile / YT, y0 = (tile % YT) * T;
     for (int g = threadIdx.x; g < T * (M + 1); g += kFftThreads) {
         const int kz = g / T, line = g % T;
-        s[line * LP + kz] = in[((size_t)kz * N + x) * N + y0 + line];
+        s[line * LP + lpos(kz)] = in[((size_t)kz * N + x) * N + y0 + line];
     }
     __syncthreads();
     // Z[k] = (X[k] + conj X[M-k]) + i W^{-k} (X[k] - conj X[M-k]), k in [0, M);
@@ -287,7 +290,7 @@ __global__ __launch_bounds__(kFftThreads) void k_zinv(const cplx_t<Rl>* __restri
         if (NPRE % kFftThreads == 0 || g < NPRE) {
             const int line = g / M, k = g % M;
             const C* row = s + line * LP;
-            C xk = row[k], xm = row[M - k];
+            C xk = row[lpos(k)], xm = row[lpos(M - k)];
             if (k == 0) {
                 xk.y = Rl(0);
                 xm.y = Rl(0);
@@ -301,13 +304,13 @@ __global__ __launch_bounds__(kFftThreads) void k_zinv(const cplx_t<Rl>* __restri
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int g = threadIdx.x + q * kFftThreads;
-        if (NPRE % kFftThreads == 0 || g < NPRE) s[(g / M) * LP + (g % M)] = z[q];
+        if (NPRE % kFftThreads == 0 || g < NPRE) s[(g / M) * LP + lpos(g % M)] = z[q];
     }
     __syncthreads();
     fft_lines<C, M, T, +1>(s, twM);
     C* dst = reinterpret_cast<C*>(out + ((size_t)x * N + y0) * N);
     for (int g = threadIdx.x; g < T * M; g += kFftThreads) {
-        const C v = s[(g / M) * LP + (g % M)];
+        const C v = s[(g / M) * LP + lpos(g % M)];
         dst[g] = C{v.x * norm, v.y * norm};
     }
 }

@@ -59,8 +59,17 @@ struct SortedScratch {
     uint32_t* key = nullptr;   // np
     uint32_t* slot = nullptr;  // np
     uint32_t* idx = nullptr;   // np
+    uint32_t* cur = nullptr;   // ntiles (scatter cursor of the key-only binning)
     void* shell = nullptr;     // ntiles * shell_size reals
 };
+// binning from the keys and counts left by the tile interpolation
+int launch_bin_from_keys(int64_t np, int ng, SortedScratch& s, hipStream_t st);
+// interpolation + kick + drift over the tile binning (phi staged in LDS);
+// with keys != 0 it also writes each particle's new tile key to s.key and
+// adds the tile counts to s.cnt for the next step's binning
+int launch_interp_tiles(int prec, double* px, double* py, double* pz, double* vx, double* vy, double* vz, int ng,
+                        const void* phi, double kick, double drift, SortedScratch& s, int keys,
+                        unsigned long long* err, hipStream_t st);
 int sorted_tile(int ng);
 size_t sorted_shell_elems(int ng);
 int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
@@ -77,8 +86,8 @@ int launch_adj_deposit(int prec, const double* px, const double* py, const doubl
                        const void* mbar, double scale, double* bx, double* by, double* bz, unsigned long long* err,
                        hipStream_t st);
 int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
-                        const double* uy, const double* uz, int ng, const SortedScratch& s, void* const U[3],
-                        void* shell3, hipStream_t st);
+                        const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s,
+                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st);
 int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st);
 int launch_adj_gather(int prec, int mode, const double* px, const double* py, const double* pz, double* vx,
                       double* vy, double* vz, const double* ux, const double* uy, const double* uz, double* bx,
@@ -93,9 +102,12 @@ int launch_fill(int prec, void* p, size_t n, double v, hipStream_t st);
 int launch_deposit_atomic(int prec, const double* px, const double* py, const double* pz,
                           int64_t np, int ng, double mass, void* rho, unsigned long long* err,
                           hipStream_t st);
+// mode 0: kick + drift, 1: forces only (into vx, vy, vz), 2: kick + drift and
+// the next step's tile keys / counts (ts = tile side, cnt, key of the binning)
 int launch_interp(int prec, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
                   int64_t np, int ng, const void* phi, double kick, double drift, int mode,
-                  unsigned long long* err, hipStream_t st);
+                  unsigned long long* err, hipStream_t st, int ts = 16, uint32_t* cnt = nullptr,
+                  uint32_t* key = nullptr);
 
 // Error reporting.
 void set_error(const std::string& msg);
@@ -121,9 +133,14 @@ struct pm_plan {
     void* specB = nullptr;
     pm::FftTables tab;
     pm::SortedScratch sorted;  // allocated on the first sorted deposit
+    // the tile interpolation of the last pm_step left the next binning's keys
+    // and counts for positions keys_x (consumed by PM_STEP_REUSE_BINS)
+    bool keys_valid = false;
+    const double* keys_x = nullptr;
     void* adjU[3] = {};        // adjoint meshes U_a (interpolation transpose)
     void* adjLam = nullptr;    // lambda = S D^T U
     void* adjShell = nullptr;  // 3 shell buffers of the vec3 deposit
+    unsigned long long* adjMax = nullptr;  // max |u_a| (fixed-point scales of the vec3 deposit)
     // Phase timing: one set of PM_NPHASES+1 events per recorded step, read
     // back (and summed) only by pm_plan_phase_ms, so that recording needs no
     // synchronisation inside a timed loop.

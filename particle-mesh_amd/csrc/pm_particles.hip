@@ -75,30 +75,15 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
     return x;
 }
 
-// update_particles + trilerp1 (utils.cpp:141-170, utils_inline.h:55-70)
-// with the centred difference of centered_derivatives (utils.cpp:130-132)
-// evaluated on the fly from the periodic potential, then the drift that
-// the reference leaves commented out (utils.cpp:145-146).
-// MODE 0: v += g*kick, x += v*drift.  MODE 1: write g to (vx, vy, vz).
-template <typename Rl, int MODE>
-__global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px, double* __restrict__ py,
-                                                         double* __restrict__ pz, double* __restrict__ vx,
-                                                         double* __restrict__ vy, double* __restrict__ vz,
-                                                         int64_t np, int ng, const Rl* __restrict__ phi,
-                                                         double kick, double drift,
-                                                         unsigned long long* __restrict__ err) {
-    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kPartThreads + threadIdx.x;
-    if (i >= np) return;
-    const double x = px[i], y = py[i], z = pz[i];
-    const double box = (double)ng;
-    if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
-        atomicAdd(err, 1ULL);
-        return;
-    }
+// Centred-difference force (centered_derivatives, utils.cpp:130-132) of the
+// periodic potential, trilinearly interpolated (trilerp1, utils_inline.h:55-70)
+// at a particle in cell (ix, iy, iz) with offsets (tx, ty, tz).
+template <typename Rl>
+__device__ __forceinline__ void cic_force(const Rl* __restrict__ phi, int N, double x, double y, double z,
+                                          double (&F)[3]) {
     const int ix = (int)x, iy = (int)y, iz = (int)z;
     const Rl tx = (Rl)(x - ix), ty = (Rl)(y - iy), tz = (Rl)(z - iz);
     const Rl mx = Rl(1) - tx, my = Rl(1) - ty, mz = Rl(1) - tz;
-    const int N = ng;
     // periodic neighbours -1, 0, +1, +2 on each axis
     const int xa[4] = {ix == 0 ? N - 1 : ix - 1, ix, ix + 1 == N ? 0 : ix + 1,
                        ix + 2 >= N ? ix + 2 - N : ix + 2};
@@ -156,45 +141,128 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
         const Rl fz1 = g00 * my + g10 * ty;
         return fz0 * mz + fz1 * tz;
     };
-    const double fx = (double)tri(gx), fy = (double)tri(gy), fz = (double)tri(gz);
-    if constexpr (MODE == 1) {
-        vx[i] = fx;
-        vy[i] = fy;
-        vz[i] = fz;
-    } else {
-        const double nvx = vx[i] + fx * kick, nvy = vy[i] + fy * kick, nvz = vz[i] + fz * kick;
-        vx[i] = nvx;
-        vy[i] = nvy;
-        vz[i] = nvz;
-        if (drift != 0.0) {
-            px[i] = wrap_pos(x + nvx * drift, box);
-            py[i] = wrap_pos(y + nvy * drift, box);
-            pz[i] = wrap_pos(z + nvz * drift, box);
+    F[0] = (double)tri(gx);
+    F[1] = (double)tri(gy);
+    F[2] = (double)tri(gz);
+}
+
+// update_particles (utils.cpp:141-170) with the force of cic_force, then the
+// drift that the reference leaves commented out (utils.cpp:145-146).
+// MODE 0: v += g*kick, x += v*drift.  MODE 1: write g to (vx, vy, vz).
+// MODE 2: as MODE 0, and bin the drifted position for the next step: its
+// tile key goes to key[i], the tile counts to cnt (aggregated per workgroup
+// in an LDS hash; a key that finds the hash full adds itself directly).
+template <typename Rl, int MODE>
+__global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px, double* __restrict__ py,
+                                                         double* __restrict__ pz, double* __restrict__ vx,
+                                                         double* __restrict__ vy, double* __restrict__ vz,
+                                                         int64_t np, int ng, const Rl* __restrict__ phi,
+                                                         double kick, double drift, int ts,
+                                                         uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
+                                                         unsigned long long* __restrict__ err) {
+    constexpr int HS = MODE == 2 ? 128 : 1;
+    __shared__ uint32_t hkey[HS], hcnt[HS];
+    if constexpr (MODE == 2) {
+        for (int s = threadIdx.x; s < HS; s += kPartThreads) {
+            hkey[s] = 0xFFFFFFFFu;
+            hcnt[s] = 0u;
         }
+        __syncthreads();
+    }
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kPartThreads + threadIdx.x;
+    const double box = (double)ng;
+    bool ok = i < np;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (ok) {
+        x = px[i];
+        y = py[i];
+        z = pz[i];
+        if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
+            atomicAdd(err, 1ULL);
+            ok = false;
+            if constexpr (MODE == 2) key[i] = 0xFFFFFFFFu;
+        }
+    }
+    if (ok) {
+        double F[3];
+        cic_force(phi, ng, x, y, z, F);
+        if constexpr (MODE == 1) {
+            vx[i] = F[0];
+            vy[i] = F[1];
+            vz[i] = F[2];
+        } else {
+            const double nvx = vx[i] + F[0] * kick, nvy = vy[i] + F[1] * kick, nvz = vz[i] + F[2] * kick;
+            vx[i] = nvx;
+            vy[i] = nvy;
+            vz[i] = nvz;
+            if (drift != 0.0) {
+                x = wrap_pos(x + nvx * drift, box);
+                y = wrap_pos(y + nvy * drift, box);
+                z = wrap_pos(z + nvz * drift, box);
+                px[i] = x;
+                py[i] = y;
+                pz[i] = z;
+            }
+            if constexpr (MODE == 2) {
+                // a particle that drifted more than a box away stays unbinned
+                // and is counted, as the next step's range check would
+                if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
+                    atomicAdd(err, 1ULL);
+                    key[i] = 0xFFFFFFFFu;
+                    ok = false;
+                }
+            }
+            if (MODE == 2 && ok) {
+                const int nt = ng / ts;
+                const uint32_t k = (uint32_t)((((int)x / ts) * nt + (int)y / ts) * nt + (int)z / ts);
+                key[i] = k;
+                uint32_t s = (k * 2654435761u) >> 25;  // 7 bits
+                bool done = false;
+                for (int probe = 0; probe < HS; ++probe) {
+                    const uint32_t old = atomicCAS(&hkey[s], 0xFFFFFFFFu, k);
+                    if (old == 0xFFFFFFFFu || old == k) {
+                        atomicAdd(&hcnt[s], 1u);
+                        done = true;
+                        break;
+                    }
+                    s = (s + 1) & (HS - 1);
+                }
+                if (!done) atomicAdd(&cnt[k], 1u);
+            }
+        }
+    }
+    if constexpr (MODE == 2) {
+        __syncthreads();
+        for (int s = threadIdx.x; s < HS; s += kPartThreads)
+            if (hkey[s] != 0xFFFFFFFFu) atomicAdd(&cnt[hkey[s]], hcnt[s]);
     }
 }
 
 int launch_interp(int prec, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
                   int64_t np, int ng, const void* phi, double kick, double drift, int mode,
-                  unsigned long long* err, hipStream_t st) {
+                  unsigned long long* err, hipStream_t st, int ts, uint32_t* cnt, uint32_t* key) {
     if (np <= 0) return PM_OK;
     const unsigned nb = (unsigned)((np + kPartThreads - 1) / kPartThreads);
     const dim3 g(nb), b(kPartThreads);
+#define PM_INTERP(RL, M)                                                                                    \
+    hipLaunchKernelGGL((k_interp<RL, M>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng, (const RL*)phi, kick, \
+                       drift, ts, cnt, key, err)
     if (prec == PM_FP32) {
         if (mode == 0)
-            hipLaunchKernelGGL((k_interp<float, 0>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng,
-                               (const float*)phi, kick, drift, err);
+            PM_INTERP(float, 0);
+        else if (mode == 1)
+            PM_INTERP(float, 1);
         else
-            hipLaunchKernelGGL((k_interp<float, 1>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng,
-                               (const float*)phi, kick, drift, err);
+            PM_INTERP(float, 2);
     } else {
         if (mode == 0)
-            hipLaunchKernelGGL((k_interp<double, 0>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng,
-                               (const double*)phi, kick, drift, err);
+            PM_INTERP(double, 0);
+        else if (mode == 1)
+            PM_INTERP(double, 1);
         else
-            hipLaunchKernelGGL((k_interp<double, 1>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng,
-                               (const double*)phi, kick, drift, err);
+            PM_INTERP(double, 2);
     }
+#undef PM_INTERP
     return check_launch("interp");
 }
 

@@ -25,6 +25,7 @@ PM_GREEN_REF_KH = 0
 PM_GREEN_CONT_K2 = 1
 PM_DEPOSIT_ATOMIC = 0
 PM_DEPOSIT_SORTED = 1
+PM_STEP_REUSE_BINS = 0x100
 PM_NPHASES = 7
 
 PHASE_NAMES = ["deposit", "fft_z_fwd", "fft_y_fwd", "fft_x_green", "fft_y_inv", "fft_z_inv", "interp_kick_drift"]

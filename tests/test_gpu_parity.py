@@ -344,3 +344,55 @@ def test_kdk_fp64_variants_match_oracle(dev, variant):
         d = host(a) - b
         d -= ng * np.round(d / ng)
         assert np.max(np.abs(d)) < 1e-7
+
+
+def test_tile_interp_bitwise_equals_particle_interp(dev):
+    """pm_step with the sorted deposit interpolates per tile from LDS; the
+    standalone pm_interp_kick_drift walks particles in memory order from
+    HBM.  Same arithmetic in the same order; only the compiler's FMA
+    contraction may differ, so the bound is a few ulps of the mesh type."""
+    import torch
+
+    ng, nrow = 32, 32
+    npart = nrow ** 3
+    noise = np.random.default_rng(21).standard_normal(npart)
+    pos0, vel0 = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 1.0, 3.0)
+    for prec in (pmhip.PM_FP32, pmhip.PM_FP64):
+        a_pos, a_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
+        b_pos, b_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
+        with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+            plan.step(a_pos, a_vel, 0.01, 0.02, pmhip.PM_DEPOSIT_SORTED)
+            plan.deposit(b_pos, pmhip.PM_DEPOSIT_SORTED)
+            plan.solve()
+            plan.interp_kick_drift(b_pos, b_vel, 0.01, 0.02)
+            torch.cuda.synchronize()
+        eps = 1e-6 if prec == pmhip.PM_FP32 else 1e-14
+        for a, b in zip(a_vel, b_vel):
+            assert float((a - b).abs().max()) <= eps * float(b.abs().max())
+        for a, b in zip(a_pos, b_pos):
+            d = a - b
+            d = d - ng * torch.round(d / ng)
+            assert float(d.abs().max()) <= eps * 0.02 * max(float(v.abs().max()) for v in b_vel) + 1e-12
+
+
+def test_reused_binning_bitwise_equals_fresh_binning(dev):
+    """pm_run_kdk bins every step after the first from the keys its tile
+    interpolation wrote; stepping with a fresh binning each time must give
+    bit-identical trajectories (the fixed-point deposit is order-free)."""
+    import torch
+
+    ng, nrow, nsteps, dt = 32, 32, 6, 0.02
+    npart = nrow ** 3
+    noise = np.random.default_rng(22).standard_normal(npart)
+    pos0, vel0 = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 1.5, 3.0)
+    a_pos, a_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
+    b_pos, b_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
+    with pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.run_kdk(a_pos, a_vel, nsteps, dt, pmhip.PM_DEPOSIT_SORTED)
+        for s in range(nsteps + 1):
+            kick = 0.5 * dt if s in (0, nsteps) else dt
+            plan.step(b_pos, b_vel, kick, 0.0 if s == nsteps else dt, pmhip.PM_DEPOSIT_SORTED)
+        torch.cuda.synchronize()
+    assert pmhip.range_errors() == 0
+    for a, b in zip(a_pos + a_vel, b_pos + b_vel):
+        assert torch.equal(a, b)
