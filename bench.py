@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-alt-deposit", action="store_true", help="skip the leg with the other deposit variant")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-vs-FP64 accuracy leg")
+    ap.add_argument("--accuracy-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -171,6 +173,48 @@ def run_adjoint_leg(pos0, vel0, args, ng, npart, variant):
             "mode": "reverse-time leapfrog recomputation, FP32 mesh, FP64 positions"}
 
 
+def run_accuracy_leg(pos0, vel0, args, ng, npart, variant, nsteps=20):
+    """FP32 mesh path against the FP64 mesh path on the same ICs: density,
+    potential and force of the initial state (relative RMS), and the
+    final-position error after nsteps KDK steps (cells, periodic)."""
+    import torch
+    import pmhip
+
+    fields = {}
+    for prec in (pmhip.PM_FP64, pmhip.PM_FP32):
+        plan = pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0)
+        plan.deposit(pos0, variant)
+        rho = plan.density().double().clone()
+        plan.solve()
+        phi = plan.potential().double().clone()
+        g = [torch.empty_like(p) for p in pos0]
+        plan.interp_force(pos0, g)
+        pos = [p.clone() for p in pos0]
+        vel = [v.clone() for v in vel0]
+        plan.run_kdk(pos, vel, nsteps, args.dt, variant)
+        torch.cuda.synchronize()
+        plan.close()
+        fields[prec] = (rho, phi, g, pos)
+    r64, p64, g64, x64 = fields[pmhip.PM_FP64]
+    r32, p32, g32, x32 = fields[pmhip.PM_FP32]
+
+    def rel(a, b):
+        return float(torch.sqrt(((a - b) ** 2).mean() / (b ** 2).mean()))
+
+    gnum = sum(float(((a - b) ** 2).mean()) for a, b in zip(g32, g64))
+    gden = sum(float((b ** 2).mean()) for b in g64)
+    d2 = None
+    for a, b in zip(x32, x64):
+        d = a - b
+        d = d - ng * torch.round(d / ng)
+        d2 = d * d if d2 is None else d2 + d * d
+    del fields
+    return {"steps": nsteps, "dt": args.dt, "density_rel_rms": rel(r32, r64), "potential_rel_rms": rel(p32, p64),
+            "force_rel_rms": (gnum / gden) ** 0.5, "final_position_rms_cells": float(torch.sqrt(d2.mean())),
+            "final_position_max_cells": float(torch.sqrt(d2.max())), "target_rms_cells": 1e-4,
+            "against": "FP64 mesh/FFT/force path on the GPU (the same step as the FP64 oracle)"}
+
+
 def cpu_baseline(pos_dev, vel_dev, ng, args):
     """The FP64 oracle (C/OpenMP restatement of the reference step) on the
     same ICs, args.cpu_steps steps, timed on the host cores."""
@@ -242,6 +286,12 @@ def main():
         log(f"FP64: {ms64:.3f} ms/step, phases {ph64}")
         res64 = {"value": npart / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "phases_ms": ph64}
 
+    acc = None
+    if not args.no_accuracy:
+        log(f"accuracy leg (FP32 vs FP64, {args.accuracy_steps} KDK steps)")
+        acc = run_accuracy_leg(pos0, vel0, args, ng, npart, variant, args.accuracy_steps)
+        log(f"accuracy: {acc}")
+
     adj = None
     if not args.no_adjoint:
         log("adjoint leg")
@@ -295,6 +345,7 @@ def main():
                   "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["phases_ms"].items()}}
                  if res64 else None),
         "alt_deposit": alt,
+        "accuracy_vs_fp64": acc,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,
         "range_errors": err32,

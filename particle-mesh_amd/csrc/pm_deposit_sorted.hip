@@ -212,7 +212,11 @@ __global__ __launch_bounds__(256) void k_scatter_keys(const uint32_t* __restrict
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kBinPer; ++j)
-        if (k[j] != kInvalidKey) idx[hcnt[h[j]] + rank[j]] = (uint32_t)(base + j);
+        if (k[j] != kInvalidKey) {
+            // keys and counts come from another kernel: never write past np
+            const uint32_t d = hcnt[h[j]] + rank[j];
+            if (d < (uint64_t)np) idx[d] = (uint32_t)(base + j);
+        }
 }
 
 // 3. particle indices in tile order.

@@ -19,7 +19,30 @@
 
 namespace pm {
 
-constexpr int kFftThreads = 256;
+// Workgroup size of each pass at the largest meshes (measured at 512^3):
+// the transposing y passes gain from more waves in flight than the z / x
+// passes, whose LDS tile already limits the workgroups per CU.
+#ifndef PM_FFT_NT_ZF
+#define PM_FFT_NT_ZF 512
+#endif
+#ifndef PM_FFT_NT_YF
+#define PM_FFT_NT_YF 1024
+#endif
+#ifndef PM_FFT_NT_X
+#define PM_FFT_NT_X 512
+#endif
+#ifndef PM_FFT_NT_YI
+#define PM_FFT_NT_YI 1024
+#endif
+#ifndef PM_FFT_NT_ZI
+#define PM_FFT_NT_ZI 512
+#endif
+// at most one thread per 4 elements of the tile (small meshes), at least a wave
+__host__ __device__ constexpr int fft_threads(int want, int tile_elems) {
+    int nt = want;
+    while (nt > 64 && nt * 4 > tile_elems) nt /= 2;
+    return nt;
+}
 
 // LDS layout of a line: element i sits at lpos(i) = i + i/8, so that the
 // stride-8 and stride-64 writes of the Stockham stages spread over the
@@ -75,17 +98,17 @@ template <int DIR, typename C> struct Dft<8, DIR, C> {
 // One radix-R Stockham autosort stage over T lines of length L held in LDS
 // (line pitch lds_pitch(L)).  p is the length of the sub-transforms already
 // done; tw[m] = exp(-2 pi i m / L).
-template <typename C, int L, int T, int R, int DIR>
+template <typename C, int L, int T, int R, int DIR, int NT>
 __device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict__ tw) {
     constexpr int Q = L / R;
     constexpr int NBF = T * Q;
-    constexpr int PER = (NBF + kFftThreads - 1) / kFftThreads;
+    constexpr int PER = (NBF + NT - 1) / NT;
     constexpr int LP = lds_pitch(L);
     C u[PER][R];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int g = threadIdx.x + q * kFftThreads;
-        if (NBF % kFftThreads == 0 || g < NBF) {
+        const int g = threadIdx.x + q * NT;
+        if (NBF % NT == 0 || g < NBF) {
             const int line = g / Q, b = g % Q;
             const C* row = s + line * LP;
 #pragma unroll
@@ -95,8 +118,8 @@ __device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict_
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int g = threadIdx.x + q * kFftThreads;
-        if (NBF % kFftThreads == 0 || g < NBF) {
+        const int g = threadIdx.x + q * NT;
+        if (NBF % NT == 0 || g < NBF) {
             const int line = g / Q, b = g % Q;
             const int k = b & (p - 1);
             if (p > 1) {
@@ -119,19 +142,19 @@ __device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict_
 
 // Full length-L FFT of T lines in LDS: radix-8 stages, then one radix-4 or
 // radix-2 stage for the remaining factor.
-template <typename C, int L, int T, int DIR>
+template <typename C, int L, int T, int DIR, int NT>
 __device__ __forceinline__ void fft_lines(C* s, const C* __restrict__ tw) {
     constexpr int lg = ilog2(L);
     int p = 1;
     if constexpr (lg >= 3) {
 #pragma unroll
         for (int st = 0; st < lg / 3; ++st) {
-            stockham_stage<C, L, T, 8, DIR>(s, p, tw);
+            stockham_stage<C, L, T, 8, DIR, NT>(s, p, tw);
             p *= 8;
         }
     }
-    if constexpr (lg % 3 == 2) stockham_stage<C, L, T, 4, DIR>(s, p, tw);
-    if constexpr (lg % 3 == 1) stockham_stage<C, L, T, 2, DIR>(s, p, tw);
+    if constexpr (lg % 3 == 2) stockham_stage<C, L, T, 4, DIR, NT>(s, p, tw);
+    if constexpr (lg % 3 == 1) stockham_stage<C, L, T, 2, DIR, NT>(s, p, tw);
 }
 
 // Lines per workgroup: 128-byte global segments (16 float2 / 8 double2),
@@ -143,8 +166,8 @@ template <typename Rl> __host__ __device__ constexpr int tile_lines(int N) {
 }
 
 // ---------------------------------------------------------------- z forward
-template <typename Rl, int N, bool SUB1>
-__global__ __launch_bounds__(kFftThreads) void k_zfwd(const Rl* __restrict__ rho,
+template <typename Rl, int N, bool SUB1, int NT>
+__global__ __launch_bounds__(NT) void k_zfwd(const Rl* __restrict__ rho,
                                                       cplx_t<Rl>* __restrict__ out,
                                                       const cplx_t<Rl>* __restrict__ twM,
                                                       const cplx_t<Rl>* __restrict__ twN) {
@@ -154,7 +177,7 @@ __global__ __launch_bounds__(kFftThreads) void k_zfwd(const Rl* __restrict__ rho
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int x = tile / YT, y0 = (tile % YT) * T;
     const C* src = reinterpret_cast<const C*>(rho + ((size_t)x * N + y0) * N);
-    for (int g = threadIdx.x; g < T * M; g += kFftThreads) {
+    for (int g = threadIdx.x; g < T * M; g += NT) {
         C v = src[g];
         if (SUB1) {
             v.x -= Rl(1);
@@ -163,9 +186,9 @@ __global__ __launch_bounds__(kFftThreads) void k_zfwd(const Rl* __restrict__ rho
         s[(g / M) * LP + lpos(g % M)] = v;
     }
     __syncthreads();
-    fft_lines<C, M, T, -1>(s, twM);
+    fft_lines<C, M, T, -1, NT>(s, twM);
     // X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2, Z[M] = Z[0]
-    for (int g = threadIdx.x; g < T * (M + 1); g += kFftThreads) {
+    for (int g = threadIdx.x; g < T * (M + 1); g += NT) {
         const int kz = g / T, line = g % T;
         const C* row = s + line * LP;
         const C zk = row[lpos(kz == M ? 0 : kz)];
@@ -178,8 +201,8 @@ __global__ __launch_bounds__(kFftThreads) void k_zfwd(const Rl* __restrict__ rho
 }
 
 // ---------------------------------------------------------------- y forward
-template <typename Rl, int N>
-__global__ __launch_bounds__(kFftThreads) void k_yfwd(const cplx_t<Rl>* __restrict__ in,
+template <typename Rl, int N, int NT>
+__global__ __launch_bounds__(NT) void k_yfwd(const cplx_t<Rl>* __restrict__ in,
                                                       cplx_t<Rl>* __restrict__ out,
                                                       const cplx_t<Rl>* __restrict__ twN) {
     using C = cplx_t<Rl>;
@@ -188,11 +211,11 @@ __global__ __launch_bounds__(kFftThreads) void k_yfwd(const cplx_t<Rl>* __restri
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / XT, x0 = (tile % XT) * T;
     const C* src = in + ((size_t)kz * N + x0) * N;
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) s[(g / N) * LP + lpos(g % N)] = src[g];
+    for (int g = threadIdx.x; g < T * N; g += NT) s[(g / N) * LP + lpos(g % N)] = src[g];
     __syncthreads();
-    fft_lines<C, N, T, -1>(s, twN);
+    fft_lines<C, N, T, -1, NT>(s, twN);
     C* dst = out + (size_t)kz * N * N + x0;
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) {
+    for (int g = threadIdx.x; g < T * N; g += NT) {
         const int ky = g / T, line = g % T;
         dst[(size_t)ky * N + line] = s[line * LP + lpos(ky)];
     }
@@ -215,8 +238,8 @@ __device__ __forceinline__ double green_value(int mode, int N, int i, int j, int
 }
 
 // ------------------------------------------------- x: forward/Green/inverse
-template <typename Rl, int N, int MODE>
-__global__ __launch_bounds__(kFftThreads) void k_x(cplx_t<Rl>* __restrict__ data,
+template <typename Rl, int N, int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_x(cplx_t<Rl>* __restrict__ data,
                                                    const cplx_t<Rl>* __restrict__ twN,
                                                    GreenArgs ga) {
     using C = cplx_t<Rl>;
@@ -225,11 +248,11 @@ __global__ __launch_bounds__(kFftThreads) void k_x(cplx_t<Rl>* __restrict__ data
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / YT, ky0 = (tile % YT) * T;
     C* base = data + ((size_t)kz * N + ky0) * N;
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) s[(g / N) * LP + lpos(g % N)] = base[g];
+    for (int g = threadIdx.x; g < T * N; g += NT) s[(g / N) * LP + lpos(g % N)] = base[g];
     __syncthreads();
-    if constexpr (MODE & kXForward) fft_lines<C, N, T, -1>(s, twN);
+    if constexpr (MODE & kXForward) fft_lines<C, N, T, -1, NT>(s, twN);
     if constexpr (MODE & kXGreen) {
-        for (int g = threadIdx.x; g < T * N; g += kFftThreads) {
+        for (int g = threadIdx.x; g < T * N; g += NT) {
             const int line = g / N, kx = g % N;
             const Rl gv = (Rl)(green_value(ga.mode, N, kx, ky0 + line, kz, ga.cos_tab) * ga.scale);
             C& v = s[line * LP + lpos(kx)];
@@ -238,13 +261,13 @@ __global__ __launch_bounds__(kFftThreads) void k_x(cplx_t<Rl>* __restrict__ data
         }
         __syncthreads();
     }
-    if constexpr (MODE & kXInverse) fft_lines<C, N, T, +1>(s, twN);
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) base[g] = s[(g / N) * LP + lpos(g % N)];
+    if constexpr (MODE & kXInverse) fft_lines<C, N, T, +1, NT>(s, twN);
+    for (int g = threadIdx.x; g < T * N; g += NT) base[g] = s[(g / N) * LP + lpos(g % N)];
 }
 
 // ---------------------------------------------------------------- y inverse
-template <typename Rl, int N>
-__global__ __launch_bounds__(kFftThreads) void k_yinv(const cplx_t<Rl>* __restrict__ in,
+template <typename Rl, int N, int NT>
+__global__ __launch_bounds__(NT) void k_yinv(const cplx_t<Rl>* __restrict__ in,
                                                       cplx_t<Rl>* __restrict__ out,
                                                       const cplx_t<Rl>* __restrict__ twN) {
     using C = cplx_t<Rl>;
@@ -253,30 +276,30 @@ __global__ __launch_bounds__(kFftThreads) void k_yinv(const cplx_t<Rl>* __restri
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / XT, x0 = (tile % XT) * T;
     const C* src = in + (size_t)kz * N * N + x0;
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) {
+    for (int g = threadIdx.x; g < T * N; g += NT) {
         const int ky = g / T, line = g % T;
         s[line * LP + lpos(ky)] = src[(size_t)ky * N + line];
     }
     __syncthreads();
-    fft_lines<C, N, T, +1>(s, twN);
+    fft_lines<C, N, T, +1, NT>(s, twN);
     C* dst = out + ((size_t)kz * N + x0) * N;
-    for (int g = threadIdx.x; g < T * N; g += kFftThreads) dst[g] = s[(g / N) * LP + lpos(g % N)];
+    for (int g = threadIdx.x; g < T * N; g += NT) dst[g] = s[(g / N) * LP + lpos(g % N)];
 }
 
 // ---------------------------------------------------------------- z inverse
-template <typename Rl, int N>
-__global__ __launch_bounds__(kFftThreads) void k_zinv(const cplx_t<Rl>* __restrict__ in,
+template <typename Rl, int N, int NT>
+__global__ __launch_bounds__(NT) void k_zinv(const cplx_t<Rl>* __restrict__ in,
                                                       Rl* __restrict__ out,
                                                       const cplx_t<Rl>* __restrict__ twM,
                                                       const cplx_t<Rl>* __restrict__ twN, Rl norm) {
     using C = cplx_t<Rl>;
     constexpr int M = N / 2, T = tile_lines<Rl>(N), LP = lds_pitch(M), YT = N / T;
     constexpr int NPRE = T * M;
-    constexpr int PER = (NPRE + kFftThreads - 1) / kFftThreads;
+    constexpr int PER = (NPRE + NT - 1) / NT;
     __shared__ C s[T * LP];
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int x = tile / YT, y0 = (tile % YT) * T;
-    for (int g = threadIdx.x; g < T * (M + 1); g += kFftThreads) {
+    for (int g = threadIdx.x; g < T * (M + 1); g += NT) {
         const int kz = g / T, line = g % T;
         s[line * LP + lpos(kz)] = in[((size_t)kz * N + x) * N + y0 + line];
     }
@@ -286,8 +309,8 @@ __global__ __launch_bounds__(kFftThreads) void k_zinv(const cplx_t<Rl>* __restri
     C z[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int g = threadIdx.x + q * kFftThreads;
-        if (NPRE % kFftThreads == 0 || g < NPRE) {
+        const int g = threadIdx.x + q * NT;
+        if (NPRE % NT == 0 || g < NPRE) {
             const int line = g / M, k = g % M;
             const C* row = s + line * LP;
             C xk = row[lpos(k)], xm = row[lpos(M - k)];
@@ -303,13 +326,13 @@ __global__ __launch_bounds__(kFftThreads) void k_zinv(const cplx_t<Rl>* __restri
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int g = threadIdx.x + q * kFftThreads;
-        if (NPRE % kFftThreads == 0 || g < NPRE) s[(g / M) * LP + lpos(g % M)] = z[q];
+        const int g = threadIdx.x + q * NT;
+        if (NPRE % NT == 0 || g < NPRE) s[(g / M) * LP + lpos(g % M)] = z[q];
     }
     __syncthreads();
-    fft_lines<C, M, T, +1>(s, twM);
+    fft_lines<C, M, T, +1, NT>(s, twM);
     C* dst = reinterpret_cast<C*>(out + ((size_t)x * N + y0) * N);
-    for (int g = threadIdx.x; g < T * M; g += kFftThreads) {
+    for (int g = threadIdx.x; g < T * M; g += NT) {
         const C v = s[(g / M) * LP + lpos(g % M)];
         dst[g] = C{v.x * norm, v.y * norm};
     }
@@ -340,37 +363,41 @@ static void launch_pass(int pass, hipStream_t st, const void* in, void* out, con
     using C = cplx_t<Rl>;
     constexpr int T = tile_lines<Rl>(N);
     constexpr int M = N / 2;
-    const dim3 blk(kFftThreads);
+    constexpr int NZF = fft_threads(PM_FFT_NT_ZF, T * M), NYF = fft_threads(PM_FFT_NT_YF, T * N);
+    constexpr int NX = fft_threads(PM_FFT_NT_X, T * N), NYI = fft_threads(PM_FFT_NT_YI, T * N);
+    constexpr int NZI = fft_threads(PM_FFT_NT_ZI, T * M);
     const unsigned zt = (unsigned)((size_t)N * N / T);            // z passes: (x, y-tile)
     const unsigned yt = (unsigned)((size_t)(M + 1) * N / T);      // y / x passes
     switch (pass) {
         case kPassZFwd:
             if (sub1)
-                hipLaunchKernelGGL((k_zfwd<Rl, N, true>), dim3(zt), blk, 0, st, (const Rl*)in, (C*)out,
+                hipLaunchKernelGGL((k_zfwd<Rl, N, true, NZF>), dim3(zt), dim3(NZF), 0, st, (const Rl*)in, (C*)out,
                                    (const C*)twM, (const C*)twN);
             else
-                hipLaunchKernelGGL((k_zfwd<Rl, N, false>), dim3(zt), blk, 0, st, (const Rl*)in, (C*)out,
+                hipLaunchKernelGGL((k_zfwd<Rl, N, false, NZF>), dim3(zt), dim3(NZF), 0, st, (const Rl*)in, (C*)out,
                                    (const C*)twM, (const C*)twN);
             break;
         case kPassYFwd:
-            hipLaunchKernelGGL((k_yfwd<Rl, N>), dim3(yt), blk, 0, st, (const C*)in, (C*)out, (const C*)twN);
+            hipLaunchKernelGGL((k_yfwd<Rl, N, NYF>), dim3(yt), dim3(NYF), 0, st, (const C*)in, (C*)out,
+                               (const C*)twN);
             break;
         case kPassXFused:
-            hipLaunchKernelGGL((k_x<Rl, N, kXForward | kXGreen | kXInverse>), dim3(yt), blk, 0, st, (C*)out,
-                               (const C*)twN, ga);
+            hipLaunchKernelGGL((k_x<Rl, N, kXForward | kXGreen | kXInverse, NX>), dim3(yt), dim3(NX), 0, st,
+                               (C*)out, (const C*)twN, ga);
             break;
         case kPassXFwd:
-            hipLaunchKernelGGL((k_x<Rl, N, kXForward>), dim3(yt), blk, 0, st, (C*)out, (const C*)twN, ga);
+            hipLaunchKernelGGL((k_x<Rl, N, kXForward, NX>), dim3(yt), dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
             break;
         case kPassXInv:
-            hipLaunchKernelGGL((k_x<Rl, N, kXInverse>), dim3(yt), blk, 0, st, (C*)out, (const C*)twN, ga);
+            hipLaunchKernelGGL((k_x<Rl, N, kXInverse, NX>), dim3(yt), dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
             break;
         case kPassYInv:
-            hipLaunchKernelGGL((k_yinv<Rl, N>), dim3(yt), blk, 0, st, (const C*)in, (C*)out, (const C*)twN);
+            hipLaunchKernelGGL((k_yinv<Rl, N, NYI>), dim3(yt), dim3(NYI), 0, st, (const C*)in, (C*)out,
+                               (const C*)twN);
             break;
         case kPassZInv:
-            hipLaunchKernelGGL((k_zinv<Rl, N>), dim3(zt), blk, 0, st, (const C*)in, (Rl*)out, (const C*)twM,
-                               (const C*)twN, (Rl)norm);
+            hipLaunchKernelGGL((k_zinv<Rl, N, NZI>), dim3(zt), dim3(NZI), 0, st, (const C*)in, (Rl*)out,
+                               (const C*)twM, (const C*)twN, (Rl)norm);
             break;
         case kPassGreen: {
             const size_t total = (size_t)(M + 1) * N * N;

@@ -75,44 +75,85 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
     return x;
 }
 
+// Loads of 4 / 2 consecutive mesh values along z from an element-aligned
+// address (one global_load_dwordx4 / dwordx2 for FP32).
+typedef float pm_f4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef float pm_f2 __attribute__((ext_vector_type(2), aligned(4)));
+typedef double pm_d2 __attribute__((ext_vector_type(2), aligned(8)));
+
+__device__ __forceinline__ void load_run4(const float* p, float (&v)[4]) {
+    const pm_f4 q = *reinterpret_cast<const pm_f4*>(p);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+__device__ __forceinline__ void load_run2(const float* p, float (&v)[2]) {
+    const pm_f2 q = *reinterpret_cast<const pm_f2*>(p);
+    v[0] = q.x; v[1] = q.y;
+}
+__device__ __forceinline__ void load_run4(const double* p, double (&v)[4]) {
+    const pm_d2 q0 = *reinterpret_cast<const pm_d2*>(p), q1 = *reinterpret_cast<const pm_d2*>(p + 2);
+    v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
+}
+__device__ __forceinline__ void load_run2(const double* p, double (&v)[2]) {
+    const pm_d2 q = *reinterpret_cast<const pm_d2*>(p);
+    v[0] = q.x; v[1] = q.y;
+}
+
+// Mesh element address: 32-bit byte offsets from the uniform base for FP32
+// (ng <= 1024 keeps them below 4 GiB), 64-bit for FP64.
+template <typename Rl>
+__device__ __forceinline__ const Rl* mesh_at(const Rl* phi, uint32_t e) {
+    if constexpr (sizeof(Rl) == 4)
+        return reinterpret_cast<const Rl*>(reinterpret_cast<const char*>(phi) + (e << 2));
+    else
+        return phi + e;
+}
+
 // Centred-difference force (centered_derivatives, utils.cpp:130-132) of the
 // periodic potential, trilinearly interpolated (trilerp1, utils_inline.h:55-70)
-// at a particle in cell (ix, iy, iz) with offsets (tx, ty, tz).
+// at a particle in cell (ix, iy, iz) with offsets (tx, ty, tz).  The 32
+// potential values sit on 12 z-runs: 4 runs of 4 (z-1..z+2) through the
+// central x/y cells and 8 runs of 2 (z, z+1) for the x and y arms; away from
+// the periodic z seam each run is a single vector load.
 template <typename Rl>
 __device__ __forceinline__ void cic_force(const Rl* __restrict__ phi, int N, double x, double y, double z,
                                           double (&F)[3]) {
     const int ix = (int)x, iy = (int)y, iz = (int)z;
     const Rl tx = (Rl)(x - ix), ty = (Rl)(y - iy), tz = (Rl)(z - iz);
     const Rl mx = Rl(1) - tx, my = Rl(1) - ty, mz = Rl(1) - tz;
-    // periodic neighbours -1, 0, +1, +2 on each axis
-    const int xa[4] = {ix == 0 ? N - 1 : ix - 1, ix, ix + 1 == N ? 0 : ix + 1,
-                       ix + 2 >= N ? ix + 2 - N : ix + 2};
-    const int ya[4] = {iy == 0 ? N - 1 : iy - 1, iy, iy + 1 == N ? 0 : iy + 1,
-                       iy + 2 >= N ? iy + 2 - N : iy + 2};
-    const int za[4] = {iz == 0 ? N - 1 : iz - 1, iz, iz + 1 == N ? 0 : iz + 1,
-                       iz + 2 >= N ? iz + 2 - N : iz + 2};
-    auto P = [&](int a, int b, int c) -> Rl {  // a, b, c in [0, 4): offsets -1..+2
-        return phi[((size_t)xa[a] * N + ya[b]) * N + za[c]];
-    };
-    // central 2x2x2 block and the six arms
-    Rl c[2][2][2], ax[2][2][2], ay[2][2][2], az[2][2][2];
+    const uint32_t n = (uint32_t)N;
+    // periodic neighbours -1, 0, +1, +2 on x and y, as element offsets
+    const uint32_t X[4] = {(ix == 0 ? n - 1 : ix - 1) * n * n, (uint32_t)ix * n * n,
+                           (ix + 1 == N ? 0u : ix + 1) * n * n, (ix + 2 >= N ? ix + 2 - N : ix + 2) * n * n};
+    const uint32_t Y[4] = {(iy == 0 ? n - 1 : iy - 1) * n, (uint32_t)iy * n, (iy + 1 == N ? 0u : iy + 1) * n,
+                           (iy + 2 >= N ? iy + 2 - N : iy + 2) * n};
+    Rl cl[2][2][4];  // cl[a][b][c]: x offset a, y offset b (0, +1), z offset c-1
+    Rl xr[2][2][2];  // xr[s][b][c]: x offset -1 (s=0) or +2 (s=1), y offset b, z offset c
+    Rl yr[2][2][2];  // yr[s][a][c]: y offset -1 (s=0) or +2 (s=1), x offset a, z offset c
+    if (iz >= 1 && iz + 2 < N) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int cc = 0; cc < 2; ++cc) c[a][b][cc] = P(a + 1, b + 1, cc + 1);
-    // ax[s][b][c]: s=0 -> x offset -1, s=1 -> x offset +2
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int cc = 0; cc < 2; ++cc) {
-                ax[s][b][cc] = P(s * 3, b + 1, cc + 1);
-                ay[s][b][cc] = P(b + 1, s * 3, cc + 1);
-                az[s][b][cc] = P(b + 1, cc + 1, s * 3);
+            for (int b = 0; b < 2; ++b) {
+                load_run4(mesh_at(phi, X[a + 1] + Y[b + 1] + iz - 1), cl[a][b]);
+                load_run2(mesh_at(phi, X[a * 3] + Y[b + 1] + iz), xr[a][b]);
+                load_run2(mesh_at(phi, X[b + 1] + Y[a * 3] + iz), yr[a][b]);
             }
+    } else {
+        const uint32_t Z[4] = {(uint32_t)(iz == 0 ? N - 1 : iz - 1), (uint32_t)iz, (uint32_t)(iz + 1 == N ? 0 : iz + 1),
+                               (uint32_t)(iz + 2 >= N ? iz + 2 - N : iz + 2)};
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) cl[a][b][c] = *mesh_at(phi, X[a + 1] + Y[b + 1] + Z[c]);
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    xr[a][b][c] = *mesh_at(phi, X[a * 3] + Y[b + 1] + Z[c + 1]);
+                    yr[a][b][c] = *mesh_at(phi, X[b + 1] + Y[a * 3] + Z[c + 1]);
+                }
+            }
+    }
     // gradient at the 8 corners: g = (phi[i-1] - phi[i+1]) / 2
     Rl gx[2][2][2], gy[2][2][2], gz[2][2][2];
     const Rl h = Rl(0.5);
@@ -122,15 +163,13 @@ __device__ __forceinline__ void cic_force(const Rl* __restrict__ phi, int N, dou
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
-                const Rl xm = a == 0 ? ax[0][b][cc] : c[0][b][cc];
-                const Rl xp = a == 0 ? c[1][b][cc] : ax[1][b][cc];
+                const Rl xm = a == 0 ? xr[0][b][cc] : cl[0][b][cc + 1];
+                const Rl xp = a == 0 ? cl[1][b][cc + 1] : xr[1][b][cc];
                 gx[a][b][cc] = (xm - xp) * h;
-                const Rl ym = b == 0 ? ay[0][a][cc] : c[a][0][cc];
-                const Rl yp = b == 0 ? c[a][1][cc] : ay[1][a][cc];
+                const Rl ym = b == 0 ? yr[0][a][cc] : cl[a][0][cc + 1];
+                const Rl yp = b == 0 ? cl[a][1][cc + 1] : yr[1][a][cc];
                 gy[a][b][cc] = (ym - yp) * h;
-                const Rl zm = cc == 0 ? az[0][a][b] : c[a][b][0];
-                const Rl zp = cc == 0 ? c[a][b][1] : az[1][a][b];
-                gz[a][b][cc] = (zm - zp) * h;
+                gz[a][b][cc] = (cl[a][b][cc] - cl[a][b][cc + 2]) * h;
             }
     auto tri = [&](const Rl (&f)[2][2][2]) -> Rl {  // trilerp1 order
         const Rl f00 = f[0][0][0] * mx + f[1][0][0] * tx;
@@ -173,6 +212,7 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
     const double box = (double)ng;
     bool ok = i < np;
     double x = 0.0, y = 0.0, z = 0.0;
+    uint32_t kk = 0xFFFFFFFFu;  // MODE 2: this particle's next tile key
     if (ok) {
         x = px[i];
         y = py[i];
@@ -214,24 +254,35 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
             }
             if (MODE == 2 && ok) {
                 const int nt = ng / ts;
-                const uint32_t k = (uint32_t)((((int)x / ts) * nt + (int)y / ts) * nt + (int)z / ts);
-                key[i] = k;
-                uint32_t s = (k * 2654435761u) >> 25;  // 7 bits
-                bool done = false;
-                for (int probe = 0; probe < HS; ++probe) {
-                    const uint32_t old = atomicCAS(&hkey[s], 0xFFFFFFFFu, k);
-                    if (old == 0xFFFFFFFFu || old == k) {
-                        atomicAdd(&hcnt[s], 1u);
-                        done = true;
-                        break;
-                    }
-                    s = (s + 1) & (HS - 1);
-                }
-                if (!done) atomicAdd(&cnt[k], 1u);
+                kk = (uint32_t)((((int)x / ts) * nt + (int)y / ts) * nt + (int)z / ts);
+                key[i] = kk;
             }
         }
     }
     if constexpr (MODE == 2) {
+        // Runs of equal keys in consecutive lanes (particles in a coherent
+        // order share tiles) are counted once: the first lane of each run
+        // adds the run length.
+        const uint32_t lane = __lane_id();
+        const uint32_t prev = __shfl_up(kk, 1);
+        const bool head = kk != 0xFFFFFFFFu && (lane == 0 || prev != kk);
+        const unsigned long long bounds = __ballot(head || kk == 0xFFFFFFFFu);
+        if (head) {
+            const unsigned long long above = lane == 63 ? 0ull : (bounds >> (lane + 1)) << (lane + 1);
+            const uint32_t run = (above ? (uint32_t)__ffsll((long long)above) - 1u : 64u) - lane;
+            uint32_t s = (kk * 2654435761u) >> 25;  // 7 bits
+            bool done = false;
+            for (int probe = 0; probe < HS; ++probe) {
+                const uint32_t old = atomicCAS(&hkey[s], 0xFFFFFFFFu, kk);
+                if (old == 0xFFFFFFFFu || old == kk) {
+                    atomicAdd(&hcnt[s], run);
+                    done = true;
+                    break;
+                }
+                s = (s + 1) & (HS - 1);
+            }
+            if (!done) atomicAdd(&cnt[kk], run);
+        }
         __syncthreads();
         for (int s = threadIdx.x; s < HS; s += kPartThreads)
             if (hkey[s] != 0xFFFFFFFFu) atomicAdd(&cnt[hkey[s]], hcnt[s]);

@@ -11,7 +11,9 @@ import os
 from pathlib import Path
 
 PKG_DIR = Path(__file__).resolve().parent.parent
-LIB_PATH = PKG_DIR / "lib" / "libpmhip.so"
+# PMHIP_LIB selects another build of the same library (e.g. one compiled
+# with different tile or workgroup constants).
+LIB_PATH = Path(os.environ.get("PMHIP_LIB", PKG_DIR / "lib" / "libpmhip.so"))
 
 PM_OK = 0
 PM_ERR_ARG = -1

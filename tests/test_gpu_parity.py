@@ -347,10 +347,11 @@ def test_kdk_fp64_variants_match_oracle(dev, variant):
 
 
 def test_tile_interp_bitwise_equals_particle_interp(dev):
-    """pm_step with the sorted deposit interpolates per tile from LDS; the
-    standalone pm_interp_kick_drift walks particles in memory order from
-    HBM.  Same arithmetic in the same order; only the compiler's FMA
-    contraction may differ, so the bound is a few ulps of the mesh type."""
+    """pm_step with the sorted deposit interpolates per tile from LDS (FP64)
+    or in memory order while writing the next binning (FP32); the standalone
+    pm_interp_kick_drift walks particles in memory order from HBM.  Same
+    arithmetic in the same order; only the compiler's FMA contraction may
+    differ, so the bound is a few ulps of the mesh type."""
     import torch
 
     ng, nrow = 32, 32
@@ -375,16 +376,134 @@ def test_tile_interp_bitwise_equals_particle_interp(dev):
             assert float(d.abs().max()) <= eps * 0.02 * max(float(v.abs().max()) for v in b_vel) + 1e-12
 
 
-def test_reused_binning_bitwise_equals_fresh_binning(dev):
-    """pm_run_kdk bins every step after the first from the keys its tile
-    interpolation wrote; stepping with a fresh binning each time must give
-    bit-identical trajectories (the fixed-point deposit is order-free)."""
+@pytest.mark.parametrize("prec", [pmhip.PM_FP32, pmhip.PM_FP64])
+@pytest.mark.parametrize("ng,nrow,shuffle", [(32, 32, False), (128, 64, True)])
+def test_reused_binning_bitwise_equals_fresh_binning(dev, prec, ng, nrow, shuffle):
+    """pm_run_kdk bins every step after the first from the keys and tile
+    counts its interpolation wrote; stepping with a fresh binning each time
+    must give bit-identical trajectories (the fixed-point deposit is
+    order-free).  Shuffled particles on 512 tiles give short key runs and
+    more distinct tiles per workgroup than its count hash holds."""
     import torch
 
-    ng, nrow, nsteps, dt = 32, 32, 6, 0.02
+    nsteps, dt = 6, 0.02
     npart = nrow ** 3
     noise = np.random.default_rng(22).standard_normal(npart)
     pos0, vel0 = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 1.5, 3.0)
+    if shuffle:
+        perm = torch.tensor(np.random.default_rng(23).permutation(npart), device=dev)
+        pos0 = [t[perm].contiguous() for t in pos0]
+        vel0 = [t[perm].contiguous() for t in vel0]
+    a_pos, a_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
+    b_pos, b_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
+    with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.run_kdk(a_pos, a_vel, nsteps, dt, pmhip.PM_DEPOSIT_SORTED)
+        for s in range(nsteps + 1):
+            kick = 0.5 * dt if s in (0, nsteps) else dt
+            plan.step(b_pos, b_vel, kick, 0.0 if s == nsteps else dt, pmhip.PM_DEPOSIT_SORTED)
+        torch.cuda.synchronize()
+    assert pmhip.range_errors() == 0
+    for a, b in zip(a_pos + a_vel, b_pos + b_vel):
+        assert torch.equal(a, b)
+
+
+# ------------------------------------------------ benchmark-size meshes
+def _plane_wave(ng, m, dev):
+    """theta . i on the mesh (FP64, [x][y][z]) for integer mode m."""
+    import torch
+
+    th = [2.0 * np.pi * mi / ng for mi in m]
+    i = torch.arange(ng, dtype=torch.float64, device=dev)
+    return (th[0] * i).view(-1, 1, 1) + (th[1] * i).view(1, -1, 1) + (th[2] * i).view(1, 1, -1), th
+
+
+@pytest.mark.parametrize("ng,prec", [(512, pmhip.PM_FP32), (1024, pmhip.PM_FP32), (512, pmhip.PM_FP64),
+                                     (1024, pmhip.PM_FP64)])
+def test_large_mesh_plane_wave_solve_and_force(dev, ng, prec):
+    """Size-independent check at the benchmark meshes.  For delta =
+    A cos(theta . i) (theta = 2 pi m / N) the CONT_K2 potential is exactly
+    A G(m) cos(theta . i), G = -N^2 / (pi |m|^2); the centred difference of
+    B cos(theta . i) along e is B sin(theta_e) sin(theta . i), so the force
+    at a particle is the trilinear blend of that at its 8 corners.  Particles
+    near the top corner of the box exercise the largest mesh offsets."""
+    import torch
+
+    print(f"case plane wave ng={ng} prec={prec}", flush=True)
+    m, amp = (3, 5, 7), 0.5
+    npart = 1 << 20
+    with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        ph, th = _plane_wave(ng, m, dev)
+        plan.density().copy_(1.0 + amp * torch.cos(ph))
+        plan.solve()
+        torch.cuda.synchronize()
+        b = amp * (-(ng * ng) / (np.pi * sum(mi * mi for mi in m)))
+        want = torch.cos(ph)
+        want *= b
+        err = float((plan.potential().double() - want).abs().max()) / abs(b)
+        del want, ph
+        tol_phi, tol_f = (1e-5, 1e-4) if prec == pmhip.PM_FP32 else (1e-11, 1e-10)
+        assert err < tol_phi, err
+        g = torch.Generator(device=dev)
+        g.manual_seed(ng)
+        pos = [torch.rand(npart, dtype=torch.float64, device=dev, generator=g) * ng for _ in range(3)]
+        corner = npart // 8  # the last eighth sits in the top cell of every axis
+        for p in pos:
+            p[-corner:] = (ng - 1) + torch.rand(corner, dtype=torch.float64, device=dev, generator=g) * 0.999
+        f = [torch.empty(npart, dtype=torch.float64, device=dev) for _ in range(3)]
+        plan.interp_force(pos, f)
+        torch.cuda.synchronize()
+    i0 = [torch.floor(p) for p in pos]
+    t = [p - i for p, i in zip(pos, i0)]
+    want = [torch.zeros(npart, dtype=torch.float64, device=dev) for _ in range(3)]
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                w = ((t[0] if dx else 1 - t[0]) * (t[1] if dy else 1 - t[1]) * (t[2] if dz else 1 - t[2]))
+                s = torch.sin(th[0] * (i0[0] + dx) + th[1] * (i0[1] + dy) + th[2] * (i0[2] + dz))
+                for e in range(3):
+                    want[e] += w * (b * np.sin(th[e])) * s
+    fmax = max(float(v.abs().max()) for v in want)
+    for a, w in zip(f, want):
+        assert float((a - w).abs().max()) < tol_f * fmax
+
+
+@pytest.mark.parametrize("ng", [512, 1024])
+def test_large_mesh_sorted_deposit_vs_atomic_fp64(dev, ng):
+    """Sorted FP32 deposit (262144 tiles at 1024^3) against the FP64 atomic
+    deposit: same CIC weights, so cells and the total mass differ by FP32
+    rounding of the mesh only."""
+    import torch
+
+    print(f"case large deposit ng={ng}", flush=True)
+    npart = 1 << 21
+    g = torch.Generator(device=dev)
+    g.manual_seed(ng + 5)
+    pos = [torch.rand(npart, dtype=torch.float64, device=dev, generator=g) * ng for _ in range(3)]
+    for p in pos:  # a dense clump at the top corner and particles on the seams
+        p[:4096] = ng - 0.5 + 0.49 * torch.rand(4096, dtype=torch.float64, device=dev, generator=g)
+        p[4096:8192] = 0.0
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as p64:
+        p64.deposit(pos, pmhip.PM_DEPOSIT_ATOMIC)
+        torch.cuda.synchronize()
+        ref = p64.density().clone()
+    with pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as p32:
+        p32.deposit(pos, pmhip.PM_DEPOSIT_SORTED)
+        torch.cuda.synchronize()
+        rho = p32.density().double()
+        assert pmhip.range_errors() == 0
+        assert float((rho - ref).abs().max()) < 2e-6 * float(ref.abs().max())
+        assert abs(float(rho.sum()) - float(ref.sum())) < 1e-3 * ng ** 3 * 1e-6 + 1e-2
+
+
+def test_large_mesh_reused_binning_1024(dev):
+    """Key reuse across steps at 1024^3 (262144 tiles): run_kdk against
+    fresh binning every step, bit-identical."""
+    import torch
+
+    ng, nrow, nsteps, dt = 1024, 128, 3, 0.05
+    npart = nrow ** 3
+    noise = np.random.default_rng(24).standard_normal(npart)
+    pos0, vel0 = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 2.0, 3.0)
     a_pos, a_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
     b_pos, b_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
     with pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
