@@ -33,7 +33,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ng", type=int, default=512)
     ap.add_argument("--nrow", type=int, default=512)
     ap.add_argument("--rms", type=float, default=1.5)

@@ -13,6 +13,7 @@
 //   (D_a^T U)[c] = (U[c + e_a] - U[c - e_a]) / 2.
 // The CIC weights are piecewise linear: grad_x W(x, corner) takes the +-1
 // slope along the axis and the linear weights along the other two.
+#include "pm_gather.h"
 #include "pm_internal.h"
 
 namespace pm {
@@ -230,8 +231,43 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
     }
 }
 
-// max |u_a| over all particles (bit patterns of non-negative doubles order
-// like the values, so an integer atomic max works)
+// Workgroup (256 threads) max of m[0..3) into one of kMaxSlots slots per
+// component (umax[3 + d * kMaxSlots + slot]): bit patterns of non-negative
+// doubles order like the values, so an integer atomic max works, and the
+// slots spread the atomics; k_max_slots reduces them into umax[0..3).
+__device__ __forceinline__ void wg_max_to_slots(double (&m)[3], unsigned long long* __restrict__ umax) {
+    __shared__ double wm[3][256 / 64];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        for (int o = 32; o > 0; o >>= 1) m[d] = fmax(m[d], __shfl_xor(m[d], o));
+        if (__lane_id() == 0) wm[d][threadIdx.x >> 6] = m[d];
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double v = wm[threadIdx.x][0];
+        for (int w = 1; w < 256 / 64; ++w) v = fmax(v, wm[threadIdx.x][w]);
+        atomicMax(&umax[3 + threadIdx.x * kMaxSlots + (blockIdx.x & (kMaxSlots - 1))],
+                  (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+// umax[d] = max over the slots of component d; the slots are cleared
+__global__ __launch_bounds__(kMaxSlots) void k_max_slots(unsigned long long* __restrict__ umax) {
+    __shared__ unsigned long long r[3][kMaxSlots];
+    for (int d = 0; d < 3; ++d) {
+        r[d][threadIdx.x] = umax[3 + d * kMaxSlots + threadIdx.x];
+        umax[3 + d * kMaxSlots + threadIdx.x] = 0ULL;
+    }
+    __syncthreads();
+    for (int h = kMaxSlots / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int d = 0; d < 3; ++d) r[d][threadIdx.x] = max(r[d][threadIdx.x], r[d][threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) umax[threadIdx.x] = r[threadIdx.x][0];
+}
+
+// max |u_a| over all particles into the slots
 __global__ __launch_bounds__(256) void k_absmax3(const double* __restrict__ ux, const double* __restrict__ uy,
                                                  const double* __restrict__ uz, int64_t np,
                                                  unsigned long long* __restrict__ umax) {
@@ -241,11 +277,7 @@ __global__ __launch_bounds__(256) void k_absmax3(const double* __restrict__ ux, 
         m[1] = fmax(m[1], fabs(uy[i]));
         m[2] = fmax(m[2], fabs(uz[i]));
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        for (int o = 32; o > 0; o >>= 1) m[a] = fmax(m[a], __shfl_xor(m[a], o));
-        if (__lane_id() == 0) atomicMax(&umax[a], (unsigned long long)__double_as_longlong(m[a]));
-    }
+    wg_max_to_slots(m, umax);
 }
 
 // ------------------------------------------------- mu = sum_a D_a^T U_a
@@ -269,55 +301,85 @@ __global__ __launch_bounds__(256) void k_div3(const Rl* __restrict__ U0, const R
 }
 
 // -------------------------------------------------- reverse-step gather
-// v -= kick * F(x); xbar += kick * (term A + term B); MODE 1 skips the
-// velocity update (pure VJP).
+// MODE 1: xbar += vjp_scale * (term A + term B) (pure VJP).
+// MODE 0: also v -= kick * F(x) (undoes the step's kick).
+// MODE 2: as MODE 0, then the undrift that opens the next reverse step,
+//         x -= next_drift * v and u = vbar + next_drift * xbar (in place on
+//         u), with max |u_a| into umax for that step's vec3 deposit.
 template <typename Rl, int MODE>
-__global__ __launch_bounds__(256) void k_adj_gather(const double* __restrict__ px, const double* __restrict__ py,
-                                                    const double* __restrict__ pz, double* __restrict__ vx,
+__global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, double* __restrict__ py,
+                                                    double* __restrict__ pz, double* __restrict__ vx,
                                                     double* __restrict__ vy, double* __restrict__ vz,
-                                                    const double* __restrict__ ux, const double* __restrict__ uy,
-                                                    const double* __restrict__ uz, double* __restrict__ bx,
+                                                    double* __restrict__ ux, double* __restrict__ uy,
+                                                    double* __restrict__ uz, double* __restrict__ bx,
                                                     double* __restrict__ by, double* __restrict__ bz, int64_t np,
                                                     int N, const Rl* __restrict__ phi, const Rl* __restrict__ lam,
-                                                    double mass, double kick, double vjp_scale,
+                                                    double mass, double kick, double vjp_scale, double next_drift,
+                                                    unsigned long long* __restrict__ umax,
                                                     unsigned long long* __restrict__ err) {
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (i >= np) return;
-    const double x = px[i], y = py[i], z = pz[i];
     const double box = (double)N;
-    if (!(in_box_a(x, box) && in_box_a(y, box) && in_box_a(z, box))) {
-        atomicAdd(err, 1ULL);
-        return;
+    bool ok = i < np;
+    double x = 0.0, y = 0.0, z = 0.0;
+    double m[3] = {0.0, 0.0, 0.0};  // MODE 2: |u| of the next step
+    if (ok) {
+        x = px[i];
+        y = py[i];
+        z = pz[i];
+        if (!(in_box_a(x, box) && in_box_a(y, box) && in_box_a(z, box))) {
+            atomicAdd(err, 1ULL);
+            ok = false;
+        }
     }
-    int xa[4], ya[4], za[4];
-    Rl tx, ty, tz;
-    corner_setup<Rl>(x, y, z, N, xa, ya, za, tx, ty, tz);
-    Rl gx[2][2][2], gy[2][2][2], gz[2][2][2], lf[2][2][2];
-    corner_gradients(phi, N, xa, ya, za, gx, gy, gz);
+    if (ok) {
+        const CellIdx<Rl> c = cell_of<Rl>(x, y, z, N);
+        Rl cl[2][2][4], xr[2][2][2], yr[2][2][2];
+        gather_runs(phi, N, c, cl, xr, yr);
+        Rl lf[2][2][2];
+        gather_corners(lam, N, c, lf);
+        Rl gx[2][2][2], gy[2][2][2], gz[2][2][2];
+        corner_differences(cl, xr, yr, gx, gy, gz);
+        Rl F[3], dG[3][3], dL[3], lv;
+        tri_val_grad(gx, c.tx, c.ty, c.tz, F[0], dG[0]);
+        tri_val_grad(gy, c.tx, c.ty, c.tz, F[1], dG[1]);
+        tri_val_grad(gz, c.tx, c.ty, c.tz, F[2], dG[2]);
+        tri_val_grad(lf, c.tx, c.ty, c.tz, lv, dL);
+        const double u[3] = {ux[i], uy[i], uz[i]};
+        double nb[3];
+        const double b0[3] = {bx[i], by[i], bz[i]};
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int d = 0; d < 3; ++d)
+            nb[d] = b0[d] + vjp_scale * (u[0] * (double)dG[0][d] + u[1] * (double)dG[1][d] +
+                                         u[2] * (double)dG[2][d] + mass * (double)dL[d]);
+        bx[i] = nb[0];
+        by[i] = nb[1];
+        bz[i] = nb[2];
+        if constexpr (MODE != 1) {
+            const double nv[3] = {vx[i] - kick * (double)F[0], vy[i] - kick * (double)F[1],
+                                  vz[i] - kick * (double)F[2]};
+            vx[i] = nv[0];
+            vy[i] = nv[1];
+            vz[i] = nv[2];
+            if constexpr (MODE == 2) {
+                auto wrap = [&](double v) {
+                    if (v < 0.0) v += box;
+                    if (v >= box) v -= box;
+                    return v;
+                };
+                px[i] = wrap(x - next_drift * nv[0]);
+                py[i] = wrap(y - next_drift * nv[1]);
+                pz[i] = wrap(z - next_drift * nv[2]);
+                const double nu[3] = {u[0] + next_drift * nb[0], u[1] + next_drift * nb[1],
+                                      u[2] + next_drift * nb[2]};
+                ux[i] = nu[0];
+                uy[i] = nu[1];
+                uz[i] = nu[2];
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int c = 0; c < 2; ++c) lf[a][b][c] = lam[((size_t)xa[a + 1] * N + ya[b + 1]) * N + za[c + 1]];
-    Rl F[3], dG[3][3], dL[3], lv;
-    tri_val_grad(gx, tx, ty, tz, F[0], dG[0]);
-    tri_val_grad(gy, tx, ty, tz, F[1], dG[1]);
-    tri_val_grad(gz, tx, ty, tz, F[2], dG[2]);
-    tri_val_grad(lf, tx, ty, tz, lv, dL);
-    const double u[3] = {ux[i], uy[i], uz[i]};
-    double gb[3];
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-        gb[b] = u[0] * (double)dG[0][b] + u[1] * (double)dG[1][b] + u[2] * (double)dG[2][b] + mass * (double)dL[b];
-    bx[i] += vjp_scale * gb[0];
-    by[i] += vjp_scale * gb[1];
-    bz[i] += vjp_scale * gb[2];
-    if constexpr (MODE == 0) {
-        vx[i] -= kick * (double)F[0];
-        vy[i] -= kick * (double)F[1];
-        vz[i] -= kick * (double)F[2];
+                for (int d = 0; d < 3; ++d) m[d] = fabs(nu[d]);
+            }
+        }
     }
+    if constexpr (MODE == 2) wg_max_to_slots(m, umax);
 }
 
 // x -= drift * v (wrapped); u = vbar + drift * xbar (in place on vbar)
@@ -360,16 +422,18 @@ int launch_adj_deposit(int prec, const double* px, const double* py, const doubl
 
 int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
                         const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s,
-                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st) {
+                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st, bool umax_ready) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
     const size_t sh = sorted_shell_elems(ng);
     const size_t rs = prec == PM_FP32 ? sizeof(float) : sizeof(double);
-    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, st, umax, (size_t)3, 0ULL);
-    if (np > 0) {
-        const unsigned nb = (unsigned)std::min<int64_t>((np + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_absmax3, dim3(nb), dim3(256), 0, st, ux, uy, uz, np, umax);
+    if (!umax_ready) {  // else the previous reverse-step gather left max |u_a|
+        if (np > 0) {
+            const unsigned nb = (unsigned)std::min<int64_t>((np + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_absmax3, dim3(nb), dim3(256), 0, st, ux, uy, uz, np, umax);
+        }
+        hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
     }
 #define PM_VEC3(RL, TSV)                                                                                            \
     hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles), dim3(kVec3Threads), 0, st, px, py, pz, ux, uy, \
@@ -405,27 +469,33 @@ int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st) {
     return check_launch("div3");
 }
 
-int launch_adj_gather(int prec, int mode, const double* px, const double* py, const double* pz, double* vx,
-                      double* vy, double* vz, const double* ux, const double* uy, const double* uz, double* bx,
-                      double* by, double* bz, int64_t np, int ng, const void* phi, const void* lam, double mass,
-                      double kick, double vjp_scale, unsigned long long* err, hipStream_t st) {
+int launch_adj_gather(int prec, int mode, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
+                      double* ux, double* uy, double* uz, double* bx, double* by, double* bz, int64_t np, int ng,
+                      const void* phi, const void* lam, double mass, double kick, double vjp_scale,
+                      unsigned long long* err, hipStream_t st, double next_drift, unsigned long long* umax) {
     if (np <= 0) return PM_OK;
+    if (mode == 2 && !umax) return fail(PM_ERR_ARG, "adjoint gather: no max buffer");
     const unsigned nb = (unsigned)((np + 255) / 256);
 #define PM_GATHER(RL, M)                                                                                          \
     hipLaunchKernelGGL((k_adj_gather<RL, M>), dim3(nb), dim3(256), 0, st, px, py, pz, vx, vy, vz, ux, uy, uz, bx, \
-                       by, bz, np, ng, (const RL*)phi, (const RL*)lam, mass, kick, vjp_scale, err)
+                       by, bz, np, ng, (const RL*)phi, (const RL*)lam, mass, kick, vjp_scale, next_drift, umax, err)
     if (prec == PM_FP32) {
         if (mode == 0)
             PM_GATHER(float, 0);
-        else
+        else if (mode == 1)
             PM_GATHER(float, 1);
+        else
+            PM_GATHER(float, 2);
     } else {
         if (mode == 0)
             PM_GATHER(double, 0);
-        else
+        else if (mode == 1)
             PM_GATHER(double, 1);
+        else
+            PM_GATHER(double, 2);
     }
 #undef PM_GATHER
+    if (mode == 2) hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
     return check_launch("adjoint_gather");
 }
 

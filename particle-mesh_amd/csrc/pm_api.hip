@@ -175,8 +175,9 @@ extern "C" void* pm_fft_spectrum(pm_fft_t* f) { return f ? f->specB : nullptr; }
 extern "C" int pm_plan_destroy(pm_plan_t* p) {
     if (!p) return PM_OK;
     tables_destroy(&p->tab);
-    void* bufs[11] = {p->rho,        p->phi,         p->specA,        p->specB,       p->sorted.cnt,  p->sorted.off,
-                      p->sorted.key, p->sorted.slot, p->sorted.idx, p->sorted.cur, p->sorted.shell};
+    void* bufs[15] = {p->rho,          p->phi,          p->specA,         p->specB,        p->sorted.cnt,
+                      p->sorted.off,   p->sorted.key,   p->sorted.slot,   p->sorted.idx,   p->sorted.cur,
+                      p->sorted.shell, p->sorted.fill,  p->sorted.coff,   p->sorted.spill, p->sorted.stage};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     void* abufs[6] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell, p->adjMax};
@@ -271,13 +272,28 @@ static int ensure_sorted(pm_plan* p) {
     if (!rc) rc = hip_ok(hipMalloc(&s.shell, rs * sorted_shell_elems(p->ng)), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.cur, 4 * ntiles), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.idx, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
+    // staging buckets: count + count/8 + 32 slots per tile (u32 slot numbers)
+    const size_t cap_total = (size_t)p->np + (size_t)p->np / 8 + 32 * ntiles;
+    if (!rc && cap_total < 0xFFFFFFF0ull) {
+        rc = hip_ok(hipMalloc((void**)&s.fill, 4 * ntiles), "sorted deposit: hipMalloc");
+        if (!rc) rc = hip_ok(hipMalloc((void**)&s.coff, 4 * (ntiles + 1)), "sorted deposit: hipMalloc");
+        if (!rc) rc = hip_ok(hipMalloc((void**)&s.spill, 4), "sorted deposit: hipMalloc");
+        if (!rc) rc = hip_ok(hipMalloc(&s.stage, 3 * (rs == 4 ? 4 : 8) * cap_total), "sorted deposit: hipMalloc");
+        if (!rc) s.cap_total = cap_total;
+    }
     if (rc) {
-        void* bufs[7] = {s.cnt, s.off, s.key, s.slot, s.shell, s.cur, s.idx};
+        void* bufs[11] = {s.cnt, s.off, s.key, s.slot, s.shell, s.cur, s.idx, s.fill, s.coff, s.spill, s.stage};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         s = SortedScratch();
     }
     return rc;
+}
+
+// Other users of the binning buffers or of x end the step-to-step reuse.
+static void invalidate_reuse(pm_plan* p) {
+    p->keys_valid = false;
+    p->staged_valid = false;
 }
 
 static int deposit_impl(pm_plan* p, const double* x, const double* y, const double* z, int variant) {
@@ -286,6 +302,7 @@ static int deposit_impl(pm_plan* p, const double* x, const double* y, const doub
     if (variant == PM_DEPOSIT_SORTED) {
         int rc = ensure_sorted(p);
         if (rc) return rc;
+        invalidate_reuse(p);
         return launch_deposit_sorted(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
                                      p->stream);
     }
@@ -333,6 +350,7 @@ extern "C" int pm_solve(pm_plan_t* p) {
 extern "C" int pm_interp_kick_drift(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy,
                                     double* vz, double kick, double drift) {
     if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_interp_kick_drift: null argument");
+    invalidate_reuse(p);
     return launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 0, range_counter(),
                          p->stream);
 }
@@ -352,35 +370,47 @@ extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx
     if (variant != PM_DEPOSIT_SORTED && variant != PM_DEPOSIT_ATOMIC) return fail(PM_ERR_ARG, "pm_step: bad variant");
     mark(p, 0);
     int rc;
+    const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
+    bool staged = false;  // deposit from / interpolation into the staging buckets
     if (variant == PM_DEPOSIT_SORTED) {
-        // binning of x: from the keys the previous step's interpolation left
-        // (caller's guarantee that x is unchanged since), else from scratch
         rc = ensure_sorted(p);
-        if (!rc) {
-            const bool keys = reuse && p->keys_valid && p->keys_x == x;
+        staged = p->sorted.stage != nullptr;
+        const bool same_x = reuse && p->keys_x == x;
+        if (rc) {
+        } else if (staged && same_x && p->staged_valid) {
+            // positions staged in tile order by the previous step's interpolation
+            rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
+                                       p->stream);
+        } else {
+            // binning of x: from the keys the previous step's tile
+            // interpolation left (caller's guarantee that x is unchanged
+            // since), else from scratch
+            const bool keys = !staged && same_x && p->keys_valid;
             rc = keys ? launch_bin_from_keys(p->np, p->ng, p->sorted, p->stream)
                       : launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
+            if (!rc) rc = launch_deposit_binned(p->prec, x, y, z, p->ng, mass, p->rho, p->sorted, p->stream);
+            if (!rc && staged) rc = launch_cap_scan(p->ng, p->sorted, true, p->stream);
         }
-        const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
-        if (!rc) rc = launch_deposit_binned(p->prec, x, y, z, p->ng, mass, p->rho, p->sorted, p->stream);
     } else {
         rc = deposit_impl(p, x, y, z, variant);
     }
-    p->keys_valid = false;
+    invalidate_reuse(p);
     if (rc) return rc;
     mark(p, 1);
     rc = solve_impl(p);
     if (rc) return rc;
-    if (variant == PM_DEPOSIT_SORTED) {
-        // FP32: particles in memory order (a coherent order after the first
-        // steps keeps the gathers in L2); FP64: tile order with phi in LDS,
-        // which halves the FP64 gather traffic.  Both leave the next binning.
-        if (p->prec == PM_FP32)
-            rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 2, range_counter(),
-                               p->stream, sorted_tile(p->ng), p->sorted.cnt, p->sorted.key);
-        else
-            rc = launch_interp_tiles(p->prec, x, y, z, vx, vy, vz, p->ng, p->phi, kick, drift, p->sorted, 1,
-                                     range_counter(), p->stream);
+    if (staged) {
+        // particles in memory order (a coherent order keeps the gathers in
+        // L2), staging the drifted positions for the next deposit
+        rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 3, range_counter(),
+                           p->stream, &p->sorted);
+        p->staged_valid = rc == PM_OK;
+        p->keys_x = x;
+    } else if (variant == PM_DEPOSIT_SORTED) {
+        // FP64: tile order with phi in LDS (halves the FP64 gather traffic),
+        // leaving the next binning's keys and counts
+        rc = launch_interp_tiles(p->prec, x, y, z, vx, vy, vz, p->ng, p->phi, kick, drift, p->sorted, 1,
+                                 range_counter(), p->stream);
         p->keys_valid = rc == PM_OK;
         p->keys_x = x;
     } else {
@@ -451,7 +481,9 @@ static int ensure_adjoint(pm_plan* p) {
     for (int a = 0; a < 3 && !rc; ++a) rc = hip_ok(hipMalloc(&p->adjU[a], rs * nc), "adjoint: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc(&p->adjShell, 3 * rs * sorted_shell_elems(p->ng)), "adjoint: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc(&p->adjLam, rs * nc), "adjoint: hipMalloc");
-    if (!rc) rc = hip_ok(hipMalloc((void**)&p->adjMax, 3 * sizeof(unsigned long long)), "adjoint: hipMalloc");
+    const size_t nmax = 3 + 3 * (size_t)kMaxSlots;
+    if (!rc) rc = hip_ok(hipMalloc((void**)&p->adjMax, nmax * sizeof(unsigned long long)), "adjoint: hipMalloc");
+    if (!rc) rc = launch_fill(PM_FP64, p->adjMax, nmax, 0.0, p->stream);  // 0.0 has all bits zero
     if (rc) {
         void* bufs[5] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjShell, p->adjLam};
         for (void* b : bufs)
@@ -468,6 +500,7 @@ static int ensure_adjoint(pm_plan* p) {
 static int forward_field(pm_plan* p, const double* x, const double* y, const double* z, int variant) {
     const size_t nc = (size_t)p->ng * p->ng * p->ng;
     const double mass = (double)nc / (double)p->np;
+    invalidate_reuse(p);
     int rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
     if (rc) return rc;
     if (variant == PM_DEPOSIT_SORTED) {
@@ -485,9 +518,9 @@ static int forward_field(pm_plan* p, const double* x, const double* y, const dou
 // lambda = S sum_a D_a^T U_a with U_a the interpolation transpose of u at x
 // (binning of x already in p->sorted); mu is staged in p->rho.
 static int adjoint_field(pm_plan* p, const double* x, const double* y, const double* z, const double* ux,
-                         const double* uy, const double* uz) {
+                         const double* uy, const double* uz, bool umax_ready = false) {
     int rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
-                                 p->adjMax, p->stream);
+                                 p->adjMax, p->stream, umax_ready);
     if (!rc) rc = launch_div3(p->prec, p->adjU, p->ng, p->rho, p->stream);
     if (!rc) rc = solve_into(p, p->rho, p->adjLam, false);
     return rc;
@@ -506,6 +539,7 @@ extern "C" int pm_interp_transpose(pm_plan_t* p, const double* x, const double* 
                                    const double* ux, const double* uy, const double* uz) {
     if (!p || !x || !y || !z || !ux || !uy || !uz) return fail(PM_ERR_ARG, "pm_interp_transpose: null argument");
     int rc = ensure_adjoint(p);
+    invalidate_reuse(p);
     if (!rc) rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
     if (!rc)
         rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
@@ -543,9 +577,27 @@ extern "C" int pm_adjoint_force(pm_plan_t* p, const double* x, const double* y, 
     rc = forward_field(p, x, y, z, variant);
     if (!rc) rc = adjoint_field(p, x, y, z, ux, uy, uz);
     if (!rc)
-        rc = launch_adj_gather(p->prec, 1, x, y, z, nullptr, nullptr, nullptr, ux, uy, uz, xbar, ybar, zbar, p->np,
-                               p->ng, p->phi, p->adjLam, mass, 0.0, 1.0, range_counter(), p->stream);
+        rc = launch_adj_gather(p->prec, 1, const_cast<double*>(x), const_cast<double*>(y), const_cast<double*>(z),
+                               nullptr, nullptr, nullptr, const_cast<double*>(ux), const_cast<double*>(uy),
+                               const_cast<double*>(uz), xbar, ybar, zbar, p->np, p->ng, p->phi, p->adjLam, mass, 0.0,
+                               1.0, range_counter(), p->stream);  // mode 1 writes only xbar
     p->timing_slots = slots;
+    return rc;
+}
+
+// One reverse step once its undrift is done (x and u = vbar + drift xbar in
+// place): forward and adjoint fields at x, then the gather that undoes the
+// kick and adds kick * J_F^T u to xbar; gather mode 2 also applies the next
+// reverse step's undrift (next_drift) and leaves max |u| for its deposit.
+static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                        double* xbar, double* ybar, double* zbar, double* ux, double* uy, double* uz, double kick,
+                        int variant, int gmode, double next_drift, bool umax_ready) {
+    const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
+    int rc = forward_field(p, x, y, z, variant);
+    if (!rc) rc = adjoint_field(p, x, y, z, ux, uy, uz, umax_ready);
+    if (!rc)
+        rc = launch_adj_gather(p->prec, gmode, x, y, z, vx, vy, vz, ux, uy, uz, xbar, ybar, zbar, p->np, p->ng, p->phi,
+                               p->adjLam, mass, kick, kick, range_counter(), p->stream, next_drift, p->adjMax);
     return rc;
 }
 
@@ -558,15 +610,10 @@ extern "C" int pm_adjoint_step(pm_plan_t* p, double* x, double* y, double* z, do
     if (rc) return rc;
     const int slots = p->timing_slots;
     p->timing_slots = 0;
-    const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
+    invalidate_reuse(p);
     // undo the drift; the velocity cotangent becomes u = vbar + drift * xbar
     rc = launch_undrift(x, y, z, vx, vy, vz, xbar, ybar, zbar, vxbar, vybar, vzbar, p->np, p->ng, drift, p->stream);
-    if (!rc) rc = forward_field(p, x, y, z, variant);
-    if (!rc) rc = adjoint_field(p, x, y, z, vxbar, vybar, vzbar);
-    // undo the kick and add kick * J_F^T u to xbar
-    if (!rc)
-        rc = launch_adj_gather(p->prec, 0, x, y, z, vx, vy, vz, vxbar, vybar, vzbar, xbar, ybar, zbar, p->np, p->ng,
-                               p->phi, p->adjLam, mass, kick, kick, range_counter(), p->stream);
+    if (!rc) rc = adjoint_core(p, x, y, z, vx, vy, vz, xbar, ybar, zbar, vxbar, vybar, vzbar, kick, variant, 0, 0.0, false);
     p->timing_slots = slots;
     return rc;
 }
@@ -574,13 +621,22 @@ extern "C" int pm_adjoint_step(pm_plan_t* p, double* x, double* y, double* z, do
 extern "C" int pm_run_adjoint(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
                               double* xbar, double* ybar, double* zbar, double* vxbar, double* vybar, double* vzbar,
                               int nsteps, double dt, int variant) {
-    if (!p || nsteps < 0) return fail(PM_ERR_ARG, "pm_run_adjoint: bad arguments");
-    for (int s = nsteps; s >= 0; --s) {
+    if (!p || nsteps < 0 || !x || !y || !z || !vx || !vy || !vz || !xbar || !ybar || !zbar || !vxbar || !vybar ||
+        !vzbar)
+        return fail(PM_ERR_ARG, "pm_run_adjoint: bad arguments");
+    int rc = ensure_adjoint(p);
+    if (rc) return rc;
+    const int slots = p->timing_slots;
+    p->timing_slots = 0;
+    invalidate_reuse(p);
+    // Reverse steps s = nsteps..0 of pm_run_kdk (kick dt/2 at both ends,
+    // no drift after the last).  The last step has no drift to undo; every
+    // other step's undrift is done by the gather of the step after it.
+    for (int s = nsteps; s >= 0 && !rc; --s) {
         const double kick = (s == 0 || s == nsteps) ? 0.5 * dt : dt;
-        const double drift = s == nsteps ? 0.0 : dt;
-        const int rc =
-            pm_adjoint_step(p, x, y, z, vx, vy, vz, xbar, ybar, zbar, vxbar, vybar, vzbar, kick, drift, variant);
-        if (rc) return rc;
+        rc = adjoint_core(p, x, y, z, vx, vy, vz, xbar, ybar, zbar, vxbar, vybar, vzbar, kick, variant, s > 0 ? 2 : 0,
+                          s > 0 ? dt : 0.0, s < nsteps);
     }
-    return PM_OK;
+    p->timing_slots = slots;
+    return rc;
 }

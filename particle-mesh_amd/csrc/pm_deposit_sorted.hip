@@ -20,6 +20,8 @@
 //                    fold of reorder_slab.cpp:43-67 at tile granularity.
 // Every mesh node is written by exactly one workgroup: no global float
 // atomics, and the result does not depend on the particle order.
+#include <type_traits>
+
 #include "pm_internal.h"
 
 namespace pm {
@@ -43,8 +45,10 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
                                                    const double* __restrict__ pz, int64_t np, int ng, int ts,
                                                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
                                                    uint32_t* __restrict__ slot,
-                                                   unsigned long long* __restrict__ err) {
+                                                   unsigned long long* __restrict__ err,
+                                                   const uint32_t* __restrict__ gate) {
     static_assert((kBinHash & (kBinHash - 1)) == 0, "hash size must be a power of two");
+    if (gate && *gate == 0u) return;
     __shared__ uint32_t hkey[kBinHash];
     __shared__ uint32_t hcnt[kBinHash];
     for (int h = threadIdx.x; h < kBinHash; h += 256) {
@@ -55,7 +59,7 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
     // each thread takes kBinPer consecutive particles: neighbouring particles
     // usually share a tile, so runs of equal keys cost one hash insert
     const int64_t base = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kBinChunk + (int64_t)threadIdx.x * kBinPer;
-    const int nt = ng / ts;
+    const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;  // ts: power of two
     const double box = (double)ng;
     uint32_t k[kBinPer], h[kBinPer], rank[kBinPer];
     double x[kBinPer], y[kBinPer], z[kBinPer];
@@ -88,7 +92,7 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
         h[j] = 0u;
         rank[j] = 0u;
         if (in_box_s(x[j], box) && in_box_s(y[j], box) && in_box_s(z[j], box)) {
-            k[j] = (uint32_t)((((int)x[j] / ts) * nt + (int)y[j] / ts) * nt + (int)z[j] / ts);
+            k[j] = (uint32_t)((((int)x[j] >> lt) * nt + ((int)y[j] >> lt)) * nt + ((int)z[j] >> lt));
         } else {
             k[j] = kInvalidKey;
             if (base + j < np) atomicAdd(err, 1ULL);
@@ -136,7 +140,9 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
 // optionally also into cursor[0..n) (the scatter's running slots) and with
 // the counts zeroed behind it for the next binning.
 __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ off,
-                                               uint32_t* __restrict__ cursor, int n, int zero_counts) {
+                                               uint32_t* __restrict__ cursor, int n, int zero_counts,
+                                               const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0u) return;
     __shared__ uint32_t part[1024];
     const int t = threadIdx.x;
     const int per = (n + 1023) / 1024;
@@ -222,7 +228,8 @@ __global__ __launch_bounds__(256) void k_scatter_keys(const uint32_t* __restrict
 // 3. particle indices in tile order.
 __global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
                                                      const uint32_t* __restrict__ off, int64_t np,
-                                                     uint32_t* __restrict__ idx) {
+                                                     uint32_t* __restrict__ idx, const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0u) return;
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (i >= np) return;
     const uint32_t k = key[i];
@@ -246,14 +253,81 @@ template <int TS> __device__ __forceinline__ int shell_index(int a, int b, int c
     return E * E + TS * E + a * TS + b;
 }
 
-// 4. one workgroup per tile.
+// In-tile offsets of the FP32 path: u32 fixed point with 2^-28 cell
+// resolution (cell = q >> 28, fraction = low 28 bits).  The staged deposit
+// reads them from the interpolation's staging arrays; the binned FP32
+// deposit forms the same values from the FP64 positions, so both give
+// bit-identical meshes.  2^-28 cells is far below the FP32 mesh rounding.
+constexpr double kQScale = 268435456.0;  // 2^28
+constexpr uint32_t kQMask = 0x0FFFFFFFu;
+
+__device__ __forceinline__ uint32_t quantize_offset(double local) { return (uint32_t)(local * kQScale); }
+
+// CIC weights (cic_deposit, utils_inline.h:25-49) of one particle in local
+// cell (a, b, c) with fractional offsets (fx, fy, fz), added in 2^-32 fixed
+// point to the (TS+1)^3 LDS node tile.
+template <int TS>
+__device__ __forceinline__ void tile_cic_add(unsigned long long* acc, int a, int b, int c, double fx, double fy,
+                                             double fz, double fix) {
+    constexpr int E = TS + 1;
+    const double sx0 = 1.0 - fx, sy0 = 1.0 - fy;
+    const double sxy00 = sx0 * sy0, sxy10 = fx * sy0, sxy01 = sx0 * fy, sxy11 = fx * fy;
+    const double m0 = fix * (1.0 - fz), m1 = fix * fz;
+    unsigned long long* n0 = acc + (a * E + b) * E + c;
+    atomicAdd(n0, (unsigned long long)__double2ll_rn(sxy00 * m0));
+    atomicAdd(n0 + E * E, (unsigned long long)__double2ll_rn(sxy10 * m0));
+    atomicAdd(n0 + E, (unsigned long long)__double2ll_rn(sxy01 * m0));
+    atomicAdd(n0 + E * E + E, (unsigned long long)__double2ll_rn(sxy11 * m0));
+    atomicAdd(n0 + 1, (unsigned long long)__double2ll_rn(sxy00 * m1));
+    atomicAdd(n0 + E * E + 1, (unsigned long long)__double2ll_rn(sxy10 * m1));
+    atomicAdd(n0 + E + 1, (unsigned long long)__double2ll_rn(sxy01 * m1));
+    atomicAdd(n0 + E * E + E + 1, (unsigned long long)__double2ll_rn(sxy11 * m1));
+}
+
+// The TS^3 owned nodes of tile t go to the mesh, the upper shell to the
+// tile's shell buffer (after a barrier on the accumulation).
+template <typename Rl, int TS>
+__device__ __forceinline__ void tile_write(const unsigned long long* acc, uint32_t t, int ox, int oy, int oz,
+                                           int ng, Rl* __restrict__ rho, Rl* __restrict__ shell) {
+    constexpr int E = TS + 1, SH = shell_size<TS>();
+    const double unfix = 1.0 / 4294967296.0;
+    const size_t N = (size_t)ng;
+    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
+        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
+        rho[((size_t)(ox + a) * N + (oy + b)) * N + (oz + c)] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
+    }
+    Rl* sh = shell + (size_t)t * SH;
+    for (int i = threadIdx.x; i < SH; i += 256) {
+        int a, b, c;
+        if (i < E * E) {
+            a = TS;
+            b = i / E;
+            c = i % E;
+        } else if (i < E * E + TS * E) {
+            const int j = i - E * E;
+            a = j / E;
+            b = TS;
+            c = j % E;
+        } else {
+            const int j = i - E * E - TS * E;
+            a = j / TS;
+            b = j % TS;
+            c = TS;
+        }
+        sh[i] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
+    }
+}
+
+// 4. one workgroup per tile, particles by index (binning of positions).
 template <typename Rl, int TS>
 __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict__ px, const double* __restrict__ py,
                                                        const double* __restrict__ pz,
                                                        const uint32_t* __restrict__ idx,
                                                        const uint32_t* __restrict__ off, int ng, double mass,
-                                                       Rl* __restrict__ rho, Rl* __restrict__ shell) {
-    constexpr int E = TS + 1, E3 = E * E * E, SH = shell_size<TS>();
+                                                       Rl* __restrict__ rho, Rl* __restrict__ shell,
+                                                       const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0u) return;
+    constexpr int E = TS + 1, E3 = E * E * E;
     __shared__ unsigned long long acc[E3];
     const int nt = ng / TS;
     const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
@@ -285,51 +359,109 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict_
 #pragma unroll
         for (int j = 0; j < kDepBatch; ++j) {
             if (p[j] == kInvalidKey) continue;
-            const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
-            const double fx = x[j] - ix, fy = y[j] - iy, fz = z[j] - iz;
-            // cic_deposit weights (utils_inline.h:25-49)
-            const double sx0 = 1.0 - fx, sy0 = 1.0 - fy;
-            const double sxy00 = sx0 * sy0, sxy10 = fx * sy0, sxy01 = sx0 * fy, sxy11 = fx * fy;
-            const double m0 = fix * (1.0 - fz), m1 = fix * fz;
-            const int a = ix - ox, b = iy - oy, c = iz - oz;
-            unsigned long long* n0 = acc + (a * E + b) * E + c;
-            atomicAdd(n0, (unsigned long long)__double2ll_rn(sxy00 * m0));
-            atomicAdd(n0 + E * E, (unsigned long long)__double2ll_rn(sxy10 * m0));
-            atomicAdd(n0 + E, (unsigned long long)__double2ll_rn(sxy01 * m0));
-            atomicAdd(n0 + E * E + E, (unsigned long long)__double2ll_rn(sxy11 * m0));
-            atomicAdd(n0 + 1, (unsigned long long)__double2ll_rn(sxy00 * m1));
-            atomicAdd(n0 + E * E + 1, (unsigned long long)__double2ll_rn(sxy10 * m1));
-            atomicAdd(n0 + E + 1, (unsigned long long)__double2ll_rn(sxy01 * m1));
-            atomicAdd(n0 + E * E + E + 1, (unsigned long long)__double2ll_rn(sxy11 * m1));
+            if constexpr (sizeof(Rl) == 4) {
+                const uint32_t qx = quantize_offset(x[j] - ox), qy = quantize_offset(y[j] - oy),
+                               qz = quantize_offset(z[j] - oz);
+                tile_cic_add<TS>(acc, (int)(qx >> 28), (int)(qy >> 28), (int)(qz >> 28), (qx & kQMask) / kQScale,
+                                 (qy & kQMask) / kQScale, (qz & kQMask) / kQScale, fix);
+            } else {
+                const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
+                tile_cic_add<TS>(acc, ix - ox, iy - oy, iz - oz, x[j] - ix, y[j] - iy, z[j] - iz, fix);
+            }
         }
     }
     __syncthreads();
-    const double unfix = 1.0 / 4294967296.0;
-    const size_t N = (size_t)ng;
-    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
-        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
-        rho[((size_t)(ox + a) * N + (oy + b)) * N + (oz + c)] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
-    }
-    Rl* sh = shell + (size_t)t * SH;
-    for (int i = threadIdx.x; i < SH; i += 256) {
-        int a, b, c;
-        if (i < E * E) {
-            a = TS;
-            b = i / E;
-            c = i % E;
-        } else if (i < E * E + TS * E) {
-            const int j = i - E * E;
-            a = j / E;
-            b = TS;
-            c = j % E;
-        } else {
-            const int j = i - E * E - TS * E;
-            a = j / TS;
-            b = j % TS;
-            c = TS;
+    tile_write<Rl, TS>(acc, t, ox, oy, oz, ng, rho, shell);
+}
+
+// 4'. one workgroup per tile, particles from the staging arrays that the
+// interpolation of the previous step filled in tile order: tile t holds
+// fill[t] entries from coff[t], in-tile offsets as u32 at 2^-28 cells (FP32
+// mesh) or exact doubles (FP64 mesh, giving the same weights as the binned
+// path bit for bit).  Skipped when a tile overflowed its capacity (the
+// binned path runs instead).
+template <typename Rl, int TS>
+__global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__ stage, size_t cap_total,
+                                                        const uint32_t* __restrict__ coff,
+                                                        const uint32_t* __restrict__ fill,
+                                                        const uint32_t* __restrict__ spill, int ng, double mass,
+                                                        Rl* __restrict__ rho, Rl* __restrict__ shell) {
+    if (*spill != 0u) return;
+    using St = typename std::conditional<sizeof(Rl) == 4, uint32_t, double>::type;
+    const St* __restrict__ sx = static_cast<const St*>(stage);
+    const St* __restrict__ sy = sx + cap_total;
+    const St* __restrict__ sz = sx + 2 * cap_total;
+    constexpr int E = TS + 1, E3 = E * E * E;
+    __shared__ unsigned long long acc[E3];
+    const int nt = ng / TS;
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
+              tz = (int)(t % (uint32_t)nt);
+    for (int i = threadIdx.x; i < E3; i += 256) acc[i] = 0ULL;
+    __syncthreads();
+    const uint32_t beg = coff[t], end = beg + fill[t];
+    const double fix = 4294967296.0 * mass;
+    for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += 256 * kDepBatch) {
+        St a[kDepBatch], b[kDepBatch], c[kDepBatch];
+#pragma unroll
+        for (int j = 0; j < kDepBatch; ++j) {
+            const uint32_t q = q0 + 256 * j;
+            if (q < end) {
+                a[j] = sx[q];
+                b[j] = sy[q];
+                c[j] = sz[q];
+            }
         }
-        sh[i] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
+#pragma unroll
+        for (int j = 0; j < kDepBatch; ++j) {
+            if (q0 + 256 * j >= end) continue;
+            if constexpr (sizeof(Rl) == 4) {
+                tile_cic_add<TS>(acc, (int)(a[j] >> 28), (int)(b[j] >> 28), (int)(c[j] >> 28),
+                                 (a[j] & kQMask) / kQScale, (b[j] & kQMask) / kQScale, (c[j] & kQMask) / kQScale, fix);
+            } else {
+                const int ia = (int)a[j], ib = (int)b[j], ic = (int)c[j];
+                tile_cic_add<TS>(acc, ia, ib, ic, a[j] - ia, b[j] - ib, c[j] - ic, fix);
+            }
+        }
     }
+    __syncthreads();
+    tile_write<Rl, TS>(acc, t, tx * TS, ty * TS, tz * TS, ng, rho, shell);
+}
+
+// Capacities of the staging buckets for the next step: count + count/8 + 32
+// per tile (counts from the binning offsets on a first step, else from the
+// fill counters of the staged step), exclusive scan into coff[0..n]; the
+// fill counters and the overflow flag are cleared for the next staging.
+__global__ __launch_bounds__(1024) void k_cap_scan(uint32_t* __restrict__ fill, const uint32_t* __restrict__ off,
+                                                   uint32_t* __restrict__ coff, uint32_t* __restrict__ spill, int n,
+                                                   int from_off) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(n, b + per);
+    auto cap = [&](int i) -> uint32_t {
+        const uint32_t c = from_off ? off[i + 1] - off[i] : fill[i];
+        return c + (c >> 3) + 32u;
+    };
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += cap(i);
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t == 0 ? 0u : part[t - 1];
+    for (int i = b; i < e; ++i) {
+        const uint32_t c = cap(i);
+        coff[i] = run;
+        run += c;
+        fill[i] = 0u;
+    }
+    if (t == 1023) coff[n] = part[1023];
+    if (t == 0) *spill = 0u;
 }
 
 // 5. add the lower neighbours' shells to each tile's lower-face nodes.
@@ -386,7 +518,7 @@ size_t sorted_shell_elems(int ng) {
 }
 
 int launch_bin(const double* px, const double* py, const double* pz, int64_t np, int ng, SortedScratch& s,
-               unsigned long long* err, hipStream_t st) {
+               unsigned long long* err, hipStream_t st, const uint32_t* gate) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
@@ -396,12 +528,12 @@ int launch_bin(const double* px, const double* py, const double* pz, int64_t np,
     if (np > 0) {
         const unsigned nchunk = (unsigned)((np + kBinChunk - 1) / kBinChunk);
         hipLaunchKernelGGL(k_bin_count, dim3(nchunk), dim3(256), 0, st, px, py, pz, np, ng, ts, s.cnt, s.key, s.slot,
-                           err);
+                           err, gate);
     }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, (uint32_t*)nullptr, (int)ntiles, 1);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, (uint32_t*)nullptr, (int)ntiles, 1, gate);
     if (np > 0) {
         const unsigned nb = (unsigned)((np + 255) / 256);
-        hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, s.key, s.slot, s.off, np, s.idx);
+        hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, s.key, s.slot, s.off, np, s.idx, gate);
     }
     return check_launch("bin");
 }
@@ -411,7 +543,8 @@ int launch_bin_from_keys(int64_t np, int ng, SortedScratch& s, hipStream_t st) {
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
     // the tile interpolation left the keys in s.key and the counts in s.cnt
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, s.cur, (int)ntiles, 1);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, s.cur, (int)ntiles, 1,
+                       (const uint32_t*)nullptr);
     if (np > 0) {
         const unsigned nchunk = (unsigned)((np + kBinChunk - 1) / kBinChunk);
         hipLaunchKernelGGL(k_scatter_keys, dim3(nchunk), dim3(256), 0, st, s.key, np, s.cur, s.idx);
@@ -441,28 +574,66 @@ int launch_fold_shells(int prec, int ts, int ng, const void* shell, void* rho, h
 }
 
 int launch_deposit_binned(int prec, const double* px, const double* py, const double* pz, int ng, double mass,
-                          void* rho, SortedScratch& s, hipStream_t st) {
+                          void* rho, SortedScratch& s, hipStream_t st, const uint32_t* gate, bool fold) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
+#define PM_DT(RL, TSV)                                                                                                 \
+    hipLaunchKernelGGL((k_deposit_tiles<RL, TSV>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx, s.off, ng, mass, \
+                       (RL*)rho, (RL*)s.shell, gate)
     if (prec == PM_FP32) {
         if (ts == 16)
-            hipLaunchKernelGGL((k_deposit_tiles<float, 16>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx, s.off,
-                               ng, mass, (float*)rho, (float*)s.shell);
+            PM_DT(float, 16);
         else
-            hipLaunchKernelGGL((k_deposit_tiles<float, 8>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx, s.off,
-                               ng, mass, (float*)rho, (float*)s.shell);
+            PM_DT(float, 8);
     } else {
         if (ts == 16)
-            hipLaunchKernelGGL((k_deposit_tiles<double, 16>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx,
-                               s.off, ng, mass, (double*)rho, (double*)s.shell);
+            PM_DT(double, 16);
         else
-            hipLaunchKernelGGL((k_deposit_tiles<double, 8>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx,
-                               s.off, ng, mass, (double*)rho, (double*)s.shell);
+            PM_DT(double, 8);
     }
+#undef PM_DT
     int rc = check_launch("deposit_tiles");
-    if (rc) return rc;
+    if (rc || !fold) return rc;
     return launch_fold_shells(prec, ts, ng, s.shell, rho, st);
+}
+
+int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st) {
+    const int ts = sorted_tile(ng);
+    const int nt = ng / ts;
+    hipLaunchKernelGGL(k_cap_scan, dim3(1), dim3(1024), 0, st, s.fill, s.off, s.coff, s.spill, nt * nt * nt,
+                       from_off ? 1 : 0);
+    return check_launch("cap_scan");
+}
+
+int launch_deposit_staged(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
+                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st) {
+    const int ts = sorted_tile(ng);
+    const int nt = ng / ts;
+    const uint32_t ntiles = (uint32_t)nt * nt * nt;
+    // fallback, run only if some tile overflowed its staging capacity
+    int rc = launch_bin(px, py, pz, np, ng, s, err, st, s.spill);
+    if (!rc) rc = launch_deposit_binned(prec, px, py, pz, ng, mass, rho, s, st, s.spill, false);
+    if (rc) return rc;
+#define PM_DS(RL, TSV)                                                                                               \
+    hipLaunchKernelGGL((k_deposit_staged<RL, TSV>), dim3(ntiles), dim3(256), 0, st, s.stage, s.cap_total, s.coff,     \
+                       s.fill, s.spill, ng, mass, (RL*)rho, (RL*)s.shell)
+    if (prec == PM_FP32) {
+        if (ts == 16)
+            PM_DS(float, 16);
+        else
+            PM_DS(float, 8);
+    } else {
+        if (ts == 16)
+            PM_DS(double, 16);
+        else
+            PM_DS(double, 8);
+    }
+#undef PM_DS
+    rc = check_launch("deposit_staged");
+    if (!rc) rc = launch_fold_shells(prec, ts, ng, s.shell, rho, st);
+    if (!rc) rc = launch_cap_scan(ng, s, false, st);
+    return rc;
 }
 
 int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,

@@ -61,6 +61,15 @@ struct SortedScratch {
     uint32_t* idx = nullptr;   // np
     uint32_t* cur = nullptr;   // ntiles (scatter cursor of the key-only binning)
     void* shell = nullptr;     // ntiles * shell_size reals
+    // Staging in tile order, filled by the interpolation for the next
+    // step's deposit: bucket t holds fill[t] in-tile offsets from coff[t]
+    // (capacity coff[t+1] - coff[t]); spill != 0 if a bucket overflowed.
+    uint32_t* fill = nullptr;   // ntiles
+    uint32_t* coff = nullptr;   // ntiles + 1
+    uint32_t* spill = nullptr;  // 1
+    void* stage = nullptr;      // 3 * cap_total in-tile offsets (x, y, z): u32 at
+                                // 2^-28 cells (FP32 mesh) or exact doubles (FP64)
+    size_t cap_total = 0;
 };
 // binning from the keys and counts left by the tile interpolation
 int launch_bin_from_keys(int64_t np, int ng, SortedScratch& s, hipStream_t st);
@@ -75,24 +84,40 @@ size_t sorted_shell_elems(int ng);
 int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
                           double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st);
 // binning only (count, scan, scatter) and the per-tile deposit of a binning
+// (a non-null gate makes every kernel a no-op unless *gate != 0)
 int launch_bin(const double* px, const double* py, const double* pz, int64_t np, int ng, SortedScratch& s,
-               unsigned long long* err, hipStream_t st);
+               unsigned long long* err, hipStream_t st, const uint32_t* gate = nullptr);
 int launch_deposit_binned(int prec, const double* px, const double* py, const double* pz, int ng, double mass,
-                          void* rho, SortedScratch& s, hipStream_t st);
+                          void* rho, SortedScratch& s, hipStream_t st, const uint32_t* gate = nullptr,
+                          bool fold = true);
+// staging capacities for the next step (from the binning offsets, or from
+// the fill counts of a staged step)
+int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st);
+// deposit from the staging arrays (binned fallback if a bucket overflowed),
+// then the next capacities
+int launch_deposit_staged(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
+                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st);
 int launch_fold_shells(int prec, int ts, int ng, const void* shell, void* rho, hipStream_t st);
 
 // Adjoint kernels (pm_adjoint.hip).
 int launch_adj_deposit(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
                        const void* mbar, double scale, double* bx, double* by, double* bz, unsigned long long* err,
                        hipStream_t st);
+// (umax_ready: umax already holds max |u_a|, left by a mode-2 gather)
 int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
                         const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s,
-                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st);
+                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st,
+                        bool umax_ready = false);
 int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st);
-int launch_adj_gather(int prec, int mode, const double* px, const double* py, const double* pz, double* vx,
-                      double* vy, double* vz, const double* ux, const double* uy, const double* uz, double* bx,
-                      double* by, double* bz, int64_t np, int ng, const void* phi, const void* lam, double mass,
-                      double kick, double vjp_scale, unsigned long long* err, hipStream_t st);
+// mode 1: xbar += VJP; 0: also undo the kick; 2: also the next reverse
+// step's undrift (next_drift) with max |u| into umax[0..3) (umax holds
+// 3 + 3 * kMaxSlots entries, the slots zero between calls)
+constexpr int kMaxSlots = 256;
+int launch_adj_gather(int prec, int mode, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
+                      double* ux, double* uy, double* uz, double* bx, double* by, double* bz, int64_t np, int ng,
+                      const void* phi, const void* lam, double mass, double kick, double vjp_scale,
+                      unsigned long long* err, hipStream_t st, double next_drift = 0.0,
+                      unsigned long long* umax = nullptr);
 int launch_undrift(double* px, double* py, double* pz, const double* vx, const double* vy, const double* vz,
                    const double* bx, const double* by, const double* bz, double* ubx, double* uby, double* ubz,
                    int64_t np, int ng, double drift, hipStream_t st);
@@ -102,12 +127,11 @@ int launch_fill(int prec, void* p, size_t n, double v, hipStream_t st);
 int launch_deposit_atomic(int prec, const double* px, const double* py, const double* pz,
                           int64_t np, int ng, double mass, void* rho, unsigned long long* err,
                           hipStream_t st);
-// mode 0: kick + drift, 1: forces only (into vx, vy, vz), 2: kick + drift and
-// the next step's tile keys / counts (ts = tile side, cnt, key of the binning)
+// mode 0: kick + drift, 1: forces only (into vx, vy, vz), 3: kick + drift and
+// the drifted positions staged in tile order into s (fill, coff, stage)
 int launch_interp(int prec, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
                   int64_t np, int ng, const void* phi, double kick, double drift, int mode,
-                  unsigned long long* err, hipStream_t st, int ts = 16, uint32_t* cnt = nullptr,
-                  uint32_t* key = nullptr);
+                  unsigned long long* err, hipStream_t st, SortedScratch* s = nullptr);
 
 // Error reporting.
 void set_error(const std::string& msg);
@@ -137,10 +161,13 @@ struct pm_plan {
     // and counts for positions keys_x (consumed by PM_STEP_REUSE_BINS)
     bool keys_valid = false;
     const double* keys_x = nullptr;
+    // the interpolation of the last pm_step staged positions keys_x in tile
+    // order (sorted.stage) for the next deposit
+    bool staged_valid = false;
     void* adjU[3] = {};        // adjoint meshes U_a (interpolation transpose)
     void* adjLam = nullptr;    // lambda = S D^T U
     void* adjShell = nullptr;  // 3 shell buffers of the vec3 deposit
-    unsigned long long* adjMax = nullptr;  // max |u_a| (fixed-point scales of the vec3 deposit)
+    unsigned long long* adjMax = nullptr;  // max |u_a| (fixed-point scales of the vec3 deposit) + slots
     // Phase timing: one set of PM_NPHASES+1 events per recorded step, read
     // back (and summed) only by pm_plan_phase_ms, so that recording needs no
     // synchronisation inside a timed loop.

@@ -515,3 +515,29 @@ def test_large_mesh_reused_binning_1024(dev):
     assert pmhip.range_errors() == 0
     for a, b in zip(a_pos + a_vel, b_pos + b_vel):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("prec", [pmhip.PM_FP32, pmhip.PM_FP64])
+def test_reused_binning_after_bucket_overflow(dev, prec):
+    """A drift that packs every particle into the central eighth of the box
+    overflows the tile buckets that the FP32 interpolation stages into (each
+    holds its previous count + 1/8 + 32): the next deposit must fall back to
+    a full binning and still match fresh binning bit for bit."""
+    import torch
+
+    ng, nrow, dt = 64, 32, 0.1
+    npart = nrow ** 3
+    noise = np.random.default_rng(25).standard_normal(npart)
+    pos0, vel0 = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 0.5, 0.0)
+    c = ng / 2.0
+    vel0 = [(c - p) * (0.5 / dt) for p in pos0]  # x -> (x + c) / 2 in one drift
+    a_pos, a_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
+    b_pos, b_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
+    with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.run_kdk(a_pos, a_vel, 2, dt, pmhip.PM_DEPOSIT_SORTED)
+        for s, (k, d) in enumerate([(0.5 * dt, dt), (dt, dt), (0.5 * dt, 0.0)]):
+            plan.step(b_pos, b_vel, k, d, pmhip.PM_DEPOSIT_SORTED)
+        torch.cuda.synchronize()
+    assert pmhip.range_errors() == 0
+    for a, b in zip(a_pos + a_vel, b_pos + b_vel):
+        assert torch.equal(a, b)
