@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-vs-FP64 accuracy leg")
     ap.add_argument("--accuracy-steps", type=int, default=20)
+    ap.add_argument("--no-large-adjoint", action="store_true", help="skip the 1024^3-mesh adjoint leg")
+    ap.add_argument("--large-nrow", type=int, default=512)
+    ap.add_argument("--large-ng", type=int, default=1024)
+    ap.add_argument("--large-steps", type=int, default=30)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -57,21 +61,54 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(ng: int, npart: int, prec: int):
-    """Compulsory HBM bytes per launch of each phase (b = mesh real bytes;
-    positions and velocities FP64).  Spectrum: (ng/2+1) ng^2 complex."""
+def algorithmic_bytes(ng: int, npart: int, prec: int, staged: bool = True):
+    """Compulsory HBM bytes per launch of each phase of a steady-state step
+    (b = mesh real bytes; positions and velocities FP64; spectrum
+    (ng/2+1) ng^2 complex).  With the sorted deposit the interpolation
+    stages every drifted position in tile order as three in-tile offsets of
+    s bytes (u32 for an FP32 mesh, FP64 otherwise) and the next deposit reads
+    them back: 3 s per particle written once and read once."""
     b = 4 if prec == 0 else 8
+    st = (3 * (4 if prec == 0 else 8)) if staged else 0
     nc = ng ** 3
     nk = (ng // 2 + 1) * ng * ng
     return {
-        "deposit": 24 * npart + b * nc,                 # read x; write the mesh once
-        "fft_z_fwd": b * nc + 2 * b * nk,               # real in, half spectrum out
-        "fft_y_fwd": 4 * b * nk,                        # spectrum in + out
+        "deposit": (st if staged else 24) * npart + b * nc,  # read positions; write the mesh once
+        "fft_z_fwd": b * nc + 2 * b * nk,                    # real in, half spectrum out
+        "fft_y_fwd": 4 * b * nk,                             # spectrum in + out
         "fft_x_green": 4 * b * nk,
         "fft_y_inv": 4 * b * nk,
         "fft_z_inv": 2 * b * nk + b * nc,
-        "interp_kick_drift": b * nc + 96 * npart,       # read phi; read+write x, v
+        "interp_kick_drift": b * nc + (96 + st) * npart,     # read phi; read+write x, v; write staging
     }
+
+
+# kernel that carries each phase of the FP32 sorted step, as rocprofv3 names it
+PHASE_KERNELS = {
+    "deposit": "void pm::k_deposit_staged<float, 16>",
+    "fft_z_fwd": "void pm::k_zfwd<float, 512, true, 512>",
+    "fft_y_fwd": "void pm::k_yfwd<float, 512, 1024>",
+    "fft_x_green": "void pm::k_x<float, 512, 7, 512>",
+    "fft_y_inv": "void pm::k_yinv<float, 512, 1024>",
+    "fft_z_inv": "void pm::k_zinv<float, 512, 512>",
+    "interp_kick_drift": "void pm::k_interp<float, 2>",
+}
+
+
+def measured_traffic(kernel: str, ng: int, npart: int):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what
+    the counter reads for a plain copy of known size); None if absent."""
+    path = ROOT / "profiles" / f"pmc_traffic_{ng}.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("#") or "|" not in line:
+            continue
+        cols = [c.strip() for c in line.split("|")]
+        if len(cols) >= 6 and cols[0] == kernel and cols[1] == str(npart):
+            return int(float(cols[5]))
+    return None
 
 
 def make_ics(nrow, ng, rms, seed, dev):
@@ -215,6 +252,57 @@ def run_accuracy_leg(pos0, vel0, args, ng, npart, variant, nsteps=20):
             "against": "FP64 mesh/FFT/force path on the GPU (the same step as the FP64 oracle)"}
 
 
+def run_large_adjoint_leg(args, dev):
+    """512^3 particles on a 1024^3 mesh, 1.0-cell rms Zel'dovich ICs, 30 KDK
+    steps with the FP32 mesh, then the full reverse sweep (no stored
+    trajectory): time ratio, reversal drift of the recovered initial state
+    against the FP64 initial positions, and device memory in use."""
+    import torch
+    import pmhip
+
+    nrow, ng, n = args.large_nrow, args.large_ng, args.large_steps
+    npart = nrow ** 3
+    torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
+    pos0, vel0 = make_ics(nrow, ng, 1.0, args.seed + 1, dev)
+    pos = [p.clone() for p in pos0]
+    vel = [v.clone() for v in vel0]
+    xb = [torch.full_like(p, 1e-3) for p in pos]
+    vb = [torch.zeros_like(p) for p in pos]
+    plan = pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0)
+    variant = pmhip.PM_DEPOSIT_SORTED
+    w = [p.clone() for p in pos0], [v.clone() for v in vel0]
+    plan.run_kdk(w[0], w[1], 1, args.dt, variant)  # warm-up (allocates the binning)
+    del w
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    plan.run_kdk(pos, vel, n, args.dt, variant)
+    ev[1].record(stream)
+    plan.run_adjoint(pos, vel, xb, vb, n, args.dt, variant)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    t_f, t_a = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    d2 = None
+    for a, b in zip(pos, pos0):
+        d = a - b
+        d = d - ng * torch.round(d / ng)
+        d2 = d * d if d2 is None else d2 + d * d
+    out = {"np": npart, "ng": ng, "steps": n, "rms_cells": 1.0, "forward_ms": round(t_f, 3),
+           "adjoint_ms": round(t_a, 3), "ratio": round(t_a / t_f, 3),
+           "reversal_drift_rms_cells": float(torch.sqrt(d2.mean())),
+           "reversal_drift_max_cells": float(torch.sqrt(d2.max())),
+           "device_memory_in_use_GB": round((total - free1) / 1e9, 2),
+           "device_memory_added_GB": round((free0 - free1) / 1e9, 2),
+           "mode": "FP32 mesh, FP64 positions, reverse-time leapfrog recomputation"}
+    plan.close()
+    del pos, vel, xb, vb, pos0, vel0, d2
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(pos_dev, vel_dev, ng, args):
     """The FP64 oracle (C/OpenMP restatement of the reference step) on the
     same ICs, args.cpu_steps steps, timed on the host cores."""
@@ -298,6 +386,12 @@ def main():
         adj = run_adjoint_leg(pos0, vel0, args, ng, npart, variant)
         log(f"adjoint: {adj}")
 
+    large = None
+    if not args.no_large_adjoint:
+        log(f"large adjoint leg ({args.large_nrow}^3 particles, {args.large_ng}^3 mesh, {args.large_steps} steps)")
+        large = run_large_adjoint_leg(args, dev)
+        log(f"large adjoint: {large}")
+
     if args.copy_calibration:
         # a plain streaming copy of 1 GiB (read 1 GiB, write 1 GiB) for PMC byte-counter scaling
         a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
@@ -307,12 +401,15 @@ def main():
             torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
 
-    nbytes = algorithmic_bytes(ng, npart, pmhip.PM_FP32)
+    nbytes = algorithmic_bytes(ng, npart, pmhip.PM_FP32, variant == pmhip.PM_DEPOSIT_SORTED)
     dom = max((k for k in ph32 if ph32[k]), key=lambda k: ph32[k])
     achieved = nbytes[dom] / (ph32[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": nbytes[dom], "avg_ms": round(ph32[dom], 4)}
+    kname = PHASE_KERNELS.get(dom) if variant == pmhip.PM_DEPOSIT_SORTED else None
+    traffic = measured_traffic(kname, ng, npart) if kname else None
+    roofline = {"bound": "hbm", "kernel": kname or dom, "phase": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes": nbytes[dom], "avg_ms": round(ph32[dom], 4),
+                "traffic_source": f"profiles/pmc_traffic_{ng}.txt" if traffic else None}
     step_bytes = sum(nbytes.values())
     cpu = None
     if not args.no_cpu:
@@ -348,6 +445,7 @@ def main():
         "accuracy_vs_fp64": acc,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,
+        "large_adjoint": large,
         "range_errors": err32,
         "roofline": roofline,
         "cpu_baseline": cpu,

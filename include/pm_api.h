@@ -52,9 +52,10 @@ extern "C" {
                                 accumulation in LDS, deterministic (no global float atomics) */
 /* Flag for pm_step's variant: the positions are exactly those written by the
  * previous pm_step of this plan (same arrays, untouched since), so the
- * binning can start from the tile keys that step's interpolation left
- * behind instead of re-reading the positions.  Only honoured with
- * PM_DEPOSIT_SORTED; pm_run_kdk sets it itself after its first step. */
+ * deposit can read the copy of them that step's interpolation staged in
+ * tile order instead of binning the positions again.  Only honoured with
+ * PM_DEPOSIT_SORTED (results are bit-identical with and without it);
+ * pm_run_kdk sets it itself after its first step. */
 #define PM_STEP_REUSE_BINS 0x100
 
 /* Last error message of the calling thread ("" if none). */

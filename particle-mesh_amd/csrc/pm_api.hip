@@ -175,9 +175,9 @@ extern "C" void* pm_fft_spectrum(pm_fft_t* f) { return f ? f->specB : nullptr; }
 extern "C" int pm_plan_destroy(pm_plan_t* p) {
     if (!p) return PM_OK;
     tables_destroy(&p->tab);
-    void* bufs[15] = {p->rho,          p->phi,          p->specA,         p->specB,        p->sorted.cnt,
-                      p->sorted.off,   p->sorted.key,   p->sorted.slot,   p->sorted.idx,   p->sorted.cur,
-                      p->sorted.shell, p->sorted.fill,  p->sorted.coff,   p->sorted.spill, p->sorted.stage};
+    void* bufs[14] = {p->rho,          p->phi,           p->specA,        p->specB,        p->sorted.cnt,
+                      p->sorted.off,   p->sorted.key,    p->sorted.slot,  p->sorted.idx,   p->sorted.shell,
+                      p->sorted.fill,  p->sorted.coff,   p->sorted.spill, p->sorted.stage};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     void* abufs[6] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell, p->adjMax};
@@ -270,7 +270,6 @@ static int ensure_sorted(pm_plan* p) {
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.key, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.slot, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc(&s.shell, rs * sorted_shell_elems(p->ng)), "sorted deposit: hipMalloc");
-    if (!rc) rc = hip_ok(hipMalloc((void**)&s.cur, 4 * ntiles), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.idx, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
     // staging buckets: count + count/8 + 32 slots per tile (u32 slot numbers)
     const size_t cap_total = (size_t)p->np + (size_t)p->np / 8 + 32 * ntiles;
@@ -282,7 +281,7 @@ static int ensure_sorted(pm_plan* p) {
         if (!rc) s.cap_total = cap_total;
     }
     if (rc) {
-        void* bufs[11] = {s.cnt, s.off, s.key, s.slot, s.shell, s.cur, s.idx, s.fill, s.coff, s.spill, s.stage};
+        void* bufs[10] = {s.cnt, s.off, s.key, s.slot, s.shell, s.idx, s.fill, s.coff, s.spill, s.stage};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         s = SortedScratch();
@@ -291,10 +290,7 @@ static int ensure_sorted(pm_plan* p) {
 }
 
 // Other users of the binning buffers or of x end the step-to-step reuse.
-static void invalidate_reuse(pm_plan* p) {
-    p->keys_valid = false;
-    p->staged_valid = false;
-}
+static void invalidate_reuse(pm_plan* p) { p->staged_valid = false; }
 
 static int deposit_impl(pm_plan* p, const double* x, const double* y, const double* z, int variant) {
     const size_t nc = (size_t)p->ng * p->ng * p->ng;
@@ -374,20 +370,15 @@ extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx
     bool staged = false;  // deposit from / interpolation into the staging buckets
     if (variant == PM_DEPOSIT_SORTED) {
         rc = ensure_sorted(p);
-        staged = p->sorted.stage != nullptr;
-        const bool same_x = reuse && p->keys_x == x;
+        staged = !rc && p->sorted.stage != nullptr;
         if (rc) {
-        } else if (staged && same_x && p->staged_valid) {
-            // positions staged in tile order by the previous step's interpolation
+        } else if (staged && reuse && p->staged_valid && p->staged_x == x) {
+            // positions staged in tile order by the previous step's
+            // interpolation (caller's guarantee that x is unchanged since)
             rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
                                        p->stream);
         } else {
-            // binning of x: from the keys the previous step's tile
-            // interpolation left (caller's guarantee that x is unchanged
-            // since), else from scratch
-            const bool keys = !staged && same_x && p->keys_valid;
-            rc = keys ? launch_bin_from_keys(p->np, p->ng, p->sorted, p->stream)
-                      : launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
+            rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
             if (!rc) rc = launch_deposit_binned(p->prec, x, y, z, p->ng, mass, p->rho, p->sorted, p->stream);
             if (!rc && staged) rc = launch_cap_scan(p->ng, p->sorted, true, p->stream);
         }
@@ -399,23 +390,14 @@ extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx
     mark(p, 1);
     rc = solve_impl(p);
     if (rc) return rc;
+    // particles in memory order (a coherent order keeps the potential's
+    // gathers in L2); with the sorted deposit the drifted positions are
+    // staged in tile order for the next step's deposit
+    rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, staged ? 2 : 0,
+                       range_counter(), p->stream, staged ? &p->sorted : nullptr);
     if (staged) {
-        // particles in memory order (a coherent order keeps the gathers in
-        // L2), staging the drifted positions for the next deposit
-        rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 3, range_counter(),
-                           p->stream, &p->sorted);
         p->staged_valid = rc == PM_OK;
-        p->keys_x = x;
-    } else if (variant == PM_DEPOSIT_SORTED) {
-        // FP64: tile order with phi in LDS (halves the FP64 gather traffic),
-        // leaving the next binning's keys and counts
-        rc = launch_interp_tiles(p->prec, x, y, z, vx, vy, vz, p->ng, p->phi, kick, drift, p->sorted, 1,
-                                 range_counter(), p->stream);
-        p->keys_valid = rc == PM_OK;
-        p->keys_x = x;
-    } else {
-        rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 0, range_counter(),
-                           p->stream);
+        p->staged_x = x;
     }
     if (rc) return rc;
     mark(p, 7);

@@ -136,11 +136,9 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
     }
 }
 
-// 2. exclusive scan of n counts into off[0..n] (one workgroup of 1024);
-// optionally also into cursor[0..n) (the scatter's running slots) and with
+// 2. exclusive scan of n counts into off[0..n] (one workgroup of 1024),
 // the counts zeroed behind it for the next binning.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ off,
-                                               uint32_t* __restrict__ cursor, int n, int zero_counts,
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, int n,
                                                const uint32_t* __restrict__ gate) {
     if (gate && *gate == 0u) return;
     __shared__ uint32_t part[1024];
@@ -160,69 +158,10 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt, uint3
     uint32_t run = t == 0 ? 0u : part[t - 1];
     for (int i = b; i < e; ++i) {
         off[i] = run;
-        if (cursor) cursor[i] = run;
         run += cnt[i];
-        if (zero_counts) cnt[i] = 0u;
+        cnt[i] = 0u;
     }
     if (t == 1023) off[n] = part[1023];
-}
-
-// 3'. particle indices in tile order from keys alone (the keys and tile
-// counts of the positions were produced by the tile interpolation of the
-// previous step): chunks of kBinChunk particles, runs of equal keys
-// aggregated in an LDS hash, one returning global atomic on the tile cursor
-// per distinct tile of the chunk.
-__global__ __launch_bounds__(256) void k_scatter_keys(const uint32_t* __restrict__ key, int64_t np,
-                                                      uint32_t* __restrict__ cursor, uint32_t* __restrict__ idx) {
-    __shared__ uint32_t hkey[kBinHash];
-    __shared__ uint32_t hcnt[kBinHash];
-    for (int h = threadIdx.x; h < kBinHash; h += 256) {
-        hkey[h] = kInvalidKey;
-        hcnt[h] = 0u;
-    }
-    __syncthreads();
-    const int64_t base = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kBinChunk + (int64_t)threadIdx.x * kBinPer;
-    uint32_t k[kBinPer], h[kBinPer], rank[kBinPer];
-#pragma unroll
-    for (int j = 0; j < kBinPer; ++j) {
-        k[j] = base + j < np ? key[base + j] : kInvalidKey;
-        h[j] = 0u;
-        rank[j] = 0u;
-    }
-    int start = 0;
-#pragma unroll
-    for (int j = 1; j <= kBinPer; ++j) {
-        if (j < kBinPer && k[j] == k[start]) continue;
-        if (k[start] != kInvalidKey) {
-            uint32_t s = bin_hash(k[start]) & (kBinHash - 1);
-            while (true) {
-                const uint32_t old = atomicCAS(&hkey[s], kInvalidKey, k[start]);
-                if (old == kInvalidKey || old == k[start]) break;
-                s = (s + 1) & (kBinHash - 1);
-            }
-            const uint32_t r = atomicAdd(&hcnt[s], (uint32_t)(j - start));
-#pragma unroll
-            for (int m = 0; m < kBinPer; ++m)
-                if (m >= start && m < j) {
-                    h[m] = s;
-                    rank[m] = r + (uint32_t)(m - start);
-                }
-        }
-        start = j;
-    }
-    __syncthreads();
-    for (int s = threadIdx.x; s < kBinHash; s += 256) {
-        const uint32_t kk = hkey[s];
-        if (kk != kInvalidKey) hcnt[s] = atomicAdd(&cursor[kk], hcnt[s]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kBinPer; ++j)
-        if (k[j] != kInvalidKey) {
-            // keys and counts come from another kernel: never write past np
-            const uint32_t d = hcnt[h[j]] + rank[j];
-            if (d < (uint64_t)np) idx[d] = (uint32_t)(base + j);
-        }
 }
 
 // 3. particle indices in tile order.
@@ -230,11 +169,11 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ off, int64_t np,
                                                      uint32_t* __restrict__ idx, const uint32_t* __restrict__ gate) {
     if (gate && *gate == 0u) return;
-    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (i >= np) return;
-    const uint32_t k = key[i];
-    if (k == kInvalidKey) return;
-    idx[off[k] + slot[i]] = (uint32_t)i;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x; i < np; i += stride) {
+        const uint32_t k = key[i];
+        if (k != kInvalidKey) idx[off[k] + slot[i]] = (uint32_t)i;
+    }
 }
 
 constexpr int kDepBatch = 4;  // particles in flight per thread in the tile kernels
@@ -530,26 +469,14 @@ int launch_bin(const double* px, const double* py, const double* pz, int64_t np,
         hipLaunchKernelGGL(k_bin_count, dim3(nchunk), dim3(256), 0, st, px, py, pz, np, ng, ts, s.cnt, s.key, s.slot,
                            err, gate);
     }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, (uint32_t*)nullptr, (int)ntiles, 1, gate);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, (int)ntiles, gate);
     if (np > 0) {
         const unsigned nb = (unsigned)((np + 255) / 256);
-        hipLaunchKernelGGL(k_bin_scatter, dim3(nb), dim3(256), 0, st, s.key, s.slot, s.off, np, s.idx, gate);
+        // grid-stride over at most 16384 workgroups (a gated no-op stays cheap)
+        hipLaunchKernelGGL(k_bin_scatter, dim3(std::min(nb, 16384u)), dim3(256), 0, st, s.key, s.slot, s.off, np,
+                           s.idx, gate);
     }
     return check_launch("bin");
-}
-
-int launch_bin_from_keys(int64_t np, int ng, SortedScratch& s, hipStream_t st) {
-    const int ts = sorted_tile(ng);
-    const int nt = ng / ts;
-    const uint32_t ntiles = (uint32_t)nt * nt * nt;
-    // the tile interpolation left the keys in s.key and the counts in s.cnt
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, s.cur, (int)ntiles, 1,
-                       (const uint32_t*)nullptr);
-    if (np > 0) {
-        const unsigned nchunk = (unsigned)((np + kBinChunk - 1) / kBinChunk);
-        hipLaunchKernelGGL(k_scatter_keys, dim3(nchunk), dim3(256), 0, st, s.key, np, s.cur, s.idx);
-    }
-    return check_launch("bin_from_keys");
 }
 
 int launch_fold_shells(int prec, int ts, int ng, const void* shell, void* rho, hipStream_t st) {

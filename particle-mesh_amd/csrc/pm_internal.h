@@ -59,7 +59,6 @@ struct SortedScratch {
     uint32_t* key = nullptr;   // np
     uint32_t* slot = nullptr;  // np
     uint32_t* idx = nullptr;   // np
-    uint32_t* cur = nullptr;   // ntiles (scatter cursor of the key-only binning)
     void* shell = nullptr;     // ntiles * shell_size reals
     // Staging in tile order, filled by the interpolation for the next
     // step's deposit: bucket t holds fill[t] in-tile offsets from coff[t]
@@ -71,14 +70,6 @@ struct SortedScratch {
                                 // 2^-28 cells (FP32 mesh) or exact doubles (FP64)
     size_t cap_total = 0;
 };
-// binning from the keys and counts left by the tile interpolation
-int launch_bin_from_keys(int64_t np, int ng, SortedScratch& s, hipStream_t st);
-// interpolation + kick + drift over the tile binning (phi staged in LDS);
-// with keys != 0 it also writes each particle's new tile key to s.key and
-// adds the tile counts to s.cnt for the next step's binning
-int launch_interp_tiles(int prec, double* px, double* py, double* pz, double* vx, double* vy, double* vz, int ng,
-                        const void* phi, double kick, double drift, SortedScratch& s, int keys,
-                        unsigned long long* err, hipStream_t st);
 int sorted_tile(int ng);
 size_t sorted_shell_elems(int ng);
 int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
@@ -127,7 +118,7 @@ int launch_fill(int prec, void* p, size_t n, double v, hipStream_t st);
 int launch_deposit_atomic(int prec, const double* px, const double* py, const double* pz,
                           int64_t np, int ng, double mass, void* rho, unsigned long long* err,
                           hipStream_t st);
-// mode 0: kick + drift, 1: forces only (into vx, vy, vz), 3: kick + drift and
+// mode 0: kick + drift, 1: forces only (into vx, vy, vz), 2: kick + drift and
 // the drifted positions staged in tile order into s (fill, coff, stage)
 int launch_interp(int prec, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
                   int64_t np, int ng, const void* phi, double kick, double drift, int mode,
@@ -157,13 +148,10 @@ struct pm_plan {
     void* specB = nullptr;
     pm::FftTables tab;
     pm::SortedScratch sorted;  // allocated on the first sorted deposit
-    // the tile interpolation of the last pm_step left the next binning's keys
-    // and counts for positions keys_x (consumed by PM_STEP_REUSE_BINS)
-    bool keys_valid = false;
-    const double* keys_x = nullptr;
-    // the interpolation of the last pm_step staged positions keys_x in tile
-    // order (sorted.stage) for the next deposit
+    // the interpolation of the last pm_step staged positions staged_x in tile
+    // order (sorted.stage) for the next deposit (PM_STEP_REUSE_BINS)
     bool staged_valid = false;
+    const double* staged_x = nullptr;
     void* adjU[3] = {};        // adjoint meshes U_a (interpolation transpose)
     void* adjLam = nullptr;    // lambda = S D^T U
     void* adjShell = nullptr;  // 3 shell buffers of the vec3 deposit

@@ -70,26 +70,6 @@ int launch_deposit_atomic(int prec, const double* px, const double* py, const do
     return check_launch("deposit_atomic");
 }
 
-// Particle-state streams (read once, written once per step) bypass L2
-// retention so that the potential's gather window stays cached.
-#ifndef PM_NT_STREAM
-#define PM_NT_STREAM 0
-#endif
-template <typename T> __device__ __forceinline__ T ld_stream(const T* p) {
-#if PM_NT_STREAM
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
-}
-template <typename T> __device__ __forceinline__ void st_stream(T* p, T v) {
-#if PM_NT_STREAM
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
-
 __device__ __forceinline__ double wrap_pos(double x, double box) {
     if (x < 0.0) x += box;
     if (x >= box) x -= box;
@@ -99,14 +79,11 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
 // update_particles (utils.cpp:141-170) with the force of cic_force, then the
 // drift that the reference leaves commented out (utils.cpp:145-146).
 // MODE 0: v += g*kick, x += v*drift.  MODE 1: write g to (vx, vy, vz).
-// MODE 3: as MODE 0, then the drifted position is staged for the next
+// MODE 2: as MODE 0, then the drifted position is staged for the next
 // step's deposit: its tile's bucket gets a slot (fill counter), and the
 // in-tile offsets (FP32 mesh: u32 at 2^-28 cells; FP64 mesh: exact doubles)
 // go to the staging arrays at coff[tile] + slot; a slot past the bucket's
 // capacity raises *spill.
-#ifndef PM_STAGE_WAVE
-#define PM_STAGE_WAVE 0
-#endif
 constexpr int kStageHash = 128;
 
 template <typename Rl, int MODE>
@@ -120,9 +97,9 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
                                                          uint32_t* __restrict__ spill, void* __restrict__ stage,
                                                          size_t cap_total, unsigned long long* __restrict__ err) {
     constexpr uint32_t kNo = 0xFFFFFFFFu;
-    constexpr int HS = MODE == 3 ? kStageHash : 1;
+    constexpr int HS = MODE == 2 ? kStageHash : 1;
     __shared__ uint32_t hkey[HS], hcnt[HS], hbase[HS];
-    if constexpr (MODE == 3 && !PM_STAGE_WAVE) {
+    if constexpr (MODE == 2) {
         for (int s = threadIdx.x; s < HS; s += kPartThreads) {
             hkey[s] = kNo;
             hcnt[s] = 0u;
@@ -133,11 +110,11 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
     const double box = (double)ng;
     bool ok = i < np;
     double x = 0.0, y = 0.0, z = 0.0;
-    uint32_t kk = kNo;  // MODE 3: the drifted position's tile
+    uint32_t kk = kNo;  // MODE 2: the drifted position's tile
     if (ok) {
-        x = ld_stream(px + i);
-        y = ld_stream(py + i);
-        z = ld_stream(pz + i);
+        x = px[i];
+        y = py[i];
+        z = pz[i];
         if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
             atomicAdd(err, 1ULL);
             ok = false;
@@ -147,24 +124,24 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
         double F[3];
         cic_force(phi, ng, x, y, z, F);
         if constexpr (MODE == 1) {
-            st_stream(vx + i, F[0]);
-            st_stream(vy + i, F[1]);
-            st_stream(vz + i, F[2]);
+            vx[i] = F[0];
+            vy[i] = F[1];
+            vz[i] = F[2];
         } else {
-            const double nvx = ld_stream(vx + i) + F[0] * kick, nvy = ld_stream(vy + i) + F[1] * kick,
-                         nvz = ld_stream(vz + i) + F[2] * kick;
-            st_stream(vx + i, nvx);
-            st_stream(vy + i, nvy);
-            st_stream(vz + i, nvz);
+            const double nvx = vx[i] + F[0] * kick, nvy = vy[i] + F[1] * kick,
+                         nvz = vz[i] + F[2] * kick;
+            vx[i] = nvx;
+            vy[i] = nvy;
+            vz[i] = nvz;
             if (drift != 0.0) {
                 x = wrap_pos(x + nvx * drift, box);
                 y = wrap_pos(y + nvy * drift, box);
                 z = wrap_pos(z + nvz * drift, box);
-                st_stream(px + i, x);
-                st_stream(py + i, y);
-                st_stream(pz + i, z);
+                px[i] = x;
+                py[i] = y;
+                pz[i] = z;
             }
-            if constexpr (MODE == 3) {
+            if constexpr (MODE == 2) {
                 // a particle that drifted more than a box away stays unstaged
                 // and is counted, as the next step's range check would
                 if (in_box(x, box) && in_box(y, box) && in_box(z, box)) {
@@ -176,7 +153,7 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
             }
         }
     }
-    if constexpr (MODE == 3) {
+    if constexpr (MODE == 2) {
         // Runs of equal tiles in consecutive lanes (particles in a coherent
         // order share tiles) reserve their slots together: the first lane of
         // a run takes run-length slots, first within the workgroup (LDS
@@ -186,15 +163,6 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
         const bool head = kk != kNo && (lane == 0 || prev != kk);
         const unsigned long long bounds = __ballot(head || kk == kNo);
         uint32_t hs = kNo, lrank = 0u, gb = 0u;
-#if PM_STAGE_WAVE
-        if (head) {
-            const unsigned long long above = lane == 63 ? 0ull : (bounds >> (lane + 1)) << (lane + 1);
-            const uint32_t run = (above ? (uint32_t)__ffsll((long long)above) - 1u : 64u) - lane;
-            gb = atomicAdd(&fill[kk], run);
-        }
-        (void)hs;
-        (void)lrank;
-#else
         if (head) {
             const unsigned long long above = lane == 63 ? 0ull : (bounds >> (lane + 1)) << (lane + 1);
             const uint32_t run = (above ? (uint32_t)__ffsll((long long)above) - 1u : 64u) - lane;
@@ -215,7 +183,6 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
             if (hkey[s] != kNo) hbase[s] = atomicAdd(&fill[hkey[s]], hcnt[s]);
         __syncthreads();
         if (hs != kNo) gb = hbase[hs] + lrank;
-#endif
         // every lane of a run takes its head's base plus its distance
         const unsigned long long upto = bounds & (lane == 63 ? ~0ull : (2ull << lane) - 1ull);
         const int hl = upto ? 63 - __clzll((long long)upto) : 0;
@@ -229,14 +196,14 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
                 const double lx = x - ((int)x & msk), ly = y - ((int)y & msk), lz = z - ((int)z & msk);
                 if constexpr (sizeof(Rl) == 4) {  // u32, 2^-28 cells
                     uint32_t* st = static_cast<uint32_t*>(stage);
-                    st_stream(st + q, (uint32_t)(lx * 268435456.0));
-                    st_stream(st + q + cap_total, (uint32_t)(ly * 268435456.0));
-                    st_stream(st + q + 2 * cap_total, (uint32_t)(lz * 268435456.0));
+                    st[q] = (uint32_t)(lx * 268435456.0);
+                    st[q + cap_total] = (uint32_t)(ly * 268435456.0);
+                    st[q + 2 * cap_total] = (uint32_t)(lz * 268435456.0);
                 } else {  // exact FP64 offsets
                     double* st = static_cast<double*>(stage);
-                    st_stream(st + q, lx);
-                    st_stream(st + q + cap_total, ly);
-                    st_stream(st + q + 2 * cap_total, lz);
+                    st[q] = lx;
+                    st[q + cap_total] = ly;
+                    st[q + 2 * cap_total] = lz;
                 }
             } else {
                 *spill = 1u;
@@ -249,7 +216,7 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
                   int64_t np, int ng, const void* phi, double kick, double drift, int mode,
                   unsigned long long* err, hipStream_t st, SortedScratch* s) {
     if (np <= 0) return PM_OK;
-    if (mode == 3 && (!s || !s->stage)) return fail(PM_ERR_ARG, "interp: staging buffers missing");
+    if (mode == 2 && (!s || !s->stage)) return fail(PM_ERR_ARG, "interp: staging buffers missing");
     const unsigned nb = (unsigned)((np + kPartThreads - 1) / kPartThreads);
     const dim3 g(nb), b(kPartThreads);
     const int ts = sorted_tile(ng);
@@ -267,14 +234,14 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
         else if (mode == 1)
             PM_INTERP(float, 1);
         else
-            PM_INTERP(float, 3);
+            PM_INTERP(float, 2);
     } else {
         if (mode == 0)
             PM_INTERP(double, 0);
         else if (mode == 1)
             PM_INTERP(double, 1);
         else
-            PM_INTERP(double, 3);
+            PM_INTERP(double, 2);
     }
 #undef PM_INTERP
     return check_launch("interp");

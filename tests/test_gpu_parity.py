@@ -346,12 +346,12 @@ def test_kdk_fp64_variants_match_oracle(dev, variant):
         assert np.max(np.abs(d)) < 1e-7
 
 
-def test_tile_interp_bitwise_equals_particle_interp(dev):
-    """pm_step with the sorted deposit interpolates per tile from LDS (FP64)
-    or in memory order while writing the next binning (FP32); the standalone
-    pm_interp_kick_drift walks particles in memory order from HBM.  Same
-    arithmetic in the same order; only the compiler's FMA contraction may
-    differ, so the bound is a few ulps of the mesh type."""
+def test_step_interp_matches_standalone_interp(dev):
+    """pm_step with the sorted deposit interpolates while staging the
+    drifted positions for the next deposit; the standalone
+    pm_interp_kick_drift does the interpolation alone.  Same arithmetic in
+    the same order; only the compiler's FMA contraction may differ between
+    the two kernels, so the bound is a few ulps of the mesh type."""
     import torch
 
     ng, nrow = 32, 32
