@@ -277,7 +277,8 @@ static int ensure_sorted(pm_plan* p) {
         rc = hip_ok(hipMalloc((void**)&s.fill, 4 * ntiles), "sorted deposit: hipMalloc");
         if (!rc) rc = hip_ok(hipMalloc((void**)&s.coff, 4 * (ntiles + 1)), "sorted deposit: hipMalloc");
         if (!rc) rc = hip_ok(hipMalloc((void**)&s.spill, 4), "sorted deposit: hipMalloc");
-        if (!rc) rc = hip_ok(hipMalloc(&s.stage, 3 * (rs == 4 ? 4 : 8) * cap_total), "sorted deposit: hipMalloc");
+        const size_t rec = rs == 4 ? sizeof(StageQ) : sizeof(StageD);
+        if (!rc) rc = hip_ok(hipMalloc(&s.stage, rec * cap_total), "sorted deposit: hipMalloc");
         if (!rc) s.cap_total = cap_total;
     }
     if (rc) {

@@ -315,21 +315,19 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict_
 
 // 4'. one workgroup per tile, particles from the staging arrays that the
 // interpolation of the previous step filled in tile order: tile t holds
-// fill[t] entries from coff[t], in-tile offsets as u32 at 2^-28 cells (FP32
+// fill[t] records from coff[t], in-tile offsets as u32 at 2^-28 cells (FP32
 // mesh) or exact doubles (FP64 mesh, giving the same weights as the binned
 // path bit for bit).  Skipped when a tile overflowed its capacity (the
 // binned path runs instead).
 template <typename Rl, int TS>
-__global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__ stage, size_t cap_total,
+__global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__ stage,
                                                         const uint32_t* __restrict__ coff,
                                                         const uint32_t* __restrict__ fill,
                                                         const uint32_t* __restrict__ spill, int ng, double mass,
                                                         Rl* __restrict__ rho, Rl* __restrict__ shell) {
     if (*spill != 0u) return;
-    using St = typename std::conditional<sizeof(Rl) == 4, uint32_t, double>::type;
-    const St* __restrict__ sx = static_cast<const St*>(stage);
-    const St* __restrict__ sy = sx + cap_total;
-    const St* __restrict__ sz = sx + 2 * cap_total;
+    using St = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
+    const St* __restrict__ sp = static_cast<const St*>(stage);
     constexpr int E = TS + 1, E3 = E * E * E;
     __shared__ unsigned long long acc[E3];
     const int nt = ng / TS;
@@ -341,25 +339,22 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
     const uint32_t beg = coff[t], end = beg + fill[t];
     const double fix = 4294967296.0 * mass;
     for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += 256 * kDepBatch) {
-        St a[kDepBatch], b[kDepBatch], c[kDepBatch];
+        St v[kDepBatch];
 #pragma unroll
         for (int j = 0; j < kDepBatch; ++j) {
             const uint32_t q = q0 + 256 * j;
-            if (q < end) {
-                a[j] = sx[q];
-                b[j] = sy[q];
-                c[j] = sz[q];
-            }
+            if (q < end) v[j] = sp[q];
         }
 #pragma unroll
         for (int j = 0; j < kDepBatch; ++j) {
             if (q0 + 256 * j >= end) continue;
             if constexpr (sizeof(Rl) == 4) {
-                tile_cic_add<TS>(acc, (int)(a[j] >> 28), (int)(b[j] >> 28), (int)(c[j] >> 28),
-                                 (a[j] & kQMask) / kQScale, (b[j] & kQMask) / kQScale, (c[j] & kQMask) / kQScale, fix);
+                tile_cic_add<TS>(acc, (int)(v[j].x >> 28), (int)(v[j].y >> 28), (int)(v[j].z >> 28),
+                                 (v[j].x & kQMask) / kQScale, (v[j].y & kQMask) / kQScale,
+                                 (v[j].z & kQMask) / kQScale, fix);
             } else {
-                const int ia = (int)a[j], ib = (int)b[j], ic = (int)c[j];
-                tile_cic_add<TS>(acc, ia, ib, ic, a[j] - ia, b[j] - ib, c[j] - ic, fix);
+                const int ia = (int)v[j].x, ib = (int)v[j].y, ic = (int)v[j].z;
+                tile_cic_add<TS>(acc, ia, ib, ic, v[j].x - ia, v[j].y - ib, v[j].z - ic, fix);
             }
         }
     }
@@ -543,8 +538,8 @@ int launch_deposit_staged(int prec, const double* px, const double* py, const do
     if (!rc) rc = launch_deposit_binned(prec, px, py, pz, ng, mass, rho, s, st, s.spill, false);
     if (rc) return rc;
 #define PM_DS(RL, TSV)                                                                                               \
-    hipLaunchKernelGGL((k_deposit_staged<RL, TSV>), dim3(ntiles), dim3(256), 0, st, s.stage, s.cap_total, s.coff,     \
-                       s.fill, s.spill, ng, mass, (RL*)rho, (RL*)s.shell)
+    hipLaunchKernelGGL((k_deposit_staged<RL, TSV>), dim3(ntiles), dim3(256), 0, st, s.stage, s.coff, s.fill,          \
+                       s.spill, ng, mass, (RL*)rho, (RL*)s.shell)
     if (prec == PM_FP32) {
         if (ts == 16)
             PM_DS(float, 16);

@@ -52,6 +52,15 @@ __global__ __launch_bounds__(256) void k_fill(T* __restrict__ p, size_t n, T v) 
         p[i] = v;
 }
 
+// A staged particle: its offsets inside its tile, u32 at 2^-28 cells for an
+// FP32 mesh, exact doubles for an FP64 mesh (one record = one vector store).
+struct StageQ {
+    uint32_t x, y, z;
+};
+struct StageD {
+    double x, y, z;
+};
+
 // Work space of the sorted deposit (pm_deposit_sorted.hip).
 struct SortedScratch {
     uint32_t* cnt = nullptr;   // ntiles
@@ -66,8 +75,8 @@ struct SortedScratch {
     uint32_t* fill = nullptr;   // ntiles
     uint32_t* coff = nullptr;   // ntiles + 1
     uint32_t* spill = nullptr;  // 1
-    void* stage = nullptr;      // 3 * cap_total in-tile offsets (x, y, z): u32 at
-                                // 2^-28 cells (FP32 mesh) or exact doubles (FP64)
+    void* stage = nullptr;      // cap_total records of in-tile offsets (x, y, z):
+                                // StageQ (FP32 mesh) or StageD (FP64 mesh)
     size_t cap_total = 0;
 };
 int sorted_tile(int ng);

@@ -84,10 +84,16 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
 // in-tile offsets (FP32 mesh: u32 at 2^-28 cells; FP64 mesh: exact doubles)
 // go to the staging arrays at coff[tile] + slot; a slot past the bucket's
 // capacity raises *spill.
-constexpr int kStageHash = 128;
+// Workgroup of the staging interpolation: bigger workgroups give longer
+// runs of one tile's slots (the buckets are written in longer pieces).
+#ifndef PM_STAGE_THREADS
+#define PM_STAGE_THREADS 256
+#endif
+constexpr int kStageThreads = PM_STAGE_THREADS;
+constexpr int kStageHash = 2 * kStageThreads / 4;
 
-template <typename Rl, int MODE>
-__global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px, double* __restrict__ py,
+template <typename Rl, int MODE, int NT = kPartThreads>
+__global__ __launch_bounds__(NT) void k_interp(double* __restrict__ px, double* __restrict__ py,
                                                          double* __restrict__ pz, double* __restrict__ vx,
                                                          double* __restrict__ vy, double* __restrict__ vz,
                                                          int64_t np, int ng, const Rl* __restrict__ phi,
@@ -95,18 +101,19 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
                                                          uint32_t* __restrict__ fill,
                                                          const uint32_t* __restrict__ coff,
                                                          uint32_t* __restrict__ spill, void* __restrict__ stage,
-                                                         size_t cap_total, unsigned long long* __restrict__ err) {
+                                                         unsigned long long* __restrict__ err) {
     constexpr uint32_t kNo = 0xFFFFFFFFu;
     constexpr int HS = MODE == 2 ? kStageHash : 1;
+    static_assert((HS & (HS - 1)) == 0, "hash size must be a power of two");
     __shared__ uint32_t hkey[HS], hcnt[HS], hbase[HS];
     if constexpr (MODE == 2) {
-        for (int s = threadIdx.x; s < HS; s += kPartThreads) {
+        for (int s = threadIdx.x; s < HS; s += NT) {
             hkey[s] = kNo;
             hcnt[s] = 0u;
         }
         __syncthreads();
     }
-    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kPartThreads + threadIdx.x;
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
     const double box = (double)ng;
     bool ok = i < np;
     double x = 0.0, y = 0.0, z = 0.0;
@@ -166,7 +173,7 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
         if (head) {
             const unsigned long long above = lane == 63 ? 0ull : (bounds >> (lane + 1)) << (lane + 1);
             const uint32_t run = (above ? (uint32_t)__ffsll((long long)above) - 1u : 64u) - lane;
-            uint32_t s = (kk * 2654435761u) >> 25;  // 7 bits
+            uint32_t s = (kk * 2654435761u) >> (32 - ilog2(HS));
             for (int probe = 0; probe < HS; ++probe) {
                 const uint32_t old = atomicCAS(&hkey[s], kNo, kk);
                 if (old == kNo || old == kk) {
@@ -179,7 +186,7 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
             if (hs == kNo) gb = atomicAdd(&fill[kk], run);  // hash full: reserve directly
         }
         __syncthreads();
-        for (int s = threadIdx.x; s < HS; s += kPartThreads)
+        for (int s = threadIdx.x; s < HS; s += NT)
             if (hkey[s] != kNo) hbase[s] = atomicAdd(&fill[hkey[s]], hcnt[s]);
         __syncthreads();
         if (hs != kNo) gb = hbase[hs] + lrank;
@@ -194,17 +201,12 @@ __global__ __launch_bounds__(kPartThreads) void k_interp(double* __restrict__ px
                 const int msk = ~(ts - 1);  // tile origin = cell with the low bits cleared
                 const size_t q = (size_t)c0 + slot;
                 const double lx = x - ((int)x & msk), ly = y - ((int)y & msk), lz = z - ((int)z & msk);
-                if constexpr (sizeof(Rl) == 4) {  // u32, 2^-28 cells
-                    uint32_t* st = static_cast<uint32_t*>(stage);
-                    st[q] = (uint32_t)(lx * 268435456.0);
-                    st[q + cap_total] = (uint32_t)(ly * 268435456.0);
-                    st[q + 2 * cap_total] = (uint32_t)(lz * 268435456.0);
-                } else {  // exact FP64 offsets
-                    double* st = static_cast<double*>(stage);
-                    st[q] = lx;
-                    st[q + cap_total] = ly;
-                    st[q + 2 * cap_total] = lz;
-                }
+                if constexpr (sizeof(Rl) == 4)  // u32, 2^-28 cells
+                    static_cast<StageQ*>(stage)[q] = StageQ{(uint32_t)(lx * 268435456.0),
+                                                            (uint32_t)(ly * 268435456.0),
+                                                            (uint32_t)(lz * 268435456.0)};
+                else  // exact FP64 offsets
+                    static_cast<StageD*>(stage)[q] = StageD{lx, ly, lz};
             } else {
                 *spill = 1u;
             }
@@ -217,31 +219,31 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
                   unsigned long long* err, hipStream_t st, SortedScratch* s) {
     if (np <= 0) return PM_OK;
     if (mode == 2 && (!s || !s->stage)) return fail(PM_ERR_ARG, "interp: staging buffers missing");
-    const unsigned nb = (unsigned)((np + kPartThreads - 1) / kPartThreads);
-    const dim3 g(nb), b(kPartThreads);
+    const int nt = mode == 2 ? kStageThreads : kPartThreads;
+    const unsigned nb = (unsigned)((np + nt - 1) / nt);
+    const dim3 g(nb), b(nt);
     const int ts = sorted_tile(ng);
     uint32_t* fill = s ? s->fill : nullptr;
     const uint32_t* coff = s ? s->coff : nullptr;
     uint32_t* spill = s ? s->spill : nullptr;
     void* stage = s ? s->stage : nullptr;
-    const size_t cap = s ? s->cap_total : 0;
-#define PM_INTERP(RL, M)                                                                                    \
-    hipLaunchKernelGGL((k_interp<RL, M>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng, (const RL*)phi, kick, \
-                       drift, ts, fill, coff, spill, stage, cap, err)
+#define PM_INTERP(RL, M, ...)                                                                          \
+    hipLaunchKernelGGL((k_interp<RL, M, ##__VA_ARGS__>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng,         \
+                       (const RL*)phi, kick, drift, ts, fill, coff, spill, stage, err)
     if (prec == PM_FP32) {
         if (mode == 0)
             PM_INTERP(float, 0);
         else if (mode == 1)
             PM_INTERP(float, 1);
         else
-            PM_INTERP(float, 2);
+            PM_INTERP(float, 2, kStageThreads);
     } else {
         if (mode == 0)
             PM_INTERP(double, 0);
         else if (mode == 1)
             PM_INTERP(double, 1);
         else
-            PM_INTERP(double, 2);
+            PM_INTERP(double, 2, kStageThreads);
     }
 #undef PM_INTERP
     return check_launch("interp");
