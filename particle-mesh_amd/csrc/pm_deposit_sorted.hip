@@ -23,6 +23,7 @@
 #include <type_traits>
 
 #include "pm_internal.h"
+#include "pm_tiles.h"
 
 namespace pm {
 
@@ -177,20 +178,6 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t* __restrict_
 }
 
 constexpr int kDepBatch = 4;  // particles in flight per thread in the tile kernels
-
-template <int TS> __host__ __device__ constexpr int shell_size() {
-    return (TS + 1) * (TS + 1) * (TS + 1) - TS * TS * TS;
-}
-
-// Shell index of a node with local coordinates (a, b, c), max == TS:
-// face x (a == TS): (TS+1)^2 entries, face y (b == TS, a < TS): TS (TS+1),
-// face z (c == TS, a, b < TS): TS^2.
-template <int TS> __device__ __forceinline__ int shell_index(int a, int b, int c) {
-    constexpr int E = TS + 1;
-    if (a == TS) return b * E + c;
-    if (b == TS) return E * E + a * E + c;
-    return E * E + TS * E + a * TS + b;
-}
 
 // In-tile offsets of the FP32 path: u32 fixed point with 2^-28 cell
 // resolution (cell = q >> 28, fraction = low 28 bits).  The staged deposit
@@ -401,7 +388,6 @@ __global__ __launch_bounds__(1024) void k_cap_scan(uint32_t* __restrict__ fill, 
 // 5. add the lower neighbours' shells to each tile's lower-face nodes.
 template <typename Rl, int TS>
 __global__ __launch_bounds__(256) void k_fold_shells(const Rl* __restrict__ shell, int ng, Rl* __restrict__ rho) {
-    constexpr int SH = shell_size<TS>();
     const int nt = ng / TS;
     const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
@@ -429,17 +415,7 @@ __global__ __launch_bounds__(256) void k_fold_shells(const Rl* __restrict__ shel
         }
         const int on = (a == 0) | ((b == 0) << 1) | ((c == 0) << 2);
         const size_t node = ((size_t)(tx * TS + a) * N + (ty * TS + b)) * N + (tz * TS + c);
-        Rl v = rho[node];
-        for (int d = 1; d < 8; ++d) {
-            if ((d & on) != d) continue;
-            const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-            const int sx = tx - dx < 0 ? nt - 1 : tx - dx;
-            const int sy = ty - dy < 0 ? nt - 1 : ty - dy;
-            const int sz = tz - dz < 0 ? nt - 1 : tz - dz;
-            const size_t st = ((size_t)sx * nt + sy) * nt + sz;
-            v += shell[st * SH + shell_index<TS>(a + TS * dx, b + TS * dy, c + TS * dz)];
-        }
-        rho[node] = v;
+        rho[node] = fold_node<Rl, TS>(rho[node], shell, ng, tx * TS + a, ty * TS + b, tz * TS + c, on);
     }
 }
 

@@ -541,3 +541,25 @@ def test_reused_binning_after_bucket_overflow(dev, prec):
     assert pmhip.range_errors() == 0
     for a, b in zip(a_pos + a_vel, b_pos + b_vel):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("prec", [pmhip.PM_FP32, pmhip.PM_FP64])
+def test_step_density_equals_standalone_deposit(dev, prec):
+    """pm_step leaves the tile-shell fold of the sorted deposit to the
+    solve's first FFT pass, which writes the folded density back: after a
+    step (binned first step, then a staged one) the plan's density must be
+    bit for bit pm_deposit's density of the same positions."""
+    import torch
+
+    ng, nrow = 64, 32
+    npart = nrow ** 3
+    noise = np.random.default_rng(27).standard_normal(npart)
+    pos, vel = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 1.5, 3.0)
+    with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan, \
+            pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as ref:
+        for s in range(2):
+            before = [p.clone() for p in pos]
+            plan.step(pos, vel, 0.01, 0.05, pmhip.PM_DEPOSIT_SORTED | (pmhip.PM_STEP_REUSE_BINS if s else 0))
+            ref.deposit(before, pmhip.PM_DEPOSIT_SORTED)
+            torch.cuda.synchronize()
+            assert torch.equal(plan.density(), ref.density()), s
