@@ -138,6 +138,20 @@ int pm_fft_backward(pm_fft_t* fft, void* slab);
 /* Device pointer of the spectrum ((ng/2+1)*ng*ng complex, interleaved). */
 void* pm_fft_spectrum(pm_fft_t* fft);
 
+/* ---- particle snapshots (no counterpart in the reference) --------------- */
+/* A versioned binary file: 64-byte header (magic "PMSNAP01", version 1,
+ * header size, np, ng, step, a) then x, y, z, vx, vy, vz as float64 arrays
+ * of np values.  Device arrays are copied through a pinned host buffer in
+ * chunks on `stream`; the calls return when the file is complete. */
+int pm_snapshot_write(const char* path, int64_t np, int ng, int step, double a, const double* posx,
+                      const double* posy, const double* posz, const double* vx, const double* vy,
+                      const double* vz, void* stream);
+/* Header fields of a snapshot (any pointer may be NULL). */
+int pm_snapshot_info(const char* path, int64_t* np, int* ng, int* step, double* a);
+/* Read a snapshot of exactly np particles into device arrays. */
+int pm_snapshot_read(const char* path, int64_t np, double* posx, double* posy, double* posz, double* vx,
+                     double* vy, double* vz, void* stream);
+
 /* ---- initial conditions (no counterpart in the reference) -------------- */
 /* Zel'dovich ICs on a displaced nrow^3 particle lattice in an ng^3 mesh
  * (ng % nrow == 0): q = (i+1/2, j+1/2, k+1/2) * ng/nrow, psi_hat_a =
@@ -185,6 +199,17 @@ int pm_step(pm_plan_t* plan, double* posx, double* posy, double* posz, double* v
  * force evaluations: kick dt/2 first, dt between, dt/2 last). */
 int pm_run_kdk(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
                double* vz, int nsteps, double dt, int variant);
+/* Cosmological kick-drift-kick in the expansion factor a, from a0 in nsteps
+ * steps of da: the reference's kick p += g(a) f(a) da (update_particles,
+ * utils.cpp:141-170, f(a) of utils_inline.h:51-53, g(a) with the 1/a of
+ * apply_green(a), fftw/fft.cpp:75-82) with the drift it leaves commented
+ * out (utils.cpp:145-146), x += p f(a) da / a^2 at the half step.  The plan's
+ * Green's function is the one at a = 1 (e.g. PM_GREEN_REF_KH, scale 1); the
+ * run divides each kick by a.  nsteps + 1 force evaluations: kick weights
+ * 1/2, 1, ..., 1, 1/2.  nsteps = 0, da = 2 * 0.001, a0 = 0.9 is exactly the
+ * reference's single update_particles at Start_a (main.cpp:83). */
+int pm_run_cosmo(pm_plan_t* plan, double* posx, double* posy, double* posz, double* px, double* py,
+                 double* pz, double a0, double da, int nsteps, int variant);
 
 /* ---- 3. adjoint (no counterpart in the reference: it has no gradients) ---
  * Reverse mode of the fused step.  With F(x) the per-particle force of

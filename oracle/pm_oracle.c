@@ -627,3 +627,37 @@ int orc_run_kdk(int64_t np, double* posx, double* posy, double* posz, double* vx
     free(phi);
     return 0;
 }
+
+/* Cosmological KDK in the expansion factor a: the reference's kick
+ * p += g(a) f(a) da (update_particles, utils.cpp:141-170; f(a) of
+ * utils_inline.h:51-53; g(a) carries the 1/a of apply_green(a),
+ * fftw/fft.cpp:75-82) with the drift it leaves commented out
+ * (utils.cpp:145-146) as x += p f(a) da / a^2 at the half step.
+ * The mesh solve uses green_scale with a = 1; step s of nsteps + 1 force
+ * evaluations kicks by w_s da f(a_s) / a_s (w = 1/2 at both ends, 1
+ * between) and drifts by da f(a_{s+1/2}) / a_{s+1/2}^2 (not after the last). */
+int orc_run_cosmo(int64_t np, double* posx, double* posy, double* posz, double* px, double* py, double* pz,
+                  int ng, int nsteps, double a0, double da, int green_mode, double green_scale) {
+    if (!is_pow2(ng) || np <= 0 || !(a0 > 0.0)) return -1;
+    const size_t nc = (size_t)ng * ng * ng;
+    double* delta = (double*)malloc(sizeof(double) * nc);
+    double* phi = (double*)malloc(sizeof(double) * nc);
+    if (!delta || !phi) {
+        free(delta);
+        free(phi);
+        return -1;
+    }
+    const double mass = (double)nc / (double)np;
+    for (int s = 0; s <= nsteps; ++s) {
+        const double a = a0 + s * da, ah = a + 0.5 * da;
+        orc_deposit_periodic(np, posx, posy, posz, mass, ng, delta);
+        orc_solve(ng, green_mode, green_scale, delta, phi);
+        const double w = (s == 0 || s == nsteps) ? 0.5 : 1.0;
+        const double kick = w * da * orc_function_a(a) / a;
+        const double drift = (s == nsteps) ? 0.0 : da * orc_function_a(ah) / (ah * ah);
+        orc_interp_kick_drift(np, posx, posy, posz, px, py, pz, phi, ng, kick, drift);
+    }
+    free(delta);
+    free(phi);
+    return 0;
+}

@@ -95,6 +95,10 @@ void orc_interp_kick_drift(int64_t np, double* posx, double* posy, double* posz,
 int orc_run_kdk(int64_t np, double* posx, double* posy, double* posz,
                 double* vx, double* vy, double* vz, int ng, int nsteps, double dt,
                 int green_mode, double green_scale);
+/* Cosmological KDK in a (kick f(a) da / a, drift f(a) da / a^2 at the half
+ * step), nsteps + 1 force evaluations; green_scale for a = 1. */
+int orc_run_cosmo(int64_t np, double* posx, double* posy, double* posz, double* px, double* py, double* pz,
+                  int ng, int nsteps, double a0, double da, int green_mode, double green_scale);
 
 /* ---- threading ------------------------------------------------------- */
 int orc_max_threads(void);

@@ -62,6 +62,7 @@ def orc() -> ctypes.CDLL:
             "orc_interp_force": (None, [_I64, _P, _P, _P, _P, _I, _P, _P, _P]),
             "orc_interp_kick_drift": (None, [_I64, _P, _P, _P, _P, _P, _P, _P, _I, _D, _D]),
             "orc_run_kdk": (_I, [_I64, _P, _P, _P, _P, _P, _P, _I, _I, _D, _I, _D]),
+            "orc_run_cosmo": (_I, [_I64, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D, _I, _D]),
             "orc_max_threads": (_I, []),
             "orc_set_threads": (None, [_I]),
         }
@@ -188,6 +189,13 @@ def interp_kick_drift(pos, vel, phi: np.ndarray, kick: float, drift: float):
 def run_kdk(pos, vel, ng: int, nsteps: int, dt: float, mode: int, scale: float):
     """In place on the float64 arrays pos and vel."""
     rc = orc().orc_run_kdk(pos[0].size, *(_p(v) for v in pos), *(_p(v) for v in vel), ng, nsteps, dt, mode, scale)
+    assert rc == 0
+
+
+def run_cosmo(pos, mom, ng: int, nsteps: int, a0: float, da: float, mode: int, scale: float):
+    """In place on the float64 arrays pos and mom (cosmological KDK in a)."""
+    rc = orc().orc_run_cosmo(pos[0].size, *(_p(v) for v in pos), *(_p(v) for v in mom), ng, nsteps, a0, da, mode,
+                             scale)
     assert rc == 0
 
 

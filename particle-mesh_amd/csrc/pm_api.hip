@@ -419,6 +419,22 @@ extern "C" int pm_run_kdk(pm_plan_t* p, double* x, double* y, double* z, double*
     return PM_OK;
 }
 
+extern "C" int pm_run_cosmo(pm_plan_t* p, double* x, double* y, double* z, double* px, double* py, double* pz,
+                            double a0, double da, int nsteps, int variant) {
+    if (!p || nsteps < 0 || !(a0 > 0.0) || !(a0 + nsteps * da > 0.0))
+        return fail(PM_ERR_ARG, "pm_run_cosmo: bad arguments (a must stay positive)");
+    variant &= ~PM_STEP_REUSE_BINS;
+    for (int s = 0; s <= nsteps; ++s) {
+        const double a = a0 + s * da, ah = a + 0.5 * da;
+        const double w = (s == 0 || s == nsteps) ? 0.5 : 1.0;
+        const double kick = w * da * pm_function_a(a) / a;
+        const double drift = s == nsteps ? 0.0 : da * pm_function_a(ah) / (ah * ah);
+        const int rc = pm_step(p, x, y, z, px, py, pz, kick, drift, s == 0 ? variant : variant | PM_STEP_REUSE_BINS);
+        if (rc) return rc;
+    }
+    return PM_OK;
+}
+
 extern "C" int pm_plan_enable_timing(pm_plan_t* p, int max_steps) {
     if (!p || max_steps < 0) return fail(PM_ERR_ARG, "pm_plan_enable_timing: bad argument");
     collect(p);

@@ -78,6 +78,7 @@ SIGNATURES = {
     "pm_interp_force": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "pm_step": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl, _c_int]),
     "pm_run_kdk": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_dbl, _c_int]),
+    "pm_run_cosmo": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl, _c_int, _c_int]),
     "pm_adjoint_deposit": (_c_int, [_c_vp] * 8),
     "pm_interp_transpose": (_c_int, [_c_vp] * 7),
     "pm_adjoint_solve": (_c_int, [_c_vp]),
@@ -85,6 +86,10 @@ SIGNATURES = {
     "pm_adjoint_force": (_c_int, [_c_vp] * 10 + [_c_int]),
     "pm_adjoint_step": (_c_int, [_c_vp] * 13 + [_c_dbl, _c_dbl, _c_int]),
     "pm_run_adjoint": (_c_int, [_c_vp] * 13 + [_c_int, _c_dbl, _c_int]),
+    "pm_snapshot_write": (_c_int, [ctypes.c_char_p, _c_i64, _c_int, _c_int, _c_dbl] + [_c_vp] * 7),
+    "pm_snapshot_info": (_c_int, [ctypes.c_char_p, ctypes.POINTER(_c_i64), ctypes.POINTER(_c_int),
+                                  ctypes.POINTER(_c_int), ctypes.POINTER(_c_dbl)]),
+    "pm_snapshot_read": (_c_int, [ctypes.c_char_p, _c_i64] + [_c_vp] * 7),
     "pm_plan_enable_timing": (_c_int, [_c_vp, _c_int]),
     "pm_plan_phase_ms": (_c_int, [_c_vp, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_i64)]),
     "pm_plan_reset_timing": (_c_int, [_c_vp]),
@@ -222,6 +227,10 @@ class Plan:
     def run_kdk(self, pos, vel, nsteps: int, dt: float, variant: int = PM_DEPOSIT_ATOMIC):
         check(lib().pm_run_kdk(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel), nsteps, dt, variant))
 
+    def run_cosmo(self, pos, mom, a0: float, da: float, nsteps: int, variant: int = PM_DEPOSIT_SORTED):
+        """Cosmological KDK in the expansion factor (pm_run_cosmo)."""
+        check(lib().pm_run_cosmo(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in mom), a0, da, nsteps, variant))
+
     # ---------------------------------------------------------- adjoint
     def adjoint_deposit(self, pos, mesh_bar, xbar):
         """xbar += m grad_x sum_c W(x, c) mesh_bar[c] (transposed CIC)."""
@@ -320,3 +329,50 @@ def range_errors() -> int:
     if n < 0:
         check(int(n))
     return int(n)
+
+
+# ------------------------------------------------------------- snapshots
+SNAP_MAGIC = b"PMSNAP01"
+SNAP_HEADER = 64
+
+
+def write_snapshot(path, pos, vel, ng: int, step: int = 0, a: float = 1.0, stream: int | None = None):
+    """pm_snapshot_write of (3, np) device arrays (x, y, z and vx, vy, vz)."""
+    np_ = pos[0].numel()
+    st = current_stream() if stream is None else stream
+    check(lib().pm_snapshot_write(str(path).encode(), np_, ng, step, a, *(dptr(t) for t in pos),
+                                  *(dptr(t) for t in vel), st))
+
+
+def snapshot_info(path):
+    """(np, ng, step, a) from a snapshot header."""
+    n, g, s, a = _c_i64(), _c_int(), _c_int(), _c_dbl()
+    check(lib().pm_snapshot_info(str(path).encode(), ctypes.byref(n), ctypes.byref(g), ctypes.byref(s),
+                                 ctypes.byref(a)))
+    return n.value, g.value, s.value, a.value
+
+
+def read_snapshot(path, pos, vel, stream: int | None = None):
+    """pm_snapshot_read into (3, np) device arrays."""
+    st = current_stream() if stream is None else stream
+    check(lib().pm_snapshot_read(str(path).encode(), pos[0].numel(), *(dptr(t) for t in pos),
+                                 *(dptr(t) for t in vel), st))
+
+
+def load_snapshot_numpy(path):
+    """Host-side reader of the same format (numpy only, no GPU): returns
+    (header dict, positions (3, np), velocities (3, np))."""
+    import numpy as np
+
+    with open(path, "rb") as f:
+        head = f.read(SNAP_HEADER)
+    if len(head) != SNAP_HEADER or head[:8] != SNAP_MAGIC:
+        raise ValueError(f"{path}: not a PM snapshot")
+    version, hsize = np.frombuffer(head[8:16], "<u4")
+    if version != 1 or hsize != SNAP_HEADER:
+        raise ValueError(f"{path}: unsupported snapshot version {version}")
+    n = int(np.frombuffer(head[16:24], "<i8")[0])
+    ng, step = (int(v) for v in np.frombuffer(head[24:32], "<i4"))
+    a = float(np.frombuffer(head[32:40], "<f8")[0])
+    data = np.fromfile(path, dtype="<f8", offset=SNAP_HEADER, count=6 * n).reshape(6, n)
+    return {"np": n, "ng": ng, "step": step, "a": a}, data[:3], data[3:]

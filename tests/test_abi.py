@@ -67,3 +67,39 @@ def test_null_arguments_rejected():
     assert lib.pm_update_particles(None, None, None, None, None, None, None, None, None, 0.9, 10, 5, 5, 5, 0, 0, 0,
                                    None) == pmhip.PM_ERR_ARG
     assert lib.pm_init_buf(None, 5, 5, 5, None) == pmhip.PM_ERR_ARG
+
+
+def _write_snapshot_numpy(path, n, ng, step, a, magic=b"PMSNAP01", version=1):
+    import numpy as np
+
+    head = bytearray(64)
+    head[0:8] = magic
+    head[8:16] = np.array([version, 64], "<u4").tobytes()
+    head[16:24] = np.array([n], "<i8").tobytes()
+    head[24:32] = np.array([ng, step], "<i4").tobytes()
+    head[32:40] = np.array([a], "<f8").tobytes()
+    data = np.arange(6 * n, dtype="<f8")
+    with open(path, "wb") as f:
+        f.write(bytes(head))
+        f.write(data.tobytes())
+    return data.reshape(6, n)
+
+
+def test_snapshot_header_parsed_by_library_and_numpy(tmp_path):
+    """pm_snapshot_info reads only the file (no GPU): it must decode a header
+    written independently with numpy, agree with the numpy loader, and
+    reject a wrong magic or version."""
+    p = tmp_path / "snap.bin"
+    data = _write_snapshot_numpy(p, 1000, 64, 7, 0.625)
+    assert pmhip.snapshot_info(p) == (1000, 64, 7, 0.625)
+    head, pos, vel = pmhip.load_snapshot_numpy(p)
+    assert head == {"np": 1000, "ng": 64, "step": 7, "a": 0.625}
+    assert (pos == data[:3]).all() and (vel == data[3:]).all()
+    for kw in ({"magic": b"PMSNAP02"}, {"version": 2}):
+        q = tmp_path / "bad.bin"
+        _write_snapshot_numpy(q, 10, 8, 0, 1.0, **kw)
+        with pytest.raises(pmhip.PMError) as e:
+            pmhip.snapshot_info(q)
+        assert e.value.code == pmhip.PM_ERR_ARG
+    with pytest.raises(pmhip.PMError):
+        pmhip.snapshot_info(tmp_path / "missing.bin")

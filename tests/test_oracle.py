@@ -234,3 +234,49 @@ def test_interp_is_deposit_transpose():
 def test_function_a():
     a = 0.9
     assert abs(pyoracle.function_a(a) - 1.0 / np.sqrt(0.3 / a + 0.7 * a * a)) < 1e-15
+
+
+# ------------------------------------------------- cosmological time stepping
+@pytest.mark.parametrize("name", ["ref_kick_ng16_np4096", "ref_kick_ng32_np4096"])
+def test_cosmo_single_kick_is_reference_kick(golden_dir, name):
+    """A cosmological run of zero steps from a = Start_a with da = 2 Da is
+    one half kick of da f(a) / a with the a = 1 Green's function: exactly the
+    reference's update_particles at Start_a (the fixture's momenta)."""
+    f = np.load(golden_dir / f"{name}.npz")
+    ng = int(f["ng"])
+    pos = [f["posx"].copy(), f["posy"].copy(), f["posz"].copy()]
+    p = [np.zeros(pos[0].size) for _ in range(3)]
+    pyoracle.run_cosmo(pos, p, ng, 0, float(f["a"]), 2 * 0.001, pyoracle.GREEN_REF_KH, 1.0)
+    for a, key in zip(p, ("px", "py", "pz")):
+        assert rel(a, f[key]) < 1e-12
+    for a, key in zip(pos, ("posx", "posy", "posz")):
+        assert np.array_equal(a, f[key])  # no drift after the last kick
+
+
+def test_cosmo_free_streaming_drift_factor():
+    """A uniform lattice feels no force, so each particle drifts by
+    p * sum_s da f(a_{s+1/2}) / a_{s+1/2}^2: the drift factors of the run."""
+    ng, n, a0, da = 8, 5, 0.5, 0.05
+    idx = np.arange(ng, dtype=np.float64)
+    X, Y, Z = np.meshgrid(idx, idx, idx, indexing="ij")
+    pos = [X.ravel().copy(), Y.ravel().copy(), Z.ravel().copy()]
+    p = [np.full(ng ** 3, 0.3), np.zeros(ng ** 3), np.full(ng ** 3, -0.2)]
+    pyoracle.run_cosmo(pos, p, ng, n, a0, da, pyoracle.GREEN_REF_KH, 1.0)
+    ah = a0 + (np.arange(n) + 0.5) * da
+    d = np.sum(da / np.sqrt(0.3 / ah + 0.7 * ah ** 2) / ah ** 2)
+    assert np.allclose(pos[0], np.mod(X.ravel() + 0.3 * d, ng), atol=1e-12)
+    assert np.allclose(pos[2], np.mod(Z.ravel() - 0.2 * d, ng), atol=1e-12)
+    assert np.allclose(p[0], 0.3, atol=1e-12)
+
+
+def test_cosmo_zero_step_is_identity():
+    """With da = 0 every kick and drift factor vanishes: nothing moves."""
+    ng, npart = 16, 2000
+    rng = np.random.default_rng(17)
+    pos = [rng.uniform(0, ng, npart) for _ in range(3)]
+    p = [np.zeros(npart) for _ in range(3)]
+    pc = [v.copy() for v in pos]
+    pyoracle.run_cosmo(pc, p, ng, 3, 0.9, 0.0, pyoracle.GREEN_REF_KH, 1.0)
+    for a, b in zip(pc, pos):
+        assert np.array_equal(a, b)
+    assert max(np.abs(v).max() for v in p) == 0.0
