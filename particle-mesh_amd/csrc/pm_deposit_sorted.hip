@@ -177,7 +177,6 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t* __restrict_
     }
 }
 
-constexpr int kDepBatch = 4;  // particles in flight per thread in the tile kernels
 
 // In-tile offsets of the FP32 path: u32 fixed point with 2^-28 cell
 // resolution (cell = q >> 28, fraction = low 28 bits).  The staged deposit
@@ -188,61 +187,6 @@ constexpr double kQScale = 268435456.0;  // 2^28
 constexpr uint32_t kQMask = 0x0FFFFFFFu;
 
 __device__ __forceinline__ uint32_t quantize_offset(double local) { return (uint32_t)(local * kQScale); }
-
-// CIC weights (cic_deposit, utils_inline.h:25-49) of one particle in local
-// cell (a, b, c) with fractional offsets (fx, fy, fz), added in 2^-32 fixed
-// point to the (TS+1)^3 LDS node tile.
-template <int TS>
-__device__ __forceinline__ void tile_cic_add(unsigned long long* acc, int a, int b, int c, double fx, double fy,
-                                             double fz, double fix) {
-    constexpr int E = TS + 1;
-    const double sx0 = 1.0 - fx, sy0 = 1.0 - fy;
-    const double sxy00 = sx0 * sy0, sxy10 = fx * sy0, sxy01 = sx0 * fy, sxy11 = fx * fy;
-    const double m0 = fix * (1.0 - fz), m1 = fix * fz;
-    unsigned long long* n0 = acc + (a * E + b) * E + c;
-    atomicAdd(n0, (unsigned long long)__double2ll_rn(sxy00 * m0));
-    atomicAdd(n0 + E * E, (unsigned long long)__double2ll_rn(sxy10 * m0));
-    atomicAdd(n0 + E, (unsigned long long)__double2ll_rn(sxy01 * m0));
-    atomicAdd(n0 + E * E + E, (unsigned long long)__double2ll_rn(sxy11 * m0));
-    atomicAdd(n0 + 1, (unsigned long long)__double2ll_rn(sxy00 * m1));
-    atomicAdd(n0 + E * E + 1, (unsigned long long)__double2ll_rn(sxy10 * m1));
-    atomicAdd(n0 + E + 1, (unsigned long long)__double2ll_rn(sxy01 * m1));
-    atomicAdd(n0 + E * E + E + 1, (unsigned long long)__double2ll_rn(sxy11 * m1));
-}
-
-// The TS^3 owned nodes of tile t go to the mesh, the upper shell to the
-// tile's shell buffer (after a barrier on the accumulation).
-template <typename Rl, int TS>
-__device__ __forceinline__ void tile_write(const unsigned long long* acc, uint32_t t, int ox, int oy, int oz,
-                                           int ng, Rl* __restrict__ rho, Rl* __restrict__ shell) {
-    constexpr int E = TS + 1, SH = shell_size<TS>();
-    const double unfix = 1.0 / 4294967296.0;
-    const size_t N = (size_t)ng;
-    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
-        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
-        rho[((size_t)(ox + a) * N + (oy + b)) * N + (oz + c)] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
-    }
-    Rl* sh = shell + (size_t)t * SH;
-    for (int i = threadIdx.x; i < SH; i += 256) {
-        int a, b, c;
-        if (i < E * E) {
-            a = TS;
-            b = i / E;
-            c = i % E;
-        } else if (i < E * E + TS * E) {
-            const int j = i - E * E;
-            a = j / E;
-            b = TS;
-            c = j % E;
-        } else {
-            const int j = i - E * E - TS * E;
-            a = j / TS;
-            b = j % TS;
-            c = TS;
-        }
-        sh[i] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
-    }
-}
 
 // 4. one workgroup per tile, particles by index (binning of positions).
 template <typename Rl, int TS>

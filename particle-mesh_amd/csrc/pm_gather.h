@@ -47,13 +47,13 @@ template <typename Rl> struct CellIdx {
     Rl tx, ty, tz;
 };
 
+// From an integer cell and the offsets inside it.
 template <typename Rl>
-__device__ __forceinline__ CellIdx<Rl> cell_of(double x, double y, double z, int N) {
+__device__ __forceinline__ CellIdx<Rl> cell_at(int ix, int iy, int iz, Rl tx, Rl ty, Rl tz, int N) {
     CellIdx<Rl> c;
-    const int ix = (int)x, iy = (int)y, iz = (int)z;
-    c.tx = (Rl)(x - ix);
-    c.ty = (Rl)(y - iy);
-    c.tz = (Rl)(z - iz);
+    c.tx = tx;
+    c.ty = ty;
+    c.tz = tz;
     const uint32_t n = (uint32_t)N;
     c.X[0] = (ix == 0 ? n - 1 : ix - 1) * n * n;
     c.X[1] = (uint32_t)ix * n * n;
@@ -65,6 +65,13 @@ __device__ __forceinline__ CellIdx<Rl> cell_of(double x, double y, double z, int
     c.Y[3] = (iy + 2 >= N ? iy + 2 - N : iy + 2) * n;
     c.iz = iz;
     return c;
+}
+
+// From an FP64 position (cell by truncation, utils.cpp:79-88).
+template <typename Rl>
+__device__ __forceinline__ CellIdx<Rl> cell_of(double x, double y, double z, int N) {
+    const int ix = (int)x, iy = (int)y, iz = (int)z;
+    return cell_at<Rl>(ix, iy, iz, (Rl)(x - ix), (Rl)(y - iy), (Rl)(z - iz), N);
 }
 
 // The 32 potential values that the centred differences at the 8 corners of
@@ -163,16 +170,24 @@ __device__ __forceinline__ Rl trilerp(const Rl (&f)[2][2][2], Rl tx, Rl ty, Rl t
 // Centred-difference force of the periodic potential, trilinearly
 // interpolated at a particle (centered_derivatives + trilerp1).
 template <typename Rl>
-__device__ __forceinline__ void cic_force(const Rl* __restrict__ phi, int N, double x, double y, double z,
-                                          double (&F)[3]) {
-    const CellIdx<Rl> c = cell_of<Rl>(x, y, z, N);
+__device__ __forceinline__ void force_at(const Rl* __restrict__ phi, int N, const CellIdx<Rl>& c, Rl (&F)[3]) {
     Rl cl[2][2][4], xr[2][2][2], yr[2][2][2];
     gather_runs(phi, N, c, cl, xr, yr);
     Rl gx[2][2][2], gy[2][2][2], gz[2][2][2];
     corner_differences(cl, xr, yr, gx, gy, gz);
-    F[0] = (double)trilerp(gx, c.tx, c.ty, c.tz);
-    F[1] = (double)trilerp(gy, c.tx, c.ty, c.tz);
-    F[2] = (double)trilerp(gz, c.tx, c.ty, c.tz);
+    F[0] = trilerp(gx, c.tx, c.ty, c.tz);
+    F[1] = trilerp(gy, c.tx, c.ty, c.tz);
+    F[2] = trilerp(gz, c.tx, c.ty, c.tz);
+}
+
+template <typename Rl>
+__device__ __forceinline__ void cic_force(const Rl* __restrict__ phi, int N, double x, double y, double z,
+                                          double (&F)[3]) {
+    Rl f[3];
+    force_at(phi, N, cell_of<Rl>(x, y, z, N), f);
+    F[0] = (double)f[0];
+    F[1] = (double)f[1];
+    F[2] = (double)f[2];
 }
 
 }  // namespace pm

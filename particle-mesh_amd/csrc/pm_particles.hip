@@ -1,8 +1,11 @@
 // Particle kernels of the fused one-box step: CIC deposit (atomic variant)
 // and interpolation of the centred-difference force fused with kick and
 // drift.  Positions and velocities are FP64; the mesh is FP32 or FP64.
+#include <type_traits>
+
 #include "pm_gather.h"
 #include "pm_internal.h"
+#include "pm_tiles.h"
 
 namespace pm {
 
@@ -76,140 +79,112 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
     return x;
 }
 
+// Shape of the staging interpolation (MODE 2): 256 threads with two
+// particles each, whose loads and gathers are in flight together (measured
+// against one particle per thread and against 512 / 1024 threads), and an
+// LDS hash of 128 tiles per pass.
+#ifndef PM_STAGE_PER
+#define PM_STAGE_PER 2
+#endif
+constexpr int kStageThreads = 256;
+constexpr int kStagePer = PM_STAGE_PER;
+constexpr int kStageHash = 128;
+
+// One particle of k_interp: returns its next tile (kNo for none) and, in
+// MODE 2, its staged record.
+template <typename Rl, int MODE, typename Rec>
+__device__ __forceinline__ uint32_t interp_one(int64_t i, double* __restrict__ px, double* __restrict__ py,
+                                               double* __restrict__ pz, double* __restrict__ vx,
+                                               double* __restrict__ vy, double* __restrict__ vz, int ng,
+                                               const Rl* __restrict__ phi, double kick, double drift, int ts,
+                                               unsigned long long* __restrict__ err, Rec& rec) {
+    constexpr uint32_t kNo = 0xFFFFFFFFu;
+    const double box = (double)ng;
+    double x = px[i], y = py[i], z = pz[i];
+    if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
+        atomicAdd(err, 1ULL);
+        return kNo;
+    }
+    double F[3];
+    cic_force(phi, ng, x, y, z, F);
+    if constexpr (MODE == 1) {
+        vx[i] = F[0];
+        vy[i] = F[1];
+        vz[i] = F[2];
+        return kNo;
+    } else {
+        const double nvx = vx[i] + F[0] * kick, nvy = vy[i] + F[1] * kick, nvz = vz[i] + F[2] * kick;
+        vx[i] = nvx;
+        vy[i] = nvy;
+        vz[i] = nvz;
+        if (drift != 0.0) {
+            x = wrap_pos(x + nvx * drift, box);
+            y = wrap_pos(y + nvy * drift, box);
+            z = wrap_pos(z + nvz * drift, box);
+            px[i] = x;
+            py[i] = y;
+            pz[i] = z;
+        }
+        if constexpr (MODE != 2) {
+            return kNo;
+        } else {
+            // a particle that drifted more than a box away stays unstaged and
+            // is counted, as the next step's range check would
+            if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
+                atomicAdd(err, 1ULL);
+                return kNo;
+            }
+            const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;  // ts: power of two
+            // in-tile offsets: u32 at 2^-28 cells (FP32 mesh) or exact doubles
+            const int msk = ~(ts - 1);  // tile origin = cell with the low bits cleared
+            const double lx = x - ((int)x & msk), ly = y - ((int)y & msk), lz = z - ((int)z & msk);
+            if constexpr (sizeof(Rl) == 4)
+                rec = StageQ{(uint32_t)(lx * 268435456.0), (uint32_t)(ly * 268435456.0),
+                             (uint32_t)(lz * 268435456.0)};
+            else
+                rec = StageD{lx, ly, lz};
+            return (uint32_t)((((int)x >> lt) * nt + ((int)y >> lt)) * nt + ((int)z >> lt));
+        }
+    }
+}
+
 // update_particles (utils.cpp:141-170) with the force of cic_force, then the
-// drift that the reference leaves commented out (utils.cpp:145-146).
+// drift that the reference leaves commented out (utils.cpp:145-146), for
+// PER particles per thread (NT apart: their loads are in flight together).
 // MODE 0: v += g*kick, x += v*drift.  MODE 1: write g to (vx, vy, vz).
 // MODE 2: as MODE 0, then the drifted position is staged for the next
-// step's deposit: its tile's bucket gets a slot (fill counter), and the
-// in-tile offsets (FP32 mesh: u32 at 2^-28 cells; FP64 mesh: exact doubles)
-// go to the staging arrays at coff[tile] + slot; a slot past the bucket's
-// capacity raises *spill.
-// Workgroup of the staging interpolation: bigger workgroups give longer
-// runs of one tile's slots (the buckets are written in longer pieces).
-#ifndef PM_STAGE_THREADS
-#define PM_STAGE_THREADS 256
-#endif
-constexpr int kStageThreads = PM_STAGE_THREADS;
-constexpr int kStageHash = 2 * kStageThreads / 4;
-
-template <typename Rl, int MODE, int NT = kPartThreads>
+// step's deposit (stage_commit).
+template <typename Rl, int MODE, int NT, int PER>
 __global__ __launch_bounds__(NT) void k_interp(double* __restrict__ px, double* __restrict__ py,
-                                                         double* __restrict__ pz, double* __restrict__ vx,
-                                                         double* __restrict__ vy, double* __restrict__ vz,
-                                                         int64_t np, int ng, const Rl* __restrict__ phi,
-                                                         double kick, double drift, int ts,
-                                                         uint32_t* __restrict__ fill,
-                                                         const uint32_t* __restrict__ coff,
-                                                         uint32_t* __restrict__ spill, void* __restrict__ stage,
-                                                         unsigned long long* __restrict__ err) {
+                                               double* __restrict__ pz, double* __restrict__ vx,
+                                               double* __restrict__ vy, double* __restrict__ vz, int64_t np,
+                                               int ng, const Rl* __restrict__ phi, double kick, double drift, int ts,
+                                               uint32_t* __restrict__ fill, const uint32_t* __restrict__ coff,
+                                               uint32_t* __restrict__ spill, void* __restrict__ stage,
+                                               unsigned long long* __restrict__ err) {
     constexpr uint32_t kNo = 0xFFFFFFFFu;
+    using Rec = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
     constexpr int HS = MODE == 2 ? kStageHash : 1;
     static_assert((HS & (HS - 1)) == 0, "hash size must be a power of two");
     __shared__ uint32_t hkey[HS], hcnt[HS], hbase[HS];
-    if constexpr (MODE == 2) {
-        for (int s = threadIdx.x; s < HS; s += NT) {
-            hkey[s] = kNo;
-            hcnt[s] = 0u;
-        }
-        __syncthreads();
-    }
-    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * NT + threadIdx.x;
-    const double box = (double)ng;
-    bool ok = i < np;
-    double x = 0.0, y = 0.0, z = 0.0;
-    uint32_t kk = kNo;  // MODE 2: the drifted position's tile
-    if (ok) {
-        x = px[i];
-        y = py[i];
-        z = pz[i];
-        if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
-            atomicAdd(err, 1ULL);
-            ok = false;
-        }
-    }
-    if (ok) {
-        double F[3];
-        cic_force(phi, ng, x, y, z, F);
-        if constexpr (MODE == 1) {
-            vx[i] = F[0];
-            vy[i] = F[1];
-            vz[i] = F[2];
-        } else {
-            const double nvx = vx[i] + F[0] * kick, nvy = vy[i] + F[1] * kick,
-                         nvz = vz[i] + F[2] * kick;
-            vx[i] = nvx;
-            vy[i] = nvy;
-            vz[i] = nvz;
-            if (drift != 0.0) {
-                x = wrap_pos(x + nvx * drift, box);
-                y = wrap_pos(y + nvy * drift, box);
-                z = wrap_pos(z + nvz * drift, box);
-                px[i] = x;
-                py[i] = y;
-                pz[i] = z;
-            }
-            if constexpr (MODE == 2) {
-                // a particle that drifted more than a box away stays unstaged
-                // and is counted, as the next step's range check would
-                if (in_box(x, box) && in_box(y, box) && in_box(z, box)) {
-                    const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;  // ts: power of two
-                    kk = (uint32_t)((((int)x >> lt) * nt + ((int)y >> lt)) * nt + ((int)z >> lt));
-                } else {
-                    atomicAdd(err, 1ULL);
-                }
-            }
-        }
+    const int64_t i0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * (NT * PER) + threadIdx.x;
+    uint32_t kk[PER];
+    Rec rec[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int64_t i = i0 + (int64_t)q * NT;
+        kk[q] = i < np ? interp_one<Rl, MODE>(i, px, py, pz, vx, vy, vz, ng, phi, kick, drift, ts, err, rec[q]) : kNo;
     }
     if constexpr (MODE == 2) {
-        // Runs of equal tiles in consecutive lanes (particles in a coherent
-        // order share tiles) reserve their slots together: the first lane of
-        // a run takes run-length slots, first within the workgroup (LDS
-        // hash), then one global fill atomic per distinct tile.
-        const uint32_t lane = __lane_id();
-        const uint32_t prev = __shfl_up(kk, 1);
-        const bool head = kk != kNo && (lane == 0 || prev != kk);
-        const unsigned long long bounds = __ballot(head || kk == kNo);
-        uint32_t hs = kNo, lrank = 0u, gb = 0u;
-        if (head) {
-            const unsigned long long above = lane == 63 ? 0ull : (bounds >> (lane + 1)) << (lane + 1);
-            const uint32_t run = (above ? (uint32_t)__ffsll((long long)above) - 1u : 64u) - lane;
-            uint32_t s = (kk * 2654435761u) >> (32 - ilog2(HS));
-            for (int probe = 0; probe < HS; ++probe) {
-                const uint32_t old = atomicCAS(&hkey[s], kNo, kk);
-                if (old == kNo || old == kk) {
-                    hs = s;
-                    lrank = atomicAdd(&hcnt[s], run);
-                    break;
-                }
-                s = (s + 1) & (HS - 1);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            for (int s = threadIdx.x; s < HS; s += NT) {
+                hkey[s] = kNo;
+                hcnt[s] = 0u;
             }
-            if (hs == kNo) gb = atomicAdd(&fill[kk], run);  // hash full: reserve directly
-        }
-        __syncthreads();
-        for (int s = threadIdx.x; s < HS; s += NT)
-            if (hkey[s] != kNo) hbase[s] = atomicAdd(&fill[hkey[s]], hcnt[s]);
-        __syncthreads();
-        if (hs != kNo) gb = hbase[hs] + lrank;
-        // every lane of a run takes its head's base plus its distance
-        const unsigned long long upto = bounds & (lane == 63 ? ~0ull : (2ull << lane) - 1ull);
-        const int hl = upto ? 63 - __clzll((long long)upto) : 0;
-        const uint32_t base = __shfl(gb, hl);
-        if (kk != kNo) {
-            const uint32_t slot = base + (lane - (uint32_t)hl);
-            const uint32_t c0 = coff[kk];
-            if (slot < coff[kk + 1] - c0) {
-                const int msk = ~(ts - 1);  // tile origin = cell with the low bits cleared
-                const size_t q = (size_t)c0 + slot;
-                const double lx = x - ((int)x & msk), ly = y - ((int)y & msk), lz = z - ((int)z & msk);
-                if constexpr (sizeof(Rl) == 4)  // u32, 2^-28 cells
-                    static_cast<StageQ*>(stage)[q] = StageQ{(uint32_t)(lx * 268435456.0),
-                                                            (uint32_t)(ly * 268435456.0),
-                                                            (uint32_t)(lz * 268435456.0)};
-                else  // exact FP64 offsets
-                    static_cast<StageD*>(stage)[q] = StageD{lx, ly, lz};
-            } else {
-                *spill = 1u;
-            }
+            __syncthreads();
+            stage_commit<HS, NT>(kk[q], rec[q], hkey, hcnt, hbase, fill, coff, spill, static_cast<Rec*>(stage));
+            __syncthreads();
         }
     }
 }
@@ -219,31 +194,31 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
                   unsigned long long* err, hipStream_t st, SortedScratch* s) {
     if (np <= 0) return PM_OK;
     if (mode == 2 && (!s || !s->stage)) return fail(PM_ERR_ARG, "interp: staging buffers missing");
-    const int nt = mode == 2 ? kStageThreads : kPartThreads;
-    const unsigned nb = (unsigned)((np + nt - 1) / nt);
+    const int nt = mode == 2 ? kStageThreads : kPartThreads, per = mode == 2 ? kStagePer : 1;
+    const unsigned nb = (unsigned)((np + (int64_t)nt * per - 1) / ((int64_t)nt * per));
     const dim3 g(nb), b(nt);
     const int ts = sorted_tile(ng);
     uint32_t* fill = s ? s->fill : nullptr;
     const uint32_t* coff = s ? s->coff : nullptr;
     uint32_t* spill = s ? s->spill : nullptr;
     void* stage = s ? s->stage : nullptr;
-#define PM_INTERP(RL, M, ...)                                                                          \
-    hipLaunchKernelGGL((k_interp<RL, M, ##__VA_ARGS__>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng,         \
+#define PM_INTERP(RL, M, NTV, PERV)                                                                          \
+    hipLaunchKernelGGL((k_interp<RL, M, NTV, PERV>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng,          \
                        (const RL*)phi, kick, drift, ts, fill, coff, spill, stage, err)
     if (prec == PM_FP32) {
         if (mode == 0)
-            PM_INTERP(float, 0);
+            PM_INTERP(float, 0, kPartThreads, 1);
         else if (mode == 1)
-            PM_INTERP(float, 1);
+            PM_INTERP(float, 1, kPartThreads, 1);
         else
-            PM_INTERP(float, 2, kStageThreads);
+            PM_INTERP(float, 2, kStageThreads, kStagePer);
     } else {
         if (mode == 0)
-            PM_INTERP(double, 0);
+            PM_INTERP(double, 0, kPartThreads, 1);
         else if (mode == 1)
-            PM_INTERP(double, 1);
+            PM_INTERP(double, 1, kPartThreads, 1);
         else
-            PM_INTERP(double, 2, kStageThreads);
+            PM_INTERP(double, 2, kStageThreads, kStagePer);
     }
 #undef PM_INTERP
     return check_launch("interp");
