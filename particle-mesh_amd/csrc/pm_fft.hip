@@ -159,8 +159,19 @@ __device__ __forceinline__ void fft_lines(C* s, const C* __restrict__ tw) {
 
 // Lines per workgroup: 128-byte global segments (16 float2 / 8 double2),
 // halved at N >= 1024 to keep the LDS tile at 64 KB.
-template <typename Rl> __host__ __device__ constexpr int tile_lines(int N) {
-    const int base = sizeof(Rl) == 4 ? 16 : 8;
+#ifndef PM_FFT_LINES32
+#define PM_FFT_LINES32 16
+#endif
+#ifndef PM_FFT_LINES64
+#define PM_FFT_LINES64 8
+#endif
+// The fused x pass (contiguous lines on both sides) prefers half the lines:
+// a smaller LDS tile, more workgroups per CU.
+#ifndef PM_FFT_XLINES32
+#define PM_FFT_XLINES32 8
+#endif
+template <typename Rl, bool XPASS = false> __host__ __device__ constexpr int tile_lines(int N) {
+    const int base = sizeof(Rl) == 4 ? (XPASS ? PM_FFT_XLINES32 : PM_FFT_LINES32) : PM_FFT_LINES64;
     const int t = N >= 1024 ? base / 2 : base;
     return t < N ? t : N;
 }
@@ -243,7 +254,7 @@ __global__ __launch_bounds__(NT) void k_x(cplx_t<Rl>* __restrict__ data,
                                                    const cplx_t<Rl>* __restrict__ twN,
                                                    GreenArgs ga) {
     using C = cplx_t<Rl>;
-    constexpr int T = tile_lines<Rl>(N), LP = lds_pitch(N), YT = N / T;
+    constexpr int T = tile_lines<Rl, true>(N), LP = lds_pitch(N), YT = N / T;
     __shared__ C s[T * LP];
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / YT, ky0 = (tile % YT) * T;
@@ -361,13 +372,14 @@ template <typename Rl, int N>
 static void launch_pass(int pass, hipStream_t st, const void* in, void* out, const void* twM,
                         const void* twN, const GreenArgs& ga, double norm, bool sub1) {
     using C = cplx_t<Rl>;
-    constexpr int T = tile_lines<Rl>(N);
+    constexpr int T = tile_lines<Rl>(N), TX = tile_lines<Rl, true>(N);
     constexpr int M = N / 2;
     constexpr int NZF = fft_threads(PM_FFT_NT_ZF, T * M), NYF = fft_threads(PM_FFT_NT_YF, T * N);
-    constexpr int NX = fft_threads(PM_FFT_NT_X, T * N), NYI = fft_threads(PM_FFT_NT_YI, T * N);
+    constexpr int NX = fft_threads(PM_FFT_NT_X, TX * N), NYI = fft_threads(PM_FFT_NT_YI, T * N);
     constexpr int NZI = fft_threads(PM_FFT_NT_ZI, T * M);
     const unsigned zt = (unsigned)((size_t)N * N / T);            // z passes: (x, y-tile)
-    const unsigned yt = (unsigned)((size_t)(M + 1) * N / T);      // y / x passes
+    const unsigned yt = (unsigned)((size_t)(M + 1) * N / T);      // y passes
+    const unsigned xt = (unsigned)((size_t)(M + 1) * N / TX);     // x pass
     switch (pass) {
         case kPassZFwd:
             if (sub1)
@@ -382,14 +394,14 @@ static void launch_pass(int pass, hipStream_t st, const void* in, void* out, con
                                (const C*)twN);
             break;
         case kPassXFused:
-            hipLaunchKernelGGL((k_x<Rl, N, kXForward | kXGreen | kXInverse, NX>), dim3(yt), dim3(NX), 0, st,
+            hipLaunchKernelGGL((k_x<Rl, N, kXForward | kXGreen | kXInverse, NX>), dim3(xt), dim3(NX), 0, st,
                                (C*)out, (const C*)twN, ga);
             break;
         case kPassXFwd:
-            hipLaunchKernelGGL((k_x<Rl, N, kXForward, NX>), dim3(yt), dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
+            hipLaunchKernelGGL((k_x<Rl, N, kXForward, NX>), dim3(xt), dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
             break;
         case kPassXInv:
-            hipLaunchKernelGGL((k_x<Rl, N, kXInverse, NX>), dim3(yt), dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
+            hipLaunchKernelGGL((k_x<Rl, N, kXInverse, NX>), dim3(xt), dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
             break;
         case kPassYInv:
             hipLaunchKernelGGL((k_yinv<Rl, N, NYI>), dim3(yt), dim3(NYI), 0, st, (const C*)in, (C*)out,
