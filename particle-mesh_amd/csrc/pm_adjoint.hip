@@ -15,10 +15,9 @@
 // slope along the axis and the linear weights along the other two.
 #include "pm_gather.h"
 #include "pm_internal.h"
+#include "pm_tiles.h"
 
 namespace pm {
-
-__device__ __forceinline__ bool in_box_a(double x, double box) { return x >= 0.0 && x < box; }
 
 // ---------------------------------------------------------------- helpers
 template <typename Rl> struct Corner {
@@ -100,7 +99,7 @@ __global__ __launch_bounds__(256) void k_adj_deposit(const double* __restrict__ 
     if (i >= np) return;
     const double x = px[i], y = py[i], z = pz[i];
     const double box = (double)N;
-    if (!(in_box_a(x, box) && in_box_a(y, box) && in_box_a(z, box))) {
+    if (!in_box3(x, y, z, box)) {
         atomicAdd(err, 1ULL);
         return;
     }
@@ -145,90 +144,64 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
     const double* __restrict__ ux, const double* __restrict__ uy, const double* __restrict__ uz,
     const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off, int ng,
     const unsigned long long* __restrict__ umax, Rl* __restrict__ U0, Rl* __restrict__ U1, Rl* __restrict__ U2,
-    Rl* __restrict__ shell) {
-    constexpr int E = TS + 1, E3 = E * E * E, SH = (TS + 1) * (TS + 1) * (TS + 1) - TS * TS * TS;
+    Rl* __restrict__ shell, size_t shell_stride) {
+    constexpr int E = TS + 1, E3 = E * E * E;
     constexpr int NT = kVec3Threads;
     __shared__ unsigned long long acc[3][E3];
-    const int nt = ng / TS;
-    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
-    const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
-              tz = (int)(t % (uint32_t)nt);
-    const int ox = tx * TS, oy = ty * TS, oz = tz * TS;
+    const WalkSeg sw = walk_segment(ng, TS);
     const double sc[3] = {vec3_scale(umax[0]), vec3_scale(umax[1]), vec3_scale(umax[2])};
+    const double inv[3] = {1.0 / sc[0], 1.0 / sc[1], 1.0 / sc[2]};
+    Rl* const out[3] = {U0, U1, U2};
+    Rl* const sh[3] = {shell, shell + shell_stride, shell + 2 * shell_stride};
     for (int i = threadIdx.x; i < 3 * E3; i += NT) (&acc[0][0])[i] = 0ULL;
     __syncthreads();
-    const uint32_t beg = off[t], end = off[t + 1];
-    constexpr int B = 2;  // particles in flight per thread
-    for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += NT * B) {
-        uint32_t p[B];
-        double x[B], y[B], z[B], u[B][3];
+    const int ox = sw.tx * TS, oy = sw.ty * TS;
+    for (int k = 0; k < sw.seg; ++k) {
+        const int tz = sw.sg * sw.seg + k, oz = tz * TS;
+        const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;
+        const uint32_t beg = off[t], end = off[t + 1];
+        constexpr int B = 2;  // particles in flight per thread
+        for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += NT * B) {
+            uint32_t p[B];
+            double x[B], y[B], z[B], u[B][3];
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            const uint32_t q = q0 + NT * j;
-            p[j] = q < end ? idx[q] : 0xFFFFFFFFu;
-        }
+            for (int j = 0; j < B; ++j) {
+                const uint32_t q = q0 + NT * j;
+                p[j] = q < end ? idx[q] : 0xFFFFFFFFu;
+            }
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            if (p[j] == 0xFFFFFFFFu) continue;
-            x[j] = px[p[j]];
-            y[j] = py[p[j]];
-            z[j] = pz[p[j]];
-            u[j][0] = ux[p[j]] * sc[0];
-            u[j][1] = uy[p[j]] * sc[1];
-            u[j][2] = uz[p[j]] * sc[2];
-        }
+            for (int j = 0; j < B; ++j) {
+                if (p[j] == 0xFFFFFFFFu) continue;
+                x[j] = px[p[j]];
+                y[j] = py[p[j]];
+                z[j] = pz[p[j]];
+                u[j][0] = ux[p[j]] * sc[0];
+                u[j][1] = uy[p[j]] * sc[1];
+                u[j][2] = uz[p[j]] * sc[2];
+            }
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            if (p[j] == 0xFFFFFFFFu) continue;
-            const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
-            const double fx = x[j] - ix, fy = y[j] - iy, fz = z[j] - iz;
-            const double w[8] = {(1 - fx) * (1 - fy) * (1 - fz), (1 - fx) * (1 - fy) * fz, (1 - fx) * fy * (1 - fz),
-                                 (1 - fx) * fy * fz,             fx * (1 - fy) * (1 - fz), fx * (1 - fy) * fz,
-                                 fx * fy * (1 - fz),             fx * fy * fz};
-            const int n0 = ((ix - ox) * E + (iy - oy)) * E + (iz - oz);
+            for (int j = 0; j < B; ++j) {
+                if (p[j] == 0xFFFFFFFFu) continue;
+                const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
+                const double fx = x[j] - ix, fy = y[j] - iy, fz = z[j] - iz;
+                const double w[8] = {(1 - fx) * (1 - fy) * (1 - fz), (1 - fx) * (1 - fy) * fz,
+                                     (1 - fx) * fy * (1 - fz),       (1 - fx) * fy * fz,
+                                     fx * (1 - fy) * (1 - fz),       fx * (1 - fy) * fz,
+                                     fx * fy * (1 - fz),             fx * fy * fz};
+                const int n0 = ((ix - ox) * E + (iy - oy)) * E + (iz - oz);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int n = n0 + ((k >> 2) & 1) * E * E + ((k >> 1) & 1) * E + (k & 1);
+                for (int m = 0; m < 8; ++m) {
+                    const int n = n0 + ((m >> 2) & 1) * E * E + ((m >> 1) & 1) * E + (m & 1);
 #pragma unroll
-                for (int a = 0; a < 3; ++a)
-                    atomicAdd(&acc[a][n], (unsigned long long)__double2ll_rn(w[k] * u[j][a]));
+                    for (int a = 0; a < 3; ++a)
+                        atomicAdd(&acc[a][n], (unsigned long long)__double2ll_rn(w[m] * u[j][a]));
+                }
             }
         }
+        __syncthreads();
+        walk_tile_out<Rl, TS, 3, NT>(&acc[0][0], inv, t, ox, oy, oz, ng, out, sh);
     }
-    __syncthreads();
-    const double inv[3] = {1.0 / sc[0], 1.0 / sc[1], 1.0 / sc[2]};
-    const size_t N = (size_t)ng;
-    Rl* out[3] = {U0, U1, U2};
-    for (int i = threadIdx.x; i < TS * TS * TS; i += NT) {
-        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
-        const size_t node = ((size_t)(ox + a) * N + (oy + b)) * N + (oz + c);
-        const int n = (a * E + b) * E + c;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) out[d][node] = (Rl)((double)(long long)acc[d][n] * inv[d]);
-    }
-    const size_t ntiles = (size_t)nt * nt * nt;
-    for (int i = threadIdx.x; i < SH; i += NT) {
-        int a, b, c;
-        if (i < E * E) {
-            a = TS;
-            b = i / E;
-            c = i % E;
-        } else if (i < E * E + TS * E) {
-            const int j = i - E * E;
-            a = j / E;
-            b = TS;
-            c = j % E;
-        } else {
-            const int j = i - E * E - TS * E;
-            a = j / TS;
-            b = j % TS;
-            c = TS;
-        }
-        const int n = (a * E + b) * E + c;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-            shell[((size_t)d * ntiles + t) * SH + i] = (Rl)((double)(long long)acc[d][n] * inv[d]);
-    }
+    walk_seam_out<Rl, TS, 3, NT>(&acc[0][0], inv, sw, sh);
 }
 
 // Workgroup (256 threads) max of m[0..3) into one of kMaxSlots slots per
@@ -326,7 +299,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, dou
         x = px[i];
         y = py[i];
         z = pz[i];
-        if (!(in_box_a(x, box) && in_box_a(y, box) && in_box_a(z, box))) {
+        if (!in_box3(x, y, z, box)) {
             atomicAdd(err, 1ULL);
             ok = false;
         }
@@ -436,8 +409,8 @@ int launch_deposit_vec3(int prec, const double* px, const double* py, const doub
         hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
     }
 #define PM_VEC3(RL, TSV)                                                                                            \
-    hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles), dim3(kVec3Threads), 0, st, px, py, pz, ux, uy, \
-                       uz, s.idx, s.off, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3);                    \
+    hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles / walk_seg(nt)), dim3(kVec3Threads), 0, st, px, \
+                       py, pz, ux, uy, uz, s.idx, s.off, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3, sh);  \
     for (int d = 0; d < 3; ++d)                                                                                       \
         launch_fold_shells(prec, ts, ng, (const char*)shell3 + d * sh * rs, U[d], st);
     if (prec == PM_FP32) {

@@ -18,6 +18,14 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
     return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
+// Position inside the periodic box [0, box)^3 (false for NaN).  The six
+// compares are combined with bitwise &: with a short-circuit && the compiler
+// sinks the y and z loads behind the x test, and the three loads of a
+// position then wait on each other.
+__device__ __forceinline__ bool in_box3(double x, double y, double z, double box) {
+    return (x >= 0.0) & (x < box) & (y >= 0.0) & (y < box) & (z >= 0.0) & (z < box);
+}
+
 template <typename R> struct vec2;
 template <> struct vec2<float> { using type = float2; };
 template <> struct vec2<double> { using type = double2; };

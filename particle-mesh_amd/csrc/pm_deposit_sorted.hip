@@ -29,8 +29,6 @@ namespace pm {
 
 constexpr uint32_t kInvalidKey = 0xFFFFFFFFu;
 
-__device__ __forceinline__ bool in_box_s(double x, double box) { return x >= 0.0 && x < box; }
-
 // 1. tile id and slot of every particle.  A workgroup takes a chunk of
 // kBinChunk particles, counts them per tile in an LDS hash table (open
 // addressing), and issues one global atomic per distinct tile of the chunk,
@@ -92,7 +90,7 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
     for (int j = 0; j < kBinPer; ++j) {
         h[j] = 0u;
         rank[j] = 0u;
-        if (in_box_s(x[j], box) && in_box_s(y[j], box) && in_box_s(z[j], box)) {
+        if (in_box3(x[j], y[j], z[j], box)) {
             k[j] = (uint32_t)((((int)x[j] >> lt) * nt + ((int)y[j] >> lt)) * nt + ((int)z[j] >> lt));
         } else {
             k[j] = kInvalidKey;
@@ -188,7 +186,8 @@ constexpr uint32_t kQMask = 0x0FFFFFFFu;
 
 __device__ __forceinline__ uint32_t quantize_offset(double local) { return (uint32_t)(local * kQScale); }
 
-// 4. one workgroup per tile, particles by index (binning of positions).
+// 4. one workgroup per segment of the column walk, particles by index
+// (binning of positions).
 template <typename Rl, int TS>
 __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict__ px, const double* __restrict__ py,
                                                        const double* __restrict__ pz,
@@ -199,57 +198,63 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict_
     if (gate && *gate == 0u) return;
     constexpr int E = TS + 1, E3 = E * E * E;
     __shared__ unsigned long long acc[E3];
-    const int nt = ng / TS;
-    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
-    const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
-              tz = (int)(t % (uint32_t)nt);
-    const int ox = tx * TS, oy = ty * TS, oz = tz * TS;
+    const WalkSeg sw = walk_segment(ng, TS);
+    const double scale[1] = {1.0 / 4294967296.0};
+    Rl* const out[1] = {rho};
+    Rl* const sh[1] = {shell};
     for (int i = threadIdx.x; i < E3; i += 256) acc[i] = 0ULL;
     __syncthreads();
-    const uint32_t beg = off[t], end = off[t + 1];
     const double fix = 4294967296.0 * mass;  // 2^32 * m
-    // kDepBatch particles per thread per pass: their index and position loads
-    // are all in flight before the first LDS atomic
-    for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += 256 * kDepBatch) {
-        uint32_t p[kDepBatch];
-        double x[kDepBatch], y[kDepBatch], z[kDepBatch];
+    const int ox = sw.tx * TS, oy = sw.ty * TS;
+    for (int k = 0; k < sw.seg; ++k) {
+        const int tz = sw.sg * sw.seg + k, oz = tz * TS;
+        const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;
+        const uint32_t beg = off[t], end = off[t + 1];
+        // kDepBatch particles per thread per pass: their index and position
+        // loads are all in flight before the first LDS atomic
+        for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += 256 * kDepBatch) {
+            uint32_t p[kDepBatch];
+            double x[kDepBatch], y[kDepBatch], z[kDepBatch];
 #pragma unroll
-        for (int j = 0; j < kDepBatch; ++j) {
-            const uint32_t q = q0 + 256 * j;
-            p[j] = q < end ? idx[q] : kInvalidKey;
-        }
+            for (int j = 0; j < kDepBatch; ++j) {
+                const uint32_t q = q0 + 256 * j;
+                p[j] = q < end ? idx[q] : kInvalidKey;
+            }
 #pragma unroll
-        for (int j = 0; j < kDepBatch; ++j) {
-            if (p[j] != kInvalidKey) {
-                x[j] = px[p[j]];
-                y[j] = py[p[j]];
-                z[j] = pz[p[j]];
+            for (int j = 0; j < kDepBatch; ++j) {
+                if (p[j] != kInvalidKey) {
+                    x[j] = px[p[j]];
+                    y[j] = py[p[j]];
+                    z[j] = pz[p[j]];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kDepBatch; ++j) {
+                if (p[j] == kInvalidKey) continue;
+                if constexpr (sizeof(Rl) == 4) {
+                    const uint32_t qx = quantize_offset(x[j] - ox), qy = quantize_offset(y[j] - oy),
+                                   qz = quantize_offset(z[j] - oz);
+                    tile_cic_add<TS>(acc, (int)(qx >> 28), (int)(qy >> 28), (int)(qz >> 28),
+                                     (qx & kQMask) / kQScale, (qy & kQMask) / kQScale, (qz & kQMask) / kQScale,
+                                     fix);
+                } else {
+                    const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
+                    tile_cic_add<TS>(acc, ix - ox, iy - oy, iz - oz, x[j] - ix, y[j] - iy, z[j] - iz, fix);
+                }
             }
         }
-#pragma unroll
-        for (int j = 0; j < kDepBatch; ++j) {
-            if (p[j] == kInvalidKey) continue;
-            if constexpr (sizeof(Rl) == 4) {
-                const uint32_t qx = quantize_offset(x[j] - ox), qy = quantize_offset(y[j] - oy),
-                               qz = quantize_offset(z[j] - oz);
-                tile_cic_add<TS>(acc, (int)(qx >> 28), (int)(qy >> 28), (int)(qz >> 28), (qx & kQMask) / kQScale,
-                                 (qy & kQMask) / kQScale, (qz & kQMask) / kQScale, fix);
-            } else {
-                const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
-                tile_cic_add<TS>(acc, ix - ox, iy - oy, iz - oz, x[j] - ix, y[j] - iy, z[j] - iz, fix);
-            }
-        }
+        __syncthreads();
+        walk_tile_out<Rl, TS, 1, 256>(acc, scale, t, ox, oy, oz, ng, out, sh);
     }
-    __syncthreads();
-    tile_write<Rl, TS>(acc, t, ox, oy, oz, ng, rho, shell);
+    walk_seam_out<Rl, TS, 1, 256>(acc, scale, sw, sh);
 }
 
-// 4'. one workgroup per tile, particles from the staging arrays that the
-// interpolation of the previous step filled in tile order: tile t holds
-// fill[t] records from coff[t], in-tile offsets as u32 at 2^-28 cells (FP32
-// mesh) or exact doubles (FP64 mesh, giving the same weights as the binned
-// path bit for bit).  Skipped when a tile overflowed its capacity (the
-// binned path runs instead).
+// 4'. as 4, particles from the staging arrays that the interpolation of the
+// previous step filled in tile order: tile t holds fill[t] records from
+// coff[t], in-tile offsets as u32 at 2^-28 cells (FP32 mesh) or exact
+// doubles (FP64 mesh, giving the same weights as the binned path bit for
+// bit).  Skipped when a tile overflowed its capacity (the binned path runs
+// instead).
 template <typename Rl, int TS>
 __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__ stage,
                                                         const uint32_t* __restrict__ coff,
@@ -261,36 +266,41 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
     const St* __restrict__ sp = static_cast<const St*>(stage);
     constexpr int E = TS + 1, E3 = E * E * E;
     __shared__ unsigned long long acc[E3];
-    const int nt = ng / TS;
-    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
-    const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
-              tz = (int)(t % (uint32_t)nt);
+    const WalkSeg sw = walk_segment(ng, TS);
+    const double scale[1] = {1.0 / 4294967296.0};
+    Rl* const out[1] = {rho};
+    Rl* const sh[1] = {shell};
     for (int i = threadIdx.x; i < E3; i += 256) acc[i] = 0ULL;
     __syncthreads();
-    const uint32_t beg = coff[t], end = beg + fill[t];
     const double fix = 4294967296.0 * mass;
-    for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += 256 * kDepBatch) {
-        St v[kDepBatch];
+    for (int k = 0; k < sw.seg; ++k) {
+        const int tz = sw.sg * sw.seg + k;
+        const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;
+        const uint32_t beg = coff[t], end = beg + fill[t];
+        for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += 256 * kDepBatch) {
+            St v[kDepBatch];
 #pragma unroll
-        for (int j = 0; j < kDepBatch; ++j) {
-            const uint32_t q = q0 + 256 * j;
-            if (q < end) v[j] = sp[q];
-        }
+            for (int j = 0; j < kDepBatch; ++j) {
+                const uint32_t q = q0 + 256 * j;
+                if (q < end) v[j] = sp[q];
+            }
 #pragma unroll
-        for (int j = 0; j < kDepBatch; ++j) {
-            if (q0 + 256 * j >= end) continue;
-            if constexpr (sizeof(Rl) == 4) {
-                tile_cic_add<TS>(acc, (int)(v[j].x >> 28), (int)(v[j].y >> 28), (int)(v[j].z >> 28),
-                                 (v[j].x & kQMask) / kQScale, (v[j].y & kQMask) / kQScale,
-                                 (v[j].z & kQMask) / kQScale, fix);
-            } else {
-                const int ia = (int)v[j].x, ib = (int)v[j].y, ic = (int)v[j].z;
-                tile_cic_add<TS>(acc, ia, ib, ic, v[j].x - ia, v[j].y - ib, v[j].z - ic, fix);
+            for (int j = 0; j < kDepBatch; ++j) {
+                if (q0 + 256 * j >= end) continue;
+                if constexpr (sizeof(Rl) == 4) {
+                    tile_cic_add<TS>(acc, (int)(v[j].x >> 28), (int)(v[j].y >> 28), (int)(v[j].z >> 28),
+                                     (v[j].x & kQMask) / kQScale, (v[j].y & kQMask) / kQScale,
+                                     (v[j].z & kQMask) / kQScale, fix);
+                } else {
+                    const int ia = (int)v[j].x, ib = (int)v[j].y, ic = (int)v[j].z;
+                    tile_cic_add<TS>(acc, ia, ib, ic, v[j].x - ia, v[j].y - ib, v[j].z - ic, fix);
+                }
             }
         }
+        __syncthreads();
+        walk_tile_out<Rl, TS, 1, 256>(acc, scale, t, sw.tx * TS, sw.ty * TS, tz * TS, ng, out, sh);
     }
-    __syncthreads();
-    tile_write<Rl, TS>(acc, t, tx * TS, ty * TS, tz * TS, ng, rho, shell);
+    walk_seam_out<Rl, TS, 1, 256>(acc, scale, sw, sh);
 }
 
 // Capacities of the staging buckets for the next step: count + count/8 + 32
@@ -329,37 +339,59 @@ __global__ __launch_bounds__(1024) void k_cap_scan(uint32_t* __restrict__ fill, 
     if (t == 0) *spill = 0u;
 }
 
-// 5. add the lower neighbours' shells to each tile's lower-face nodes.
+// 5. add the lower neighbours' shells to each tile's lower-face nodes
+// (one workgroup per tile): on the faces a == 0 and b == 0 the xy shells of
+// the x, y and xy neighbours in the same z layer; on the first plane of a
+// segment (c == 0) also the z seams of the previous segment of the own, x,
+// y and xy columns.  Fixed order d = 1..7 (bit 0: x neighbour, 1: y, 2: z).
 template <typename Rl, int TS>
 __global__ __launch_bounds__(256) void k_fold_shells(const Rl* __restrict__ shell, int ng, Rl* __restrict__ rho) {
-    const int nt = ng / TS;
+    constexpr int E = TS + 1, XYS = xy_shell_size<TS>(), ZS = zseam_size<TS>();
+    const int nt = ng / TS, seg = walk_seg(nt), nseg = nt / seg;
     const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
               tz = (int)(t % (uint32_t)nt);
+    const int sg = tz / seg;
+    const bool first = tz == sg * seg;
+    const Rl* __restrict__ seam = shell + (size_t)nt * nt * nt * XYS;
     const size_t N = (size_t)ng;
-    // lower-face nodes: a == 0 (TS^2), then b == 0 < a (TS (TS-1)), then
-    // c == 0 < a, b ((TS-1)^2)
-    constexpr int NF = TS * TS + TS * (TS - 1) + (TS - 1) * (TS - 1);
-    for (int i = threadIdx.x; i < NF; i += 256) {
+    // a == 0 (TS^2 nodes), then b == 0 < a (TS (TS-1)), then on a segment's
+    // first plane c == 0 < a, b ((TS-1)^2)
+    constexpr int NXY = TS * TS + TS * (TS - 1);
+    const int nf = NXY + (first ? (TS - 1) * (TS - 1) : 0);
+    for (int i = threadIdx.x; i < nf; i += 256) {
         int a, b, c;
         if (i < TS * TS) {
             a = 0;
             b = i / TS;
             c = i % TS;
-        } else if (i < TS * TS + TS * (TS - 1)) {
+        } else if (i < NXY) {
             const int j = i - TS * TS;
             a = 1 + j / TS;
             b = 0;
             c = j % TS;
         } else {
-            const int j = i - TS * TS - TS * (TS - 1);
+            const int j = i - NXY;
             a = 1 + j / (TS - 1);
             b = 1 + j % (TS - 1);
             c = 0;
         }
-        const int on = (a == 0) | ((b == 0) << 1) | ((c == 0) << 2);
+        const int on = (a == 0) | ((b == 0) << 1) | ((first && c == 0) << 2);
         const size_t node = ((size_t)(tx * TS + a) * N + (ty * TS + b)) * N + (tz * TS + c);
-        rho[node] = fold_node<Rl, TS>(rho[node], shell, ng, tx * TS + a, ty * TS + b, tz * TS + c, on);
+        Rl v = rho[node];
+        for (int d = 1; d < 8; ++d) {
+            if ((d & on) != d) continue;
+            const int dx = d & 1, dy = (d >> 1) & 1;
+            const int sx = tx - dx < 0 ? nt - 1 : tx - dx;
+            const int sy = ty - dy < 0 ? nt - 1 : ty - dy;
+            if (d < 4) {
+                v += shell[(((size_t)sx * nt + sy) * nt + tz) * XYS + xy_shell_index<TS>(a + TS * dx, b + TS * dy, c)];
+            } else {
+                const int ss = sg == 0 ? nseg - 1 : sg - 1;
+                v += seam[(((size_t)sx * nt + sy) * nseg + ss) * ZS + (a + TS * dx) * E + (b + TS * dy)];
+            }
+        }
+        rho[node] = v;
     }
 }
 
@@ -368,7 +400,9 @@ int sorted_tile(int ng) { return ng >= 16 ? 16 : 8; }
 size_t sorted_shell_elems(int ng) {
     const int ts = sorted_tile(ng);
     const size_t nt = (size_t)(ng / ts);
-    return nt * nt * nt * (size_t)(ts == 16 ? shell_size<16>() : shell_size<8>());
+    const size_t nseg = nt * nt * (nt / (size_t)walk_seg((int)nt));
+    return ts == 16 ? nt * nt * nt * xy_shell_size<16>() + nseg * zseam_size<16>()
+                    : nt * nt * nt * xy_shell_size<8>() + nseg * zseam_size<8>();
 }
 
 int launch_bin(const double* px, const double* py, const double* pz, int64_t np, int ng, SortedScratch& s,
@@ -418,11 +452,11 @@ int launch_fold_shells(int prec, int ts, int ng, const void* shell, void* rho, h
 int launch_deposit_binned(int prec, const double* px, const double* py, const double* pz, int ng, double mass,
                           void* rho, SortedScratch& s, hipStream_t st, const uint32_t* gate, bool fold) {
     const int ts = sorted_tile(ng);
-    const int nt = ng / ts;
+    const int nt = ng / ts, seg = walk_seg(nt);
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
 #define PM_DT(RL, TSV)                                                                                                 \
-    hipLaunchKernelGGL((k_deposit_tiles<RL, TSV>), dim3(ntiles), dim3(256), 0, st, px, py, pz, s.idx, s.off, ng, mass, \
-                       (RL*)rho, (RL*)s.shell, gate)
+    hipLaunchKernelGGL((k_deposit_tiles<RL, TSV>), dim3(ntiles / seg), dim3(256), 0, st, px, py, pz, s.idx, s.off, ng,  \
+                       mass, (RL*)rho, (RL*)s.shell, gate)
     if (prec == PM_FP32) {
         if (ts == 16)
             PM_DT(float, 16);
@@ -451,14 +485,14 @@ int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st) {
 int launch_deposit_staged(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
                           double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st) {
     const int ts = sorted_tile(ng);
-    const int nt = ng / ts;
+    const int nt = ng / ts, seg = walk_seg(nt);
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
     // fallback, run only if some tile overflowed its staging capacity
     int rc = launch_bin(px, py, pz, np, ng, s, err, st, s.spill);
     if (!rc) rc = launch_deposit_binned(prec, px, py, pz, ng, mass, rho, s, st, s.spill, false);
     if (rc) return rc;
 #define PM_DS(RL, TSV)                                                                                               \
-    hipLaunchKernelGGL((k_deposit_staged<RL, TSV>), dim3(ntiles), dim3(256), 0, st, s.stage, s.coff, s.fill,          \
+    hipLaunchKernelGGL((k_deposit_staged<RL, TSV>), dim3(ntiles / seg), dim3(256), 0, st, s.stage, s.coff, s.fill,    \
                        s.spill, ng, mass, (RL*)rho, (RL*)s.shell)
     if (prec == PM_FP32) {
         if (ts == 16)

@@ -81,36 +81,54 @@ __device__ __forceinline__ CellIdx<Rl> cell_of(double x, double y, double z, int
 //   cl[a][b][c]: x offset a, y offset b (0, +1), z offset c-1
 //   xr[s][b][c]: x offset -1 (s=0) or +2 (s=1), y offset b, z offset c
 //   yr[s][a][c]: y offset -1 (s=0) or +2 (s=1), x offset a, z offset c
+// The gather is split in two so that a thread can have the loads of several
+// particles in flight at once: gather_runs_main issues the 12 vector loads
+// with their z starts clamped into the mesh (always in bounds, no branch);
+// gather_runs_seam then reloads value by value, with wrapped indices, the
+// runs of a particle whose cell touches the seam (iz == 0 or iz + 2 >= N),
+// for which the clamped runs hold the wrong values.  N >= 8.
+template <typename Rl>
+__device__ __forceinline__ void gather_runs_main(const Rl* __restrict__ phi, int N, const CellIdx<Rl>& c,
+                                                 Rl (&cl)[2][2][4], Rl (&xr)[2][2][2], Rl (&yr)[2][2][2]) {
+    const int z4 = min(max(c.iz - 1, 0), N - 4), z2 = min(c.iz, N - 2);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            load_run4(mesh_at(phi, c.X[a + 1] + c.Y[b + 1] + z4), cl[a][b]);
+            load_run2(mesh_at(phi, c.X[a * 3] + c.Y[b + 1] + z2), xr[a][b]);
+            load_run2(mesh_at(phi, c.X[b + 1] + c.Y[a * 3] + z2), yr[a][b]);
+        }
+}
+
+template <typename Rl>
+__device__ __forceinline__ void gather_runs_seam(const Rl* __restrict__ phi, int N, const CellIdx<Rl>& c,
+                                                 Rl (&cl)[2][2][4], Rl (&xr)[2][2][2], Rl (&yr)[2][2][2]) {
+    const int iz = c.iz;
+    if (iz >= 1 && iz + 2 < N) return;
+    const uint32_t Z[4] = {(uint32_t)(iz == 0 ? N - 1 : iz - 1), (uint32_t)iz, (uint32_t)(iz + 1 == N ? 0 : iz + 1),
+                           (uint32_t)(iz + 2 >= N ? iz + 2 - N : iz + 2)};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) cl[a][b][k] = *mesh_at(phi, c.X[a + 1] + c.Y[b + 1] + Z[k]);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                xr[a][b][k] = *mesh_at(phi, c.X[a * 3] + c.Y[b + 1] + Z[k + 1]);
+                yr[a][b][k] = *mesh_at(phi, c.X[b + 1] + c.Y[a * 3] + Z[k + 1]);
+            }
+        }
+}
+
 template <typename Rl>
 __device__ __forceinline__ void gather_runs(const Rl* __restrict__ phi, int N, const CellIdx<Rl>& c,
                                             Rl (&cl)[2][2][4], Rl (&xr)[2][2][2], Rl (&yr)[2][2][2]) {
-    const int iz = c.iz;
-    if (iz >= 1 && iz + 2 < N) {
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                load_run4(mesh_at(phi, c.X[a + 1] + c.Y[b + 1] + iz - 1), cl[a][b]);
-                load_run2(mesh_at(phi, c.X[a * 3] + c.Y[b + 1] + iz), xr[a][b]);
-                load_run2(mesh_at(phi, c.X[b + 1] + c.Y[a * 3] + iz), yr[a][b]);
-            }
-    } else {
-        const uint32_t Z[4] = {(uint32_t)(iz == 0 ? N - 1 : iz - 1), (uint32_t)iz, (uint32_t)(iz + 1 == N ? 0 : iz + 1),
-                               (uint32_t)(iz + 2 >= N ? iz + 2 - N : iz + 2)};
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) cl[a][b][k] = *mesh_at(phi, c.X[a + 1] + c.Y[b + 1] + Z[k]);
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    xr[a][b][k] = *mesh_at(phi, c.X[a * 3] + c.Y[b + 1] + Z[k + 1]);
-                    yr[a][b][k] = *mesh_at(phi, c.X[b + 1] + c.Y[a * 3] + Z[k + 1]);
-                }
-            }
-    }
+    gather_runs_main(phi, N, c, cl, xr, yr);
+    gather_runs_seam(phi, N, c, cl, xr, yr);
 }
+
 
 // Values of a mesh at the 8 corners of the cell (4 z-runs of 2).
 template <typename Rl>
@@ -170,14 +188,20 @@ __device__ __forceinline__ Rl trilerp(const Rl (&f)[2][2][2], Rl tx, Rl ty, Rl t
 // Centred-difference force of the periodic potential, trilinearly
 // interpolated at a particle (centered_derivatives + trilerp1).
 template <typename Rl>
+__device__ __forceinline__ void force_from_runs(const Rl (&cl)[2][2][4], const Rl (&xr)[2][2][2],
+                                                const Rl (&yr)[2][2][2], Rl tx, Rl ty, Rl tz, Rl (&F)[3]) {
+    Rl gx[2][2][2], gy[2][2][2], gz[2][2][2];
+    corner_differences(cl, xr, yr, gx, gy, gz);
+    F[0] = trilerp(gx, tx, ty, tz);
+    F[1] = trilerp(gy, tx, ty, tz);
+    F[2] = trilerp(gz, tx, ty, tz);
+}
+
+template <typename Rl>
 __device__ __forceinline__ void force_at(const Rl* __restrict__ phi, int N, const CellIdx<Rl>& c, Rl (&F)[3]) {
     Rl cl[2][2][4], xr[2][2][2], yr[2][2][2];
     gather_runs(phi, N, c, cl, xr, yr);
-    Rl gx[2][2][2], gy[2][2][2], gz[2][2][2];
-    corner_differences(cl, xr, yr, gx, gy, gz);
-    F[0] = trilerp(gx, c.tx, c.ty, c.tz);
-    F[1] = trilerp(gy, c.tx, c.ty, c.tz);
-    F[2] = trilerp(gz, c.tx, c.ty, c.tz);
+    force_from_runs(cl, xr, yr, c.tx, c.ty, c.tz, F);
 }
 
 template <typename Rl>

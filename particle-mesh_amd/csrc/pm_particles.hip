@@ -20,8 +20,6 @@ int launch_fill(int prec, void* p, size_t n, double v, hipStream_t st) {
     return check_launch("fill");
 }
 
-__device__ __forceinline__ bool in_box(double x, double box) { return x >= 0.0 && x < box; }
-
 // deposit_particles + cic_deposit (utils.cpp:70-102, utils_inline.h:25-49)
 // on the periodic mesh: node ix+1 == ng wraps to 0, which is the ghost fold
 // of reorder_slab.cpp:43-67 done in place.  Weights are formed in FP64 with
@@ -36,8 +34,7 @@ __global__ __launch_bounds__(kPartThreads) void k_deposit_atomic(const double* _
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kPartThreads + threadIdx.x;
     if (i >= np) return;
     const double x = px[i], y = py[i], z = pz[i];
-    const double box = (double)ng;
-    if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
+    if (!in_box3(x, y, z, (double)ng)) {
         atomicAdd(err, 1ULL);
         return;
     }
@@ -79,78 +76,29 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
     return x;
 }
 
-// Shape of the staging interpolation (MODE 2): 256 threads with two
-// particles each, whose loads and gathers are in flight together (measured
-// against one particle per thread and against 512 / 1024 threads), and an
-// LDS hash of 128 tiles per pass.
-#ifndef PM_STAGE_PER
-#define PM_STAGE_PER 2
+// Shape of the staging interpolation (MODE 2): 256 threads with PER
+// particles each and an LDS hash of 128 tiles per pass.  One particle per
+// thread for both meshes: with two, the gathers of both are in flight
+// together but the registers they take halve the waves per SIMD, which
+// measured slower (FP32 3.8 against 3.3 ms at 512^3).
+#ifndef PM_STAGE_PER32
+#define PM_STAGE_PER32 1
+#endif
+#ifndef PM_STAGE_PER64
+#define PM_STAGE_PER64 1
 #endif
 constexpr int kStageThreads = 256;
-constexpr int kStagePer = PM_STAGE_PER;
+constexpr int kStagePer32 = PM_STAGE_PER32, kStagePer64 = PM_STAGE_PER64;
 constexpr int kStageHash = 128;
-
-// One particle of k_interp: returns its next tile (kNo for none) and, in
-// MODE 2, its staged record.
-template <typename Rl, int MODE, typename Rec>
-__device__ __forceinline__ uint32_t interp_one(int64_t i, double* __restrict__ px, double* __restrict__ py,
-                                               double* __restrict__ pz, double* __restrict__ vx,
-                                               double* __restrict__ vy, double* __restrict__ vz, int ng,
-                                               const Rl* __restrict__ phi, double kick, double drift, int ts,
-                                               unsigned long long* __restrict__ err, Rec& rec) {
-    constexpr uint32_t kNo = 0xFFFFFFFFu;
-    const double box = (double)ng;
-    double x = px[i], y = py[i], z = pz[i];
-    if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
-        atomicAdd(err, 1ULL);
-        return kNo;
-    }
-    double F[3];
-    cic_force(phi, ng, x, y, z, F);
-    if constexpr (MODE == 1) {
-        vx[i] = F[0];
-        vy[i] = F[1];
-        vz[i] = F[2];
-        return kNo;
-    } else {
-        const double nvx = vx[i] + F[0] * kick, nvy = vy[i] + F[1] * kick, nvz = vz[i] + F[2] * kick;
-        vx[i] = nvx;
-        vy[i] = nvy;
-        vz[i] = nvz;
-        if (drift != 0.0) {
-            x = wrap_pos(x + nvx * drift, box);
-            y = wrap_pos(y + nvy * drift, box);
-            z = wrap_pos(z + nvz * drift, box);
-            px[i] = x;
-            py[i] = y;
-            pz[i] = z;
-        }
-        if constexpr (MODE != 2) {
-            return kNo;
-        } else {
-            // a particle that drifted more than a box away stays unstaged and
-            // is counted, as the next step's range check would
-            if (!(in_box(x, box) && in_box(y, box) && in_box(z, box))) {
-                atomicAdd(err, 1ULL);
-                return kNo;
-            }
-            const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;  // ts: power of two
-            // in-tile offsets: u32 at 2^-28 cells (FP32 mesh) or exact doubles
-            const int msk = ~(ts - 1);  // tile origin = cell with the low bits cleared
-            const double lx = x - ((int)x & msk), ly = y - ((int)y & msk), lz = z - ((int)z & msk);
-            if constexpr (sizeof(Rl) == 4)
-                rec = StageQ{(uint32_t)(lx * 268435456.0), (uint32_t)(ly * 268435456.0),
-                             (uint32_t)(lz * 268435456.0)};
-            else
-                rec = StageD{lx, ly, lz};
-            return (uint32_t)((((int)x >> lt) * nt + ((int)y >> lt)) * nt + ((int)z >> lt));
-        }
-    }
-}
 
 // update_particles (utils.cpp:141-170) with the force of cic_force, then the
 // drift that the reference leaves commented out (utils.cpp:145-146), for
-// PER particles per thread (NT apart: their loads are in flight together).
+// PER particles per thread, NT apart.  The kernel runs in three phases so
+// that each phase has all its loads in flight together: (1) the positions
+// and velocities of the thread's particles, (2) their 12 potential runs
+// each, (3) force, kick, drift and stores.  A lane past the end or with a
+// position outside the box (counted in err) gathers around cell 0 and
+// stores nothing.
 // MODE 0: v += g*kick, x += v*drift.  MODE 1: write g to (vx, vy, vz).
 // MODE 2: as MODE 0, then the drifted position is staged for the next
 // step's deposit (stage_commit).
@@ -168,12 +116,84 @@ __global__ __launch_bounds__(NT) void k_interp(double* __restrict__ px, double* 
     static_assert((HS & (HS - 1)) == 0, "hash size must be a power of two");
     __shared__ uint32_t hkey[HS], hcnt[HS], hbase[HS];
     const int64_t i0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * (NT * PER) + threadIdx.x;
+    const double box = (double)ng;
+    int64_t idx[PER];
+    bool live[PER], ok[PER];
+    double x[PER], y[PER], z[PER], u[PER][3];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        idx[q] = i0 + (int64_t)q * NT;
+        live[q] = idx[q] < np;
+        const int64_t j = live[q] ? idx[q] : 0;
+        x[q] = px[j];
+        y[q] = py[j];
+        z[q] = pz[j];
+        if constexpr (MODE != 1) {
+            u[q][0] = vx[j];
+            u[q][1] = vy[j];
+            u[q][2] = vz[j];
+        }
+    }
+    CellIdx<Rl> c[PER];
+    Rl cl[PER][2][2][4], xr[PER][2][2][2], yr[PER][2][2][2];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        ok[q] = live[q] & in_box3(x[q], y[q], z[q], box);
+        c[q] = cell_of<Rl>(ok[q] ? x[q] : 0.0, ok[q] ? y[q] : 0.0, ok[q] ? z[q] : 0.0, ng);
+        gather_runs_main(phi, ng, c[q], cl[q], xr[q], yr[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) gather_runs_seam(phi, ng, c[q], cl[q], xr[q], yr[q]);
     uint32_t kk[PER];
     Rec rec[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int64_t i = i0 + (int64_t)q * NT;
-        kk[q] = i < np ? interp_one<Rl, MODE>(i, px, py, pz, vx, vy, vz, ng, phi, kick, drift, ts, err, rec[q]) : kNo;
+        kk[q] = kNo;
+        Rl f[3];
+        force_from_runs(cl[q], xr[q], yr[q], c[q].tx, c[q].ty, c[q].tz, f);
+        const int64_t i = idx[q];
+        if (!ok[q]) {
+            if (live[q]) atomicAdd(err, 1ULL);
+            continue;
+        }
+        if constexpr (MODE == 1) {
+            vx[i] = (double)f[0];
+            vy[i] = (double)f[1];
+            vz[i] = (double)f[2];
+        } else {
+            const double nvx = u[q][0] + (double)f[0] * kick, nvy = u[q][1] + (double)f[1] * kick,
+                         nvz = u[q][2] + (double)f[2] * kick;
+            vx[i] = nvx;
+            vy[i] = nvy;
+            vz[i] = nvz;
+            double xn = x[q], yn = y[q], zn = z[q];
+            if (drift != 0.0) {
+                xn = wrap_pos(xn + nvx * drift, box);
+                yn = wrap_pos(yn + nvy * drift, box);
+                zn = wrap_pos(zn + nvz * drift, box);
+                px[i] = xn;
+                py[i] = yn;
+                pz[i] = zn;
+            }
+            if constexpr (MODE == 2) {
+                // a particle that drifted more than a box away stays unstaged
+                // and is counted, as the next step's range check would
+                if (!in_box3(xn, yn, zn, box)) {
+                    atomicAdd(err, 1ULL);
+                    continue;
+                }
+                const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;  // ts: power of two
+                // in-tile offsets: u32 at 2^-28 cells (FP32 mesh) or exact doubles
+                const int msk = ~(ts - 1);  // tile origin = cell with the low bits cleared
+                const double lx = xn - ((int)xn & msk), ly = yn - ((int)yn & msk), lz = zn - ((int)zn & msk);
+                if constexpr (sizeof(Rl) == 4)
+                    rec[q] = StageQ{(uint32_t)(lx * 268435456.0), (uint32_t)(ly * 268435456.0),
+                                    (uint32_t)(lz * 268435456.0)};
+                else
+                    rec[q] = StageD{lx, ly, lz};
+                kk[q] = (uint32_t)((((int)xn >> lt) * nt + ((int)yn >> lt)) * nt + ((int)zn >> lt));
+            }
+        }
     }
     if constexpr (MODE == 2) {
 #pragma unroll
@@ -194,7 +214,8 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
                   unsigned long long* err, hipStream_t st, SortedScratch* s) {
     if (np <= 0) return PM_OK;
     if (mode == 2 && (!s || !s->stage)) return fail(PM_ERR_ARG, "interp: staging buffers missing");
-    const int nt = mode == 2 ? kStageThreads : kPartThreads, per = mode == 2 ? kStagePer : 1;
+    const int nt = mode == 2 ? kStageThreads : kPartThreads,
+              per = mode != 2 ? 1 : prec == PM_FP32 ? kStagePer32 : kStagePer64;
     const unsigned nb = (unsigned)((np + (int64_t)nt * per - 1) / ((int64_t)nt * per));
     const dim3 g(nb), b(nt);
     const int ts = sorted_tile(ng);
@@ -211,14 +232,14 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
         else if (mode == 1)
             PM_INTERP(float, 1, kPartThreads, 1);
         else
-            PM_INTERP(float, 2, kStageThreads, kStagePer);
+            PM_INTERP(float, 2, kStageThreads, kStagePer32);
     } else {
         if (mode == 0)
             PM_INTERP(double, 0, kPartThreads, 1);
         else if (mode == 1)
             PM_INTERP(double, 1, kPartThreads, 1);
         else
-            PM_INTERP(double, 2, kStageThreads, kStagePer);
+            PM_INTERP(double, 2, kStageThreads, kStagePer64);
     }
 #undef PM_INTERP
     return check_launch("interp");

@@ -1,46 +1,108 @@
-// Tile shells of the sorted deposit, shared by the deposit kernels and the
-// FFT pass that can fold them while it reads the density.
+// Tiles of the sorted deposit (column walk, shells, CIC adds in LDS fixed
+// point) and the staging commit of the interpolation.
 #pragma once
 
 #include "pm_common.h"
 
 namespace pm {
 
-template <int TS> __host__ __device__ constexpr int shell_size() {
-    return (TS + 1) * (TS + 1) * (TS + 1) - TS * TS * TS;
+// Column walk of the sorted deposits.  A workgroup deposits a segment of
+// consecutive tiles along z of one (x, y) tile column, accumulating each
+// tile's (TS+1)^3 nodes in LDS fixed point.  The upper z plane (c == TS) of
+// a tile is the lower plane (c == 0) of the next one, so it is carried into
+// the next tile's accumulator instead of being written out: within a
+// segment the z part of the ghost fold (reorder_slab.cpp:43-67) is exact
+// integer addition.  Written out per tile: the TS^3 owned nodes to the mesh
+// and, for c < TS, the upper x and y faces to the tile's xy shell; per
+// segment: the upper z plane of its last tile to the segment's z seam.
+// k_fold_shells then adds the lower neighbours' xy shells to each tile's
+// lower x and y faces, and the seams to the first plane of each segment.
+#ifndef PM_WALK_SEG
+#define PM_WALK_SEG 8
+#endif
+constexpr int kWalkSeg = PM_WALK_SEG;  // tiles per segment (a power of two)
+__host__ __device__ inline int walk_seg(int nt) { return nt < kWalkSeg ? nt : kWalkSeg; }
+
+// xy shell of a tile: x face (a == TS, b = 0..TS, c < TS) at b*TS + c, then
+// the y face (b == TS, a < TS, c < TS) at (TS+1)*TS + a*TS + c.
+template <int TS> __host__ __device__ constexpr int xy_shell_size() { return (TS + 1) * TS + TS * TS; }
+template <int TS> __device__ __forceinline__ int xy_shell_index(int a, int b, int c) {
+    return a == TS ? b * TS + c : (TS + 1) * TS + a * TS + c;
+}
+// z seam of a segment: the nodes (a, b) = 0..TS of its last tile's plane c == TS.
+template <int TS> __host__ __device__ constexpr int zseam_size() { return (TS + 1) * (TS + 1); }
+
+// Position of a workgroup's segment: column (tx, ty), segment sg of nseg.
+struct WalkSeg {
+    int nt, seg, nseg, tx, ty, sg;
+    uint32_t w;
+};
+__device__ __forceinline__ WalkSeg walk_segment(int ng, int TS) {
+    WalkSeg s;
+    s.nt = ng / TS;
+    s.seg = walk_seg(s.nt);
+    s.nseg = s.nt / s.seg;
+    s.w = xcd_remap(blockIdx.x, gridDim.x);
+    s.tx = (int)(s.w / (uint32_t)(s.nt * s.nseg));
+    s.ty = (int)((s.w / (uint32_t)s.nseg) % (uint32_t)s.nt);
+    s.sg = (int)(s.w % (uint32_t)s.nseg);
+    return s;
 }
 
-// Shell index of a node with local coordinates (a, b, c), max == TS:
-// face x (a == TS): (TS+1)^2 entries, face y (b == TS, a < TS): TS (TS+1),
-// face z (c == TS, a, b < TS): TS^2.
-template <int TS> __device__ __forceinline__ int shell_index(int a, int b, int c) {
-    constexpr int E = TS + 1;
-    if (a == TS) return b * E + c;
-    if (b == TS) return E * E + a * E + c;
-    return E * E + TS * E + a * TS + b;
-}
-
-// The periodic ghost fold (reorder_slab.cpp:43-67) at tile granularity for
-// one mesh node (x, y, z) on a lower face of its tile: v (the tile's own
-// sum) plus the shells of the lower neighbour tiles that hold the node, in
-// the fixed order d = 1..7 (bit 0: x-neighbour, 1: y, 2: z).  on: which of
-// a == 0, b == 0, c == 0 hold (same bits).
-template <typename Rl, int TS>
-__device__ __forceinline__ Rl fold_node(Rl v, const Rl* __restrict__ shell, int ng, int x, int y, int z, int on) {
-    constexpr int SH = shell_size<TS>();
-    const int nt = ng / TS;
-    const int tx = x / TS, ty = y / TS, tz = z / TS;
-    const int a = x - tx * TS, b = y - ty * TS, c = z - tz * TS;
-    for (int d = 1; d < 8; ++d) {
-        if ((d & on) != d) continue;
-        const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-        const int sx = tx - dx < 0 ? nt - 1 : tx - dx;
-        const int sy = ty - dy < 0 ? nt - 1 : ty - dy;
-        const int sz = tz - dz < 0 ? nt - 1 : tz - dz;
-        const size_t st = ((size_t)sx * nt + sy) * nt + sz;
-        v += shell[st * SH + shell_index<TS>(a + TS * dx, b + TS * dy, c + TS * dz)];
+// End of one tile of the walk (after a barrier on its accumulation): D
+// accumulators acc[d * E^3 + node] (signed fixed point, value = acc *
+// scale[d]) go out as the owned nodes and the xy shell of tile t, then the
+// plane c == TS moves to c == 0 and the rest is cleared for the next tile.
+template <typename Rl, int TS, int D, int NT>
+__device__ __forceinline__ void walk_tile_out(unsigned long long* acc, const double (&scale)[D], uint32_t t, int ox,
+                                              int oy, int oz, int ng, Rl* const (&out)[D],
+                                              Rl* const (&shell)[D]) {
+    constexpr int E = TS + 1, E3 = E * E * E, XYS = xy_shell_size<TS>();
+    const size_t N = (size_t)ng;
+    for (int i = threadIdx.x; i < TS * TS * TS; i += NT) {
+        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
+        const size_t node = ((size_t)(ox + a) * N + (oy + b)) * N + (oz + c);
+        const int n = (a * E + b) * E + c;
+#pragma unroll
+        for (int d = 0; d < D; ++d) out[d][node] = (Rl)((double)(long long)acc[d * E3 + n] * scale[d]);
     }
-    return v;
+    for (int i = threadIdx.x; i < XYS; i += NT) {
+        int a, b, c;
+        if (i < E * TS) {
+            a = TS;
+            b = i / TS;
+            c = i % TS;
+        } else {
+            const int j = i - E * TS;
+            a = j / TS;
+            b = TS;
+            c = j % TS;
+        }
+        const int n = (a * E + b) * E + c;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            shell[d][(size_t)t * XYS + i] = (Rl)((double)(long long)acc[d * E3 + n] * scale[d]);
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < D * E * E; r += NT) {  // one thread per z row
+        unsigned long long* row = acc + (r / (E * E)) * E3 + (r % (E * E)) * E;
+        row[0] = row[TS];
+#pragma unroll
+        for (int c = 1; c <= TS; ++c) row[c] = 0ULL;
+    }
+    __syncthreads();
+}
+
+// End of the segment: the carried plane (now at c == 0) is its z seam.
+template <typename Rl, int TS, int D, int NT>
+__device__ __forceinline__ void walk_seam_out(const unsigned long long* acc, const double (&scale)[D],
+                                              const WalkSeg& s, Rl* const (&shell)[D]) {
+    constexpr int E = TS + 1, E3 = E * E * E, XYS = xy_shell_size<TS>(), ZS = zseam_size<TS>();
+    const size_t seam0 = (size_t)s.nt * s.nt * s.nt * XYS + (size_t)s.w * ZS;
+    for (int r = threadIdx.x; r < D * ZS; r += NT) {
+        const int d = r / ZS, n = r % ZS;
+        shell[d][seam0 + n] = (Rl)((double)(long long)acc[d * E3 + n * E] * scale[d]);
+    }
 }
 
 constexpr int kDepBatch = 4;  // particles in flight per thread in the tile kernels
@@ -64,40 +126,6 @@ __device__ __forceinline__ void tile_cic_add(unsigned long long* acc, int a, int
     atomicAdd(n0 + E * E + 1, (unsigned long long)__double2ll_rn(sxy10 * m1));
     atomicAdd(n0 + E + 1, (unsigned long long)__double2ll_rn(sxy01 * m1));
     atomicAdd(n0 + E * E + E + 1, (unsigned long long)__double2ll_rn(sxy11 * m1));
-}
-
-// The TS^3 owned nodes of tile t go to the mesh, the upper shell to the
-// tile's shell buffer (after a barrier on the accumulation).
-template <typename Rl, int TS>
-__device__ __forceinline__ void tile_write(const unsigned long long* acc, uint32_t t, int ox, int oy, int oz,
-                                           int ng, Rl* __restrict__ rho, Rl* __restrict__ shell) {
-    constexpr int E = TS + 1, SH = shell_size<TS>();
-    const double unfix = 1.0 / 4294967296.0;
-    const size_t N = (size_t)ng;
-    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
-        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
-        rho[((size_t)(ox + a) * N + (oy + b)) * N + (oz + c)] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
-    }
-    Rl* sh = shell + (size_t)t * SH;
-    for (int i = threadIdx.x; i < SH; i += 256) {
-        int a, b, c;
-        if (i < E * E) {
-            a = TS;
-            b = i / E;
-            c = i % E;
-        } else if (i < E * E + TS * E) {
-            const int j = i - E * E;
-            a = j / E;
-            b = TS;
-            c = j % E;
-        } else {
-            const int j = i - E * E - TS * E;
-            a = j / TS;
-            b = j % TS;
-            c = TS;
-        }
-        sh[i] = (Rl)((double)acc[(a * E + b) * E + c] * unfix);
-    }
 }
 
 // Slot reservation in the staging buckets and the write of a staged record

@@ -545,10 +545,10 @@ def test_reused_binning_after_bucket_overflow(dev, prec):
 
 @pytest.mark.parametrize("prec", [pmhip.PM_FP32, pmhip.PM_FP64])
 def test_step_density_equals_standalone_deposit(dev, prec):
-    """pm_step leaves the tile-shell fold of the sorted deposit to the
-    solve's first FFT pass, which writes the folded density back: after a
-    step (binned first step, then a staged one) the plan's density must be
-    bit for bit pm_deposit's density of the same positions."""
+    """pm_step deposits through the staging of the previous step's
+    interpolation (column walk over the staged records, then the shell
+    fold): after a step (binned first step, then a staged one) the plan's
+    density must be bit for bit pm_deposit's density of the same positions."""
     import torch
 
     ng, nrow = 64, 32
