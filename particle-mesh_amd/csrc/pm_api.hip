@@ -266,6 +266,7 @@ static int ensure_sorted(pm_plan* p) {
     const size_t ntiles = nt * nt * nt;
     const size_t rs = p->prec == PM_FP32 ? sizeof(float) : sizeof(double);
     int rc = hip_ok(hipMalloc((void**)&s.cnt, 4 * ntiles), "sorted deposit: hipMalloc");
+    if (!rc) rc = hip_ok(hipMemsetAsync(s.cnt, 0, 4 * ntiles, p->stream), "sorted deposit: clear counts");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.off, 4 * (ntiles + 1)), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.key, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.slot, 4 * (size_t)p->np), "sorted deposit: hipMalloc");

@@ -50,6 +50,10 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
     if (gate && *gate == 0u) return;
     __shared__ uint32_t hkey[kBinHash];
     __shared__ uint32_t hcnt[kBinHash];
+    const int64_t nchunk = (np + kBinChunk - 1) / kBinChunk;
+    // grid-stride over the chunks (the grid is capped, so that a gated no-op
+    // launch stays cheap)
+    for (int64_t ch = xcd_remap(blockIdx.x, gridDim.x); ch < nchunk; ch += gridDim.x) {
     for (int h = threadIdx.x; h < kBinHash; h += 256) {
         hkey[h] = kInvalidKey;
         hcnt[h] = 0u;
@@ -57,7 +61,7 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
     __syncthreads();
     // each thread takes kBinPer consecutive particles: neighbouring particles
     // usually share a tile, so runs of equal keys cost one hash insert
-    const int64_t base = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kBinChunk + (int64_t)threadIdx.x * kBinPer;
+    const int64_t base = ch * kBinChunk + (int64_t)threadIdx.x * kBinPer;
     const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;  // ts: power of two
     const double box = (double)ng;
     uint32_t k[kBinPer], h[kBinPer], rank[kBinPer];
@@ -133,34 +137,82 @@ __global__ __launch_bounds__(256) void k_bin_count(const double* __restrict__ px
             slot[i] = k[j] == kInvalidKey ? 0u : hcnt[h[j]] + rank[j];
         }
     }
+    __syncthreads();  // the hash is reset for the next chunk
+    }
 }
 
-// 2. exclusive scan of n counts into off[0..n] (one workgroup of 1024),
-// the counts zeroed behind it for the next binning.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, int n,
-                                               const uint32_t* __restrict__ gate) {
+// Exclusive scan by one workgroup of 1024 threads of the n values load(i)
+// into out[0..n] (out[n] = total).  The values pass through LDS in chunks of
+// 16384 with coalesced loads and stores, each thread scanning 16 consecutive
+// ones; then a wave scan by shuffles and one over the 16 wave totals.
+constexpr int kScanThreads = 1024;
+
+template <typename Load>
+__device__ __forceinline__ void scan_workgroup(int n, Load load, uint32_t* __restrict__ out) {
+    constexpr int T = kScanThreads, PER = 16, CH = T * PER;
+    __shared__ uint32_t buf[CH + CH / PER];  // entry k at k + k / 16: the thread rows have stride 17
+    __shared__ uint32_t wsum[T / 64];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    uint32_t carry = 0u;
+    for (int base = 0; base < n; base += CH) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int k = j * T + t, i = base + k;
+            buf[k + k / PER] = i < n ? load(i) : 0u;
+        }
+        __syncthreads();
+        uint32_t v[PER], sum = 0u;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            v[j] = buf[t * (PER + 1) + j];
+            sum += v[j];
+        }
+        uint32_t incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        uint32_t before = 0u, total = 0u;
+#pragma unroll
+        for (int w = 0; w < T / 64; ++w) {
+            const uint32_t x = wsum[w];
+            before += w < wv ? x : 0u;
+            total += x;
+        }
+        uint32_t run = carry + before + incl - sum;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            buf[t * (PER + 1) + j] = run;
+            run += v[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int k = j * T + t, i = base + k;
+            if (i < n) out[i] = buf[k + k / PER];
+        }
+        carry += total;
+        __syncthreads();  // buf and wsum are refilled by the next chunk
+    }
+    if (t == 0) out[n] = carry;
+}
+
+// 2. exclusive scan of the n tile counts into off[0..n], the counts zeroed
+// behind it for the next binning.
+__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, int n,
+                                                       const uint32_t* __restrict__ gate) {
     if (gate && *gate == 0u) return;
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = t * per, e = min(n, b + per);
-    uint32_t s = 0;
-    for (int i = b; i < e; ++i) s += cnt[i];
-    part[t] = s;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {  // Hillis-Steele inclusive scan
-        const uint32_t v = t >= d ? part[t - d] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = t == 0 ? 0u : part[t - 1];
-    for (int i = b; i < e; ++i) {
-        off[i] = run;
-        run += cnt[i];
-        cnt[i] = 0u;
-    }
-    if (t == 1023) off[n] = part[1023];
+    scan_workgroup(
+        n,
+        [&](int i) {
+            const uint32_t c = cnt[i];
+            cnt[i] = 0u;
+            return c;
+        },
+        off);
 }
 
 // 3. particle indices in tile order.
@@ -307,36 +359,19 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
 // per tile (counts from the binning offsets on a first step, else from the
 // fill counters of the staged step), exclusive scan into coff[0..n]; the
 // fill counters and the overflow flag are cleared for the next staging.
-__global__ __launch_bounds__(1024) void k_cap_scan(uint32_t* __restrict__ fill, const uint32_t* __restrict__ off,
-                                                   uint32_t* __restrict__ coff, uint32_t* __restrict__ spill, int n,
-                                                   int from_off) {
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = t * per, e = min(n, b + per);
-    auto cap = [&](int i) -> uint32_t {
-        const uint32_t c = from_off ? off[i + 1] - off[i] : fill[i];
-        return c + (c >> 3) + 32u;
-    };
-    uint32_t s = 0;
-    for (int i = b; i < e; ++i) s += cap(i);
-    part[t] = s;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const uint32_t v = t >= d ? part[t - d] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = t == 0 ? 0u : part[t - 1];
-    for (int i = b; i < e; ++i) {
-        const uint32_t c = cap(i);
-        coff[i] = run;
-        run += c;
-        fill[i] = 0u;
-    }
-    if (t == 1023) coff[n] = part[1023];
-    if (t == 0) *spill = 0u;
+__global__ __launch_bounds__(kScanThreads) void k_cap_scan(uint32_t* __restrict__ fill,
+                                                           const uint32_t* __restrict__ off,
+                                                           uint32_t* __restrict__ coff, uint32_t* __restrict__ spill,
+                                                           int n, int from_off) {
+    scan_workgroup(
+        n,
+        [&](int i) {
+            const uint32_t c = from_off ? off[i + 1] - off[i] : fill[i];
+            fill[i] = 0u;
+            return c + (c >> 3) + 32u;
+        },
+        coff);
+    if (threadIdx.x == 0) *spill = 0u;
 }
 
 // 5. add the lower neighbours' shells to each tile's lower-face nodes
@@ -411,14 +446,13 @@ int launch_bin(const double* px, const double* py, const double* pz, int64_t np,
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
     if (np > 0xFFFFFFF0LL) return fail(PM_ERR_ARG, "sorted deposit: more than 2^32 particles");
-    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(std::min<uint32_t>((ntiles + 255) / 256, 8192)), dim3(256), 0, st,
-                       s.cnt, (size_t)ntiles, 0u);
+    // the counts are zero here: cleared at allocation, then by every k_scan
     if (np > 0) {
         const unsigned nchunk = (unsigned)((np + kBinChunk - 1) / kBinChunk);
-        hipLaunchKernelGGL(k_bin_count, dim3(nchunk), dim3(256), 0, st, px, py, pz, np, ng, ts, s.cnt, s.key, s.slot,
-                           err, gate);
+        hipLaunchKernelGGL(k_bin_count, dim3(std::min(nchunk, 4096u)), dim3(256), 0, st, px, py, pz, np, ng, ts,
+                           s.cnt, s.key, s.slot, err, gate);
     }
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s.cnt, s.off, (int)ntiles, gate);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, st, s.cnt, s.off, (int)ntiles, gate);
     if (np > 0) {
         const unsigned nb = (unsigned)((np + 255) / 256);
         // grid-stride over at most 16384 workgroups (a gated no-op stays cheap)
@@ -477,7 +511,7 @@ int launch_deposit_binned(int prec, const double* px, const double* py, const do
 int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
-    hipLaunchKernelGGL(k_cap_scan, dim3(1), dim3(1024), 0, st, s.fill, s.off, s.coff, s.spill, nt * nt * nt,
+    hipLaunchKernelGGL(k_cap_scan, dim3(1), dim3(kScanThreads), 0, st, s.fill, s.off, s.coff, s.spill, nt * nt * nt,
                        from_off ? 1 : 0);
     return check_launch("cap_scan");
 }
