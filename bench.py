@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--large-nrow", type=int, default=512)
     ap.add_argument("--large-ng", type=int, default=1024)
     ap.add_argument("--large-steps", type=int, default=30)
+    ap.add_argument("--no-tune", action="store_true", help="use the default FFT pass shapes (no pm_plan_tune)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -134,6 +135,11 @@ def run_gpu(prec, pos0, vel0, args, ng, npart, variant):
     pos = [p.clone() for p in pos0]
     vel = [v.clone() for v in vel0]
     plan = pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0)
+    if not args.no_tune:
+        # the FFT pass shapes are chosen by timing them on this plan (and cached
+        # for the later plans of this process), as FFTW plans before it runs
+        plan.tune(5)
+    shape = plan.fft_shape
     dt = args.dt
     step = 0
 
@@ -165,6 +171,7 @@ def run_gpu(prec, pos0, vel0, args, ng, npart, variant):
     nerr = pmhip.range_errors()
     plan.close()
     phases = {n: (m / c if c else None) for n, m, c in zip(pmhip.PHASE_NAMES, ms, cnt)}
+    phases["fft_shape"] = shape
     return ev_ms / args.steps, wall / args.steps * 1e3, phases, nerr, (pos, vel)
 
 
@@ -401,6 +408,8 @@ def main():
             torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
 
+    shape32 = ph32.pop("fft_shape")
+    shape64 = res64["phases_ms"].pop("fft_shape") if res64 else None
     nbytes = algorithmic_bytes(ng, npart, pmhip.PM_FP32, variant == pmhip.PM_DEPOSIT_SORTED)
     dom = max((k for k in ph32 if ph32[k]), key=lambda k: ph32[k])
     achieved = nbytes[dom] / (ph32[dom] * 1e-3) / 1e9
@@ -441,6 +450,8 @@ def main():
         "fp64": ({"value": round(res64["value"], 2), "ms_per_step": round(res64["ms_per_step"], 4),
                   "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["phases_ms"].items()}}
                  if res64 else None),
+        "fft_shape": {"fp32": shape32, "fp64": shape64, "tuned": not args.no_tune,
+                      "slots": ["z_fwd", "y_fwd", "x_green", "y_inv", "z_inv"]},
         "alt_deposit": alt,
         "accuracy_vs_fp64": acc,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,

@@ -5,9 +5,14 @@
 // spectrum lives in the handle.  As in ffte1, backward() is normalised by
 // 1/Ng^3 and apply_green() multiplies by the reference's KH Green's function
 // times 1/a (ffte1/fft.cpp:28-35,46-52).  color() is always 1 and there is
-// no communicator: one rank holds the only slab.
+// no communicator: one rank holds the only slab.  As the reference's FFTW
+// class does with its wisdom file (fftw/fft.cpp:28-49,63-70), the
+// constructor imports wisdom_<n>_hip.dat (from $PM_WISDOM_DIR, else the
+// working directory) when it exists, else times the FFT pass shapes on buf
+// (overwriting it) and the destructor exports the cache to that file.
 #pragma once
 #include <cstddef>
+#include <string>
 
 struct pm_fft;
 
@@ -33,4 +38,6 @@ private:
     int n_;
     double* real_;
     pm_fft* h_;
+    std::string filename_;  // wisdom file (fftw/fft.cpp:28)
+    bool need_export_;
 };

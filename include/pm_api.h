@@ -261,6 +261,41 @@ int pm_plan_enable_timing(pm_plan_t* plan, int max_steps);
 int pm_plan_phase_ms(pm_plan_t* plan, double* ms, int64_t* counts);
 int pm_plan_reset_timing(pm_plan_t* plan);
 
+/* ---- 4. FFT shapes and the tuning cache ----------------------------------
+ * The analogue of the reference's FFTW wisdom (fftw/fft.cpp:28-49,63-70:
+ * the FFT constructor imports wisdom_<n>_mpi_<size>.dat if present, plans
+ * FFTW_WISDOM_ONLY, else plans FFTW_EXHAUSTIVE and exports at destruction).
+ * Each of the five tunable FFT passes (slots 0 z fwd, 1 y fwd, 2 x (fused
+ * forward + Green + inverse, or the forward / inverse alone), 3 y inv,
+ * 4 z inv) runs in one of PM_FFT_VARIANTS shapes (lines per workgroup x
+ * threads per workgroup); variant 0 is the built-in default, and every
+ * variant gives bit-identical results.  Meshes below 64 always use
+ * variant 0.  Plans and FFT handles take the cached shape for their
+ * (ng, precision) when they are created. */
+#define PM_FFT_SLOTS 5
+#define PM_FFT_VARIANTS 6
+/* Time every variant of every slot on the plan's own buffers (`repeats`
+ * launches each, after one warm-up; the density, potential and spectra are
+ * overwritten, as FFTW_MEASURE overwrites its arrays), keep the fastest for
+ * the plan and store it in the cache.  solve_ms (may be NULL) receives the
+ * summed pass times of the chosen shape. */
+int pm_plan_tune(pm_plan_t* plan, int repeats, double* solve_ms);
+int pm_plan_set_fft_shape(pm_plan_t* plan, const int variants[PM_FFT_SLOTS]);
+int pm_plan_get_fft_shape(const pm_plan_t* plan, int variants[PM_FFT_SLOTS]);
+/* As pm_plan_tune for an FFT handle; slab (ng^3 reals, overwritten) is
+ * the scratch real mesh, slot 2 is timed as forward + inverse x passes. */
+int pm_fft_tune(pm_fft_t* fft, void* slab, int repeats, double* fft_ms);
+int pm_fft_get_shape(const pm_fft_t* fft, int variants[PM_FFT_SLOTS]);
+/* The cache as a text file: a "pm-wisdom 1" line, then one line per entry
+ * "ng fp32|fp64 v0 v1 v2 v3 v4 solve_ms".  Import merges (an entry for the
+ * same ng and precision is replaced); a malformed file imports nothing and
+ * returns PM_ERR_ARG. */
+int pm_wisdom_import(const char* path);
+int pm_wisdom_export(const char* path);
+/* *found = 1 and the variants if the cache holds (ng, precision), else 0. */
+int pm_wisdom_lookup(int ng, int precision, int variants[PM_FFT_SLOTS], int* found);
+void pm_wisdom_forget(void);
+
 #ifdef __cplusplus
 }
 #endif

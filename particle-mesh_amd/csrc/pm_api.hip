@@ -122,6 +122,7 @@ extern "C" int pm_fft_create(pm_fft_t** out, int ng, int precision, void* stream
     f->ng = ng;
     f->prec = precision;
     f->stream = (hipStream_t)stream;
+    wisdom_find(ng, precision, &f->shape);  // the cached shape, else the default
     const size_t cs = precision == PM_FP32 ? sizeof(float2) : sizeof(double2);
     int rc = tables_create(&f->tab, ng, precision, f->stream);
     if (!rc) rc = hip_ok(hipMalloc(&f->specA, cs * spec_elems(ng)), "pm_fft_create: hipMalloc");
@@ -146,9 +147,9 @@ extern "C" int pm_fft_destroy(pm_fft_t* f) {
 extern "C" int pm_fft_forward(pm_fft_t* f, const void* slab) {
     if (!f || !slab) return fail(PM_ERR_ARG, "pm_fft_forward: null argument");
     const GreenArgs ga{PM_GREEN_REF_KH, 1.0, f->tab.cos_tab};
-    fft_pass(f->prec, f->ng, kPassZFwd, f->stream, slab, f->specA, f->tab.twM, f->tab.twN, ga, 1.0, false);
-    fft_pass(f->prec, f->ng, kPassYFwd, f->stream, f->specA, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false);
-    fft_pass(f->prec, f->ng, kPassXFwd, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false);
+    fft_pass(f->prec, f->ng, kPassZFwd, f->stream, slab, f->specA, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
+    fft_pass(f->prec, f->ng, kPassYFwd, f->stream, f->specA, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
+    fft_pass(f->prec, f->ng, kPassXFwd, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
     return check_launch("pm_fft_forward");
 }
 
@@ -163,9 +164,9 @@ extern "C" int pm_fft_backward(pm_fft_t* f, void* slab) {
     if (!f || !slab) return fail(PM_ERR_ARG, "pm_fft_backward: null argument");
     const GreenArgs ga{PM_GREEN_REF_KH, 1.0, f->tab.cos_tab};
     const double n3 = (double)f->ng * f->ng * f->ng;
-    fft_pass(f->prec, f->ng, kPassXInv, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false);
-    fft_pass(f->prec, f->ng, kPassYInv, f->stream, f->specB, f->specA, f->tab.twM, f->tab.twN, ga, 1.0, false);
-    fft_pass(f->prec, f->ng, kPassZInv, f->stream, f->specA, slab, f->tab.twM, f->tab.twN, ga, 1.0 / n3, false);
+    fft_pass(f->prec, f->ng, kPassXInv, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
+    fft_pass(f->prec, f->ng, kPassYInv, f->stream, f->specB, f->specA, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
+    fft_pass(f->prec, f->ng, kPassZInv, f->stream, f->specA, slab, f->tab.twM, f->tab.twN, ga, 1.0 / n3, false, &f->shape);
     return check_launch("pm_fft_backward");
 }
 
@@ -206,6 +207,7 @@ extern "C" int pm_plan_create(pm_plan_t** out, int ng, int64_t np, int precision
     p->green_mode = green_mode;
     p->green_scale = green_scale;
     p->stream = (hipStream_t)stream;
+    wisdom_find(ng, precision, &p->fft_shape);  // the cached shape, else the default
     const size_t rs = precision == PM_FP32 ? sizeof(float) : sizeof(double);
     const size_t nc = (size_t)ng * ng * ng;
     int rc = tables_create(&p->tab, ng, precision, p->stream);
@@ -316,15 +318,15 @@ static int solve_into(pm_plan* p, const void* in, void* out, bool sub1) {
     const GreenArgs ga{p->green_mode, p->green_scale / ((double)N * N * N), p->tab.cos_tab};
     const void* tm = p->tab.twM;
     const void* tn = p->tab.twN;
-    fft_pass(p->prec, N, kPassZFwd, p->stream, in, p->specA, tm, tn, ga, 1.0, sub1);
+    fft_pass(p->prec, N, kPassZFwd, p->stream, in, p->specA, tm, tn, ga, 1.0, sub1, &p->fft_shape);
     mark(p, 2);
-    fft_pass(p->prec, N, kPassYFwd, p->stream, p->specA, p->specB, tm, tn, ga, 1.0, false);
+    fft_pass(p->prec, N, kPassYFwd, p->stream, p->specA, p->specB, tm, tn, ga, 1.0, false, &p->fft_shape);
     mark(p, 3);
-    fft_pass(p->prec, N, kPassXFused, p->stream, nullptr, p->specB, tm, tn, ga, 1.0, false);
+    fft_pass(p->prec, N, kPassXFused, p->stream, nullptr, p->specB, tm, tn, ga, 1.0, false, &p->fft_shape);
     mark(p, 4);
-    fft_pass(p->prec, N, kPassYInv, p->stream, p->specB, p->specA, tm, tn, ga, 1.0, false);
+    fft_pass(p->prec, N, kPassYInv, p->stream, p->specB, p->specA, tm, tn, ga, 1.0, false, &p->fft_shape);
     mark(p, 5);
-    fft_pass(p->prec, N, kPassZInv, p->stream, p->specA, out, tm, tn, ga, 1.0, false);
+    fft_pass(p->prec, N, kPassZInv, p->stream, p->specA, out, tm, tn, ga, 1.0, false, &p->fft_shape);
     mark(p, 6);
     return check_launch("pm_solve");
 }

@@ -177,13 +177,13 @@ template <typename Rl, bool XPASS = false> __host__ __device__ constexpr int til
 }
 
 // ---------------------------------------------------------------- z forward
-template <typename Rl, int N, bool SUB1, int NT>
+template <typename Rl, int N, bool SUB1, int T, int NT>
 __global__ __launch_bounds__(NT) void k_zfwd(const Rl* __restrict__ rho,
                                                       cplx_t<Rl>* __restrict__ out,
                                                       const cplx_t<Rl>* __restrict__ twM,
                                                       const cplx_t<Rl>* __restrict__ twN) {
     using C = cplx_t<Rl>;
-    constexpr int M = N / 2, T = tile_lines<Rl>(N), LP = lds_pitch(M), YT = N / T;
+    constexpr int M = N / 2, LP = lds_pitch(M), YT = N / T;
     __shared__ C s[T * LP];
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int x = tile / YT, y0 = (tile % YT) * T;
@@ -212,12 +212,12 @@ __global__ __launch_bounds__(NT) void k_zfwd(const Rl* __restrict__ rho,
 }
 
 // ---------------------------------------------------------------- y forward
-template <typename Rl, int N, int NT>
+template <typename Rl, int N, int T, int NT>
 __global__ __launch_bounds__(NT) void k_yfwd(const cplx_t<Rl>* __restrict__ in,
                                                       cplx_t<Rl>* __restrict__ out,
                                                       const cplx_t<Rl>* __restrict__ twN) {
     using C = cplx_t<Rl>;
-    constexpr int T = tile_lines<Rl>(N), LP = lds_pitch(N), XT = N / T;
+    constexpr int LP = lds_pitch(N), XT = N / T;
     __shared__ C s[T * LP];
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / XT, x0 = (tile % XT) * T;
@@ -249,12 +249,12 @@ __device__ __forceinline__ double green_value(int mode, int N, int i, int j, int
 }
 
 // ------------------------------------------------- x: forward/Green/inverse
-template <typename Rl, int N, int MODE, int NT>
+template <typename Rl, int N, int MODE, int T, int NT>
 __global__ __launch_bounds__(NT) void k_x(cplx_t<Rl>* __restrict__ data,
                                                    const cplx_t<Rl>* __restrict__ twN,
                                                    GreenArgs ga) {
     using C = cplx_t<Rl>;
-    constexpr int T = tile_lines<Rl, true>(N), LP = lds_pitch(N), YT = N / T;
+    constexpr int LP = lds_pitch(N), YT = N / T;
     __shared__ C s[T * LP];
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / YT, ky0 = (tile % YT) * T;
@@ -277,12 +277,12 @@ __global__ __launch_bounds__(NT) void k_x(cplx_t<Rl>* __restrict__ data,
 }
 
 // ---------------------------------------------------------------- y inverse
-template <typename Rl, int N, int NT>
+template <typename Rl, int N, int T, int NT>
 __global__ __launch_bounds__(NT) void k_yinv(const cplx_t<Rl>* __restrict__ in,
                                                       cplx_t<Rl>* __restrict__ out,
                                                       const cplx_t<Rl>* __restrict__ twN) {
     using C = cplx_t<Rl>;
-    constexpr int T = tile_lines<Rl>(N), LP = lds_pitch(N), XT = N / T;
+    constexpr int LP = lds_pitch(N), XT = N / T;
     __shared__ C s[T * LP];
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / XT, x0 = (tile % XT) * T;
@@ -298,13 +298,13 @@ __global__ __launch_bounds__(NT) void k_yinv(const cplx_t<Rl>* __restrict__ in,
 }
 
 // ---------------------------------------------------------------- z inverse
-template <typename Rl, int N, int NT>
+template <typename Rl, int N, int T, int NT>
 __global__ __launch_bounds__(NT) void k_zinv(const cplx_t<Rl>* __restrict__ in,
                                                       Rl* __restrict__ out,
                                                       const cplx_t<Rl>* __restrict__ twM,
                                                       const cplx_t<Rl>* __restrict__ twN, Rl norm) {
     using C = cplx_t<Rl>;
-    constexpr int M = N / 2, T = tile_lines<Rl>(N), LP = lds_pitch(M), YT = N / T;
+    constexpr int M = N / 2, LP = lds_pitch(M), YT = N / T;
     constexpr int NPRE = T * M;
     constexpr int PER = (NPRE + NT - 1) / NT;
     __shared__ C s[T * LP];
@@ -368,48 +368,74 @@ __global__ __launch_bounds__(256) void k_apply_green(cplx_t<Rl>* __restrict__ sp
 }
 
 // ================================================================ launchers
-template <typename Rl, int N>
-static void launch_pass(int pass, hipStream_t st, const void* in, void* out, const void* twM,
-                        const void* twN, const GreenArgs& ga, double norm, bool sub1) {
+// Shape variants of the five tunable passes (slots zfwd, yfwd, x, yinv,
+// zinv): variant v takes the default lines per tile (v < 3) or half of them
+// (v >= 3), and threads per workgroup: the pass's default (v % 3 == 0), then
+// the smaller and the larger of the other two of 256 / 512 / 1024, clamped by
+// fft_threads.  Variant 0 is the default shape; every variant computes the
+// same transform bit for bit (only the work split differs).  Meshes below
+// kTuneMinN (pm_internal.h) always take variant 0.
+
+__host__ __device__ constexpr int slot_default_threads(int slot) {
+    return slot == 0 ? PM_FFT_NT_ZF : slot == 1 ? PM_FFT_NT_YF : slot == 2 ? PM_FFT_NT_X
+         : slot == 3 ? PM_FFT_NT_YI : PM_FFT_NT_ZI;
+}
+__host__ __device__ constexpr int variant_threads(int slot, int v) {
+    const int d = slot_default_threads(slot), i = v % 3;
+    const int lo = d == 256 ? 512 : 256, hi = d == 1024 ? 512 : 1024;
+    return i == 0 ? d : i == 1 ? lo : hi;
+}
+template <typename Rl> __host__ __device__ constexpr int variant_lines(int slot, int N, int v) {
+    const int base = slot == 2 ? tile_lines<Rl, true>(N) : tile_lines<Rl>(N);
+    return v < 3 || base < 2 ? base : base / 2;
+}
+
+template <typename Rl, int N, int V>
+static void launch_variant(int pass, hipStream_t st, const void* in, void* out, const void* twM, const void* twN,
+                           const GreenArgs& ga, double norm, bool sub1) {
     using C = cplx_t<Rl>;
-    constexpr int T = tile_lines<Rl>(N), TX = tile_lines<Rl, true>(N);
     constexpr int M = N / 2;
-    constexpr int NZF = fft_threads(PM_FFT_NT_ZF, T * M), NYF = fft_threads(PM_FFT_NT_YF, T * N);
-    constexpr int NX = fft_threads(PM_FFT_NT_X, TX * N), NYI = fft_threads(PM_FFT_NT_YI, T * N);
-    constexpr int NZI = fft_threads(PM_FFT_NT_ZI, T * M);
-    const unsigned zt = (unsigned)((size_t)N * N / T);            // z passes: (x, y-tile)
-    const unsigned yt = (unsigned)((size_t)(M + 1) * N / T);      // y passes
-    const unsigned xt = (unsigned)((size_t)(M + 1) * N / TX);     // x pass
+    constexpr int TZF = variant_lines<Rl>(0, N, V), TYF = variant_lines<Rl>(1, N, V);
+    constexpr int TX = variant_lines<Rl>(2, N, V), TYI = variant_lines<Rl>(3, N, V);
+    constexpr int TZI = variant_lines<Rl>(4, N, V);
+    constexpr int NZF = fft_threads(variant_threads(0, V), TZF * M), NYF = fft_threads(variant_threads(1, V), TYF * N);
+    constexpr int NX = fft_threads(variant_threads(2, V), TX * N), NYI = fft_threads(variant_threads(3, V), TYI * N);
+    constexpr int NZI = fft_threads(variant_threads(4, V), TZI * M);
     switch (pass) {
-        case kPassZFwd:
+        case kPassZFwd: {
+            const unsigned g = (unsigned)((size_t)N * N / TZF);  // (x, y-tile)
             if (sub1)
-                hipLaunchKernelGGL((k_zfwd<Rl, N, true, NZF>), dim3(zt), dim3(NZF), 0, st, (const Rl*)in, (C*)out,
+                hipLaunchKernelGGL((k_zfwd<Rl, N, true, TZF, NZF>), dim3(g), dim3(NZF), 0, st, (const Rl*)in, (C*)out,
                                    (const C*)twM, (const C*)twN);
             else
-                hipLaunchKernelGGL((k_zfwd<Rl, N, false, NZF>), dim3(zt), dim3(NZF), 0, st, (const Rl*)in, (C*)out,
-                                   (const C*)twM, (const C*)twN);
+                hipLaunchKernelGGL((k_zfwd<Rl, N, false, TZF, NZF>), dim3(g), dim3(NZF), 0, st, (const Rl*)in,
+                                   (C*)out, (const C*)twM, (const C*)twN);
             break;
+        }
         case kPassYFwd:
-            hipLaunchKernelGGL((k_yfwd<Rl, N, NYF>), dim3(yt), dim3(NYF), 0, st, (const C*)in, (C*)out,
-                               (const C*)twN);
+            hipLaunchKernelGGL((k_yfwd<Rl, N, TYF, NYF>), dim3((unsigned)((size_t)(M + 1) * N / TYF)), dim3(NYF), 0,
+                               st, (const C*)in, (C*)out, (const C*)twN);
             break;
         case kPassXFused:
-            hipLaunchKernelGGL((k_x<Rl, N, kXForward | kXGreen | kXInverse, NX>), dim3(xt), dim3(NX), 0, st,
-                               (C*)out, (const C*)twN, ga);
-            break;
         case kPassXFwd:
-            hipLaunchKernelGGL((k_x<Rl, N, kXForward, NX>), dim3(xt), dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
+        case kPassXInv: {
+            const dim3 g((unsigned)((size_t)(M + 1) * N / TX));
+            if (pass == kPassXFused)
+                hipLaunchKernelGGL((k_x<Rl, N, kXForward | kXGreen | kXInverse, TX, NX>), g, dim3(NX), 0, st, (C*)out,
+                                   (const C*)twN, ga);
+            else if (pass == kPassXFwd)
+                hipLaunchKernelGGL((k_x<Rl, N, kXForward, TX, NX>), g, dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
+            else
+                hipLaunchKernelGGL((k_x<Rl, N, kXInverse, TX, NX>), g, dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
             break;
-        case kPassXInv:
-            hipLaunchKernelGGL((k_x<Rl, N, kXInverse, NX>), dim3(xt), dim3(NX), 0, st, (C*)out, (const C*)twN, ga);
-            break;
+        }
         case kPassYInv:
-            hipLaunchKernelGGL((k_yinv<Rl, N, NYI>), dim3(yt), dim3(NYI), 0, st, (const C*)in, (C*)out,
-                               (const C*)twN);
+            hipLaunchKernelGGL((k_yinv<Rl, N, TYI, NYI>), dim3((unsigned)((size_t)(M + 1) * N / TYI)), dim3(NYI), 0,
+                               st, (const C*)in, (C*)out, (const C*)twN);
             break;
         case kPassZInv:
-            hipLaunchKernelGGL((k_zinv<Rl, N, NZI>), dim3(zt), dim3(NZI), 0, st, (const C*)in, (Rl*)out,
-                               (const C*)twM, (const C*)twN, (Rl)norm);
+            hipLaunchKernelGGL((k_zinv<Rl, N, TZI, NZI>), dim3((unsigned)((size_t)N * N / TZI)), dim3(NZI), 0, st,
+                               (const C*)in, (Rl*)out, (const C*)twM, (const C*)twN, (Rl)norm);
             break;
         case kPassGreen: {
             const size_t total = (size_t)(M + 1) * N * N;
@@ -422,27 +448,60 @@ static void launch_pass(int pass, hipStream_t st, const void* in, void* out, con
     }
 }
 
+template <typename Rl, int N>
+static void launch_pass(int pass, int v, hipStream_t st, const void* in, void* out, const void* twM,
+                        const void* twN, const GreenArgs& ga, double norm, bool sub1) {
+    if constexpr (N < kTuneMinN) {
+        launch_variant<Rl, N, 0>(pass, st, in, out, twM, twN, ga, norm, sub1);
+    } else {
+        switch (v) {
+            case 1: launch_variant<Rl, N, 1>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+            case 2: launch_variant<Rl, N, 2>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+            case 3: launch_variant<Rl, N, 3>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+            case 4: launch_variant<Rl, N, 4>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+            case 5: launch_variant<Rl, N, 5>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+            default: launch_variant<Rl, N, 0>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        }
+    }
+}
+
 template <typename Rl>
-static int dispatch_pass(int N, int pass, hipStream_t st, const void* in, void* out, const void* twM,
+static int dispatch_pass(int N, int pass, int v, hipStream_t st, const void* in, void* out, const void* twM,
                          const void* twN, const GreenArgs& ga, double norm, bool sub1) {
     switch (N) {
-        case 8: launch_pass<Rl, 8>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
-        case 16: launch_pass<Rl, 16>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
-        case 32: launch_pass<Rl, 32>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
-        case 64: launch_pass<Rl, 64>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
-        case 128: launch_pass<Rl, 128>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
-        case 256: launch_pass<Rl, 256>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
-        case 512: launch_pass<Rl, 512>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
-        case 1024: launch_pass<Rl, 1024>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 8: launch_pass<Rl, 8>(pass, v, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 16: launch_pass<Rl, 16>(pass, v, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 32: launch_pass<Rl, 32>(pass, v, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 64: launch_pass<Rl, 64>(pass, v, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 128: launch_pass<Rl, 128>(pass, v, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 256: launch_pass<Rl, 256>(pass, v, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 512: launch_pass<Rl, 512>(pass, v, st, in, out, twM, twN, ga, norm, sub1); break;
+        case 1024: launch_pass<Rl, 1024>(pass, v, st, in, out, twM, twN, ga, norm, sub1); break;
         default: return -1;
     }
     return 0;
 }
 
+int fft_slot(int pass) {
+    switch (pass) {
+        case kPassZFwd: return 0;
+        case kPassYFwd: return 1;
+        case kPassXFused:
+        case kPassXFwd:
+        case kPassXInv: return 2;
+        case kPassYInv: return 3;
+        case kPassZInv: return 4;
+        default: return -1;
+    }
+}
+
 int fft_pass(int prec, int N, int pass, hipStream_t st, const void* in, void* out, const void* twM,
-             const void* twN, const GreenArgs& ga, double norm, bool sub1) {
-    return prec == PM_FP32 ? dispatch_pass<float>(N, pass, st, in, out, twM, twN, ga, norm, sub1)
-                           : dispatch_pass<double>(N, pass, st, in, out, twM, twN, ga, norm, sub1);
+             const void* twN, const GreenArgs& ga, double norm, bool sub1, const FftShape* shape) {
+    const int slot = fft_slot(pass);
+    const int v = shape && slot >= 0 ? shape->v[slot] : 0;
+    if (v < 0 || v >= kFftVariants) return -1;
+    return prec == PM_FP32 ? dispatch_pass<float>(N, pass, v, st, in, out, twM, twN, ga, norm, sub1)
+                           : dispatch_pass<double>(N, pass, v, st, in, out, twM, twN, ga, norm, sub1);
 }
 
 }  // namespace pm

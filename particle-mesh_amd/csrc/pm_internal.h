@@ -40,9 +40,28 @@ struct FftTables {
 int tables_create(FftTables* t, int ng, int prec, hipStream_t st);
 void tables_destroy(FftTables* t);
 
-// Launch one FFT pass (see pm_fft.hip).  Returns -1 for an unsupported N.
+// Shape choice of the five tunable FFT passes (slots zfwd, yfwd, x, yinv,
+// zinv): a variant index each, 0 = the default shape (pm_fft.hip).
+constexpr int kFftSlots = PM_FFT_SLOTS;
+constexpr int kFftVariants = PM_FFT_VARIANTS;
+constexpr int kTuneMinN = 64;  // smaller meshes always take variant 0
+struct FftShape {
+    int v[kFftSlots] = {0, 0, 0, 0, 0};
+};
+
+// Slot of a pass (-1 for the standalone Green's multiply).
+int fft_slot(int pass);
+
+// Launch one FFT pass (see pm_fft.hip) in the shape that 'shape' selects for
+// its slot (nullptr: the default).  Returns -1 for an unsupported N or variant.
 int fft_pass(int prec, int N, int pass, hipStream_t st, const void* in, void* out, const void* twM,
-             const void* twN, const GreenArgs& ga, double norm, bool sub1);
+             const void* twN, const GreenArgs& ga, double norm, bool sub1, const FftShape* shape = nullptr);
+
+// Process-wide tuning cache (pm_wisdom.hip): the shape chosen for a mesh
+// side and precision by pm_plan_tune / pm_fft_tune or read by
+// pm_wisdom_import.
+bool wisdom_find(int ng, int prec, FftShape* shape);
+void wisdom_store(int ng, int prec, const FftShape& shape, double solve_ms);
 
 // Grid-stride fill (used instead of hipMemsetAsync so that the step can be
 // captured in a graph).
@@ -156,6 +175,7 @@ struct pm_plan {
     void* specA = nullptr;   // (ng/2+1) ng^2 complex
     void* specB = nullptr;
     pm::FftTables tab;
+    pm::FftShape fft_shape;  // the FFT pass shapes (default, cached or tuned)
     pm::SortedScratch sorted;  // allocated on the first sorted deposit
     // the interpolation of the last pm_step staged positions staged_x in tile
     // order (sorted.stage) for the next deposit (PM_STEP_REUSE_BINS)
@@ -182,4 +202,5 @@ struct pm_fft {
     void* specA = nullptr;
     void* specB = nullptr;
     pm::FftTables tab;
+    pm::FftShape shape;
 };

@@ -1,6 +1,8 @@
 // The FFT class at one box over pm_fft_* (reference: src/fft/ffte1/fft.cpp).
 #include "fft.h"
 
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -10,14 +12,44 @@ namespace {
 void check(int rc, const char* what) {
     if (rc != PM_OK) throw std::runtime_error(std::string(what) + ": " + pm_last_error());
 }
-}  // namespace
 
-FFT::FFT(int n, double* buf) : n_(n), real_(buf), h_(nullptr) {
-    if (!buf) throw std::invalid_argument("FFT: null buffer");
-    check(pm_fft_create(&h_, n, PM_FP64, nullptr), "FFT::FFT");
+// wisdom_<n>_mpi_<size>.dat of fftw/fft.cpp:28 for the one-GPU build, in
+// $PM_WISDOM_DIR if set, else in the working directory as the reference does.
+std::string wisdom_file(int n) {
+    const std::string name = "wisdom_" + std::to_string(n) + "_hip.dat";
+    const char* dir = std::getenv("PM_WISDOM_DIR");
+    return dir && *dir ? std::string(dir) + "/" + name : name;
 }
 
-FFT::~FFT() { pm_fft_destroy(h_); }
+bool readable(const std::string& path) {
+    std::FILE* fp = std::fopen(path.c_str(), "r");
+    if (fp) std::fclose(fp);
+    return fp != nullptr;
+}
+}  // namespace
+
+// fftw/fft.cpp:28-49: import the wisdom file if it is there; otherwise
+// measure (here: time every FFT pass shape on buf, which is overwritten as
+// FFTW_EXHAUSTIVE planning overwrites its arrays) and export at destruction.
+FFT::FFT(int n, double* buf) : n_(n), real_(buf), h_(nullptr), filename_(wisdom_file(n)), need_export_(false) {
+    if (!buf) throw std::invalid_argument("FFT: null buffer");
+    const bool loaded = readable(filename_) && pm_wisdom_import(filename_.c_str()) == PM_OK;
+    check(pm_fft_create(&h_, n, PM_FP64, nullptr), "FFT::FFT");
+    if (!loaded) {
+        const int rc = pm_fft_tune(h_, buf, 5, nullptr);
+        if (rc != PM_OK) {
+            pm_fft_destroy(h_);
+            check(rc, "FFT::FFT tune");
+        }
+        need_export_ = true;
+    }
+}
+
+// fftw/fft.cpp:63-70
+FFT::~FFT() {
+    if (need_export_) (void)pm_wisdom_export(filename_.c_str());
+    pm_fft_destroy(h_);
+}
 
 int FFT::color() const { return 1; }
 

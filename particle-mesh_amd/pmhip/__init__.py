@@ -29,6 +29,8 @@ PM_DEPOSIT_ATOMIC = 0
 PM_DEPOSIT_SORTED = 1
 PM_STEP_REUSE_BINS = 0x100
 PM_NPHASES = 7
+PM_FFT_SLOTS = 5
+PM_FFT_VARIANTS = 6
 
 PHASE_NAMES = ["deposit", "fft_z_fwd", "fft_y_fwd", "fft_x_green", "fft_y_inv", "fft_z_inv", "interp_kick_drift"]
 
@@ -93,6 +95,15 @@ SIGNATURES = {
     "pm_plan_enable_timing": (_c_int, [_c_vp, _c_int]),
     "pm_plan_phase_ms": (_c_int, [_c_vp, ctypes.POINTER(_c_dbl), ctypes.POINTER(_c_i64)]),
     "pm_plan_reset_timing": (_c_int, [_c_vp]),
+    "pm_plan_tune": (_c_int, [_c_vp, _c_int, ctypes.POINTER(_c_dbl)]),
+    "pm_plan_set_fft_shape": (_c_int, [_c_vp, ctypes.POINTER(_c_int)]),
+    "pm_plan_get_fft_shape": (_c_int, [_c_vp, ctypes.POINTER(_c_int)]),
+    "pm_fft_tune": (_c_int, [_c_vp, _c_vp, _c_int, ctypes.POINTER(_c_dbl)]),
+    "pm_fft_get_shape": (_c_int, [_c_vp, ctypes.POINTER(_c_int)]),
+    "pm_wisdom_import": (_c_int, [ctypes.c_char_p]),
+    "pm_wisdom_export": (_c_int, [ctypes.c_char_p]),
+    "pm_wisdom_lookup": (_c_int, [_c_int, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
+    "pm_wisdom_forget": (None, []),
 }
 
 _lib = None
@@ -274,6 +285,26 @@ class Plan:
         check(lib().pm_plan_phase_ms(self.h, ms, cnt))
         return list(ms), list(cnt)
 
+    # ------------------------------------------------- FFT shapes / tuning
+    def tune(self, repeats: int = 10) -> float:
+        """Time every FFT pass shape on this plan (overwrites its density,
+        potential and spectra), keep the fastest and cache it (pm_plan_tune).
+        Returns the summed pass milliseconds of the chosen shape."""
+        ms = ctypes.c_double()
+        check(lib().pm_plan_tune(self.h, repeats, ctypes.byref(ms)))
+        return ms.value
+
+    @property
+    def fft_shape(self) -> list[int]:
+        v = (ctypes.c_int * PM_FFT_SLOTS)()
+        check(lib().pm_plan_get_fft_shape(self.h, v))
+        return list(v)
+
+    @fft_shape.setter
+    def fft_shape(self, variants) -> None:
+        v = (ctypes.c_int * PM_FFT_SLOTS)(*variants)
+        check(lib().pm_plan_set_fft_shape(self.h, v))
+
 
 class FFT:
     """The reference's FFT class at one box (pm_fft_* of include/pm_api.h)."""
@@ -303,6 +334,18 @@ class FFT:
 
     def backward(self, slab):
         check(lib().pm_fft_backward(self.h, dptr(slab)))
+
+    def tune(self, slab, repeats: int = 10) -> float:
+        """pm_fft_tune: slab (ng^3 reals, overwritten) is the scratch mesh."""
+        ms = ctypes.c_double()
+        check(lib().pm_fft_tune(self.h, dptr(slab), repeats, ctypes.byref(ms)))
+        return ms.value
+
+    @property
+    def shape(self) -> list[int]:
+        v = (ctypes.c_int * PM_FFT_SLOTS)()
+        check(lib().pm_fft_get_shape(self.h, v))
+        return list(v)
 
     def spectrum(self):
         """Complex view [kz][ky][kx] (kz in [0, ng/2]) of the handle's spectrum."""
@@ -376,3 +419,25 @@ def load_snapshot_numpy(path):
     a = float(np.frombuffer(head[32:40], "<f8")[0])
     data = np.fromfile(path, dtype="<f8", offset=SNAP_HEADER, count=6 * n).reshape(6, n)
     return {"np": n, "ng": ng, "step": step, "a": a}, data[:3], data[3:]
+
+
+# ------------------------------------------------------------ tuning cache
+def wisdom_import(path) -> None:
+    """Merge a wisdom file into the process-wide FFT shape cache."""
+    check(lib().pm_wisdom_import(str(path).encode()))
+
+
+def wisdom_export(path) -> None:
+    check(lib().pm_wisdom_export(str(path).encode()))
+
+
+def wisdom_lookup(ng: int, precision: int):
+    """The cached shape of (ng, precision) as a list of 5 variants, or None."""
+    v = (ctypes.c_int * PM_FFT_SLOTS)()
+    found = ctypes.c_int()
+    check(lib().pm_wisdom_lookup(ng, precision, v, ctypes.byref(found)))
+    return list(v) if found.value else None
+
+
+def wisdom_forget() -> None:
+    lib().pm_wisdom_forget()

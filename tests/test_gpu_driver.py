@@ -3,6 +3,7 @@ main.cpp loop over the drop-in FFT / TransposeSlab / ReorderSlab /
 ReorderSlabBack / TransposeSlabBack classes and utils functions) reproduces
 the reference's own kick: positions never move in that loop, so after W + L
 iterations the momenta are (W + L) times the golden one-kick momenta."""
+import os
 import subprocess
 from pathlib import Path
 
@@ -24,7 +25,7 @@ def test_driver_reproduces_reference_kicks(dev, golden_dir, tmp_path, name):
     out = tmp_path / "p.bin"
     print(f"case driver {name}", flush=True)
     r = subprocess.run([str(EXE), str(ng), str(nrow), "--warmup", "1", "--loop", "2", "--dump", str(out)],
-                       capture_output=True, text=True, timeout=120)
+                       capture_output=True, text=True, timeout=120, env=dict(os.environ, PM_WISDOM_DIR=str(tmp_path)))
     assert r.returncode == 0, r.stderr
     times = [float(v) for v in r.stdout.split()]
     assert len(times) == 9 and all(t >= 0 for t in times)
