@@ -84,14 +84,16 @@ def algorithmic_bytes(ng: int, npart: int, prec: int, staged: bool = True):
     }
 
 
-# kernel that carries each phase of the FP32 sorted step, as rocprofv3 names it
+# kernel that carries each phase of the FP32 sorted step, as rocprofv3 names it;
+# for the FFT passes a name prefix: the lines / threads template arguments that
+# follow it are the shape that pm_plan_tune chose
 PHASE_KERNELS = {
     "deposit": "void pm::k_deposit_staged<float, 16>",
-    "fft_z_fwd": "void pm::k_zfwd<float, 512, true, 512>",
-    "fft_y_fwd": "void pm::k_yfwd<float, 512, 1024>",
-    "fft_x_green": "void pm::k_x<float, 512, 7, 512>",
-    "fft_y_inv": "void pm::k_yinv<float, 512, 1024>",
-    "fft_z_inv": "void pm::k_zinv<float, 512, 512>",
+    "fft_z_fwd": "void pm::k_zfwd<float, 512, true,",
+    "fft_y_fwd": "void pm::k_yfwd<float, 512,",
+    "fft_x_green": "void pm::k_x<float, 512, 7,",
+    "fft_y_inv": "void pm::k_yinv<float, 512,",
+    "fft_z_inv": "void pm::k_zinv<float, 512,",
     "interp_kick_drift": "void pm::k_interp<float, 2, 256, 1>",
 }
 
