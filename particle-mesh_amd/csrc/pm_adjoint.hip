@@ -121,13 +121,13 @@ __global__ __launch_bounds__(256) void k_adj_deposit(const double* __restrict__ 
 }
 
 // ------------------------------------------- interpolation transpose
-// U_a[c] += W(x_p, c) u_p^a for a = 0..2, one workgroup per tile of the
-// sorted binning.  The cotangents have arbitrary sign and scale, so each
+// U_a[c] += W(x_p, c) u_p^a for a = 0..2, one workgroup per segment of the
+// column walk over the tiles of the sorted binning.  The cotangents have arbitrary sign and scale, so each
 // component is accumulated in signed 64-bit fixed point with a power-of-two
 // scale 2^40 / 2^ceil(log2 max|u_a|) taken from a global max (k_absmax3):
 // integer LDS adds are exact and order-free (LDS float atomics serialise on
-// this part).  Owned nodes and shells are then written as in the forward
-// deposit.
+// this part).  Owned nodes, xy shells and z seams are written as in the
+// forward deposit, and folded by the same kernel.
 constexpr int kVec3Threads = 1024;
 
 __device__ __forceinline__ double vec3_scale(unsigned long long maxbits) {

@@ -268,7 +268,11 @@ static int ensure_sorted(pm_plan* p) {
     const size_t ntiles = nt * nt * nt;
     const size_t rs = p->prec == PM_FP32 ? sizeof(float) : sizeof(double);
     int rc = hip_ok(hipMalloc((void**)&s.cnt, 4 * ntiles), "sorted deposit: hipMalloc");
-    if (!rc) rc = hip_ok(hipMemsetAsync(s.cnt, 0, 4 * ntiles, p->stream), "sorted deposit: clear counts");
+    if (!rc) {  // a fill kernel, as everywhere on the step path (no memset nodes if a caller captures it)
+        hipLaunchKernelGGL(k_fill<uint32_t>, dim3((unsigned)std::min<size_t>((ntiles + 255) / 256, 8192)), dim3(256),
+                           0, p->stream, s.cnt, ntiles, 0u);
+        rc = check_launch("sorted deposit: clear counts");
+    }
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.off, 4 * (ntiles + 1)), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.key, 4 * (size_t)p->np), "sorted deposit: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc((void**)&s.slot, 4 * (size_t)p->np), "sorted deposit: hipMalloc");

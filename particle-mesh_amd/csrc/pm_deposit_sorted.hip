@@ -9,15 +9,19 @@
 //                    tile per chunk of 2048 particles)
 //   2. scan        : exclusive prefix sum of the tile counts
 //   3. bin_scatter : particle indices in tile order
-//   4. tiles       : one workgroup per tile of TS^3 cells; the CIC weights of
-//                    its particles are added in 2^-32 fixed point to a
-//                    (TS+1)^3 LDS node tile (integer adds: exact, order-free),
-//                    the TS^3 owned nodes are written to the mesh and the
-//                    upper shell (nodes TS on some axis, owned by the next
-//                    tiles) to a per-tile shell buffer
-//   5. fold        : each tile adds the shells of its lower neighbours to its
-//                    lower-face nodes in a fixed order -- the periodic ghost
-//                    fold of reorder_slab.cpp:43-67 at tile granularity.
+//   4. tiles       : one workgroup per segment of consecutive tiles of TS^3
+//                    cells along z (the column walk of pm_tiles.h); the CIC
+//                    weights of a tile's particles are added in 2^-32 fixed
+//                    point to a (TS+1)^3 LDS node tile (integer adds: exact,
+//                    order-free), the TS^3 owned nodes are written to the
+//                    mesh, the upper x / y faces to the tile's xy shell, and
+//                    the upper z plane is carried in LDS into the next tile
+//                    (the segment's last one goes to its z seam)
+//   5. fold        : each tile adds the xy shells of its lower neighbours to
+//                    its lower x / y faces, and a segment's first tile the
+//                    seams to its first plane, in a fixed order -- the
+//                    periodic ghost fold of reorder_slab.cpp:43-67 at tile
+//                    granularity.
 // Every mesh node is written by exactly one workgroup: no global float
 // atomics, and the result does not depend on the particle order.
 #include <type_traits>

@@ -87,7 +87,7 @@ struct SortedScratch {
     uint32_t* key = nullptr;   // np
     uint32_t* slot = nullptr;  // np
     uint32_t* idx = nullptr;   // np
-    void* shell = nullptr;     // ntiles * shell_size reals
+    void* shell = nullptr;     // sorted_shell_elems reals: xy shells, then z seams
     // Staging in tile order, filled by the interpolation for the next
     // step's deposit: bucket t holds fill[t] in-tile offsets from coff[t]
     // (capacity coff[t+1] - coff[t]); spill != 0 if a bucket overflowed.
