@@ -88,6 +88,26 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
 #define PM_STAGE_PER64 1
 #endif
 constexpr int kStageThreads = 256;
+
+// Loads and stores of the particle streams, each read once and written
+// once: non-temporal, so that the 13 GB per launch (512^3 particles) stream
+// through L2 without evicting the potential's planes, which the gathers of
+// neighbouring particle rows re-read.  Measured in an interleaved A/B at
+// 512^3: 3.86 -> 3.40 ms (FP32 mesh), 5.41 -> 4.95 ms (FP64); stores alone
+// 3.81, loads alone 3.61.  Bit 0: loads, bit 1: stores.
+#ifndef PM_INTERP_NT
+#define PM_INTERP_NT 3
+#endif
+__device__ __forceinline__ double stream_load(const double* p) {
+    if constexpr (PM_INTERP_NT & 1) return __builtin_nontemporal_load(p);
+    return *p;
+}
+__device__ __forceinline__ void stream_store(double* p, double v) {
+    if constexpr (PM_INTERP_NT & 2)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
 constexpr int kStagePer32 = PM_STAGE_PER32, kStagePer64 = PM_STAGE_PER64;
 constexpr int kStageHash = 128;
 
@@ -125,13 +145,13 @@ __global__ __launch_bounds__(NT) void k_interp(double* __restrict__ px, double* 
         idx[q] = i0 + (int64_t)q * NT;
         live[q] = idx[q] < np;
         const int64_t j = live[q] ? idx[q] : 0;
-        x[q] = px[j];
-        y[q] = py[j];
-        z[q] = pz[j];
+        x[q] = stream_load(px + j);
+        y[q] = stream_load(py + j);
+        z[q] = stream_load(pz + j);
         if constexpr (MODE != 1) {
-            u[q][0] = vx[j];
-            u[q][1] = vy[j];
-            u[q][2] = vz[j];
+            u[q][0] = stream_load(vx + j);
+            u[q][1] = stream_load(vy + j);
+            u[q][2] = stream_load(vz + j);
         }
     }
     CellIdx<Rl> c[PER];
@@ -163,17 +183,17 @@ __global__ __launch_bounds__(NT) void k_interp(double* __restrict__ px, double* 
         } else {
             const double nvx = u[q][0] + (double)f[0] * kick, nvy = u[q][1] + (double)f[1] * kick,
                          nvz = u[q][2] + (double)f[2] * kick;
-            vx[i] = nvx;
-            vy[i] = nvy;
-            vz[i] = nvz;
+            stream_store(vx + i, nvx);
+            stream_store(vy + i, nvy);
+            stream_store(vz + i, nvz);
             double xn = x[q], yn = y[q], zn = z[q];
             if (drift != 0.0) {
                 xn = wrap_pos(xn + nvx * drift, box);
                 yn = wrap_pos(yn + nvy * drift, box);
                 zn = wrap_pos(zn + nvz * drift, box);
-                px[i] = xn;
-                py[i] = yn;
-                pz[i] = zn;
+                stream_store(px + i, xn);
+                stream_store(py + i, yn);
+                stream_store(pz + i, zn);
             }
             if constexpr (MODE == 2) {
                 // a particle that drifted more than a box away stays unstaged
