@@ -292,61 +292,65 @@ __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, dou
                                                     unsigned long long* __restrict__ err) {
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const double box = (double)N;
-    bool ok = i < np;
-    double x = 0.0, y = 0.0, z = 0.0;
-    double m[3] = {0.0, 0.0, 0.0};  // MODE 2: |u| of the next step
-    if (ok) {
-        x = px[i];
-        y = py[i];
-        z = pz[i];
-        if (!in_box3(x, y, z, box)) {
-            atomicAdd(err, 1ULL);
-            ok = false;
-        }
+    // three load phases, as in k_interp: the particle's streams, then the
+    // gathers around its cell (cell 0 for a lane that stores nothing), then
+    // the arithmetic and the stores
+    const bool live = i < np;
+    const int64_t j = live ? i : 0;
+    const double x = stream_load(px + j), y = stream_load(py + j), z = stream_load(pz + j);
+    const double u[3] = {stream_load(ux + j), stream_load(uy + j), stream_load(uz + j)};
+    const double b0[3] = {stream_load(bx + j), stream_load(by + j), stream_load(bz + j)};
+    double v0[3] = {0.0, 0.0, 0.0};
+    if constexpr (MODE != 1) {
+        v0[0] = stream_load(vx + j);
+        v0[1] = stream_load(vy + j);
+        v0[2] = stream_load(vz + j);
     }
+    const bool ok = live & in_box3(x, y, z, box);
+    if (live && !ok) atomicAdd(err, 1ULL);
+    const CellIdx<Rl> c = cell_of<Rl>(ok ? x : 0.0, ok ? y : 0.0, ok ? z : 0.0, N);
+    Rl cl[2][2][4], xr[2][2][2], yr[2][2][2], lf[2][2][2];
+    gather_runs_main(phi, N, c, cl, xr, yr);
+    gather_corners_main(lam, N, c, lf);
+    gather_runs_seam(phi, N, c, cl, xr, yr);
+    gather_corners_seam(lam, N, c, lf);
+    double m[3] = {0.0, 0.0, 0.0};  // MODE 2: |u| of the next step
+    Rl gx[2][2][2], gy[2][2][2], gz[2][2][2];
+    corner_differences(cl, xr, yr, gx, gy, gz);
+    Rl F[3], dG[3][3], dL[3], lv;
+    tri_val_grad(gx, c.tx, c.ty, c.tz, F[0], dG[0]);
+    tri_val_grad(gy, c.tx, c.ty, c.tz, F[1], dG[1]);
+    tri_val_grad(gz, c.tx, c.ty, c.tz, F[2], dG[2]);
+    tri_val_grad(lf, c.tx, c.ty, c.tz, lv, dL);
     if (ok) {
-        const CellIdx<Rl> c = cell_of<Rl>(x, y, z, N);
-        Rl cl[2][2][4], xr[2][2][2], yr[2][2][2];
-        gather_runs(phi, N, c, cl, xr, yr);
-        Rl lf[2][2][2];
-        gather_corners(lam, N, c, lf);
-        Rl gx[2][2][2], gy[2][2][2], gz[2][2][2];
-        corner_differences(cl, xr, yr, gx, gy, gz);
-        Rl F[3], dG[3][3], dL[3], lv;
-        tri_val_grad(gx, c.tx, c.ty, c.tz, F[0], dG[0]);
-        tri_val_grad(gy, c.tx, c.ty, c.tz, F[1], dG[1]);
-        tri_val_grad(gz, c.tx, c.ty, c.tz, F[2], dG[2]);
-        tri_val_grad(lf, c.tx, c.ty, c.tz, lv, dL);
-        const double u[3] = {ux[i], uy[i], uz[i]};
         double nb[3];
-        const double b0[3] = {bx[i], by[i], bz[i]};
 #pragma unroll
         for (int d = 0; d < 3; ++d)
             nb[d] = b0[d] + vjp_scale * (u[0] * (double)dG[0][d] + u[1] * (double)dG[1][d] +
                                          u[2] * (double)dG[2][d] + mass * (double)dL[d]);
-        bx[i] = nb[0];
-        by[i] = nb[1];
-        bz[i] = nb[2];
+        stream_store(bx + i, nb[0]);
+        stream_store(by + i, nb[1]);
+        stream_store(bz + i, nb[2]);
         if constexpr (MODE != 1) {
-            const double nv[3] = {vx[i] - kick * (double)F[0], vy[i] - kick * (double)F[1],
-                                  vz[i] - kick * (double)F[2]};
-            vx[i] = nv[0];
-            vy[i] = nv[1];
-            vz[i] = nv[2];
+            const double nv[3] = {v0[0] - kick * (double)F[0], v0[1] - kick * (double)F[1],
+                                  v0[2] - kick * (double)F[2]};
+            stream_store(vx + i, nv[0]);
+            stream_store(vy + i, nv[1]);
+            stream_store(vz + i, nv[2]);
             if constexpr (MODE == 2) {
                 auto wrap = [&](double v) {
                     if (v < 0.0) v += box;
                     if (v >= box) v -= box;
                     return v;
                 };
-                px[i] = wrap(x - next_drift * nv[0]);
-                py[i] = wrap(y - next_drift * nv[1]);
-                pz[i] = wrap(z - next_drift * nv[2]);
+                stream_store(px + i, wrap(x - next_drift * nv[0]));
+                stream_store(py + i, wrap(y - next_drift * nv[1]));
+                stream_store(pz + i, wrap(z - next_drift * nv[2]));
                 const double nu[3] = {u[0] + next_drift * nb[0], u[1] + next_drift * nb[1],
                                       u[2] + next_drift * nb[2]};
-                ux[i] = nu[0];
-                uy[i] = nu[1];
-                uz[i] = nu[2];
+                stream_store(ux + i, nu[0]);
+                stream_store(uy + i, nu[1]);
+                stream_store(uz + i, nu[2]);
 #pragma unroll
                 for (int d = 0; d < 3; ++d) m[d] = fabs(nu[d]);
             }

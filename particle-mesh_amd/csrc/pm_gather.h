@@ -130,23 +130,38 @@ __device__ __forceinline__ void gather_runs(const Rl* __restrict__ phi, int N, c
 }
 
 
-// Values of a mesh at the 8 corners of the cell (4 z-runs of 2).
+// Values of a mesh at the 8 corners of the cell (4 z-runs of 2), in the
+// same two parts as gather_runs: the runs at z starts clamped into the mesh,
+// then the values of a cell on the seam (iz + 1 == N) reloaded one by one.
 template <typename Rl>
-__device__ __forceinline__ void gather_corners(const Rl* __restrict__ m, int N, const CellIdx<Rl>& c,
-                                               Rl (&f)[2][2][2]) {
+__device__ __forceinline__ void gather_corners_main(const Rl* __restrict__ m, int N, const CellIdx<Rl>& c,
+                                                    Rl (&f)[2][2][2]) {
+    const int z2 = min(c.iz, N - 2);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) load_run2(mesh_at(m, c.X[a + 1] + c.Y[b + 1] + z2), f[a][b]);
+}
+
+template <typename Rl>
+__device__ __forceinline__ void gather_corners_seam(const Rl* __restrict__ m, int N, const CellIdx<Rl>& c,
+                                                    Rl (&f)[2][2][2]) {
     const int iz = c.iz;
-    const uint32_t z1 = (uint32_t)(iz + 1 == N ? 0 : iz + 1);
+    if (iz + 1 < N) return;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            if (iz + 1 < N) {
-                load_run2(mesh_at(m, c.X[a + 1] + c.Y[b + 1] + iz), f[a][b]);
-            } else {
-                f[a][b][0] = *mesh_at(m, c.X[a + 1] + c.Y[b + 1] + iz);
-                f[a][b][1] = *mesh_at(m, c.X[a + 1] + c.Y[b + 1] + z1);
-            }
+            f[a][b][0] = *mesh_at(m, c.X[a + 1] + c.Y[b + 1] + iz);
+            f[a][b][1] = *mesh_at(m, c.X[a + 1] + c.Y[b + 1] + 0u);
         }
+}
+
+template <typename Rl>
+__device__ __forceinline__ void gather_corners(const Rl* __restrict__ m, int N, const CellIdx<Rl>& c,
+                                               Rl (&f)[2][2][2]) {
+    gather_corners_main(m, N, c, f);
+    gather_corners_seam(m, N, c, f);
 }
 
 // Centred differences g = (phi[i-1] - phi[i+1]) / 2 (centered_derivatives,

@@ -80,7 +80,8 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
 // particles each and an LDS hash of 128 tiles per pass.  One particle per
 // thread for both meshes: with two, the gathers of both are in flight
 // together but the registers they take halve the waves per SIMD, which
-// measured slower (FP32 3.8 against 3.3 ms at 512^3).
+// measured slower in an interleaved A/B at 512^3 (FP32 3.88 against 3.82
+// ms, FP64 5.93 against 5.41 ms).
 #ifndef PM_STAGE_PER32
 #define PM_STAGE_PER32 1
 #endif
@@ -88,26 +89,6 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
 #define PM_STAGE_PER64 1
 #endif
 constexpr int kStageThreads = 256;
-
-// Loads and stores of the particle streams, each read once and written
-// once: non-temporal, so that the 13 GB per launch (512^3 particles) stream
-// through L2 without evicting the potential's planes, which the gathers of
-// neighbouring particle rows re-read.  Measured in an interleaved A/B at
-// 512^3: 3.86 -> 3.40 ms (FP32 mesh), 5.41 -> 4.95 ms (FP64); stores alone
-// 3.81, loads alone 3.61.  Bit 0: loads, bit 1: stores.
-#ifndef PM_INTERP_NT
-#define PM_INTERP_NT 3
-#endif
-__device__ __forceinline__ double stream_load(const double* p) {
-    if constexpr (PM_INTERP_NT & 1) return __builtin_nontemporal_load(p);
-    return *p;
-}
-__device__ __forceinline__ void stream_store(double* p, double v) {
-    if constexpr (PM_INTERP_NT & 2)
-        __builtin_nontemporal_store(v, p);
-    else
-        *p = v;
-}
 constexpr int kStagePer32 = PM_STAGE_PER32, kStagePer64 = PM_STAGE_PER64;
 constexpr int kStageHash = 128;
 
