@@ -318,6 +318,35 @@ def test_sorted_deposit_clustered_and_edges(dev):
     assert abs(rho.sum() - ng ** 3) < 1e-6
 
 
+@pytest.mark.parametrize("prec", [pmhip.PM_FP64, pmhip.PM_FP32])
+def test_sorted_deposit_across_segment_seams(dev, prec):
+    """ng = 256: a tile column of 16 tiles is walked in two segments of 8,
+    so z = 128 is a seam between segments and z = 0 the periodic one; the
+    particles sit on and beside those planes, on the tile faces x, y = 16 k,
+    at the box edge, over a uniform background.  Against the oracle's
+    deposit: 1e-8 of the particle mass (FP64: each weight is rounded to
+    2^-32 of the mass, a node sums a few of them) and 1e-6 of
+    the largest node value (FP32 mesh: a node sums up to 8 FP32-rounded
+    parts, its own and its neighbours' shells and seams; a lost or doubled
+    shell or seam value is ~1e-3 of it)."""
+    ng = 256
+    print(f"case sorted deposit seams prec={prec}", flush=True)
+    rng = np.random.default_rng(61)
+    n_bg, n_s = 300000, 40000
+    bg = [rng.uniform(0, ng, n_bg) for _ in range(3)]
+    planes = np.array([127.9999, 128.0, 128.5, 255.99999, 0.0, 0.3, 15.9, 16.0, 143.99, 144.0, 240.0, 239.5])
+    zs = rng.choice(planes, n_s) + rng.uniform(0, 1e-6, n_s) * (rng.random(n_s) < 0.5)
+    zs = np.minimum(zs, np.nextafter(ng, 0))
+    face = lambda: np.where(rng.random(n_s) < 0.4, 16.0 * rng.integers(0, 16, n_s), rng.uniform(0, ng, n_s))
+    ph = [np.concatenate([bg[0], face()]), np.concatenate([bg[1], face()]), np.concatenate([bg[2], zs])]
+    rho = _plan_density(dev, ng, ph, prec, pmhip.PM_DEPOSIT_SORTED)
+    ref = pyoracle.deposit_periodic(ph, ng) + 1.0
+    mass = ng ** 3 / ph[0].size
+    tol = 1e-8 * mass if prec == pmhip.PM_FP64 else 1e-6 * np.abs(ref).max()
+    assert np.max(np.abs(rho - ref)) < tol
+    assert pmhip.range_errors() == 0
+
+
 def test_sorted_deposit_skips_out_of_box(dev):
     ng = 16
     pmhip.range_errors()
@@ -543,15 +572,15 @@ def test_reused_binning_after_bucket_overflow(dev, prec):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("ng,nrow", [(64, 32), (256, 128)])
 @pytest.mark.parametrize("prec", [pmhip.PM_FP32, pmhip.PM_FP64])
-def test_step_density_equals_standalone_deposit(dev, prec):
+def test_step_density_equals_standalone_deposit(dev, prec, ng, nrow):
     """pm_step deposits through the staging of the previous step's
     interpolation (column walk over the staged records, then the shell
     fold): after a step (binned first step, then a staged one) the plan's
     density must be bit for bit pm_deposit's density of the same positions."""
     import torch
 
-    ng, nrow = 64, 32
     npart = nrow ** 3
     noise = np.random.default_rng(27).standard_normal(npart)
     pos, vel = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 1.5, 3.0)
