@@ -13,6 +13,8 @@
 //   (D_a^T U)[c] = (U[c + e_a] - U[c - e_a]) / 2.
 // The CIC weights are piecewise linear: grad_x W(x, corner) takes the +-1
 // slope along the axis and the linear weights along the other two.
+#include <type_traits>
+
 #include "pm_gather.h"
 #include "pm_internal.h"
 #include "pm_tiles.h"
@@ -142,9 +144,13 @@ template <typename Rl, int TS>
 __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
     const double* __restrict__ ux, const double* __restrict__ uy, const double* __restrict__ uz,
-    const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off, int ng,
+    const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, int ng,
     const unsigned long long* __restrict__ umax, Rl* __restrict__ U0, Rl* __restrict__ U1, Rl* __restrict__ U2,
-    Rl* __restrict__ shell, size_t shell_stride) {
+    Rl* __restrict__ shell, size_t shell_stride, const uint32_t* __restrict__ gate, int run_if) {
+    // tile t's particles: idx[off[t] .. off[t] + cnt[t]) with counts (the
+    // buckets of a staging gather), else idx[off[t] .. off[t+1]) (a binning);
+    // with a gate the kernel runs only if (*gate != 0) == run_if
+    if (gate && ((*gate != 0u) != (run_if != 0))) return;
     constexpr int E = TS + 1, E3 = E * E * E;
     constexpr int NT = kVec3Threads;
     __shared__ unsigned long long acc[3][E3];
@@ -159,7 +165,7 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
     for (int k = 0; k < sw.seg; ++k) {
         const int tz = sw.sg * sw.seg + k, oz = tz * TS;
         const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;
-        const uint32_t beg = off[t], end = off[t + 1];
+        const uint32_t beg = off[t], end = cnt ? beg + cnt[t] : off[t + 1];
         constexpr int B = 2;  // particles in flight per thread
         for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += NT * B) {
             uint32_t p[B];
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(256) void k_div3(const Rl* __restrict__ U0, const R
 // MODE 2: as MODE 0, then the undrift that opens the next reverse step,
 //         x -= next_drift * v and u = vbar + next_drift * xbar (in place on
 //         u), with max |u_a| into umax for that step's vec3 deposit.
-template <typename Rl, int MODE>
+template <typename Rl, int MODE, bool STAGE>
 __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, double* __restrict__ py,
                                                     double* __restrict__ pz, double* __restrict__ vx,
                                                     double* __restrict__ vy, double* __restrict__ vz,
@@ -289,7 +295,14 @@ __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, dou
                                                     int N, const Rl* __restrict__ phi, const Rl* __restrict__ lam,
                                                     double mass, double kick, double vjp_scale, double next_drift,
                                                     unsigned long long* __restrict__ umax,
-                                                    unsigned long long* __restrict__ err) {
+                                                    unsigned long long* __restrict__ err, AdjStage stg) {
+    static_assert(!STAGE || MODE == 2, "only the undrifting gather stages");
+    using Rec = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
+    constexpr uint32_t kNo = 0xFFFFFFFFu;
+    constexpr int HS = STAGE ? 128 : 1;
+    __shared__ uint32_t hkey[HS], hcnt[HS], hbase[HS];
+    uint32_t kk = kNo;  // STAGE: tile of the undrifted position
+    Rec rec{};
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const double box = (double)N;
     // three load phases, as in k_interp: the particle's streams, then the
@@ -343,9 +356,19 @@ __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, dou
                     if (v >= box) v -= box;
                     return v;
                 };
-                stream_store(px + i, wrap(x - next_drift * nv[0]));
-                stream_store(py + i, wrap(y - next_drift * nv[1]));
-                stream_store(pz + i, wrap(z - next_drift * nv[2]));
+                const double xn = wrap(x - next_drift * nv[0]), yn = wrap(y - next_drift * nv[1]),
+                             zn = wrap(z - next_drift * nv[2]);
+                stream_store(px + i, xn);
+                stream_store(py + i, yn);
+                stream_store(pz + i, zn);
+                if constexpr (STAGE) {
+                    if (in_box3(xn, yn, zn, box)) {
+                        kk = tile_key(xn, yn, zn, N, stg.ts);
+                        rec = tile_record<Rec>(xn, yn, zn, stg.ts);
+                    } else {
+                        atomicAdd(err, 1ULL);  // as the next deposit's range check would
+                    }
+                }
                 const double nu[3] = {u[0] + next_drift * nb[0], u[1] + next_drift * nb[1],
                                       u[2] + next_drift * nb[2]};
                 stream_store(ux + i, nu[0]);
@@ -354,6 +377,13 @@ __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, dou
 #pragma unroll
                 for (int d = 0; d < 3; ++d) m[d] = fabs(nu[d]);
             }
+        }
+    }
+    if constexpr (STAGE) {
+        const int64_t pos = stage_pass<HS, 256>(kk, hkey, hcnt, hbase, stg.fill, stg.coff, stg.spill);
+        if (pos >= 0) {
+            static_cast<Rec*>(stg.rec)[pos] = rec;
+            stg.idx[pos] = (uint32_t)i;
         }
     }
     if constexpr (MODE == 2) wg_max_to_slots(m, umax);
@@ -399,7 +429,8 @@ int launch_adj_deposit(int prec, const double* px, const double* py, const doubl
 
 int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
                         const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s,
-                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st, bool umax_ready) {
+                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st, bool umax_ready,
+                        const uint32_t* bucket_idx) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
@@ -412,11 +443,19 @@ int launch_deposit_vec3(int prec, const double* px, const double* py, const doub
         }
         hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
     }
-#define PM_VEC3(RL, TSV)                                                                                            \
+    // with buckets: the bucket pass, or (overflow) the binning pass
+#define PM_VEC3_PASS(RL, TSV, IDX, OFF, CNT, GATE, RUN)                                                            \
     hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles / walk_seg(nt)), dim3(kVec3Threads), 0, st, px, \
-                       py, pz, ux, uy, uz, s.idx, s.off, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3, sh);  \
-    for (int d = 0; d < 3; ++d)                                                                                       \
-        launch_fold_shells(prec, ts, ng, (const char*)shell3 + d * sh * rs, U[d], st);
+                       py, pz, ux, uy, uz, IDX, OFF, CNT, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3, sh, \
+                       GATE, RUN)
+#define PM_VEC3(RL, TSV)                                                                                    \
+    if (bucket_idx) {                                                                                       \
+        PM_VEC3_PASS(RL, TSV, bucket_idx, s.coff, s.fill, s.spill, 0);                                      \
+        PM_VEC3_PASS(RL, TSV, s.idx, s.off, nullptr, s.spill, 1);                                           \
+    } else {                                                                                                \
+        PM_VEC3_PASS(RL, TSV, s.idx, s.off, nullptr, nullptr, 0);                                           \
+    }                                                                                                       \
+    for (int d = 0; d < 3; ++d) launch_fold_shells(prec, ts, ng, (const char*)shell3 + d * sh * rs, U[d], st);
     if (prec == PM_FP32) {
         if (ts == 16) {
             PM_VEC3(float, 16)
@@ -431,6 +470,7 @@ int launch_deposit_vec3(int prec, const double* px, const double* py, const doub
         }
     }
 #undef PM_VEC3
+#undef PM_VEC3_PASS
     return check_launch("deposit_vec3");
 }
 
@@ -449,27 +489,35 @@ int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st) {
 int launch_adj_gather(int prec, int mode, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
                       double* ux, double* uy, double* uz, double* bx, double* by, double* bz, int64_t np, int ng,
                       const void* phi, const void* lam, double mass, double kick, double vjp_scale,
-                      unsigned long long* err, hipStream_t st, double next_drift, unsigned long long* umax) {
+                      unsigned long long* err, hipStream_t st, double next_drift, unsigned long long* umax,
+                      const AdjStage* stage) {
     if (np <= 0) return PM_OK;
     if (mode == 2 && !umax) return fail(PM_ERR_ARG, "adjoint gather: no max buffer");
+    if (stage && (mode != 2 || !stage->rec || !stage->idx)) return fail(PM_ERR_ARG, "adjoint gather: bad staging");
     const unsigned nb = (unsigned)((np + 255) / 256);
-#define PM_GATHER(RL, M)                                                                                          \
-    hipLaunchKernelGGL((k_adj_gather<RL, M>), dim3(nb), dim3(256), 0, st, px, py, pz, vx, vy, vz, ux, uy, uz, bx, \
-                       by, bz, np, ng, (const RL*)phi, (const RL*)lam, mass, kick, vjp_scale, next_drift, umax, err)
+    const AdjStage stg = stage ? *stage : AdjStage();
+#define PM_GATHER(RL, M, S)                                                                                          \
+    hipLaunchKernelGGL((k_adj_gather<RL, M, S>), dim3(nb), dim3(256), 0, st, px, py, pz, vx, vy, vz, ux, uy, uz, bx, \
+                       by, bz, np, ng, (const RL*)phi, (const RL*)lam, mass, kick, vjp_scale, next_drift, umax, err,  \
+                       stg)
     if (prec == PM_FP32) {
         if (mode == 0)
-            PM_GATHER(float, 0);
+            PM_GATHER(float, 0, false);
         else if (mode == 1)
-            PM_GATHER(float, 1);
+            PM_GATHER(float, 1, false);
+        else if (!stage)
+            PM_GATHER(float, 2, false);
         else
-            PM_GATHER(float, 2);
+            PM_GATHER(float, 2, true);
     } else {
         if (mode == 0)
-            PM_GATHER(double, 0);
+            PM_GATHER(double, 0, false);
         else if (mode == 1)
-            PM_GATHER(double, 1);
+            PM_GATHER(double, 1, false);
+        else if (!stage)
+            PM_GATHER(double, 2, false);
         else
-            PM_GATHER(double, 2);
+            PM_GATHER(double, 2, true);
     }
 #undef PM_GATHER
     if (mode == 2) hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);

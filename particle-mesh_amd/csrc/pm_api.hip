@@ -181,7 +181,7 @@ extern "C" int pm_plan_destroy(pm_plan_t* p) {
                       p->sorted.fill,  p->sorted.coff,   p->sorted.spill, p->sorted.stage};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    void* abufs[6] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell, p->adjMax};
+    void* abufs[7] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell, p->adjMax, p->adjIdx};
     for (void* b : abufs)
         if (b) (void)hipFree(b);
     for (auto& e : p->ev)
@@ -490,13 +490,18 @@ static int ensure_adjoint(pm_plan* p) {
     const size_t nmax = 3 + 3 * (size_t)kMaxSlots;
     if (!rc) rc = hip_ok(hipMalloc((void**)&p->adjMax, nmax * sizeof(unsigned long long)), "adjoint: hipMalloc");
     if (!rc) rc = launch_fill(PM_FP64, p->adjMax, nmax, 0.0, p->stream);  // 0.0 has all bits zero
+    // index buckets for the staging of the reverse sweep (beside the staging records)
+    if (!rc && p->sorted.stage)
+        rc = hip_ok(hipMalloc((void**)&p->adjIdx, 4 * p->sorted.cap_total), "adjoint: hipMalloc");
     if (rc) {
         void* bufs[5] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjShell, p->adjLam};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (p->adjMax) (void)hipFree(p->adjMax);
+        if (p->adjIdx) (void)hipFree(p->adjIdx);
         p->adjU[0] = p->adjU[1] = p->adjU[2] = p->adjShell = p->adjLam = nullptr;
         p->adjMax = nullptr;
+        p->adjIdx = nullptr;
     }
     return rc;
 }
@@ -524,9 +529,10 @@ static int forward_field(pm_plan* p, const double* x, const double* y, const dou
 // lambda = S sum_a D_a^T U_a with U_a the interpolation transpose of u at x
 // (binning of x already in p->sorted); mu is staged in p->rho.
 static int adjoint_field(pm_plan* p, const double* x, const double* y, const double* z, const double* ux,
-                         const double* uy, const double* uz, bool umax_ready = false) {
+                         const double* uy, const double* uz, bool umax_ready = false,
+                         const uint32_t* bucket_idx = nullptr) {
     int rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
-                                 p->adjMax, p->stream, umax_ready);
+                                 p->adjMax, p->stream, umax_ready, bucket_idx);
     if (!rc) rc = launch_div3(p->prec, p->adjU, p->ng, p->rho, p->stream);
     if (!rc) rc = solve_into(p, p->rho, p->adjLam, false);
     return rc;
@@ -595,15 +601,42 @@ extern "C" int pm_adjoint_force(pm_plan_t* p, const double* x, const double* y, 
 // place): forward and adjoint fields at x, then the gather that undoes the
 // kick and adds kick * J_F^T u to xbar; gather mode 2 also applies the next
 // reverse step's undrift (next_drift) and leaves max |u| for its deposit.
+//
+// Staging (pm_run_adjoint with the sorted deposit): from_stage = the previous
+// reverse step's gather staged x in the tile buckets (in-tile records and
+// particle indices), so the forward deposit reads the records and the vec3
+// deposit the indices, with the full binning behind the overflow flag;
+// stage_next = this step's gather stages the undrifted positions, into
+// buckets sized from this step's counts.
 static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
                         double* xbar, double* ybar, double* zbar, double* ux, double* uy, double* uz, double kick,
-                        int variant, int gmode, double next_drift, bool umax_ready) {
+                        int variant, int gmode, double next_drift, bool umax_ready, bool from_stage = false,
+                        bool stage_next = false) {
     const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
-    int rc = forward_field(p, x, y, z, variant);
-    if (!rc) rc = adjoint_field(p, x, y, z, ux, uy, uz, umax_ready);
+    int rc;
+    if (from_stage) {
+        rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
+                                   p->stream, false);
+        if (!rc) rc = solve_into(p, p->rho, p->phi, true);
+        if (!rc) rc = adjoint_field(p, x, y, z, ux, uy, uz, umax_ready, p->adjIdx);
+    } else {
+        rc = forward_field(p, x, y, z, variant);
+        if (!rc) rc = adjoint_field(p, x, y, z, ux, uy, uz, umax_ready);
+    }
+    AdjStage stg;
+    if (!rc && stage_next) {
+        rc = launch_cap_scan(p->ng, p->sorted, !from_stage, p->stream);
+        stg.ts = sorted_tile(p->ng);
+        stg.fill = p->sorted.fill;
+        stg.coff = p->sorted.coff;
+        stg.spill = p->sorted.spill;
+        stg.rec = p->sorted.stage;
+        stg.idx = p->adjIdx;
+    }
     if (!rc)
         rc = launch_adj_gather(p->prec, gmode, x, y, z, vx, vy, vz, ux, uy, uz, xbar, ybar, zbar, p->np, p->ng, p->phi,
-                               p->adjLam, mass, kick, kick, range_counter(), p->stream, next_drift, p->adjMax);
+                               p->adjLam, mass, kick, kick, range_counter(), p->stream, next_drift, p->adjMax,
+                               stage_next ? &stg : nullptr);
     return rc;
 }
 
@@ -638,10 +671,13 @@ extern "C" int pm_run_adjoint(pm_plan_t* p, double* x, double* y, double* z, dou
     // Reverse steps s = nsteps..0 of pm_run_kdk (kick dt/2 at both ends,
     // no drift after the last).  The last step has no drift to undo; every
     // other step's undrift is done by the gather of the step after it.
+    // With the sorted deposit every gather but the last stages the positions
+    // it undrifts for the next reverse step (no rebinning from scratch).
+    const bool stage = variant == PM_DEPOSIT_SORTED && p->adjIdx && p->sorted.stage;
     for (int s = nsteps; s >= 0 && !rc; --s) {
         const double kick = (s == 0 || s == nsteps) ? 0.5 * dt : dt;
         rc = adjoint_core(p, x, y, z, vx, vy, vz, xbar, ybar, zbar, vxbar, vybar, vzbar, kick, variant, s > 0 ? 2 : 0,
-                          s > 0 ? dt : 0.0, s < nsteps);
+                          s > 0 ? dt : 0.0, s < nsteps, stage && s < nsteps, stage && s > 0);
     }
     p->timing_slots = slots;
     return rc;

@@ -521,7 +521,8 @@ int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st) {
 }
 
 int launch_deposit_staged(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
-                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st) {
+                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st,
+                          bool rescan) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts, seg = walk_seg(nt);
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
@@ -546,7 +547,7 @@ int launch_deposit_staged(int prec, const double* px, const double* py, const do
 #undef PM_DS
     rc = check_launch("deposit_staged");
     if (!rc) rc = launch_fold_shells(prec, ts, ng, s.shell, rho, st);
-    if (!rc) rc = launch_cap_scan(ng, s, false, st);
+    if (!rc && rescan) rc = launch_cap_scan(ng, s, false, st);
     return rc;
 }
 

@@ -114,29 +114,46 @@ int launch_deposit_binned(int prec, const double* px, const double* py, const do
 int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st);
 // deposit from the staging arrays (binned fallback if a bucket overflowed),
 // then the next capacities
+// (rescan = false leaves the buckets and their fill counts for a second
+// reader, the reverse sweep's vec3 deposit; launch_cap_scan follows it)
 int launch_deposit_staged(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
-                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st);
+                          double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st,
+                          bool rescan = true);
 int launch_fold_shells(int prec, int ts, int ng, const void* shell, void* rho, hipStream_t st);
 
 // Adjoint kernels (pm_adjoint.hip).
 int launch_adj_deposit(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
                        const void* mbar, double scale, double* bx, double* by, double* bz, unsigned long long* err,
                        hipStream_t st);
-// (umax_ready: umax already holds max |u_a|, left by a mode-2 gather)
+// (umax_ready: umax already holds max |u_a|, left by a mode-2 gather;
+// bucket_idx: the particles come from the index buckets that a staging
+// mode-2 gather filled (coff / fill of s), or, if a bucket overflowed
+// (s.spill), from the binning that the staged deposit's fallback made)
 int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
                         const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s,
                         void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st,
-                        bool umax_ready = false);
+                        bool umax_ready = false, const uint32_t* bucket_idx = nullptr);
 int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st);
 // mode 1: xbar += VJP; 0: also undo the kick; 2: also the next reverse
 // step's undrift (next_drift) with max |u| into umax[0..3) (umax holds
 // 3 + 3 * kMaxSlots entries, the slots zero between calls)
 constexpr int kMaxSlots = 256;
+// Staging of a mode-2 gather: the undrifted positions go to the tile
+// buckets of s (coff, fill, spill) as in-tile records (s.stage) and as
+// particle indices (idx), at the same slots, for the next reverse step.
+struct AdjStage {
+    int ts = 0;
+    uint32_t* fill = nullptr;
+    const uint32_t* coff = nullptr;
+    uint32_t* spill = nullptr;
+    void* rec = nullptr;
+    uint32_t* idx = nullptr;
+};
 int launch_adj_gather(int prec, int mode, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
                       double* ux, double* uy, double* uz, double* bx, double* by, double* bz, int64_t np, int ng,
                       const void* phi, const void* lam, double mass, double kick, double vjp_scale,
                       unsigned long long* err, hipStream_t st, double next_drift = 0.0,
-                      unsigned long long* umax = nullptr);
+                      unsigned long long* umax = nullptr, const AdjStage* stage = nullptr);
 int launch_undrift(double* px, double* py, double* pz, const double* vx, const double* vy, const double* vz,
                    const double* bx, const double* by, const double* bz, double* ubx, double* uby, double* ubz,
                    int64_t np, int ng, double drift, hipStream_t st);
@@ -185,6 +202,7 @@ struct pm_plan {
     void* adjLam = nullptr;    // lambda = S D^T U
     void* adjShell = nullptr;  // 3 shell buffers of the vec3 deposit
     unsigned long long* adjMax = nullptr;  // max |u_a| (fixed-point scales of the vec3 deposit) + slots
+    uint32_t* adjIdx = nullptr;  // index buckets of the reverse sweep (cap_total), beside sorted.stage
     // Phase timing: one set of PM_NPHASES+1 events per recorded step, read
     // back (and summed) only by pm_plan_phase_ms, so that recording needs no
     // synchronisation inside a timed loop.

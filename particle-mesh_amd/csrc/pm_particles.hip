@@ -102,7 +102,7 @@ constexpr int kStageHash = 128;
 // stores nothing.
 // MODE 0: v += g*kick, x += v*drift.  MODE 1: write g to (vx, vy, vz).
 // MODE 2: as MODE 0, then the drifted position is staged for the next
-// step's deposit (stage_commit).
+// step's deposit (stage_pass).
 template <typename Rl, int MODE, int NT, int PER>
 __global__ __launch_bounds__(NT) void k_interp(double* __restrict__ px, double* __restrict__ py,
                                                double* __restrict__ pz, double* __restrict__ vx,
@@ -183,29 +183,16 @@ __global__ __launch_bounds__(NT) void k_interp(double* __restrict__ px, double* 
                     atomicAdd(err, 1ULL);
                     continue;
                 }
-                const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;  // ts: power of two
-                // in-tile offsets: u32 at 2^-28 cells (FP32 mesh) or exact doubles
-                const int msk = ~(ts - 1);  // tile origin = cell with the low bits cleared
-                const double lx = xn - ((int)xn & msk), ly = yn - ((int)yn & msk), lz = zn - ((int)zn & msk);
-                if constexpr (sizeof(Rl) == 4)
-                    rec[q] = StageQ{(uint32_t)(lx * 268435456.0), (uint32_t)(ly * 268435456.0),
-                                    (uint32_t)(lz * 268435456.0)};
-                else
-                    rec[q] = StageD{lx, ly, lz};
-                kk[q] = (uint32_t)((((int)xn >> lt) * nt + ((int)yn >> lt)) * nt + ((int)zn >> lt));
+                rec[q] = tile_record<Rec>(xn, yn, zn, ts);
+                kk[q] = tile_key(xn, yn, zn, ng, ts);
             }
         }
     }
     if constexpr (MODE == 2) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            for (int s = threadIdx.x; s < HS; s += NT) {
-                hkey[s] = kNo;
-                hcnt[s] = 0u;
-            }
-            __syncthreads();
-            stage_commit<HS, NT>(kk[q], rec[q], hkey, hcnt, hbase, fill, coff, spill, static_cast<Rec*>(stage));
-            __syncthreads();
+            const int64_t pos = stage_pass<HS, NT>(kk[q], hkey, hcnt, hbase, fill, coff, spill);
+            if (pos >= 0) static_cast<Rec*>(stage)[pos] = rec[q];
         }
     }
 }

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "pm_common.h"
+#include "pm_internal.h"
 
 namespace pm {
 
@@ -128,18 +129,19 @@ __device__ __forceinline__ void tile_cic_add(unsigned long long* acc, int a, int
     atomicAdd(n0 + E * E + E + 1, (unsigned long long)__double2ll_rn(sxy11 * m1));
 }
 
-// Slot reservation in the staging buckets and the write of a staged record
-// (called by every thread of the workgroup; kk = kNo for no particle).
-// Runs of equal tiles in consecutive lanes (particles in a coherent order
-// share tiles) reserve their slots together: the first lane of a run takes
-// run-length slots, first within the workgroup (LDS hash of HS slots), then
-// one global fill atomic per distinct tile; a slot past the bucket's
-// capacity raises *spill instead of writing.
-template <int HS, int NT, typename Rec>
-__device__ __forceinline__ void stage_commit(uint32_t kk, const Rec& rec, uint32_t* hkey, uint32_t* hcnt,
-                                             uint32_t* hbase, uint32_t* __restrict__ fill,
-                                             const uint32_t* __restrict__ coff, uint32_t* __restrict__ spill,
-                                             Rec* __restrict__ stage) {
+// Slot reservation in the staging buckets (called by every thread of the
+// workgroup; kk = kNo for no particle); returns the record's position in the
+// bucket arrays, or -1 (no particle, or the bucket is full: then *spill is
+// raised and the caller writes nothing).  Runs of equal tiles in consecutive
+// lanes (particles in a coherent order share tiles) reserve their slots
+// together: the first lane of a run takes run-length slots, first within the
+// workgroup (LDS hash of HS slots), then one global fill atomic per distinct
+// tile.  The hash must be cleared (hkey = kNo, hcnt = 0) and a barrier
+// passed before the call.
+template <int HS, int NT>
+__device__ __forceinline__ int64_t stage_reserve(uint32_t kk, uint32_t* hkey, uint32_t* hcnt, uint32_t* hbase,
+                                                 uint32_t* __restrict__ fill, const uint32_t* __restrict__ coff,
+                                                 uint32_t* __restrict__ spill) {
     constexpr uint32_t kNo = 0xFFFFFFFFu;
     const uint32_t lane = __lane_id();
     const uint32_t prev = __shfl_up(kk, 1);
@@ -170,14 +172,45 @@ __device__ __forceinline__ void stage_commit(uint32_t kk, const Rec& rec, uint32
     const unsigned long long upto = bounds & (lane == 63 ? ~0ull : (2ull << lane) - 1ull);
     const int hl = upto ? 63 - __clzll((long long)upto) : 0;
     const uint32_t base = __shfl(gb, hl);
-    if (kk != kNo) {
-        const uint32_t slot = base + (lane - (uint32_t)hl);
-        const uint32_t c0 = coff[kk];
-        if (slot < coff[kk + 1] - c0)
-            stage[(size_t)c0 + slot] = rec;
-        else
-            *spill = 1u;
+    if (kk == kNo) return -1;
+    const uint32_t slot = base + (lane - (uint32_t)hl);
+    const uint32_t c0 = coff[kk];
+    if (slot < coff[kk + 1] - c0) return (int64_t)c0 + slot;
+    *spill = 1u;
+    return -1;
+}
+
+// Reservation of HS-hash passes: clears the hash, reserves (stage_reserve).
+template <int HS, int NT>
+__device__ __forceinline__ int64_t stage_pass(uint32_t kk, uint32_t* hkey, uint32_t* hcnt, uint32_t* hbase,
+                                              uint32_t* __restrict__ fill, const uint32_t* __restrict__ coff,
+                                              uint32_t* __restrict__ spill) {
+    constexpr uint32_t kNo = 0xFFFFFFFFu;
+    for (int s = threadIdx.x; s < HS; s += NT) {
+        hkey[s] = kNo;
+        hcnt[s] = 0u;
     }
+    __syncthreads();
+    const int64_t pos = stage_reserve<HS, NT>(kk, hkey, hcnt, hbase, fill, coff, spill);
+    __syncthreads();
+    return pos;
+}
+
+// Tile of a position in [0, ng)^3 for power-of-two tiles of ts cells, and
+// the offsets inside it: u32 at 2^-28 cells (FP32 mesh) or exact doubles.
+__device__ __forceinline__ uint32_t tile_key(double x, double y, double z, int ng, int ts) {
+    const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;
+    return (uint32_t)((((int)x >> lt) * nt + ((int)y >> lt)) * nt + ((int)z >> lt));
+}
+template <typename Rec> __device__ __forceinline__ Rec tile_record(double x, double y, double z, int ts);
+template <> __device__ __forceinline__ StageQ tile_record<StageQ>(double x, double y, double z, int ts) {
+    const int msk = ~(ts - 1);  // tile origin = cell with the low bits cleared
+    return StageQ{(uint32_t)((x - ((int)x & msk)) * 268435456.0), (uint32_t)((y - ((int)y & msk)) * 268435456.0),
+                  (uint32_t)((z - ((int)z & msk)) * 268435456.0)};
+}
+template <> __device__ __forceinline__ StageD tile_record<StageD>(double x, double y, double z, int ts) {
+    const int msk = ~(ts - 1);
+    return StageD{x - ((int)x & msk), y - ((int)y & msk), z - ((int)z & msk)};
 }
 
 }  // namespace pm

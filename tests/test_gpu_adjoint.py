@@ -271,3 +271,39 @@ def test_run_adjoint_equals_step_by_step(dev, prec):
     assert pmhip.range_errors() == 0
     for a, b in zip(*out):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("prec", [pmhip.PM_FP32, pmhip.PM_FP64])
+def test_run_adjoint_staging_overflow_falls_back(dev, prec):
+    """The reverse sweep's gather stages the undrifted positions in tile
+    buckets sized from the current counts; velocities that undrift every
+    particle towards one point overflow them, and the next reverse step
+    must fall back to a full binning: bit for bit the step-by-step sweep."""
+    import torch
+
+    ng, nrow, nsteps, dt = 32, 32, 3, 0.05
+    npart = nrow ** 3
+    rng = np.random.default_rng(33)
+    x0 = [torch.tensor(rng.uniform(0, ng, npart), device=dev) for _ in range(3)]
+    c = 8.5
+    v0 = [(x - c) / dt for x in x0]  # x - dt v = c: all particles into one tile
+    g = torch.Generator(device=dev)
+    g.manual_seed(34)
+    xb0 = [torch.randn(npart, dtype=torch.float64, device=dev, generator=g) for _ in range(3)]
+    vb0 = [torch.randn(npart, dtype=torch.float64, device=dev, generator=g) for _ in range(3)]
+    out = []
+    with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        for fused in (True, False):
+            pos, vel = [t.clone() for t in x0], [t.clone() for t in v0]
+            xb, vb = [t.clone() for t in xb0], [t.clone() for t in vb0]
+            if fused:
+                plan.run_adjoint(pos, vel, xb, vb, nsteps, dt, pmhip.PM_DEPOSIT_SORTED)
+            else:
+                for s in range(nsteps, -1, -1):
+                    kick = 0.5 * dt if s in (0, nsteps) else dt
+                    plan.adjoint_step(pos, vel, xb, vb, kick, 0.0 if s == nsteps else dt, pmhip.PM_DEPOSIT_SORTED)
+            torch.cuda.synchronize()
+            out.append(pos + vel + xb + vb)
+    assert pmhip.range_errors() == 0
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
