@@ -69,6 +69,26 @@ def test_null_arguments_rejected():
     assert lib.pm_init_buf(None, 5, 5, 5, None) == pmhip.PM_ERR_ARG
 
 
+def test_tuning_entry_points_reject_bad_arguments():
+    """The FFT shape / tuning cache entry points check their arguments before
+    touching a GPU."""
+    lib = pmhip.lib()
+    shape = (ctypes.c_int * pmhip.PM_FFT_SLOTS)()
+    ms = ctypes.c_double()
+    assert lib.pm_plan_tune(None, 5, ctypes.byref(ms)) == pmhip.PM_ERR_ARG
+    assert lib.pm_plan_set_fft_shape(None, shape) == pmhip.PM_ERR_ARG
+    assert lib.pm_plan_get_fft_shape(None, shape) == pmhip.PM_ERR_ARG
+    assert lib.pm_fft_tune(None, None, 5, None) == pmhip.PM_ERR_ARG
+    assert lib.pm_fft_get_shape(None, shape) == pmhip.PM_ERR_ARG
+    assert lib.pm_wisdom_import(None) == pmhip.PM_ERR_ARG
+    assert lib.pm_wisdom_export(None) == pmhip.PM_ERR_ARG
+    assert lib.pm_wisdom_lookup(512, pmhip.PM_FP32, shape, None) == pmhip.PM_ERR_ARG
+    found = ctypes.c_int(7)
+    pmhip.wisdom_forget()
+    assert lib.pm_wisdom_lookup(512, pmhip.PM_FP32, None, ctypes.byref(found)) == pmhip.PM_OK
+    assert found.value == 0
+
+
 def _write_snapshot_numpy(path, n, ng, step, a, magic=b"PMSNAP01", version=1):
     import numpy as np
 
