@@ -186,6 +186,9 @@ extern "C" int pm_plan_destroy(pm_plan_t* p) {
         if (b) (void)hipFree(b);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
+    if (p->adjFork) (void)hipEventDestroy(p->adjFork);
+    if (p->adjJoin) (void)hipEventDestroy(p->adjJoin);
+    if (p->adjStream) (void)hipStreamDestroy(p->adjStream);
     delete p;
     return PM_OK;
 }
@@ -493,15 +496,23 @@ static int ensure_adjoint(pm_plan* p) {
     // index buckets for the staging of the reverse sweep (beside the staging records)
     if (!rc && p->sorted.stage)
         rc = hip_ok(hipMalloc((void**)&p->adjIdx, 4 * p->sorted.cap_total), "adjoint: hipMalloc");
+    if (!rc) rc = hip_ok(hipStreamCreateWithFlags(&p->adjStream, hipStreamNonBlocking), "adjoint: stream");
+    if (!rc) rc = hip_ok(hipEventCreateWithFlags(&p->adjFork, hipEventDisableTiming), "adjoint: event");
+    if (!rc) rc = hip_ok(hipEventCreateWithFlags(&p->adjJoin, hipEventDisableTiming), "adjoint: event");
     if (rc) {
         void* bufs[5] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjShell, p->adjLam};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (p->adjMax) (void)hipFree(p->adjMax);
         if (p->adjIdx) (void)hipFree(p->adjIdx);
+        if (p->adjFork) (void)hipEventDestroy(p->adjFork);
+        if (p->adjJoin) (void)hipEventDestroy(p->adjJoin);
+        if (p->adjStream) (void)hipStreamDestroy(p->adjStream);
         p->adjU[0] = p->adjU[1] = p->adjU[2] = p->adjShell = p->adjLam = nullptr;
         p->adjMax = nullptr;
         p->adjIdx = nullptr;
+        p->adjStream = nullptr;
+        p->adjFork = p->adjJoin = nullptr;
     }
     return rc;
 }
@@ -612,17 +623,45 @@ static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx,
                         double* xbar, double* ybar, double* zbar, double* ux, double* uy, double* uz, double kick,
                         int variant, int gmode, double next_drift, bool umax_ready, bool from_stage = false,
                         bool stage_next = false) {
-    const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
+    const size_t nc = (size_t)p->ng * p->ng * p->ng;
+    const double mass = (double)nc / (double)p->np;
+    if (variant != PM_DEPOSIT_SORTED && variant != PM_DEPOSIT_ATOMIC)
+        return fail(PM_ERR_ARG, "adjoint: unknown deposit variant");
+    // 1. the tile binning of x: from scratch, or (staged) only behind the
+    //    overflow flag of the buckets
     int rc;
     if (from_stage) {
-        rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
-                                   p->stream, false);
-        if (!rc) rc = solve_into(p, p->rho, p->phi, true);
-        if (!rc) rc = adjoint_field(p, x, y, z, ux, uy, uz, umax_ready, p->adjIdx);
+        rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream, p->sorted.spill);
     } else {
-        rc = forward_field(p, x, y, z, variant);
-        if (!rc) rc = adjoint_field(p, x, y, z, ux, uy, uz, umax_ready);
+        invalidate_reuse(p);
+        rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
     }
+    // 2. the vec3 deposit U_a = interp^T(u) on the second stream, beside the
+    //    forward deposit and solve: it reads x, u and the tile lists, writes
+    //    only U_a and their shells
+    if (!rc) rc = hip_ok(hipEventRecord(p->adjFork, p->stream), "adjoint: fork");
+    if (!rc) rc = hip_ok(hipStreamWaitEvent(p->adjStream, p->adjFork, 0), "adjoint: fork");
+    if (!rc)
+        rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
+                                 p->adjMax, p->adjStream, umax_ready, from_stage ? p->adjIdx : nullptr);
+    if (!rc) rc = hip_ok(hipEventRecord(p->adjJoin, p->adjStream), "adjoint: join");
+    // 3. rho and phi at x
+    if (!rc) {
+        if (from_stage)
+            rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
+                                       p->stream, false, false);
+        else if (variant == PM_DEPOSIT_SORTED)
+            rc = launch_deposit_binned(p->prec, x, y, z, p->ng, mass, p->rho, p->sorted, p->stream);
+        else if (!(rc = launch_fill(p->prec, p->rho, nc, 0.0, p->stream)))
+            rc = launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, range_counter(), p->stream);
+    }
+    if (!rc) rc = solve_into(p, p->rho, p->phi, true);
+    // 4. join; lambda = S sum_a D_a^T U_a (mu staged in rho, which the solve has read)
+    if (!rc) rc = hip_ok(hipStreamWaitEvent(p->stream, p->adjJoin, 0), "adjoint: join");
+    if (!rc) rc = launch_div3(p->prec, p->adjU, p->ng, p->rho, p->stream);
+    if (!rc) rc = solve_into(p, p->rho, p->adjLam, false);
+    // 5. the gather (undo the kick, VJP; mode 2: the next step's undrift,
+    //    staged into buckets sized from this step's counts)
     AdjStage stg;
     if (!rc && stage_next) {
         rc = launch_cap_scan(p->ng, p->sorted, !from_stage, p->stream);

@@ -522,12 +522,12 @@ int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st) {
 
 int launch_deposit_staged(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
                           double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st,
-                          bool rescan) {
+                          bool rescan, bool bin) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts, seg = walk_seg(nt);
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
     // fallback, run only if some tile overflowed its staging capacity
-    int rc = launch_bin(px, py, pz, np, ng, s, err, st, s.spill);
+    int rc = bin ? launch_bin(px, py, pz, np, ng, s, err, st, s.spill) : PM_OK;
     if (!rc) rc = launch_deposit_binned(prec, px, py, pz, ng, mass, rho, s, st, s.spill, false);
     if (rc) return rc;
 #define PM_DS(RL, TSV)                                                                                               \

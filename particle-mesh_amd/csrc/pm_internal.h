@@ -115,10 +115,11 @@ int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st);
 // deposit from the staging arrays (binned fallback if a bucket overflowed),
 // then the next capacities
 // (rescan = false leaves the buckets and their fill counts for a second
-// reader, the reverse sweep's vec3 deposit; launch_cap_scan follows it)
+// reader, the reverse sweep's vec3 deposit; launch_cap_scan follows it.
+// bin = false: the fallback binning was launched already)
 int launch_deposit_staged(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
                           double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st,
-                          bool rescan = true);
+                          bool rescan = true, bool bin = true);
 int launch_fold_shells(int prec, int ts, int ng, const void* shell, void* rho, hipStream_t st);
 
 // Adjoint kernels (pm_adjoint.hip).
@@ -203,6 +204,10 @@ struct pm_plan {
     void* adjShell = nullptr;  // 3 shell buffers of the vec3 deposit
     unsigned long long* adjMax = nullptr;  // max |u_a| (fixed-point scales of the vec3 deposit) + slots
     uint32_t* adjIdx = nullptr;  // index buckets of the reverse sweep (cap_total), beside sorted.stage
+    // Second stream of an adjoint step (default priority): the vec3 deposit
+    // runs there beside the forward deposit and solve; fork / join events.
+    hipStream_t adjStream = nullptr;
+    hipEvent_t adjFork = nullptr, adjJoin = nullptr;
     // Phase timing: one set of PM_NPHASES+1 events per recorded step, read
     // back (and summed) only by pm_plan_phase_ms, so that recording needs no
     // synchronisation inside a timed loop.
