@@ -267,13 +267,13 @@ int pm_plan_reset_timing(pm_plan_t* plan);
  * FFTW_WISDOM_ONLY, else plans FFTW_EXHAUSTIVE and exports at destruction).
  * Each of the five tunable FFT passes (slots 0 z fwd, 1 y fwd, 2 x (fused
  * forward + Green + inverse, or the forward / inverse alone), 3 y inv,
- * 4 z inv) runs in one of PM_FFT_VARIANTS shapes (lines per workgroup x
- * threads per workgroup); variant 0 is the built-in default, and every
- * variant gives bit-identical results.  Meshes below 64 always use
+ * 4 z inv) runs in one of PM_FFT_VARIANTS shapes (lines per workgroup: the
+ * default, half, a quarter; x threads per workgroup); variant 0 is the
+ * built-in default, and every variant gives bit-identical results.  Meshes below 64 always use
  * variant 0.  Plans and FFT handles take the cached shape for their
  * (ng, precision) when they are created. */
 #define PM_FFT_SLOTS 5
-#define PM_FFT_VARIANTS 6
+#define PM_FFT_VARIANTS 9
 /* Time every variant of every slot on the plan's own buffers (`repeats`
  * launches each, after one warm-up; the density, potential and spectra are
  * overwritten, as FFTW_MEASURE overwrites its arrays), keep the fastest for

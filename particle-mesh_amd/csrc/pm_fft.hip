@@ -396,10 +396,10 @@ __global__ __launch_bounds__(256) void k_apply_green(cplx_t<Rl>* __restrict__ sp
 
 // ================================================================ launchers
 // Shape variants of the five tunable passes (slots zfwd, yfwd, x, yinv,
-// zinv): variant v takes the default lines per tile (v < 3) or half of them
-// (v >= 3), and threads per workgroup: the pass's default (v % 3 == 0), then
-// the smaller and the larger of the other two of 256 / 512 / 1024, clamped by
-// fft_threads.  Variant 0 is the default shape; every variant computes the
+// zinv): variant v takes the default lines per tile (v < 3), half of them
+// (3 <= v < 6) or a quarter (v >= 6), at least one, and threads per
+// workgroup: the pass's default (v % 3 == 0), then the smaller and the larger
+// of the other two of 256 / 512 / 1024, clamped by fft_threads.  Variant 0 is the default shape; every variant computes the
 // same transform bit for bit (only the work split differs).  Meshes below
 // kTuneMinN (pm_internal.h) always take variant 0.
 
@@ -414,7 +414,8 @@ __host__ __device__ constexpr int variant_threads(int slot, int v) {
 }
 template <typename Rl> __host__ __device__ constexpr int variant_lines(int slot, int N, int v) {
     const int base = slot == 2 ? tile_lines<Rl, true>(N) : tile_lines<Rl>(N);
-    return v < 3 || base < 2 ? base : base / 2;
+    const int t = v < 3 ? base : v < 6 ? base / 2 : base / 4;
+    return t < 1 ? 1 : t;
 }
 
 template <typename Rl, int N, int V>
@@ -487,6 +488,9 @@ static void launch_pass(int pass, int v, hipStream_t st, const void* in, void* o
             case 3: launch_variant<Rl, N, 3>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
             case 4: launch_variant<Rl, N, 4>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
             case 5: launch_variant<Rl, N, 5>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+            case 6: launch_variant<Rl, N, 6>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+            case 7: launch_variant<Rl, N, 7>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
+            case 8: launch_variant<Rl, N, 8>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
             default: launch_variant<Rl, N, 0>(pass, st, in, out, twM, twN, ga, norm, sub1); break;
         }
     }

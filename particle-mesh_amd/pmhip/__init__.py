@@ -30,7 +30,7 @@ PM_DEPOSIT_SORTED = 1
 PM_STEP_REUSE_BINS = 0x100
 PM_NPHASES = 7
 PM_FFT_SLOTS = 5
-PM_FFT_VARIANTS = 6
+PM_FFT_VARIANTS = 9
 
 PHASE_NAMES = ["deposit", "fft_z_fwd", "fft_y_fwd", "fft_x_green", "fft_y_inv", "fft_z_inv", "interp_kick_drift"]
 

@@ -54,7 +54,7 @@ def test_small_meshes_always_take_the_default_shape(tmp_path):
     "512 fp16 0 0 0 0 0 1.0",        # unknown precision
     "500 fp32 0 0 0 0 0 1.0",        # not a power of two
     "2048 fp32 0 0 0 0 0 1.0",       # above the largest mesh
-    "512 fp32 0 0 6 0 0 1.0",        # variant out of range
+    "512 fp32 0 0 9 0 0 1.0",        # variant out of range
     "512 fp32 0 0 -1 0 0 1.0",
     "512 fp32 0 0 0 0 1.0",          # a field missing
     "512 fp32 0 0 0 0 0 1.0 extra",  # trailing garbage
