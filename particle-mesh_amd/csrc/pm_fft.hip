@@ -20,8 +20,9 @@
 namespace pm {
 
 // Workgroup size of each pass at the largest meshes (measured at 512^3):
-// the transposing y passes gain from more waves in flight than the z / x
-// passes, whose LDS tile already limits the workgroups per CU.
+// the transposing y passes gain from more waves in flight than the z
+// passes, whose LDS tile already limits the workgroups per CU; the x pass
+// runs small tiles (4 lines) with 256 threads.
 #ifndef PM_FFT_NT_ZF
 #define PM_FFT_NT_ZF 512
 #endif
@@ -29,7 +30,7 @@ namespace pm {
 #define PM_FFT_NT_YF 1024
 #endif
 #ifndef PM_FFT_NT_X
-#define PM_FFT_NT_X 512
+#define PM_FFT_NT_X 256
 #endif
 #ifndef PM_FFT_NT_YI
 #define PM_FFT_NT_YI 1024
@@ -192,13 +193,18 @@ __device__ __forceinline__ void fft_lines(C* s, const C* __restrict__ tw) {
 #ifndef PM_FFT_LINES64
 #define PM_FFT_LINES64 8
 #endif
-// The fused x pass (contiguous lines on both sides) prefers half the lines:
-// a smaller LDS tile, more workgroups per CU.
+// The fused x pass (contiguous lines on both sides, two FFTs per tile)
+// prefers 4 lines with 256 threads: a small LDS tile and many workgroups per
+// CU (the shape pm_plan_tune picks at 512^3 in both precisions).
 #ifndef PM_FFT_XLINES32
-#define PM_FFT_XLINES32 8
+#define PM_FFT_XLINES32 4
+#endif
+#ifndef PM_FFT_XLINES64
+#define PM_FFT_XLINES64 4
 #endif
 template <typename Rl, bool XPASS = false> __host__ __device__ constexpr int tile_lines(int N) {
-    const int base = sizeof(Rl) == 4 ? (XPASS ? PM_FFT_XLINES32 : PM_FFT_LINES32) : PM_FFT_LINES64;
+    const int base = sizeof(Rl) == 4 ? (XPASS ? PM_FFT_XLINES32 : PM_FFT_LINES32)
+                                     : (XPASS ? PM_FFT_XLINES64 : PM_FFT_LINES64);
     const int t = N >= 1024 ? base / 2 : base;
     return t < N ? t : N;
 }
