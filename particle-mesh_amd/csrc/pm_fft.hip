@@ -409,12 +409,16 @@ __global__ __launch_bounds__(256) void k_apply_green(cplx_t<Rl>* __restrict__ sp
 // same transform bit for bit (only the work split differs).  Meshes below
 // kTuneMinN (pm_internal.h) always take variant 0.
 
-__host__ __device__ constexpr int slot_default_threads(int slot) {
-    return slot == 0 ? PM_FFT_NT_ZF : slot == 1 ? PM_FFT_NT_YF : slot == 2 ? PM_FFT_NT_X
-         : slot == 3 ? PM_FFT_NT_YI : PM_FFT_NT_ZI;
+// FP64 tiles hold half the lines of FP32 ones (the same 128-byte segments):
+// the z forward and the y passes take half the threads (measured at 512^3
+// and 1024^3 by pm_plan_tune).
+template <typename Rl> __host__ __device__ constexpr int slot_default_threads(int slot) {
+    constexpr bool d = sizeof(Rl) == 8;
+    return slot == 0 ? (d ? PM_FFT_NT_ZF / 2 : PM_FFT_NT_ZF) : slot == 1 ? (d ? PM_FFT_NT_YF / 2 : PM_FFT_NT_YF)
+         : slot == 2 ? PM_FFT_NT_X : slot == 3 ? (d ? PM_FFT_NT_YI / 2 : PM_FFT_NT_YI) : PM_FFT_NT_ZI;
 }
-__host__ __device__ constexpr int variant_threads(int slot, int v) {
-    const int d = slot_default_threads(slot), i = v % 3;
+template <typename Rl> __host__ __device__ constexpr int variant_threads(int slot, int v) {
+    const int d = slot_default_threads<Rl>(slot), i = v % 3;
     const int lo = d == 256 ? 512 : 256, hi = d == 1024 ? 512 : 1024;
     return i == 0 ? d : i == 1 ? lo : hi;
 }
@@ -432,9 +436,11 @@ static void launch_variant(int pass, hipStream_t st, const void* in, void* out, 
     constexpr int TZF = variant_lines<Rl>(0, N, V), TYF = variant_lines<Rl>(1, N, V);
     constexpr int TX = variant_lines<Rl>(2, N, V), TYI = variant_lines<Rl>(3, N, V);
     constexpr int TZI = variant_lines<Rl>(4, N, V);
-    constexpr int NZF = fft_threads(variant_threads(0, V), TZF * M), NYF = fft_threads(variant_threads(1, V), TYF * N);
-    constexpr int NX = fft_threads(variant_threads(2, V), TX * N), NYI = fft_threads(variant_threads(3, V), TYI * N);
-    constexpr int NZI = fft_threads(variant_threads(4, V), TZI * M);
+    constexpr int NZF = fft_threads(variant_threads<Rl>(0, V), TZF * M);
+    constexpr int NYF = fft_threads(variant_threads<Rl>(1, V), TYF * N);
+    constexpr int NX = fft_threads(variant_threads<Rl>(2, V), TX * N);
+    constexpr int NYI = fft_threads(variant_threads<Rl>(3, V), TYI * N);
+    constexpr int NZI = fft_threads(variant_threads<Rl>(4, V), TZI * M);
     switch (pass) {
         case kPassZFwd: {
             const unsigned g = (unsigned)((size_t)N * N / TZF);  // (x, y-tile)
