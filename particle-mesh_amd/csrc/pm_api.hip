@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pm_internal.h"
@@ -29,12 +30,15 @@ static int hip_ok(hipError_t e, const char* what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// One range-error counter per device, allocated on first use.
+// One range-error counter per device, allocated on first use (under a
+// lock: plans on one device may be driven from several host threads).
 static unsigned long long* g_range[64] = {};
+static std::mutex g_range_mu;
 
 unsigned long long* range_counter() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    std::lock_guard<std::mutex> lock(g_range_mu);
     if (!g_range[dev]) {
         void* p = nullptr;
         if (hipMalloc(&p, sizeof(unsigned long long)) != hipSuccess) return nullptr;
