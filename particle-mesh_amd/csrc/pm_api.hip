@@ -386,8 +386,8 @@ extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx
     if (variant == PM_DEPOSIT_SORTED) {
         rc = ensure_sorted(p);
         staged = !rc && p->sorted.stage != nullptr;
-        if (rc) {
-        } else if (staged && reuse && p->staged_valid && p->staged_x == x) {
+        if (rc) return rc;
+        if (staged && reuse && p->staged_valid && p->staged_x == x) {
             // positions staged in tile order by the previous step's
             // interpolation (caller's guarantee that x is unchanged since)
             rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
