@@ -176,7 +176,9 @@ int pm_plan_create(pm_plan_t** out, int ng, int64_t np, int precision, int green
 int pm_plan_destroy(pm_plan_t* plan);
 int pm_plan_set_stream(pm_plan_t* plan, void* stream);
 /* Device pointers of the plan's density (after pm_deposit: mass*CIC, without
- * the -1) and potential (after pm_solve) meshes, ng^3 in the mesh precision. */
+ * the -1) and potential (after pm_solve) meshes, ng^3 in the mesh precision.
+ * The adjoint calls (section 3) use the density mesh as scratch for the
+ * divergence sum_a D_a^T U_a, and pm_plan_tune overwrites both. */
 void* pm_plan_density(pm_plan_t* plan);
 void* pm_plan_potential(pm_plan_t* plan);
 
