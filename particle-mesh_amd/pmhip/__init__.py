@@ -272,6 +272,11 @@ class Plan:
         check(lib().pm_run_adjoint(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel),
                                    *(dptr(t) for t in xbar), *(dptr(t) for t in vbar), nsteps, dt, variant))
 
+    def set_stream(self, stream: int):
+        """Run the plan's later calls on another HIP stream (pm_plan_set_stream)."""
+        self._stream = stream
+        check(lib().pm_plan_set_stream(self.h, stream))
+
     def enable_timing(self, max_steps: int):
         """Record phase events for the next max_steps steps (0 = off)."""
         check(lib().pm_plan_enable_timing(self.h, max_steps))

@@ -653,3 +653,53 @@ def test_snapshot_round_trip(dev, tmp_path):
         assert np.array_equal(host(a), b)
     with pytest.raises(pmhip.PMError):
         pmhip.read_snapshot(path, [p[:10] for p in pos2], [v[:10] for v in vel2])
+
+
+def test_phase_timing_records_reserved_steps(dev):
+    """pm_plan_enable_timing(k) records the phases of the next k steps (no
+    synchronisation inside the loop); pm_plan_phase_ms returns the sums and
+    counts; reset clears them; k = 0 turns recording off."""
+    import torch
+
+    ng, nrow = 64, 32
+    noise = torch.tensor(np.random.default_rng(71).standard_normal(nrow ** 3), device=dev)
+    pos, vel = pmhip.zeldovich_ic(nrow, ng, noise, 1.0, 1.0)
+    with pmhip.Plan(ng, nrow ** 3, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.enable_timing(3)
+        for s in range(5):
+            plan.step(pos, vel, 0.01, 0.02, pmhip.PM_DEPOSIT_SORTED | (pmhip.PM_STEP_REUSE_BINS if s else 0))
+        ms, cnt = plan.phase_ms()
+        assert cnt == [3] * pmhip.PM_NPHASES
+        assert all(m > 0 for m in ms)
+        plan.reset_timing()
+        ms, cnt = plan.phase_ms()
+        assert cnt == [0] * pmhip.PM_NPHASES and ms == [0.0] * pmhip.PM_NPHASES
+        plan.enable_timing(0)
+        plan.step(pos, vel, 0.01, 0.02, pmhip.PM_DEPOSIT_SORTED | pmhip.PM_STEP_REUSE_BINS)
+        ms, cnt = plan.phase_ms()
+        assert cnt == [0] * pmhip.PM_NPHASES
+
+
+def test_plan_on_another_stream_matches_default_stream(dev):
+    """pm_plan_set_stream moves a plan's work to another stream (default
+    priority); the same steps give the same particles bit for bit."""
+    import torch
+
+    ng, nrow = 64, 32
+    noise = torch.tensor(np.random.default_rng(72).standard_normal(nrow ** 3), device=dev)
+    pos0, vel0 = pmhip.zeldovich_ic(nrow, ng, noise, 1.0, 1.0)
+    torch.cuda.synchronize()
+    out = []
+    side = torch.cuda.Stream()
+    for use_side in (False, True):
+        pos, vel = [p.clone() for p in pos0], [v.clone() for v in vel0]
+        torch.cuda.synchronize()
+        with pmhip.Plan(ng, nrow ** 3, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+            if use_side:
+                plan.set_stream(side.cuda_stream)
+            for s in range(3):
+                plan.step(pos, vel, 0.01, 0.02, pmhip.PM_DEPOSIT_SORTED | (pmhip.PM_STEP_REUSE_BINS if s else 0))
+            (side if use_side else torch.cuda.current_stream()).synchronize()
+            out.append([t.clone() for t in pos + vel])
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
