@@ -66,11 +66,11 @@ def algorithmic_bytes(ng: int, npart: int, prec: int, staged: bool = True):
     """Compulsory HBM bytes per launch of each phase of a steady-state step
     (b = mesh real bytes; positions and velocities FP64; spectrum
     (ng/2+1) ng^2 complex).  With the sorted deposit the interpolation
-    stages every drifted position in tile order as three in-tile offsets of
-    s bytes (u32 for an FP32 mesh, FP64 otherwise) and the next deposit reads
-    them back: 3 s per particle written once and read once."""
+    stages every drifted position in tile order as a record of its in-tile
+    offsets (12 B: 3 x u32 for an FP32 mesh; 16 B: 3 x 42 bits for an FP64
+    mesh) and the next deposit reads it back: written once and read once."""
     b = 4 if prec == 0 else 8
-    st = (3 * (4 if prec == 0 else 8)) if staged else 0
+    st = (12 if prec == 0 else 16) if staged else 0
     nc = ng ** 3
     nk = (ng // 2 + 1) * ng * ng
     return {

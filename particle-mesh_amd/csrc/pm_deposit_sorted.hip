@@ -294,8 +294,12 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict_
                                      (qx & kQMask) / kQScale, (qy & kQMask) / kQScale, (qz & kQMask) / kQScale,
                                      fix);
                 } else {
-                    const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
-                    tile_cic_add<TS>(acc, ix - ox, iy - oy, iz - oz, x[j] - ix, y[j] - iy, z[j] - iz, fix);
+                    // the staged records' 2^-38-cell offsets, formed the same way
+                    const uint64_t qx = (uint64_t)((x[j] - ox) * kQScaleD), qy = (uint64_t)((y[j] - oy) * kQScaleD),
+                                   qz = (uint64_t)((z[j] - oz) * kQScaleD);
+                    tile_cic_add<TS>(acc, (int)(qx >> 38), (int)(qy >> 38), (int)(qz >> 38),
+                                     (double)(qx & kQMaskD) / kQScaleD, (double)(qy & kQMaskD) / kQScaleD,
+                                     (double)(qz & kQMaskD) / kQScaleD, fix);
                 }
             }
         }
@@ -307,9 +311,9 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict_
 
 // 4'. as 4, particles from the staging arrays that the interpolation of the
 // previous step filled in tile order: tile t holds fill[t] records from
-// coff[t], in-tile offsets as u32 at 2^-28 cells (FP32 mesh) or exact
-// doubles (FP64 mesh, giving the same weights as the binned path bit for
-// bit).  Skipped when a tile overflowed its capacity (the binned path runs
+// coff[t], in-tile offsets in fixed point (2^-28 cells, FP32 mesh; 2^-38
+// cells, FP64 mesh), which the binned path forms in the same way, so both
+// give the same weights bit for bit.  Skipped when a tile overflowed its capacity (the binned path runs
 // instead).
 template <typename Rl, int TS>
 __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__ stage,
@@ -348,8 +352,11 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
                                      (v[j].x & kQMask) / kQScale, (v[j].y & kQMask) / kQScale,
                                      (v[j].z & kQMask) / kQScale, fix);
                 } else {
-                    const int ia = (int)v[j].x, ib = (int)v[j].y, ic = (int)v[j].z;
-                    tile_cic_add<TS>(acc, ia, ib, ic, v[j].x - ia, v[j].y - ib, v[j].z - ic, fix);
+                    uint64_t qx, qy, qz;
+                    unpack_stage_d(v[j], qx, qy, qz);
+                    tile_cic_add<TS>(acc, (int)(qx >> 38), (int)(qy >> 38), (int)(qz >> 38),
+                                     (double)(qx & kQMaskD) / kQScaleD, (double)(qy & kQMaskD) / kQScaleD,
+                                     (double)(qz & kQMaskD) / kQScaleD, fix);
                 }
             }
         }

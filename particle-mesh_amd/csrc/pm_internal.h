@@ -71,14 +71,27 @@ __global__ __launch_bounds__(256) void k_fill(T* __restrict__ p, size_t n, T v) 
         p[i] = v;
 }
 
-// A staged particle: its offsets inside its tile, u32 at 2^-28 cells for an
-// FP32 mesh, exact doubles for an FP64 mesh (one record = one vector store).
+// A staged particle: its offsets inside its tile (one record = one vector
+// store).  FP32 mesh: u32 at 2^-28 cells (cell = q >> 28).  FP64 mesh: 42
+// bits each at 2^-38 cells (cell = q >> 38) packed in 16 bytes: x in bits
+// 0..41 of lo, y in bits 42..63 of lo and 0..19 of hi, z in bits 20..61 of hi.
 struct StageQ {
     uint32_t x, y, z;
 };
 struct StageD {
-    double x, y, z;
+    uint64_t lo, hi;
 };
+constexpr double kQScaleD = 274877906944.0;  // 2^38
+constexpr uint64_t kQMaskD = (1ull << 38) - 1;
+__host__ __device__ inline StageD pack_stage_d(uint64_t qx, uint64_t qy, uint64_t qz) {
+    return StageD{qx | (qy << 42), (qy >> 22) | (qz << 20)};
+}
+__host__ __device__ inline void unpack_stage_d(const StageD& r, uint64_t& qx, uint64_t& qy, uint64_t& qz) {
+    constexpr uint64_t m42 = (1ull << 42) - 1;
+    qx = r.lo & m42;
+    qy = ((r.lo >> 42) | (r.hi << 22)) & m42;
+    qz = (r.hi >> 20) & m42;
+}
 
 // Work space of the sorted deposit (pm_deposit_sorted.hip).
 struct SortedScratch {

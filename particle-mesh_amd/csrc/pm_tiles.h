@@ -197,7 +197,8 @@ __device__ __forceinline__ int64_t stage_pass(uint32_t kk, uint32_t* hkey, uint3
 }
 
 // Tile of a position in [0, ng)^3 for power-of-two tiles of ts cells, and
-// the offsets inside it: u32 at 2^-28 cells (FP32 mesh) or exact doubles.
+// the offsets inside it: u32 at 2^-28 cells (FP32 mesh) or 42-bit at 2^-38
+// cells (FP64 mesh).
 __device__ __forceinline__ uint32_t tile_key(double x, double y, double z, int ng, int ts) {
     const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;
     return (uint32_t)((((int)x >> lt) * nt + ((int)y >> lt)) * nt + ((int)z >> lt));
@@ -209,8 +210,9 @@ template <> __device__ __forceinline__ StageQ tile_record<StageQ>(double x, doub
                   (uint32_t)((z - ((int)z & msk)) * 268435456.0)};
 }
 template <> __device__ __forceinline__ StageD tile_record<StageD>(double x, double y, double z, int ts) {
-    const int msk = ~(ts - 1);
-    return StageD{x - ((int)x & msk), y - ((int)y & msk), z - ((int)z & msk)};
+    const int msk = ~(ts - 1);  // x - origin is exact; the scale is a power of two
+    return pack_stage_d((uint64_t)((x - ((int)x & msk)) * kQScaleD), (uint64_t)((y - ((int)y & msk)) * kQScaleD),
+                        (uint64_t)((z - ((int)z & msk)) * kQScaleD));
 }
 
 }  // namespace pm
