@@ -140,17 +140,28 @@ __device__ __forceinline__ double vec3_scale(unsigned long long maxbits) {
     return ldexp(1.0, 40 - e);
 }
 
+// Particles in flight per thread.  Full reverse sweep of 20 steps at 512^3
+// in an interleaved A/B: 260.5 ms (2), 265.7 (4), 279.5 (8); 268.2 with 2
+// and the positions gathered by index instead of the staged records.
+#ifndef PM_VEC3_BATCH
+#define PM_VEC3_BATCH 2
+#endif
 template <typename Rl, int TS>
 __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
-    const double* __restrict__ ux, const double* __restrict__ uy, const double* __restrict__ uz,
-    const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, int ng,
-    const unsigned long long* __restrict__ umax, Rl* __restrict__ U0, Rl* __restrict__ U1, Rl* __restrict__ U2,
-    Rl* __restrict__ shell, size_t shell_stride, const uint32_t* __restrict__ gate, int run_if) {
+    const void* __restrict__ stage, const double* __restrict__ ux, const double* __restrict__ uy,
+    const double* __restrict__ uz, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off,
+    const uint32_t* __restrict__ cnt, int ng, const unsigned long long* __restrict__ umax, Rl* __restrict__ U0,
+    Rl* __restrict__ U1, Rl* __restrict__ U2, Rl* __restrict__ shell, size_t shell_stride,
+    const uint32_t* __restrict__ gate, int run_if) {
     // tile t's particles: idx[off[t] .. off[t] + cnt[t]) with counts (the
-    // buckets of a staging gather), else idx[off[t] .. off[t+1]) (a binning);
-    // with a gate the kernel runs only if (*gate != 0) == run_if
+    // buckets of a staging gather, whose records at the same slots hold the
+    // in-tile offsets), else idx[off[t] .. off[t+1]) (a binning; offsets
+    // formed from the positions in the same fixed point, so both give the
+    // same weights); with a gate the kernel runs only if (*gate != 0) == run_if
     if (gate && ((*gate != 0u) != (run_if != 0))) return;
+    using St = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
+    const St* __restrict__ sp = static_cast<const St*>(stage);
     constexpr int E = TS + 1, E3 = E * E * E;
     constexpr int NT = kVec3Threads;
     __shared__ unsigned long long acc[3][E3];
@@ -166,21 +177,21 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
         const int tz = sw.sg * sw.seg + k, oz = tz * TS;
         const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;
         const uint32_t beg = off[t], end = cnt ? beg + cnt[t] : off[t + 1];
-        constexpr int B = 2;  // particles in flight per thread
+        constexpr int B = PM_VEC3_BATCH;  // particles in flight per thread
         for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += NT * B) {
             uint32_t p[B];
-            double x[B], y[B], z[B], u[B][3];
+            St r[B];
+            double u[B][3];
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 const uint32_t q = q0 + NT * j;
                 p[j] = q < end ? idx[q] : 0xFFFFFFFFu;
+                if (sp && q < end) r[j] = sp[q];
             }
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 if (p[j] == 0xFFFFFFFFu) continue;
-                x[j] = px[p[j]];
-                y[j] = py[p[j]];
-                z[j] = pz[p[j]];
+                if (!sp) r[j] = tile_record_of<St>(px[p[j]] - ox, py[p[j]] - oy, pz[p[j]] - oz);
                 u[j][0] = ux[p[j]] * sc[0];
                 u[j][1] = uy[p[j]] * sc[1];
                 u[j][2] = uz[p[j]] * sc[2];
@@ -188,13 +199,13 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 if (p[j] == 0xFFFFFFFFu) continue;
-                const int ix = (int)x[j], iy = (int)y[j], iz = (int)z[j];
-                const double fx = x[j] - ix, fy = y[j] - iy, fz = z[j] - iz;
+                const TileOffs o = tile_offs(r[j]);
+                const double fx = o.fx, fy = o.fy, fz = o.fz;
                 const double w[8] = {(1 - fx) * (1 - fy) * (1 - fz), (1 - fx) * (1 - fy) * fz,
                                      (1 - fx) * fy * (1 - fz),       (1 - fx) * fy * fz,
                                      fx * (1 - fy) * (1 - fz),       fx * (1 - fy) * fz,
                                      fx * fy * (1 - fz),             fx * fy * fz};
-                const int n0 = ((ix - ox) * E + (iy - oy)) * E + (iz - oz);
+                const int n0 = (o.a * E + o.b) * E + o.c;
 #pragma unroll
                 for (int m = 0; m < 8; ++m) {
                     const int n = n0 + ((m >> 2) & 1) * E * E + ((m >> 1) & 1) * E + (m & 1);
@@ -444,16 +455,16 @@ int launch_deposit_vec3(int prec, const double* px, const double* py, const doub
         hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
     }
     // with buckets: the bucket pass, or (overflow) the binning pass
-#define PM_VEC3_PASS(RL, TSV, IDX, OFF, CNT, GATE, RUN)                                                            \
+#define PM_VEC3_PASS(RL, TSV, REC, IDX, OFF, CNT, GATE, RUN)                                                       \
     hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles / walk_seg(nt)), dim3(kVec3Threads), 0, st, px, \
-                       py, pz, ux, uy, uz, IDX, OFF, CNT, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3, sh, \
-                       GATE, RUN)
+                       py, pz, REC, ux, uy, uz, IDX, OFF, CNT, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2],          \
+                       (RL*)shell3, sh, GATE, RUN)
 #define PM_VEC3(RL, TSV)                                                                                    \
     if (bucket_idx) {                                                                                       \
-        PM_VEC3_PASS(RL, TSV, bucket_idx, s.coff, s.fill, s.spill, 0);                                      \
-        PM_VEC3_PASS(RL, TSV, s.idx, s.off, nullptr, s.spill, 1);                                           \
+        PM_VEC3_PASS(RL, TSV, s.stage, bucket_idx, s.coff, s.fill, s.spill, 0);                             \
+        PM_VEC3_PASS(RL, TSV, nullptr, s.idx, s.off, nullptr, s.spill, 1);                                  \
     } else {                                                                                                \
-        PM_VEC3_PASS(RL, TSV, s.idx, s.off, nullptr, nullptr, 0);                                           \
+        PM_VEC3_PASS(RL, TSV, nullptr, s.idx, s.off, nullptr, nullptr, 0);                                  \
     }                                                                                                       \
     for (int d = 0; d < 3; ++d) launch_fold_shells(prec, ts, ng, (const char*)shell3 + d * sh * rs, U[d], st);
     if (prec == PM_FP32) {

@@ -232,16 +232,6 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t* __restrict_
 }
 
 
-// In-tile offsets of the FP32 path: u32 fixed point with 2^-28 cell
-// resolution (cell = q >> 28, fraction = low 28 bits).  The staged deposit
-// reads them from the interpolation's staging arrays; the binned FP32
-// deposit forms the same values from the FP64 positions, so both give
-// bit-identical meshes.  2^-28 cells is far below the FP32 mesh rounding.
-constexpr double kQScale = 268435456.0;  // 2^28
-constexpr uint32_t kQMask = 0x0FFFFFFFu;
-
-__device__ __forceinline__ uint32_t quantize_offset(double local) { return (uint32_t)(local * kQScale); }
-
 // 4. one workgroup per segment of the column walk, particles by index
 // (binning of positions).
 template <typename Rl, int TS>
@@ -252,6 +242,7 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict_
                                                        Rl* __restrict__ rho, Rl* __restrict__ shell,
                                                        const uint32_t* __restrict__ gate) {
     if (gate && *gate == 0u) return;
+    using St = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
     constexpr int E = TS + 1, E3 = E * E * E;
     __shared__ unsigned long long acc[E3];
     const WalkSeg sw = walk_segment(ng, TS);
@@ -287,20 +278,9 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict_
 #pragma unroll
             for (int j = 0; j < kDepBatch; ++j) {
                 if (p[j] == kInvalidKey) continue;
-                if constexpr (sizeof(Rl) == 4) {
-                    const uint32_t qx = quantize_offset(x[j] - ox), qy = quantize_offset(y[j] - oy),
-                                   qz = quantize_offset(z[j] - oz);
-                    tile_cic_add<TS>(acc, (int)(qx >> 28), (int)(qy >> 28), (int)(qz >> 28),
-                                     (qx & kQMask) / kQScale, (qy & kQMask) / kQScale, (qz & kQMask) / kQScale,
-                                     fix);
-                } else {
-                    // the staged records' 2^-38-cell offsets, formed the same way
-                    const uint64_t qx = (uint64_t)((x[j] - ox) * kQScaleD), qy = (uint64_t)((y[j] - oy) * kQScaleD),
-                                   qz = (uint64_t)((z[j] - oz) * kQScaleD);
-                    tile_cic_add<TS>(acc, (int)(qx >> 38), (int)(qy >> 38), (int)(qz >> 38),
-                                     (double)(qx & kQMaskD) / kQScaleD, (double)(qy & kQMaskD) / kQScaleD,
-                                     (double)(qz & kQMaskD) / kQScaleD, fix);
-                }
+                // the staged records' fixed-point offsets, formed the same way
+                const TileOffs o = tile_offs(tile_record_of<St>(x[j] - ox, y[j] - oy, z[j] - oz));
+                tile_cic_add<TS>(acc, o.a, o.b, o.c, o.fx, o.fy, o.fz, fix);
             }
         }
         __syncthreads();
@@ -347,17 +327,8 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
 #pragma unroll
             for (int j = 0; j < kDepBatch; ++j) {
                 if (q0 + 256 * j >= end) continue;
-                if constexpr (sizeof(Rl) == 4) {
-                    tile_cic_add<TS>(acc, (int)(v[j].x >> 28), (int)(v[j].y >> 28), (int)(v[j].z >> 28),
-                                     (v[j].x & kQMask) / kQScale, (v[j].y & kQMask) / kQScale,
-                                     (v[j].z & kQMask) / kQScale, fix);
-                } else {
-                    uint64_t qx, qy, qz;
-                    unpack_stage_d(v[j], qx, qy, qz);
-                    tile_cic_add<TS>(acc, (int)(qx >> 38), (int)(qy >> 38), (int)(qz >> 38),
-                                     (double)(qx & kQMaskD) / kQScaleD, (double)(qy & kQMaskD) / kQScaleD,
-                                     (double)(qz & kQMaskD) / kQScaleD, fix);
-                }
+                const TileOffs o = tile_offs(v[j]);
+                tile_cic_add<TS>(acc, o.a, o.b, o.c, o.fx, o.fy, o.fz, fix);
             }
         }
         __syncthreads();

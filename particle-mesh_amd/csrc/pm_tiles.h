@@ -203,16 +203,40 @@ __device__ __forceinline__ uint32_t tile_key(double x, double y, double z, int n
     const int lt = __builtin_ctz((unsigned)ts), nt = ng >> lt;
     return (uint32_t)((((int)x >> lt) * nt + ((int)y >> lt)) * nt + ((int)z >> lt));
 }
-template <typename Rec> __device__ __forceinline__ Rec tile_record(double x, double y, double z, int ts);
-template <> __device__ __forceinline__ StageQ tile_record<StageQ>(double x, double y, double z, int ts) {
-    const int msk = ~(ts - 1);  // tile origin = cell with the low bits cleared
-    return StageQ{(uint32_t)((x - ((int)x & msk)) * 268435456.0), (uint32_t)((y - ((int)y & msk)) * 268435456.0),
-                  (uint32_t)((z - ((int)z & msk)) * 268435456.0)};
+// The record of in-tile offsets (lx, ly, lz) in [0, ts)^3 (exact: a position
+// minus its tile origin): u32 fixed point at 2^-28 cells for the FP32 mesh
+// (cell = q >> 28, fraction = the low 28 bits), 42-bit fields at 2^-38 cells
+// for the FP64 mesh, both far below the mesh's own rounding.  Truncation, so
+// that a position and its record always name the same cell.
+template <typename Rec> __device__ __forceinline__ Rec tile_record_of(double lx, double ly, double lz);
+template <> __device__ __forceinline__ StageQ tile_record_of<StageQ>(double lx, double ly, double lz) {
+    return StageQ{(uint32_t)(lx * 268435456.0), (uint32_t)(ly * 268435456.0), (uint32_t)(lz * 268435456.0)};
 }
-template <> __device__ __forceinline__ StageD tile_record<StageD>(double x, double y, double z, int ts) {
-    const int msk = ~(ts - 1);  // x - origin is exact; the scale is a power of two
-    return pack_stage_d((uint64_t)((x - ((int)x & msk)) * kQScaleD), (uint64_t)((y - ((int)y & msk)) * kQScaleD),
-                        (uint64_t)((z - ((int)z & msk)) * kQScaleD));
+template <> __device__ __forceinline__ StageD tile_record_of<StageD>(double lx, double ly, double lz) {
+    return pack_stage_d((uint64_t)(lx * kQScaleD), (uint64_t)(ly * kQScaleD), (uint64_t)(lz * kQScaleD));
+}
+template <typename Rec> __device__ __forceinline__ Rec tile_record(double x, double y, double z, int ts) {
+    const int msk = ~(ts - 1);  // tile origin = cell with the low bits cleared
+    return tile_record_of<Rec>(x - ((int)x & msk), y - ((int)y & msk), z - ((int)z & msk));
+}
+
+// Cell in the tile and offsets in the cell of a record: the deposits take
+// every particle through a record (read from the staging, or formed from its
+// position), so that staged and binned paths give the same weights.
+struct TileOffs {
+    int a, b, c;
+    double fx, fy, fz;
+};
+__device__ __forceinline__ TileOffs tile_offs(const StageQ& r) {
+    constexpr uint32_t m = 0x0FFFFFFFu;
+    constexpr double s = 1.0 / 268435456.0;
+    return TileOffs{(int)(r.x >> 28), (int)(r.y >> 28), (int)(r.z >> 28), (r.x & m) * s, (r.y & m) * s, (r.z & m) * s};
+}
+__device__ __forceinline__ TileOffs tile_offs(const StageD& r) {
+    uint64_t qx, qy, qz;
+    unpack_stage_d(r, qx, qy, qz);
+    return TileOffs{(int)(qx >> 38), (int)(qy >> 38), (int)(qz >> 38), (double)(qx & kQMaskD) / kQScaleD,
+                    (double)(qy & kQMaskD) / kQScaleD, (double)(qz & kQMaskD) / kQScaleD};
 }
 
 }  // namespace pm
