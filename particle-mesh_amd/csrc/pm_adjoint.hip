@@ -306,7 +306,8 @@ __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, dou
                                                     int N, const Rl* __restrict__ phi, const Rl* __restrict__ lam,
                                                     double mass, double kick, double vjp_scale, double next_drift,
                                                     unsigned long long* __restrict__ umax,
-                                                    unsigned long long* __restrict__ err, AdjStage stg) {
+                                                    unsigned long long* __restrict__ err, AdjStage stg,
+                                                    BandMap bm) {
     static_assert(!STAGE || MODE == 2, "only the undrifting gather stages");
     using Rec = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
     constexpr uint32_t kNo = 0xFFFFFFFFu;
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, dou
     __shared__ uint32_t hkey[HS], hcnt[HS], hbase[HS];
     uint32_t kk = kNo;  // STAGE: tile of the undrifted position
     Rec rec{};
-    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int64_t i = (int64_t)band_remap(blockIdx.x, gridDim.x, bm) * 256 + threadIdx.x;
     const double box = (double)N;
     // three load phases, as in k_interp: the particle's streams, then the
     // gathers around its cell (cell 0 for a lane that stores nothing), then
@@ -507,10 +508,11 @@ int launch_adj_gather(int prec, int mode, double* px, double* py, double* pz, do
     if (stage && (mode != 2 || !stage->rec || !stage->idx)) return fail(PM_ERR_ARG, "adjoint gather: bad staging");
     const unsigned nb = (unsigned)((np + 255) / 256);
     const AdjStage stg = stage ? *stage : AdjStage();
+    const BandMap bm = band_map(np, nb, 256);
 #define PM_GATHER(RL, M, S)                                                                                          \
     hipLaunchKernelGGL((k_adj_gather<RL, M, S>), dim3(nb), dim3(256), 0, st, px, py, pz, vx, vy, vz, ux, uy, uz, bx, \
                        by, bz, np, ng, (const RL*)phi, (const RL*)lam, mass, kick, vjp_scale, next_drift, umax, err,  \
-                       stg)
+                       stg, bm)
     if (prec == PM_FP32) {
         if (mode == 0)
             PM_GATHER(float, 0, false);

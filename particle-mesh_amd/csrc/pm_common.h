@@ -18,6 +18,27 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
     return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
+// Block order of the particle kernels.  Particles are usually stored in
+// Lagrangian order (the lattice of the initial conditions, np = L^3), so a
+// block of consecutive particles gathers from a small patch of the mesh.
+// Swept plane by plane, one mesh plane serves the particles of four lattice
+// planes, a whole plane apart, and drops out of the XCD's L2 in between.
+// With a band map each XCD sweeps its range of whole lattice planes in bands
+// of rows instead: band by band, each band through all the XCD's planes,
+// so the mesh rows around a band stay in L2 while the next planes use them.
+// A permutation of the blocks: any particle order gives the same results.
+struct BandMap {
+    uint32_t bpp, bb;  // blocks per lattice plane, blocks per band; bb == 0: plain order
+};
+__device__ __forceinline__ uint32_t band_remap(uint32_t bid, uint32_t nb, BandMap m) {
+    const uint32_t l = xcd_remap(bid, nb);
+    if (m.bb == 0) return l;
+    // the host checked nb % kXcds == 0, len % bpp == 0 and bpp % bb == 0
+    const uint32_t len = nb / kXcds, r = l % len, s = l - r;  // r: dispatch order in the XCD
+    const uint32_t planes = len / m.bpp, o = r % m.bb, q = r / m.bb;
+    return s + (q % planes) * m.bpp + (q / planes) * m.bb + o;
+}
+
 // Position inside the periodic box [0, box)^3 (false for NaN).  The six
 // compares are combined with bitwise &: with a short-circuit && the compiler
 // sinks the y and z loads behind the x test, and the three loads of a

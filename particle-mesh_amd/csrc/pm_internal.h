@@ -1,6 +1,7 @@
 // Internal declarations shared by the PM HIP sources.
 #pragma once
 
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -8,6 +9,26 @@
 #include "pm_common.h"
 
 namespace pm {
+
+// Lattice rows per band of the particle kernels' block order (band_remap);
+// 0 sweeps the particles in plain order.
+#ifndef PM_BAND_ROWS
+#define PM_BAND_ROWS 32
+#endif
+// The band map for np particles in nb blocks of per_block: a band map when
+// np = L^3 and the XCD ranges, lattice planes and bands fall on whole blocks,
+// else the plain order.
+inline BandMap band_map(int64_t np, unsigned nb, int per_block) {
+    const BandMap plain{0u, 0u};
+    if (PM_BAND_ROWS <= 0 || np <= 0 || nb % kXcds) return plain;
+    const int64_t l = std::llround(std::cbrt((double)np));
+    if (l * l * l != np || PM_BAND_ROWS >= l) return plain;
+    const int64_t plane = l * l, band = (int64_t)PM_BAND_ROWS * l;
+    if (plane % per_block || band % per_block) return plain;
+    const int64_t bpp = plane / per_block, bb = band / per_block;
+    if (bpp % bb || (nb / kXcds) % bpp) return plain;
+    return BandMap{(uint32_t)bpp, (uint32_t)bb};
+}
 
 enum Pass {
     kPassZFwd = 0,

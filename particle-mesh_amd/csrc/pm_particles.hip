@@ -110,13 +110,13 @@ __global__ __launch_bounds__(NT) void k_interp(double* __restrict__ px, double* 
                                                int ng, const Rl* __restrict__ phi, double kick, double drift, int ts,
                                                uint32_t* __restrict__ fill, const uint32_t* __restrict__ coff,
                                                uint32_t* __restrict__ spill, void* __restrict__ stage,
-                                               unsigned long long* __restrict__ err) {
+                                               unsigned long long* __restrict__ err, BandMap bm) {
     constexpr uint32_t kNo = 0xFFFFFFFFu;
     using Rec = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
     constexpr int HS = MODE == 2 ? kStageHash : 1;
     static_assert((HS & (HS - 1)) == 0, "hash size must be a power of two");
     __shared__ uint32_t hkey[HS], hcnt[HS], hbase[HS];
-    const int64_t i0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * (NT * PER) + threadIdx.x;
+    const int64_t i0 = (int64_t)band_remap(blockIdx.x, gridDim.x, bm) * (NT * PER) + threadIdx.x;
     const double box = (double)ng;
     int64_t idx[PER];
     bool live[PER], ok[PER];
@@ -206,6 +206,7 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
               per = mode != 2 ? 1 : prec == PM_FP32 ? kStagePer32 : kStagePer64;
     const unsigned nb = (unsigned)((np + (int64_t)nt * per - 1) / ((int64_t)nt * per));
     const dim3 g(nb), b(nt);
+    const BandMap bm = band_map(np, nb, nt * per);
     const int ts = sorted_tile(ng);
     uint32_t* fill = s ? s->fill : nullptr;
     const uint32_t* coff = s ? s->coff : nullptr;
@@ -213,7 +214,7 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
     void* stage = s ? s->stage : nullptr;
 #define PM_INTERP(RL, M, NTV, PERV)                                                                          \
     hipLaunchKernelGGL((k_interp<RL, M, NTV, PERV>), g, b, 0, st, px, py, pz, vx, vy, vz, np, ng,          \
-                       (const RL*)phi, kick, drift, ts, fill, coff, spill, stage, err)
+                       (const RL*)phi, kick, drift, ts, fill, coff, spill, stage, err, bm)
     if (prec == PM_FP32) {
         if (mode == 0)
             PM_INTERP(float, 0, kPartThreads, 1);
