@@ -94,7 +94,7 @@ PHASE_KERNELS = {
     "fft_x_green": "void pm::k_x<float, 512, 7,",
     "fft_y_inv": "void pm::k_yinv<float, 512,",
     "fft_z_inv": "void pm::k_zinv<float, 512,",
-    "interp_kick_drift": "void pm::k_interp<float, 2, 256, 1>",
+    "interp_kick_drift": "void pm::k_interp<float, 2, 512, 1>",
 }
 
 

@@ -221,12 +221,13 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
     walk_seam_out<Rl, TS, 3, NT>(&acc[0][0], inv, sw, sh);
 }
 
-// Workgroup (256 threads) max of m[0..3) into one of kMaxSlots slots per
+// Workgroup (NT threads) max of m[0..3) into one of kMaxSlots slots per
 // component (umax[3 + d * kMaxSlots + slot]): bit patterns of non-negative
 // doubles order like the values, so an integer atomic max works, and the
 // slots spread the atomics; k_max_slots reduces them into umax[0..3).
+template <int NT>
 __device__ __forceinline__ void wg_max_to_slots(double (&m)[3], unsigned long long* __restrict__ umax) {
-    __shared__ double wm[3][256 / 64];
+    __shared__ double wm[3][NT / 64];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         for (int o = 32; o > 0; o >>= 1) m[d] = fmax(m[d], __shfl_xor(m[d], o));
@@ -235,7 +236,7 @@ __device__ __forceinline__ void wg_max_to_slots(double (&m)[3], unsigned long lo
     __syncthreads();
     if (threadIdx.x < 3) {
         double v = wm[threadIdx.x][0];
-        for (int w = 1; w < 256 / 64; ++w) v = fmax(v, wm[threadIdx.x][w]);
+        for (int w = 1; w < NT / 64; ++w) v = fmax(v, wm[threadIdx.x][w]);
         atomicMax(&umax[3 + threadIdx.x * kMaxSlots + (blockIdx.x & (kMaxSlots - 1))],
                   (unsigned long long)__double_as_longlong(v));
     }
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(256) void k_absmax3(const double* __restrict__ ux, 
         m[1] = fmax(m[1], fabs(uy[i]));
         m[2] = fmax(m[2], fabs(uz[i]));
     }
-    wg_max_to_slots(m, umax);
+    wg_max_to_slots<256>(m, umax);
 }
 
 // ------------------------------------------------- mu = sum_a D_a^T U_a
@@ -296,8 +297,18 @@ __global__ __launch_bounds__(256) void k_div3(const Rl* __restrict__ U0, const R
 // MODE 2: as MODE 0, then the undrift that opens the next reverse step,
 //         x -= next_drift * v and u = vbar + next_drift * xbar (in place on
 //         u), with max |u_a| into umax for that step's vec3 deposit.
-template <typename Rl, int MODE, bool STAGE>
-__global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, double* __restrict__ py,
+// Workgroup size of the adjoint gather.  Full reverse sweep of 20 steps at
+// 512^3 (FP32 mesh) in an interleaved A/B, 128 / 256 / 512 / 1024 threads:
+// 265.1 / 258.6 / 261.7 / 278.7 ms.
+#ifndef PM_ADJ_THREADS32
+#define PM_ADJ_THREADS32 256
+#endif
+#ifndef PM_ADJ_THREADS64
+#define PM_ADJ_THREADS64 256
+#endif
+constexpr int kAdjThreads32 = PM_ADJ_THREADS32, kAdjThreads64 = PM_ADJ_THREADS64;
+template <typename Rl, int MODE, bool STAGE, int NT>
+__global__ __launch_bounds__(NT) void k_adj_gather(double* __restrict__ px, double* __restrict__ py,
                                                     double* __restrict__ pz, double* __restrict__ vx,
                                                     double* __restrict__ vy, double* __restrict__ vz,
                                                     double* __restrict__ ux, double* __restrict__ uy,
@@ -315,7 +326,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, dou
     __shared__ uint32_t hkey[HS], hcnt[HS], hbase[HS];
     uint32_t kk = kNo;  // STAGE: tile of the undrifted position
     Rec rec{};
-    const int64_t i = (int64_t)band_remap(blockIdx.x, gridDim.x, bm) * 256 + threadIdx.x;
+    const int64_t i = (int64_t)band_remap(blockIdx.x, gridDim.x, bm) * NT + threadIdx.x;
     const double box = (double)N;
     // three load phases, as in k_interp: the particle's streams, then the
     // gathers around its cell (cell 0 for a lane that stores nothing), then
@@ -392,13 +403,13 @@ __global__ __launch_bounds__(256) void k_adj_gather(double* __restrict__ px, dou
         }
     }
     if constexpr (STAGE) {
-        const int64_t pos = stage_pass<HS, 256>(kk, hkey, hcnt, hbase, stg.fill, stg.coff, stg.spill);
+        const int64_t pos = stage_pass<HS, NT>(kk, hkey, hcnt, hbase, stg.fill, stg.coff, stg.spill);
         if (pos >= 0) {
             static_cast<Rec*>(stg.rec)[pos] = rec;
             stg.idx[pos] = (uint32_t)i;
         }
     }
-    if constexpr (MODE == 2) wg_max_to_slots(m, umax);
+    if constexpr (MODE == 2) wg_max_to_slots<NT>(m, umax);
 }
 
 // x -= drift * v (wrapped); u = vbar + drift * xbar (in place on vbar)
@@ -506,31 +517,32 @@ int launch_adj_gather(int prec, int mode, double* px, double* py, double* pz, do
     if (np <= 0) return PM_OK;
     if (mode == 2 && !umax) return fail(PM_ERR_ARG, "adjoint gather: no max buffer");
     if (stage && (mode != 2 || !stage->rec || !stage->idx)) return fail(PM_ERR_ARG, "adjoint gather: bad staging");
-    const unsigned nb = (unsigned)((np + 255) / 256);
+    const int nt = prec == PM_FP32 ? kAdjThreads32 : kAdjThreads64;
+    const unsigned nb = (unsigned)((np + nt - 1) / nt);
     const AdjStage stg = stage ? *stage : AdjStage();
-    const BandMap bm = band_map(np, nb, 256);
-#define PM_GATHER(RL, M, S)                                                                                          \
-    hipLaunchKernelGGL((k_adj_gather<RL, M, S>), dim3(nb), dim3(256), 0, st, px, py, pz, vx, vy, vz, ux, uy, uz, bx, \
+    const BandMap bm = band_map(np, nb, nt);
+#define PM_GATHER(RL, M, S, NTV)                                                                                         \
+    hipLaunchKernelGGL((k_adj_gather<RL, M, S, NTV>), dim3(nb), dim3(NTV), 0, st, px, py, pz, vx, vy, vz, ux, uy, uz, bx, \
                        by, bz, np, ng, (const RL*)phi, (const RL*)lam, mass, kick, vjp_scale, next_drift, umax, err,  \
                        stg, bm)
     if (prec == PM_FP32) {
         if (mode == 0)
-            PM_GATHER(float, 0, false);
+            PM_GATHER(float, 0, false, kAdjThreads32);
         else if (mode == 1)
-            PM_GATHER(float, 1, false);
+            PM_GATHER(float, 1, false, kAdjThreads32);
         else if (!stage)
-            PM_GATHER(float, 2, false);
+            PM_GATHER(float, 2, false, kAdjThreads32);
         else
-            PM_GATHER(float, 2, true);
+            PM_GATHER(float, 2, true, kAdjThreads32);
     } else {
         if (mode == 0)
-            PM_GATHER(double, 0, false);
+            PM_GATHER(double, 0, false, kAdjThreads64);
         else if (mode == 1)
-            PM_GATHER(double, 1, false);
+            PM_GATHER(double, 1, false, kAdjThreads64);
         else if (!stage)
-            PM_GATHER(double, 2, false);
+            PM_GATHER(double, 2, false, kAdjThreads64);
         else
-            PM_GATHER(double, 2, true);
+            PM_GATHER(double, 2, true, kAdjThreads64);
     }
 #undef PM_GATHER
     if (mode == 2) hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);

@@ -76,19 +76,28 @@ __device__ __forceinline__ double wrap_pos(double x, double box) {
     return x;
 }
 
-// Shape of the staging interpolation (MODE 2): 256 threads with PER
-// particles each and an LDS hash of 128 tiles per pass.  One particle per
-// thread for both meshes: with two, the gathers of both are in flight
-// together but the registers they take halve the waves per SIMD, which
-// measured slower in an interleaved A/B at 512^3 (FP32 3.88 against 3.82
-// ms, FP64 5.93 against 5.41 ms).
+// Shape of the staging interpolation (MODE 2): 512 threads (FP32 mesh) or
+// 256 (FP64 mesh) with PER particles each and an LDS hash of 128 tiles per
+// pass.  Interleaved A/B at 512^3 with the band order, 256 / 512 / 1024
+// threads: 3.40 / 3.26 / 3.95 ms (FP32 mesh); 128 / 256 / 512: 4.88 / 4.22 /
+// 4.32 ms (FP64 mesh).  One particle per thread for both meshes: with two,
+// the gathers of both are in flight together but the registers they take
+// halve the waves per SIMD (FP64: 5.02 against 4.26 ms; FP32 3.27 against
+// 3.40, no better than 512 threads), and capping the FP64 kernel at 96
+// VGPRs for a fifth wave spills (5.31 ms).
 #ifndef PM_STAGE_PER32
 #define PM_STAGE_PER32 1
 #endif
 #ifndef PM_STAGE_PER64
 #define PM_STAGE_PER64 1
 #endif
-constexpr int kStageThreads = 256;
+#ifndef PM_STAGE_THREADS32
+#define PM_STAGE_THREADS32 512
+#endif
+#ifndef PM_STAGE_THREADS64
+#define PM_STAGE_THREADS64 256
+#endif
+constexpr int kStageThreads32 = PM_STAGE_THREADS32, kStageThreads64 = PM_STAGE_THREADS64;
 constexpr int kStagePer32 = PM_STAGE_PER32, kStagePer64 = PM_STAGE_PER64;
 constexpr int kStageHash = 128;
 
@@ -103,8 +112,11 @@ constexpr int kStageHash = 128;
 // MODE 0: v += g*kick, x += v*drift.  MODE 1: write g to (vx, vy, vz).
 // MODE 2: as MODE 0, then the drifted position is staged for the next
 // step's deposit (stage_pass).
+#ifndef PM_INTERP_MINW
+#define PM_INTERP_MINW 1
+#endif
 template <typename Rl, int MODE, int NT, int PER>
-__global__ __launch_bounds__(NT) void k_interp(double* __restrict__ px, double* __restrict__ py,
+__global__ __launch_bounds__(NT, PM_INTERP_MINW) void k_interp(double* __restrict__ px, double* __restrict__ py,
                                                double* __restrict__ pz, double* __restrict__ vx,
                                                double* __restrict__ vy, double* __restrict__ vz, int64_t np,
                                                int ng, const Rl* __restrict__ phi, double kick, double drift, int ts,
@@ -202,7 +214,7 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
                   unsigned long long* err, hipStream_t st, SortedScratch* s) {
     if (np <= 0) return PM_OK;
     if (mode == 2 && (!s || !s->stage)) return fail(PM_ERR_ARG, "interp: staging buffers missing");
-    const int nt = mode == 2 ? kStageThreads : kPartThreads,
+    const int nt = mode != 2 ? kPartThreads : prec == PM_FP32 ? kStageThreads32 : kStageThreads64,
               per = mode != 2 ? 1 : prec == PM_FP32 ? kStagePer32 : kStagePer64;
     const unsigned nb = (unsigned)((np + (int64_t)nt * per - 1) / ((int64_t)nt * per));
     const dim3 g(nb), b(nt);
@@ -221,14 +233,14 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
         else if (mode == 1)
             PM_INTERP(float, 1, kPartThreads, 1);
         else
-            PM_INTERP(float, 2, kStageThreads, kStagePer32);
+            PM_INTERP(float, 2, kStageThreads32, kStagePer32);
     } else {
         if (mode == 0)
             PM_INTERP(double, 0, kPartThreads, 1);
         else if (mode == 1)
             PM_INTERP(double, 1, kPartThreads, 1);
         else
-            PM_INTERP(double, 2, kStageThreads, kStagePer64);
+            PM_INTERP(double, 2, kStageThreads64, kStagePer64);
     }
 #undef PM_INTERP
     return check_launch("interp");
