@@ -185,6 +185,35 @@ __device__ __forceinline__ void fft_lines(C* s, const C* __restrict__ tw) {
     if constexpr (lg % 3 == 1) stockham_stage<C, L, T, 2, DIR, NT>(s, p, tw);
 }
 
+// Twiddle table of a pass's L-point FFTs: for the passes in the mask of
+// its precision (bit = slot: 0 z fwd, 1 y fwd, 2 x, 3 y inv, 4 z inv) a
+// copy in LDS (L complex), which the workgroup reads after its first
+// barrier, instead of the global table through L1 (the same values, so the
+// same results bit for bit).  Interleaved A/B at 512^3, global -> LDS, ms:
+//   FP32  z fwd 0.250 -> 0.214, y fwd 0.227 -> 0.180, x 0.355 -> 0.247,
+//         y inv 0.226 -> 0.179, z inv 0.261 -> 0.241 (all five)
+//   FP64  z fwd 0.455 -> 0.481, y fwd 0.400 -> 0.528, x 0.593 -> 0.543,
+//         y inv 0.409 -> 0.535, z inv 0.463 -> 0.439 (x and z inv only:
+//         the 8 KB table costs the FP64 y passes a workgroup per CU)
+#ifndef PM_FFT_TW_LDS32
+#define PM_FFT_TW_LDS32 31
+#endif
+#ifndef PM_FFT_TW_LDS64
+#define PM_FFT_TW_LDS64 20
+#endif
+template <typename C, int SLOT> __host__ __device__ constexpr bool tw_in_lds() {
+    return ((sizeof(C) == 8 ? PM_FFT_TW_LDS32 : PM_FFT_TW_LDS64) >> SLOT) & 1;
+}
+template <typename C, int L, int SLOT> using tw_lds_t = C[tw_in_lds<C, SLOT>() ? L : 1];
+template <typename C, int L, int NT, int SLOT>
+__device__ __forceinline__ const C* stage_twiddles(C* tws, const C* __restrict__ tw) {
+    if constexpr (tw_in_lds<C, SLOT>()) {
+        for (int i = threadIdx.x; i < L; i += NT) tws[i] = tw[i];
+        return tws;
+    }
+    return tw;
+}
+
 // Lines per workgroup: 128-byte global segments (16 float2 / 8 double2),
 // halved at N >= 1024 to keep the LDS tile at 64 KB.
 #ifndef PM_FFT_LINES32
@@ -218,6 +247,8 @@ __global__ __launch_bounds__(NT) void k_zfwd(const Rl* __restrict__ rho,
     using C = cplx_t<Rl>;
     constexpr int M = N / 2, LP = lds_pitch(M), YT = N / T;
     __shared__ C s[T * LP];
+    __shared__ tw_lds_t<C, M, 0> tws;
+    const C* tw = stage_twiddles<C, M, NT, 0>(tws, twM);
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int x = tile / YT, y0 = (tile % YT) * T;
     const C* src = reinterpret_cast<const C*>(rho + ((size_t)x * N + y0) * N);
@@ -230,7 +261,7 @@ __global__ __launch_bounds__(NT) void k_zfwd(const Rl* __restrict__ rho,
         s[(g / M) * LP + lpos(g % M)] = v;
     }
     __syncthreads();
-    fft_lines<C, M, T, -1, NT>(s, twM);
+    fft_lines<C, M, T, -1, NT>(s, tw);
     // X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2, Z[M] = Z[0]
     for (int g = threadIdx.x; g < T * (M + 1); g += NT) {
         const int kz = g / T, line = g % T;
@@ -252,12 +283,14 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx_t<Rl>* __restrict__ in,
     using C = cplx_t<Rl>;
     constexpr int LP = lds_pitch(N), XT = N / T;
     __shared__ C s[T * LP];
+    __shared__ tw_lds_t<C, N, 1> tws;
+    const C* tw = stage_twiddles<C, N, NT, 1>(tws, twN);
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / XT, x0 = (tile % XT) * T;
     const C* src = in + ((size_t)kz * N + x0) * N;
     for (int g = threadIdx.x; g < T * N; g += NT) s[(g / N) * LP + lpos(g % N)] = gload<Rl>(src + g);
     __syncthreads();
-    fft_lines<C, N, T, -1, NT>(s, twN);
+    fft_lines<C, N, T, -1, NT>(s, tw);
     C* dst = out + (size_t)kz * N * N + x0;
     for (int g = threadIdx.x; g < T * N; g += NT) {
         const int ky = g / T, line = g % T;
@@ -289,12 +322,14 @@ __global__ __launch_bounds__(NT) void k_x(cplx_t<Rl>* __restrict__ data,
     using C = cplx_t<Rl>;
     constexpr int LP = lds_pitch(N), YT = N / T;
     __shared__ C s[T * LP];
+    __shared__ tw_lds_t<C, N, 2> tws;
+    const C* tw = stage_twiddles<C, N, NT, 2>(tws, twN);
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / YT, ky0 = (tile % YT) * T;
     C* base = data + ((size_t)kz * N + ky0) * N;
     for (int g = threadIdx.x; g < T * N; g += NT) s[(g / N) * LP + lpos(g % N)] = gload<Rl>(base + g);
     __syncthreads();
-    if constexpr (MODE & kXForward) fft_lines<C, N, T, -1, NT>(s, twN);
+    if constexpr (MODE & kXForward) fft_lines<C, N, T, -1, NT>(s, tw);
     if constexpr (MODE & kXGreen) {
         for (int g = threadIdx.x; g < T * N; g += NT) {
             const int line = g / N, kx = g % N;
@@ -305,7 +340,7 @@ __global__ __launch_bounds__(NT) void k_x(cplx_t<Rl>* __restrict__ data,
         }
         __syncthreads();
     }
-    if constexpr (MODE & kXInverse) fft_lines<C, N, T, +1, NT>(s, twN);
+    if constexpr (MODE & kXInverse) fft_lines<C, N, T, +1, NT>(s, tw);
     for (int g = threadIdx.x; g < T * N; g += NT) gstore<Rl>(base + g, s[(g / N) * LP + lpos(g % N)]);
 }
 
@@ -317,6 +352,8 @@ __global__ __launch_bounds__(NT) void k_yinv(const cplx_t<Rl>* __restrict__ in,
     using C = cplx_t<Rl>;
     constexpr int LP = lds_pitch(N), XT = N / T;
     __shared__ C s[T * LP];
+    __shared__ tw_lds_t<C, N, 3> tws;
+    const C* tw = stage_twiddles<C, N, NT, 3>(tws, twN);
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / XT, x0 = (tile % XT) * T;
     const C* src = in + (size_t)kz * N * N + x0;
@@ -325,7 +362,7 @@ __global__ __launch_bounds__(NT) void k_yinv(const cplx_t<Rl>* __restrict__ in,
         s[line * LP + lpos(ky)] = gload<Rl>(src + (size_t)ky * N + line);
     }
     __syncthreads();
-    fft_lines<C, N, T, +1, NT>(s, twN);
+    fft_lines<C, N, T, +1, NT>(s, tw);
     C* dst = out + ((size_t)kz * N + x0) * N;
     for (int g = threadIdx.x; g < T * N; g += NT) gstore<Rl>(dst + g, s[(g / N) * LP + lpos(g % N)]);
 }
@@ -341,6 +378,8 @@ __global__ __launch_bounds__(NT) void k_zinv(const cplx_t<Rl>* __restrict__ in,
     constexpr int NPRE = T * M;
     constexpr int PER = (NPRE + NT - 1) / NT;
     __shared__ C s[T * LP];
+    __shared__ tw_lds_t<C, M, 4> tws;
+    const C* tw = stage_twiddles<C, M, NT, 4>(tws, twM);
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int x = tile / YT, y0 = (tile % YT) * T;
     for (int g = threadIdx.x; g < T * (M + 1); g += NT) {
@@ -374,7 +413,7 @@ __global__ __launch_bounds__(NT) void k_zinv(const cplx_t<Rl>* __restrict__ in,
         if (NPRE % NT == 0 || g < NPRE) s[(g / M) * LP + lpos(g % M)] = z[q];
     }
     __syncthreads();
-    fft_lines<C, M, T, +1, NT>(s, twM);
+    fft_lines<C, M, T, +1, NT>(s, tw);
     C* dst = reinterpret_cast<C*>(out + ((size_t)x * N + y0) * N);
     for (int g = threadIdx.x; g < T * M; g += NT) {
         const C v = s[(g / M) * LP + lpos(g % M)];
