@@ -56,15 +56,22 @@ int tables_create(FftTables* t, int ng, int prec, hipStream_t st) {
     const size_t es = prec == PM_FP32 ? sizeof(float2) : sizeof(double2);
     std::vector<double> twm(2 * (size_t)M), twn(2 * (size_t)ng), ct(ng);
     const double two_pi = 2.0 * M_PI;
-    for (int m = 0; m < M; ++m) {
-        twm[2 * m] = std::cos(two_pi * m / M);
-        twm[2 * m + 1] = -std::sin(two_pi * m / M);
-    }
-    for (int m = 0; m < ng; ++m) {
-        twn[2 * m] = std::cos(two_pi * m / ng);
-        twn[2 * m + 1] = -std::sin(two_pi * m / ng);
-        ct[m] = std::cos(two_pi / ng * m);  // same expression as ffte1/fft.cpp:29,33
-    }
+    // exp(-2 pi i m / L), with tw[m + L/2] = -tw[m] exactly: the FFT stages
+    // read only the first half of a table
+    auto fill = [&](std::vector<double>& tw, int L) {
+        for (int m = 0; m < L; ++m) {
+            if (m < L / 2 || L < 2) {
+                tw[2 * (size_t)m] = std::cos(two_pi * m / L);
+                tw[2 * (size_t)m + 1] = -std::sin(two_pi * m / L);
+            } else {
+                tw[2 * (size_t)m] = -tw[2 * (size_t)(m - L / 2)];
+                tw[2 * (size_t)m + 1] = -tw[2 * (size_t)(m - L / 2) + 1];
+            }
+        }
+    };
+    fill(twm, M);
+    fill(twn, ng);
+    for (int m = 0; m < ng; ++m) ct[m] = std::cos(two_pi / ng * m);  // same expression as ffte1/fft.cpp:29,33
     int rc = hip_ok(hipMalloc(&t->twM, es * M), "tables: hipMalloc");
     if (rc) return rc;
     rc = hip_ok(hipMalloc(&t->twN, es * ng), "tables: hipMalloc");

@@ -125,7 +125,9 @@ template <int DIR, typename C> struct Dft<8, DIR, C> {
 
 // One radix-R Stockham autosort stage over T lines of length L held in LDS
 // (line pitch lds_pitch(L)).  p is the length of the sub-transforms already
-// done; tw[m] = exp(-2 pi i m / L).
+// done; tw[m] = exp(-2 pi i m / L) for m < L/2, and the stage takes
+// exp(-2 pi i m / L) = -tw[m - L/2] above that (the tables are built with
+// exactly this symmetry, so only their first half is ever read).
 template <typename C, int L, int T, int R, int DIR, int NT>
 __device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict__ tw) {
     constexpr int Q = L / R;
@@ -154,7 +156,9 @@ __device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict_
                 const int step = L / (R * p);
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    const C w = tw[r * k * step];
+                    const int m = r * k * step;
+                    const C t = tw[m & (L / 2 - 1)];
+                    const C w = (m & (L / 2)) ? C{-t.x, -t.y} : t;
                     u[q][r] = DIR < 0 ? cmul(u[q][r], w) : cmulc(u[q][r], w);
                 }
             }
@@ -187,28 +191,29 @@ __device__ __forceinline__ void fft_lines(C* s, const C* __restrict__ tw) {
 
 // Twiddle table of a pass's L-point FFTs: for the passes in the mask of
 // its precision (bit = slot: 0 z fwd, 1 y fwd, 2 x, 3 y inv, 4 z inv) a
-// copy in LDS (L complex), which the workgroup reads after its first
-// barrier, instead of the global table through L1 (the same values, so the
-// same results bit for bit).  Interleaved A/B at 512^3, global -> LDS, ms:
+// copy in LDS (the L/2 complex that the stages read), which the workgroup
+// reads after its first barrier, instead of the global table through L1
+// (the same values, so the same results bit for bit).  Interleaved A/B at
+// 512^3, global -> LDS, ms:
 //   FP32  z fwd 0.250 -> 0.214, y fwd 0.227 -> 0.180, x 0.355 -> 0.247,
-//         y inv 0.226 -> 0.179, z inv 0.261 -> 0.241 (all five)
-//   FP64  z fwd 0.455 -> 0.481, y fwd 0.400 -> 0.528, x 0.593 -> 0.543,
-//         y inv 0.409 -> 0.535, z inv 0.463 -> 0.439 (x and z inv only:
-//         the 8 KB table costs the FP64 y passes a workgroup per CU)
+//         y inv 0.226 -> 0.179, z inv 0.261 -> 0.241
+//   FP64  z fwd 0.476 -> 0.450, y fwd 0.405 -> 0.359, x 0.593 -> 0.533,
+//         y inv 0.412 -> 0.371, z inv 0.463 -> 0.431 (a full L-entry copy
+//         instead of the half cost the y passes a workgroup per CU: 0.53 ms)
 #ifndef PM_FFT_TW_LDS32
 #define PM_FFT_TW_LDS32 31
 #endif
 #ifndef PM_FFT_TW_LDS64
-#define PM_FFT_TW_LDS64 20
+#define PM_FFT_TW_LDS64 31
 #endif
 template <typename C, int SLOT> __host__ __device__ constexpr bool tw_in_lds() {
     return ((sizeof(C) == 8 ? PM_FFT_TW_LDS32 : PM_FFT_TW_LDS64) >> SLOT) & 1;
 }
-template <typename C, int L, int SLOT> using tw_lds_t = C[tw_in_lds<C, SLOT>() ? L : 1];
+template <typename C, int L, int SLOT> using tw_lds_t = C[tw_in_lds<C, SLOT>() ? L / 2 : 1];
 template <typename C, int L, int NT, int SLOT>
 __device__ __forceinline__ const C* stage_twiddles(C* tws, const C* __restrict__ tw) {
     if constexpr (tw_in_lds<C, SLOT>()) {
-        for (int i = threadIdx.x; i < L; i += NT) tws[i] = tw[i];
+        for (int i = threadIdx.x; i < L / 2; i += NT) tws[i] = tw[i];
         return tws;
     }
     return tw;
