@@ -233,7 +233,10 @@ int pm_interp_transpose(pm_plan_t* plan, const double* posx, const double* posy,
                         const double* ux, const double* uy, const double* uz);
 /* Adjoint FFT solve: lambda = S sum_a D_a^T U_a into adjoint mesh 3. */
 int pm_adjoint_solve(pm_plan_t* plan);
-/* Device pointer of adjoint mesh `which` (0..2: U_a, 3: lambda), or NULL. */
+/* Device pointer of adjoint mesh `which` (0..2: U_a, 3: lambda), or NULL.
+ * pm_adjoint_force, pm_adjoint_step and pm_run_adjoint deposit the
+ * divergence sum_a D_a^T U_a directly: after them mesh 0 holds it (meshes 1
+ * and 2 are not written) and mesh 3 holds lambda. */
 void* pm_plan_adjoint_mesh(pm_plan_t* plan, int which);
 /* xbar += J_F(x)^T u (recomputes the potential at x). */
 int pm_adjoint_force(pm_plan_t* plan, const double* posx, const double* posy, const double* posz,

@@ -221,6 +221,106 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
     walk_seam_out<Rl, TS, 3, NT>(&acc[0][0], inv, sw, sh);
 }
 
+// ------------------------------------ divergence of the transpose, direct
+// mu = sum_a D_a^T U_a with U_a = interp^T(u_a) (the mesh that k_div3 forms
+// from the three k_deposit_vec3_tiles meshes), deposited in one pass: with
+// (D_a f)[i] = (f[i - e_a] - f[i + e_a]) / 2, a particle's CIC weight w_c
+// at corner c adds +u_a w_c / 2 to node c - e_a and -u_a w_c / 2 to node
+// c + e_a.  That is 32 nodes, cell offsets -1..2 on one axis and 0..1 on
+// the others; the 8 corners take one term per axis.  One signed 64-bit
+// fixed-point accumulator per node, with the scale of the largest |u_a|;
+// the column walk keeps kDivHalo = 3 halo planes above a tile, node box
+// shifted one node down (walk_halo_tile_out), in 19^3 x 8 B = 55 KB of LDS,
+// so two workgroups share a CU.  Against the vec3 deposit, divergence and
+// three folds: one mesh written instead of three, one fold, no k_div3.
+constexpr int kDivThreads = 512;
+
+template <typename Rl, int TS>
+__global__ __launch_bounds__(kDivThreads) void k_deposit_div_tiles(
+    const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+    const void* __restrict__ stage, const double* __restrict__ ux, const double* __restrict__ uy,
+    const double* __restrict__ uz, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off,
+    const uint32_t* __restrict__ cnt, int ng, const unsigned long long* __restrict__ umax, Rl* __restrict__ mu,
+    Rl* __restrict__ shell, const uint32_t* __restrict__ gate, int run_if) {
+    // particles of tile t as in k_deposit_vec3_tiles (buckets with records,
+    // or a binning with the records formed from the positions)
+    if (gate && ((*gate != 0u) != (run_if != 0))) return;
+    using St = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
+    const St* __restrict__ sp = static_cast<const St*>(stage);
+    constexpr int H = kDivHalo, E = TS + H, E3 = E * E * E, NT = kDivThreads;
+    __shared__ unsigned long long acc[E3];
+    const WalkSeg sw = walk_segment(ng, TS);
+    const double sc = vec3_scale(max(umax[0], max(umax[1], umax[2])));
+    const double hs = 0.5 * sc, inv = 1.0 / sc;
+    for (int i = threadIdx.x; i < E3; i += NT) acc[i] = 0ULL;
+    __syncthreads();
+    const int ox = sw.tx * TS, oy = sw.ty * TS;
+    for (int k = 0; k < sw.seg; ++k) {
+        const int tz = sw.sg * sw.seg + k, oz = tz * TS;
+        const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;
+        const uint32_t beg = off[t], end = cnt ? beg + cnt[t] : off[t + 1];
+        constexpr int B = 2;  // particles in flight per thread
+        for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += NT * B) {
+            uint32_t p[B];
+            St r[B];
+            double u[B][3];
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const uint32_t q = q0 + NT * j;
+                p[j] = q < end ? idx[q] : 0xFFFFFFFFu;
+                if (sp && q < end) r[j] = sp[q];
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (p[j] == 0xFFFFFFFFu) continue;
+                if (!sp) r[j] = tile_record_of<St>(px[p[j]] - ox, py[p[j]] - oy, pz[p[j]] - oz);
+                u[j][0] = ux[p[j]] * hs;
+                u[j][1] = uy[p[j]] * hs;
+                u[j][2] = uz[p[j]] * hs;
+            }
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                if (p[j] == 0xFFFFFFFFu) continue;
+                const TileOffs o = tile_offs(r[j]);
+                const double wx[2] = {1.0 - o.fx, o.fx}, wy[2] = {1.0 - o.fy, o.fy}, wz[2] = {1.0 - o.fz, o.fz};
+                // node (dx, dy, dz) relative to the cell's lower corner
+                unsigned long long* const n0 = acc + ((o.a + 1) * E + (o.b + 1)) * E + (o.c + 1);
+                auto add = [&](int dx, int dy, int dz, double v) {
+                    atomicAdd(n0 + (dx * E + dy) * E + dz, (unsigned long long)__double2ll_rn(v));
+                };
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+#pragma unroll
+                    for (int f = 0; f < 2; ++f) {
+                        add(-1, e, f, u[j][0] * wx[0] * wy[e] * wz[f]);
+                        add(2, e, f, -u[j][0] * wx[1] * wy[e] * wz[f]);
+                        add(e, -1, f, u[j][1] * wx[e] * wy[0] * wz[f]);
+                        add(e, 2, f, -u[j][1] * wx[e] * wy[1] * wz[f]);
+                        add(e, f, -1, u[j][2] * wx[e] * wy[f] * wz[0]);
+                        add(e, f, 2, -u[j][2] * wx[e] * wy[f] * wz[1]);
+                    }
+#pragma unroll
+                for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+                    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                        for (int cz = 0; cz < 2; ++cz) {
+                            const double vx = cx == 0 ? u[j][0] * wx[1] * wy[cy] * wz[cz]
+                                                      : -u[j][0] * wx[0] * wy[cy] * wz[cz];
+                            const double vy = cy == 0 ? u[j][1] * wx[cx] * wy[1] * wz[cz]
+                                                      : -u[j][1] * wx[cx] * wy[0] * wz[cz];
+                            const double vz = cz == 0 ? u[j][2] * wx[cx] * wy[cy] * wz[1]
+                                                      : -u[j][2] * wx[cx] * wy[cy] * wz[0];
+                            add(cx, cy, cz, vx + vy + vz);
+                        }
+            }
+        }
+        __syncthreads();
+        walk_halo_tile_out<Rl, TS, H, NT>(acc, inv, t, ox - 1, oy - 1, oz - 1, ng, mu, shell);
+    }
+    walk_halo_seam_out<Rl, TS, H, NT>(acc, inv, sw, shell);
+}
+
 // Workgroup (NT threads) max of m[0..3) into one of kMaxSlots slots per
 // component (umax[3 + d * kMaxSlots + slot]): bit patterns of non-negative
 // doubles order like the values, so an integer atomic max works, and the
@@ -495,6 +595,51 @@ int launch_deposit_vec3(int prec, const double* px, const double* py, const doub
 #undef PM_VEC3
 #undef PM_VEC3_PASS
     return check_launch("deposit_vec3");
+}
+
+int launch_deposit_div(int prec, const double* px, const double* py, const double* pz, const double* ux,
+                       const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s, void* mu,
+                       void* shell, unsigned long long* umax, hipStream_t st, bool umax_ready,
+                       const uint32_t* bucket_idx) {
+    const int ts = sorted_tile(ng);
+    const int nt = ng / ts;
+    const uint32_t ntiles = (uint32_t)nt * nt * nt;
+    if (!umax_ready) {  // else the previous reverse-step gather left max |u_a|
+        if (np > 0) {
+            const unsigned nb = (unsigned)std::min<int64_t>((np + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_absmax3, dim3(nb), dim3(256), 0, st, ux, uy, uz, np, umax);
+        }
+        hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
+    }
+    // with buckets: the bucket pass, or (overflow) the binning pass
+#define PM_DIV_PASS(RL, TSV, REC, IDX, OFF, CNT, GATE, RUN)                                                      \
+    hipLaunchKernelGGL((k_deposit_div_tiles<RL, TSV>), dim3(ntiles / walk_seg(nt)), dim3(kDivThreads), 0, st, px, \
+                       py, pz, REC, ux, uy, uz, IDX, OFF, CNT, ng, umax, (RL*)mu, (RL*)shell, GATE, RUN)
+#define PM_DIV(RL, TSV)                                                                                     \
+    if (bucket_idx) {                                                                                       \
+        PM_DIV_PASS(RL, TSV, s.stage, bucket_idx, s.coff, s.fill, s.spill, 0);                              \
+        PM_DIV_PASS(RL, TSV, nullptr, s.idx, s.off, nullptr, s.spill, 1);                                   \
+    } else {                                                                                                \
+        PM_DIV_PASS(RL, TSV, nullptr, s.idx, s.off, nullptr, nullptr, 0);                                   \
+    }
+    if (prec == PM_FP32) {
+        if (ts == 16) {
+            PM_DIV(float, 16)
+        } else {
+            PM_DIV(float, 8)
+        }
+    } else {
+        if (ts == 16) {
+            PM_DIV(double, 16)
+        } else {
+            PM_DIV(double, 8)
+        }
+    }
+#undef PM_DIV
+#undef PM_DIV_PASS
+    const int rc = check_launch("deposit_div");
+    if (rc) return rc;
+    return launch_fold_halo(prec, ts, ng, shell, mu, st);
 }
 
 int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st) {

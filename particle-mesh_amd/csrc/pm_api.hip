@@ -1,4 +1,5 @@
 // C ABI glue: error state, FFT tables, the FFT handle and the fused-step plan.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -499,7 +500,9 @@ static int ensure_adjoint(pm_plan* p) {
     const size_t nc = (size_t)p->ng * p->ng * p->ng;
     int rc = ensure_sorted(p);
     for (int a = 0; a < 3 && !rc; ++a) rc = hip_ok(hipMalloc(&p->adjU[a], rs * nc), "adjoint: hipMalloc");
-    if (!rc) rc = hip_ok(hipMalloc(&p->adjShell, 3 * rs * sorted_shell_elems(p->ng)), "adjoint: hipMalloc");
+    // shells of the vec3 deposit (three meshes) or of the divergence deposit
+    const size_t shell_elems = std::max(3 * sorted_shell_elems(p->ng), halo_shell_elems(p->ng));
+    if (!rc) rc = hip_ok(hipMalloc(&p->adjShell, rs * shell_elems), "adjoint: hipMalloc");
     if (!rc) rc = hip_ok(hipMalloc(&p->adjLam, rs * nc), "adjoint: hipMalloc");
     const size_t nmax = 3 + 3 * (size_t)kMaxSlots;
     if (!rc) rc = hip_ok(hipMalloc((void**)&p->adjMax, nmax * sizeof(unsigned long long)), "adjoint: hipMalloc");
@@ -549,14 +552,14 @@ static int forward_field(pm_plan* p, const double* x, const double* y, const dou
 }
 
 // lambda = S sum_a D_a^T U_a with U_a the interpolation transpose of u at x
-// (binning of x already in p->sorted); mu is staged in p->rho.
+// (binning of x already in p->sorted): the divergence mu is deposited
+// directly into adjoint mesh 0.
 static int adjoint_field(pm_plan* p, const double* x, const double* y, const double* z, const double* ux,
                          const double* uy, const double* uz, bool umax_ready = false,
                          const uint32_t* bucket_idx = nullptr) {
-    int rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
-                                 p->adjMax, p->stream, umax_ready, bucket_idx);
-    if (!rc) rc = launch_div3(p->prec, p->adjU, p->ng, p->rho, p->stream);
-    if (!rc) rc = solve_into(p, p->rho, p->adjLam, false);
+    int rc = launch_deposit_div(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU[0], p->adjShell,
+                                p->adjMax, p->stream, umax_ready, bucket_idx);
+    if (!rc) rc = solve_into(p, p->adjU[0], p->adjLam, false);
     return rc;
 }
 
@@ -647,14 +650,14 @@ static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx,
         invalidate_reuse(p);
         rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
     }
-    // 2. the vec3 deposit U_a = interp^T(u) on the second stream, beside the
-    //    forward deposit and solve: it reads x, u and the tile lists, writes
-    //    only U_a and their shells
+    // 2. the divergence deposit mu = sum_a D_a^T interp^T(u_a) on the second
+    //    stream, beside the forward deposit and solve: it reads x, u and the
+    //    tile lists, writes only adjoint mesh 0 and its shells
     if (!rc) rc = hip_ok(hipEventRecord(p->adjFork, p->stream), "adjoint: fork");
     if (!rc) rc = hip_ok(hipStreamWaitEvent(p->adjStream, p->adjFork, 0), "adjoint: fork");
     if (!rc)
-        rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
-                                 p->adjMax, p->adjStream, umax_ready, from_stage ? p->adjIdx : nullptr);
+        rc = launch_deposit_div(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU[0], p->adjShell,
+                                p->adjMax, p->adjStream, umax_ready, from_stage ? p->adjIdx : nullptr);
     if (!rc) rc = hip_ok(hipEventRecord(p->adjJoin, p->adjStream), "adjoint: join");
     // 3. rho and phi at x
     if (!rc) {
@@ -667,10 +670,9 @@ static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx,
             rc = launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, range_counter(), p->stream);
     }
     if (!rc) rc = solve_into(p, p->rho, p->phi, true);
-    // 4. join; lambda = S sum_a D_a^T U_a (mu staged in rho, which the solve has read)
+    // 4. join; lambda = S mu
     if (!rc) rc = hip_ok(hipStreamWaitEvent(p->stream, p->adjJoin, 0), "adjoint: join");
-    if (!rc) rc = launch_div3(p->prec, p->adjU, p->ng, p->rho, p->stream);
-    if (!rc) rc = solve_into(p, p->rho, p->adjLam, false);
+    if (!rc) rc = solve_into(p, p->adjU[0], p->adjLam, false);
     // 5. the gather (undo the kick, VJP; mode 2: the next step's undrift,
     //    staged into buckets sized from this step's counts)
     AdjStage stg;

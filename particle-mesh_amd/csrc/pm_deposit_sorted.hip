@@ -412,6 +412,47 @@ __global__ __launch_bounds__(256) void k_fold_shells(const Rl* __restrict__ shel
     }
 }
 
+// 5'. the fold of a walk with H halo planes and the node box shifted one
+// node down (walk_halo_tile_out): each tile's owned nodes with a < H, b < H
+// or (first tile of a segment) c < H get the x, y, xy neighbours' xy shells
+// and the previous segment's z seams of the own, x, y, xy columns, in the
+// fixed order d = 1..7 (bit 0: x neighbour, 1: y, 2: z).
+template <typename Rl, int TS, int H>
+__global__ __launch_bounds__(256) void k_fold_halo(const Rl* __restrict__ shell, int ng, Rl* __restrict__ mesh) {
+    constexpr int E = TS + H, XYS = halo_shell_size<TS, H>(), ZS = halo_seam_size<TS, H>();
+    const int nt = ng / TS, seg = walk_seg(nt), nseg = nt / seg;
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = (int)(t / (uint32_t)(nt * nt)), ty = (int)((t / (uint32_t)nt) % (uint32_t)nt),
+              tz = (int)(t % (uint32_t)nt);
+    const int sg = tz / seg;
+    const bool first = tz == sg * seg;
+    const Rl* __restrict__ seam = shell + (size_t)nt * nt * nt * XYS;
+    const size_t N = (size_t)ng;
+    const int m = ng - 1;
+    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
+        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
+        const int on = (a < H) | ((b < H) << 1) | ((first && c < H) << 2);
+        if (!on) continue;
+        const size_t node = ((size_t)((tx * TS - 1 + a) & m) * N + (size_t)((ty * TS - 1 + b) & m)) * N +
+                            (size_t)((tz * TS - 1 + c) & m);
+        Rl v = mesh[node];
+        for (int d = 1; d < 8; ++d) {
+            if ((d & on) != d) continue;
+            const int dx = d & 1, dy = (d >> 1) & 1;
+            const int sx = tx - dx < 0 ? nt - 1 : tx - dx;
+            const int sy = ty - dy < 0 ? nt - 1 : ty - dy;
+            const int na = a + TS * dx, nb = b + TS * dy;
+            if (d < 4) {
+                v += shell[(((size_t)sx * nt + sy) * nt + tz) * XYS + halo_shell_index<TS, H>(na, nb, c)];
+            } else {
+                const int ss = sg == 0 ? nseg - 1 : sg - 1;
+                v += seam[(((size_t)sx * nt + sy) * nseg + ss) * ZS + ((size_t)na * E + nb) * H + c];
+            }
+        }
+        mesh[node] = v;
+    }
+}
+
 int sorted_tile(int ng) { return ng >= 16 ? 16 : 8; }
 
 size_t sorted_shell_elems(int ng) {
@@ -420,6 +461,35 @@ size_t sorted_shell_elems(int ng) {
     const size_t nseg = nt * nt * (nt / (size_t)walk_seg((int)nt));
     return ts == 16 ? nt * nt * nt * xy_shell_size<16>() + nseg * zseam_size<16>()
                     : nt * nt * nt * xy_shell_size<8>() + nseg * zseam_size<8>();
+}
+
+size_t halo_shell_elems(int ng) {
+    const int ts = sorted_tile(ng);
+    const size_t nt = (size_t)(ng / ts);
+    const size_t nseg = nt * nt * (nt / (size_t)walk_seg((int)nt));
+    return ts == 16 ? nt * nt * nt * halo_shell_size<16, kDivHalo>() + nseg * halo_seam_size<16, kDivHalo>()
+                    : nt * nt * nt * halo_shell_size<8, kDivHalo>() + nseg * halo_seam_size<8, kDivHalo>();
+}
+
+int launch_fold_halo(int prec, int ts, int ng, const void* shell, void* mesh, hipStream_t st) {
+    const int nt = ng / ts;
+    const uint32_t ntiles = (uint32_t)nt * nt * nt;
+#define PM_FH(RL, TSV)                                                                                              \
+    hipLaunchKernelGGL((k_fold_halo<RL, TSV, kDivHalo>), dim3(ntiles), dim3(256), 0, st, (const RL*)shell, ng, \
+                       (RL*)mesh)
+    if (prec == PM_FP32) {
+        if (ts == 16)
+            PM_FH(float, 16);
+        else
+            PM_FH(float, 8);
+    } else {
+        if (ts == 16)
+            PM_FH(double, 16);
+        else
+            PM_FH(double, 8);
+    }
+#undef PM_FH
+    return check_launch("fold_halo");
 }
 
 int launch_bin(const double* px, const double* py, const double* pz, int64_t np, int ng, SortedScratch& s,

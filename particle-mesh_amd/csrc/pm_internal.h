@@ -134,6 +134,11 @@ struct SortedScratch {
 };
 int sorted_tile(int ng);
 size_t sorted_shell_elems(int ng);
+// Halo planes of the divergence deposit's column walk (its stencil reaches
+// two nodes above a cell's upper corner) and the reals of its shells.
+constexpr int kDivHalo = 3;
+size_t halo_shell_elems(int ng);
+int launch_fold_halo(int prec, int ts, int ng, const void* shell, void* mesh, hipStream_t st);
 int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
                           double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st);
 // binning only (count, scan, scatter) and the per-tile deposit of a binning
@@ -169,6 +174,13 @@ int launch_deposit_vec3(int prec, const double* px, const double* py, const doub
                         void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st,
                         bool umax_ready = false, const uint32_t* bucket_idx = nullptr);
 int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st);
+// mu = sum_a D_a^T interp^T(u_a) in one deposit (the mesh that
+// launch_deposit_vec3 + launch_div3 form), folded; shell holds
+// halo_shell_elems(ng) reals; umax_ready, bucket_idx as for the vec3 deposit
+int launch_deposit_div(int prec, const double* px, const double* py, const double* pz, const double* ux,
+                       const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s, void* mu,
+                       void* shell, unsigned long long* umax, hipStream_t st, bool umax_ready = false,
+                       const uint32_t* bucket_idx = nullptr);
 // mode 1: xbar += VJP; 0: also undo the kick; 2: also the next reverse
 // step's undrift (next_drift) with max |u| into umax[0..3) (umax holds
 // 3 + 3 * kMaxSlots entries, the slots zero between calls)

@@ -106,6 +106,74 @@ __device__ __forceinline__ void walk_seam_out(const unsigned long long* acc, con
     }
 }
 
+// Column walk with H halo planes above the tile and the node box shifted
+// one node down: local node n of a tile sits at tile origin + n - 1, so
+// local nodes 0 .. TS+H-1 cover a stencil that reaches one node below a
+// tile's cells and up to H-1 above its upper corner nodes.  A tile owns the
+// local nodes 0 .. TS-1 on each axis (global origin - 1 .. origin + TS - 2);
+// the H upper planes go out as shells (x and y) or are carried up the
+// column (z), exactly as the walk above does with H = 1.
+// xy shell: x face (a = TS..TS+H-1, b = 0..E-1, c < TS) at
+// ((a - TS) E + b) TS + c, then the y face (a < TS, b = TS..TS+H-1, c < TS).
+template <int TS, int H> __host__ __device__ constexpr int halo_shell_size() {
+    return H * (TS + H) * TS + TS * H * TS;
+}
+template <int TS, int H> __device__ __forceinline__ int halo_shell_index(int a, int b, int c) {
+    constexpr int E = TS + H;
+    return a >= TS ? ((a - TS) * E + b) * TS + c : H * E * TS + (a * H + (b - TS)) * TS + c;
+}
+// z seam of a segment: the carried planes c < H, nodes (a, b) of E x E, at (a E + b) H + c.
+template <int TS, int H> __host__ __device__ constexpr int halo_seam_size() { return H * (TS + H) * (TS + H); }
+
+template <typename Rl, int TS, int H, int NT>
+__device__ __forceinline__ void walk_halo_tile_out(unsigned long long* acc, double scale, uint32_t t, int gx0,
+                                                   int gy0, int gz0, int ng, Rl* __restrict__ out,
+                                                   Rl* __restrict__ shell) {
+    static_assert(H <= TS, "the carried planes must not overlap");
+    constexpr int E = TS + H, XYS = halo_shell_size<TS, H>();
+    const size_t N = (size_t)ng;
+    const int m = ng - 1;  // ng is a power of two
+    for (int i = threadIdx.x; i < TS * TS * TS; i += NT) {
+        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
+        const size_t node = ((size_t)((gx0 + a) & m) * N + (size_t)((gy0 + b) & m)) * N + (size_t)((gz0 + c) & m);
+        out[node] = (Rl)((double)(long long)acc[(a * E + b) * E + c] * scale);
+    }
+    for (int i = threadIdx.x; i < XYS; i += NT) {
+        int a, b, c;
+        if (i < H * E * TS) {
+            a = TS + i / (E * TS);
+            b = (i / TS) % E;
+            c = i % TS;
+        } else {
+            const int j = i - H * E * TS;
+            a = j / (H * TS);
+            b = TS + (j / TS) % H;
+            c = j % TS;
+        }
+        shell[(size_t)t * XYS + i] = (Rl)((double)(long long)acc[(a * E + b) * E + c] * scale);
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < E * E; r += NT) {  // one thread per z row
+        unsigned long long* row = acc + r * E;
+#pragma unroll
+        for (int c = 0; c < H; ++c) row[c] = row[TS + c];
+#pragma unroll
+        for (int c = H; c < E; ++c) row[c] = 0ULL;
+    }
+    __syncthreads();
+}
+
+template <typename Rl, int TS, int H, int NT>
+__device__ __forceinline__ void walk_halo_seam_out(const unsigned long long* acc, double scale, const WalkSeg& s,
+                                                   Rl* __restrict__ shell) {
+    constexpr int E = TS + H, XYS = halo_shell_size<TS, H>(), ZS = halo_seam_size<TS, H>();
+    const size_t seam0 = (size_t)s.nt * s.nt * s.nt * XYS + (size_t)s.w * ZS;
+    for (int r = threadIdx.x; r < ZS; r += NT) {
+        const int ab = r / H, c = r % H;
+        shell[seam0 + r] = (Rl)((double)(long long)acc[ab * E + c] * scale);
+    }
+}
+
 constexpr int kDepBatch = 4;  // particles in flight per thread in the tile kernels
 
 // CIC weights (cic_deposit, utils_inline.h:25-49) of one particle in local
