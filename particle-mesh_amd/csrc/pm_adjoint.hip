@@ -233,7 +233,16 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
 // shifted one node down (walk_halo_tile_out), in 19^3 x 8 B = 55 KB of LDS,
 // so two workgroups share a CU.  Against the vec3 deposit, divergence and
 // three folds: one mesh written instead of three, one fold, no k_div3.
-constexpr int kDivThreads = 512;
+// Workgroup size and particles in flight per thread.  Reverse sweep of 20
+// steps at 512^3 in an interleaved A/B: 237.2 ms (512 threads, 2 in
+// flight), 237.6 (512, 1), 239.3 (512, 4), 244.6 (256, 2).
+#ifndef PM_DIV_THREADS
+#define PM_DIV_THREADS 512
+#endif
+#ifndef PM_DIV_BATCH
+#define PM_DIV_BATCH 2
+#endif
+constexpr int kDivThreads = PM_DIV_THREADS;
 
 template <typename Rl, int TS>
 __global__ __launch_bounds__(kDivThreads) void k_deposit_div_tiles(
@@ -259,7 +268,7 @@ __global__ __launch_bounds__(kDivThreads) void k_deposit_div_tiles(
         const int tz = sw.sg * sw.seg + k, oz = tz * TS;
         const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;
         const uint32_t beg = off[t], end = cnt ? beg + cnt[t] : off[t + 1];
-        constexpr int B = 2;  // particles in flight per thread
+        constexpr int B = PM_DIV_BATCH;  // particles in flight per thread
         for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += NT * B) {
             uint32_t p[B];
             St r[B];

@@ -395,19 +395,27 @@ __global__ __launch_bounds__(256) void k_fold_shells(const Rl* __restrict__ shel
         }
         const int on = (a == 0) | ((b == 0) << 1) | ((first && c == 0) << 2);
         const size_t node = ((size_t)(tx * TS + a) * N + (ty * TS + b)) * N + (tz * TS + c);
-        Rl v = rho[node];
+        // all loads first (each under its own condition), then the adds in
+        // the fixed order
+        Rl sv[8];
+#pragma unroll
         for (int d = 1; d < 8; ++d) {
-            if ((d & on) != d) continue;
             const int dx = d & 1, dy = (d >> 1) & 1;
             const int sx = tx - dx < 0 ? nt - 1 : tx - dx;
             const int sy = ty - dy < 0 ? nt - 1 : ty - dy;
+            sv[d] = Rl(0);
+            if ((d & on) != d) continue;
             if (d < 4) {
-                v += shell[(((size_t)sx * nt + sy) * nt + tz) * XYS + xy_shell_index<TS>(a + TS * dx, b + TS * dy, c)];
+                sv[d] = shell[(((size_t)sx * nt + sy) * nt + tz) * XYS + xy_shell_index<TS>(a + TS * dx, b + TS * dy, c)];
             } else {
                 const int ss = sg == 0 ? nseg - 1 : sg - 1;
-                v += seam[(((size_t)sx * nt + sy) * nseg + ss) * ZS + (a + TS * dx) * E + (b + TS * dy)];
+                sv[d] = seam[(((size_t)sx * nt + sy) * nseg + ss) * ZS + (a + TS * dx) * E + (b + TS * dy)];
             }
         }
+        Rl v = rho[node];
+#pragma unroll
+        for (int d = 1; d < 8; ++d)
+            if ((d & on) == d) v += sv[d];
         rho[node] = v;
     }
 }
@@ -429,26 +437,52 @@ __global__ __launch_bounds__(256) void k_fold_halo(const Rl* __restrict__ shell,
     const Rl* __restrict__ seam = shell + (size_t)nt * nt * nt * XYS;
     const size_t N = (size_t)ng;
     const int m = ng - 1;
-    for (int i = threadIdx.x; i < TS * TS * TS; i += 256) {
-        const int a = i / (TS * TS), b = (i / TS) % TS, c = i % TS;
+    // a < H (H TS^2 nodes), then b < H <= a ((TS-H) H TS), then on a
+    // segment's first tile c < H <= a, b ((TS-H)^2 H)
+    constexpr int NA = H * TS * TS, NB = NA + (TS - H) * H * TS;
+    const int nf = NB + (first ? (TS - H) * (TS - H) * H : 0);
+    for (int i = threadIdx.x; i < nf; i += 256) {
+        int a, b, c;
+        if (i < NA) {
+            a = i / (TS * TS);
+            b = (i / TS) % TS;
+            c = i % TS;
+        } else if (i < NB) {
+            const int j = i - NA;
+            a = H + j / (H * TS);
+            b = (j / TS) % H;
+            c = j % TS;
+        } else {
+            const int j = i - NB;
+            a = H + j / ((TS - H) * H);
+            b = H + (j / H) % (TS - H);
+            c = j % H;
+        }
         const int on = (a < H) | ((b < H) << 1) | ((first && c < H) << 2);
-        if (!on) continue;
         const size_t node = ((size_t)((tx * TS - 1 + a) & m) * N + (size_t)((ty * TS - 1 + b) & m)) * N +
                             (size_t)((tz * TS - 1 + c) & m);
-        Rl v = mesh[node];
+        // all loads first (each under its own condition), then the adds in
+        // the fixed order
+        Rl sv[8];
+#pragma unroll
         for (int d = 1; d < 8; ++d) {
-            if ((d & on) != d) continue;
             const int dx = d & 1, dy = (d >> 1) & 1;
             const int sx = tx - dx < 0 ? nt - 1 : tx - dx;
             const int sy = ty - dy < 0 ? nt - 1 : ty - dy;
             const int na = a + TS * dx, nb = b + TS * dy;
+            sv[d] = Rl(0);
+            if ((d & on) != d) continue;
             if (d < 4) {
-                v += shell[(((size_t)sx * nt + sy) * nt + tz) * XYS + halo_shell_index<TS, H>(na, nb, c)];
+                sv[d] = shell[(((size_t)sx * nt + sy) * nt + tz) * XYS + halo_shell_index<TS, H>(na, nb, c)];
             } else {
                 const int ss = sg == 0 ? nseg - 1 : sg - 1;
-                v += seam[(((size_t)sx * nt + sy) * nseg + ss) * ZS + ((size_t)na * E + nb) * H + c];
+                sv[d] = seam[(((size_t)sx * nt + sy) * nseg + ss) * ZS + ((size_t)na * E + nb) * H + c];
             }
         }
+        Rl v = mesh[node];
+#pragma unroll
+        for (int d = 1; d < 8; ++d)
+            if ((d & on) == d) v += sv[d];
         mesh[node] = v;
     }
 }
