@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256) void k_div3(const Rl* __restrict__ U0, const R
 // MODE 0: also v -= kick * F(x) (undoes the step's kick).
 // MODE 2: as MODE 0, then the undrift that opens the next reverse step,
 //         x -= next_drift * v and u = vbar + next_drift * xbar (in place on
-//         u), with max |u_a| into umax for that step's vec3 deposit.
+//         u), with max |u_a| into umax for that step's divergence deposit.
 // Workgroup size of the adjoint gather.  Full reverse sweep of 20 steps at
 // 512^3 (FP32 mesh) in an interleaved A/B, 128 / 256 / 512 / 1024 threads:
 // 265.1 / 258.6 / 261.7 / 278.7 ms.

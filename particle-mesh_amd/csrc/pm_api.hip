@@ -629,8 +629,9 @@ extern "C" int pm_adjoint_force(pm_plan_t* p, const double* x, const double* y, 
 //
 // Staging (pm_run_adjoint with the sorted deposit): from_stage = the previous
 // reverse step's gather staged x in the tile buckets (in-tile records and
-// particle indices), so the forward deposit reads the records and the vec3
-// deposit the indices, with the full binning behind the overflow flag;
+// particle indices), so the forward deposit reads the records and the
+// divergence deposit records and indices, with the full binning behind the
+// overflow flag;
 // stage_next = this step's gather stages the undrifted positions, into
 // buckets sized from this step's counts.
 static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,

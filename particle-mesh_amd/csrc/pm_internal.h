@@ -154,7 +154,7 @@ int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st);
 // deposit from the staging arrays (binned fallback if a bucket overflowed),
 // then the next capacities
 // (rescan = false leaves the buckets and their fill counts for a second
-// reader, the reverse sweep's vec3 deposit; launch_cap_scan follows it.
+// reader, the reverse sweep's divergence deposit; launch_cap_scan follows it.
 // bin = false: the fallback binning was launched already)
 int launch_deposit_staged(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,
                           double mass, void* rho, SortedScratch& s, unsigned long long* err, hipStream_t st,
@@ -245,13 +245,14 @@ struct pm_plan {
     // order (sorted.stage) for the next deposit (PM_STEP_REUSE_BINS)
     bool staged_valid = false;
     const double* staged_x = nullptr;
-    void* adjU[3] = {};        // adjoint meshes U_a (interpolation transpose)
+    void* adjU[3] = {};        // adjoint meshes U_a (interpolation transpose); the
+                               // fused paths put the divergence in adjU[0]
     void* adjLam = nullptr;    // lambda = S D^T U
-    void* adjShell = nullptr;  // 3 shell buffers of the vec3 deposit
-    unsigned long long* adjMax = nullptr;  // max |u_a| (fixed-point scales of the vec3 deposit) + slots
+    void* adjShell = nullptr;  // shells of the vec3 deposit (3) or the divergence deposit
+    unsigned long long* adjMax = nullptr;  // max |u_a| (fixed-point scales of the adjoint deposits) + slots
     uint32_t* adjIdx = nullptr;  // index buckets of the reverse sweep (cap_total), beside sorted.stage
-    // Second stream of an adjoint step (default priority): the vec3 deposit
-    // runs there beside the forward deposit and solve; fork / join events.
+    // Second stream of an adjoint step (default priority): the divergence
+    // deposit runs there beside the forward deposit and solve; fork / join events.
     hipStream_t adjStream = nullptr;
     hipEvent_t adjFork = nullptr, adjJoin = nullptr;
     // Phase timing: one set of PM_NPHASES+1 events per recorded step, read
