@@ -159,16 +159,19 @@ extern "C" int pm_fft_destroy(pm_fft_t* f) {
 extern "C" int pm_fft_forward(pm_fft_t* f, const void* slab) {
     if (!f || !slab) return fail(PM_ERR_ARG, "pm_fft_forward: null argument");
     const GreenArgs ga{PM_GREEN_REF_KH, 1.0, f->tab.cos_tab};
-    fft_pass(f->prec, f->ng, kPassZFwd, f->stream, slab, f->specA, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
-    fft_pass(f->prec, f->ng, kPassYFwd, f->stream, f->specA, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
-    fft_pass(f->prec, f->ng, kPassXFwd, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
+    const void *tm = f->tab.twM, *tn = f->tab.twN;
+    int bad = fft_pass(f->prec, f->ng, kPassZFwd, f->stream, slab, f->specA, tm, tn, ga, 1.0, false, &f->shape);
+    bad |= fft_pass(f->prec, f->ng, kPassYFwd, f->stream, f->specA, f->specB, tm, tn, ga, 1.0, false, &f->shape);
+    bad |= fft_pass(f->prec, f->ng, kPassXFwd, f->stream, nullptr, f->specB, tm, tn, ga, 1.0, false, &f->shape);
+    if (bad) return fail(PM_ERR_ARG, "pm_fft_forward: unsupported FFT pass");
     return check_launch("pm_fft_forward");
 }
 
 extern "C" int pm_fft_apply_green(pm_fft_t* f, double a) {
     if (!f || !(a > 0.0)) return fail(PM_ERR_ARG, "pm_fft_apply_green: bad argument");
     const GreenArgs ga{PM_GREEN_REF_KH, 1.0 / a, f->tab.cos_tab};
-    fft_pass(f->prec, f->ng, kPassGreen, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false);
+    if (fft_pass(f->prec, f->ng, kPassGreen, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false))
+        return fail(PM_ERR_ARG, "pm_fft_apply_green: unsupported FFT pass");
     return check_launch("pm_fft_apply_green");
 }
 
@@ -176,9 +179,11 @@ extern "C" int pm_fft_backward(pm_fft_t* f, void* slab) {
     if (!f || !slab) return fail(PM_ERR_ARG, "pm_fft_backward: null argument");
     const GreenArgs ga{PM_GREEN_REF_KH, 1.0, f->tab.cos_tab};
     const double n3 = (double)f->ng * f->ng * f->ng;
-    fft_pass(f->prec, f->ng, kPassXInv, f->stream, nullptr, f->specB, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
-    fft_pass(f->prec, f->ng, kPassYInv, f->stream, f->specB, f->specA, f->tab.twM, f->tab.twN, ga, 1.0, false, &f->shape);
-    fft_pass(f->prec, f->ng, kPassZInv, f->stream, f->specA, slab, f->tab.twM, f->tab.twN, ga, 1.0 / n3, false, &f->shape);
+    const void *tm = f->tab.twM, *tn = f->tab.twN;
+    int bad = fft_pass(f->prec, f->ng, kPassXInv, f->stream, nullptr, f->specB, tm, tn, ga, 1.0, false, &f->shape);
+    bad |= fft_pass(f->prec, f->ng, kPassYInv, f->stream, f->specB, f->specA, tm, tn, ga, 1.0, false, &f->shape);
+    bad |= fft_pass(f->prec, f->ng, kPassZInv, f->stream, f->specA, slab, tm, tn, ga, 1.0 / n3, false, &f->shape);
+    if (bad) return fail(PM_ERR_ARG, "pm_fft_backward: unsupported FFT pass");
     return check_launch("pm_fft_backward");
 }
 
@@ -337,16 +342,17 @@ static int solve_into(pm_plan* p, const void* in, void* out, bool sub1) {
     const GreenArgs ga{p->green_mode, p->green_scale / ((double)N * N * N), p->tab.cos_tab};
     const void* tm = p->tab.twM;
     const void* tn = p->tab.twN;
-    fft_pass(p->prec, N, kPassZFwd, p->stream, in, p->specA, tm, tn, ga, 1.0, sub1, &p->fft_shape);
+    int bad = fft_pass(p->prec, N, kPassZFwd, p->stream, in, p->specA, tm, tn, ga, 1.0, sub1, &p->fft_shape);
     mark(p, 2);
-    fft_pass(p->prec, N, kPassYFwd, p->stream, p->specA, p->specB, tm, tn, ga, 1.0, false, &p->fft_shape);
+    bad |= fft_pass(p->prec, N, kPassYFwd, p->stream, p->specA, p->specB, tm, tn, ga, 1.0, false, &p->fft_shape);
     mark(p, 3);
-    fft_pass(p->prec, N, kPassXFused, p->stream, nullptr, p->specB, tm, tn, ga, 1.0, false, &p->fft_shape);
+    bad |= fft_pass(p->prec, N, kPassXFused, p->stream, nullptr, p->specB, tm, tn, ga, 1.0, false, &p->fft_shape);
     mark(p, 4);
-    fft_pass(p->prec, N, kPassYInv, p->stream, p->specB, p->specA, tm, tn, ga, 1.0, false, &p->fft_shape);
+    bad |= fft_pass(p->prec, N, kPassYInv, p->stream, p->specB, p->specA, tm, tn, ga, 1.0, false, &p->fft_shape);
     mark(p, 5);
-    fft_pass(p->prec, N, kPassZInv, p->stream, p->specA, out, tm, tn, ga, 1.0, false, &p->fft_shape);
+    bad |= fft_pass(p->prec, N, kPassZInv, p->stream, p->specA, out, tm, tn, ga, 1.0, false, &p->fft_shape);
     mark(p, 6);
+    if (bad) return fail(PM_ERR_ARG, "pm_solve: unsupported FFT pass");
     return check_launch("pm_solve");
 }
 
