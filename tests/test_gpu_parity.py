@@ -524,10 +524,13 @@ def test_large_mesh_sorted_deposit_vs_atomic_fp64(dev, ng):
         assert abs(float(rho.sum()) - float(ref.sum())) < 1e-3 * ng ** 3 * 1e-6 + 1e-2
 
 
-def test_large_mesh_reused_binning_1024(dev):
-    """Key reuse across steps at 1024^3 (262144 tiles): run_kdk against
-    fresh binning every step, bit-identical."""
+@pytest.mark.parametrize("prec", [pmhip.PM_FP32, pmhip.PM_FP64])
+def test_large_mesh_reused_binning_1024(dev, prec):
+    """Key reuse across steps at 1024^3 (262144 tiles; FP64: 42-bit staged
+    offsets): run_kdk against fresh binning every step, bit-identical."""
     import torch
+
+    print(f"case reused binning 1024 prec={prec}", flush=True)
 
     ng, nrow, nsteps, dt = 1024, 128, 3, 0.05
     npart = nrow ** 3
@@ -535,7 +538,7 @@ def test_large_mesh_reused_binning_1024(dev):
     pos0, vel0 = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 2.0, 3.0)
     a_pos, a_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
     b_pos, b_vel = [t.clone() for t in pos0], [t.clone() for t in vel0]
-    with pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+    with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
         plan.run_kdk(a_pos, a_vel, nsteps, dt, pmhip.PM_DEPOSIT_SORTED)
         for s in range(nsteps + 1):
             kick = 0.5 * dt if s in (0, nsteps) else dt
