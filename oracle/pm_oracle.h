@@ -3,8 +3,9 @@
  * TEST INFRASTRUCTURE ONLY.  This restates, in plain C with OpenMP, the
  * reference's single-box step (yukiknk/Particle-Mesh) so that the HIP path
  * can be checked against it.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load it, and only as the checker or as the
- * timed CPU baseline -- never as part of the product path.
+ * cpu_baseline leg of bench.py may load it, and only as the reference that
+ * GPU results are compared against or as the timed CPU baseline -- never as
+ * part of the product path.
  *
  * Conventions (all functions):
  *   - positions are in cell units, x in [0, ng); the node of index i sits at

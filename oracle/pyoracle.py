@@ -2,7 +2,8 @@
 built, of the reference's own operators (_ref/libpmref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-the cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.
+the cpu_baseline leg of bench.py, as the reference that GPU results are
+compared against or as the timed CPU baseline.
 """
 from __future__ import annotations
 

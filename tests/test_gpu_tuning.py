@@ -151,11 +151,11 @@ def test_fft_tune(dev):
     assert pmhip.wisdom_lookup(ng, pmhip.PM_FP64) == shape
 
 
-def test_driver_writes_then_reads_its_wisdom_file(dev, tmp_path):
-    """First run: no wisdom_64_hip.dat, the FFT class tunes and exports it;
-    second run imports it (the file is left as it was); both runs give the
-    same momenta up to the order of the driver's FP64 atomic deposit
-    (the reference-layout deposit sums with float atomics)."""
+def test_pm_main_writes_then_reads_its_wisdom_file(dev, tmp_path):
+    """First run of bin/pm_main: no wisdom_64_hip.dat, the FFT class tunes
+    and exports it; the second run imports it (the file is left as it was);
+    both runs give the same momenta up to the order of pm_main's FP64 atomic
+    deposit (the reference-layout deposit sums with float atomics)."""
     assert EXE.exists(), "build first"
     print("case driver wisdom", flush=True)
     env = dict(os.environ, PM_WISDOM_DIR=str(tmp_path))
