@@ -270,15 +270,19 @@ def test_fp32_adjoint_close_to_fp64(dev):
     assert err < 1e-3, err
 
 
+@pytest.mark.parametrize("ng,nrow", [(32, 32), (256, 64)])
 @pytest.mark.parametrize("prec", [pmhip.PM_FP64, pmhip.PM_FP32])
-def test_run_adjoint_equals_step_by_step(dev, prec):
+def test_run_adjoint_equals_step_by_step(dev, prec, ng, nrow):
     """pm_run_adjoint folds each step's undrift and the max |u| of the
-    vec3 deposit into the previous reverse step's gather; the same reverse
-    sweep as pm_adjoint_step calls with their own undrift must match it bit
-    for bit (positions, velocities and both cotangents)."""
+    divergence deposit into the previous reverse step's gather, and stages
+    the positions for the next step's deposits; the same reverse sweep as
+    pm_adjoint_step calls with their own undrift and binning must match it
+    bit for bit (positions, velocities and both cotangents).  256^3: two
+    walk segments per tile column."""
     import torch
 
-    ng, nrow, nsteps, dt = 32, 32, 4, 0.02
+    print(f"case run_adjoint vs steps ng={ng} nrow={nrow} prec={prec}", flush=True)
+    nsteps, dt = 4, 0.02
     npart = nrow ** 3
     noise = np.random.default_rng(31).standard_normal(npart)
     pos0, vel0 = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 1.0, 3.0)
