@@ -114,6 +114,24 @@ def measured_traffic(kernel: str, ng: int, npart: int):
     return None
 
 
+# the FP64 step's dominant kernel, and the PMC summary that holds its traffic
+# (columns: kernel | dispatches | FETCH_KiB | WRITE_KiB | hbm_bytes | ...)
+FP64_INTERP_KERNEL = "void pm::k_interp<double, 2, 256, 1>"
+
+
+def measured_traffic_fp64(kernel: str, ng: int):
+    path = ROOT / "profiles" / f"pmc_traffic_{ng}_fp64_adjoint.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("#") or "|" not in line:
+            continue
+        cols = [c.strip() for c in line.split("|")]
+        if len(cols) >= 5 and cols[0] == kernel:
+            return int(float(cols[4]))
+    return None
+
+
 def make_ics(nrow, ng, rms, seed, dev):
     import torch
     import pmhip
@@ -422,6 +440,18 @@ def main():
                 "algorithmic_bytes": nbytes[dom], "avg_ms": round(ph32[dom], 4),
                 "traffic_source": f"profiles/pmc_traffic_{ng}.txt" if traffic else None}
     step_bytes = sum(nbytes.values())
+    roofline64 = None
+    if res64:
+        ph64 = res64["phases_ms"]
+        nb64 = algorithmic_bytes(ng, npart, pmhip.PM_FP64, variant == pmhip.PM_DEPOSIT_SORTED)
+        dom64 = max((k for k in ph64 if ph64[k]), key=lambda k: ph64[k])
+        ach64 = nb64[dom64] / (ph64[dom64] * 1e-3) / 1e9
+        k64 = FP64_INTERP_KERNEL if (dom64 == "interp_kick_drift" and variant == pmhip.PM_DEPOSIT_SORTED) else None
+        tr64 = measured_traffic_fp64(k64, ng) if k64 else None
+        roofline64 = {"bound": "hbm", "kernel": k64 or dom64, "phase": dom64, "achieved": round(ach64, 1),
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
+                      "traffic": tr64, "algorithmic_bytes": nb64[dom64], "avg_ms": round(ph64[dom64], 4),
+                      "traffic_source": f"profiles/pmc_traffic_{ng}_fp64_adjoint.txt" if tr64 else None}
     cpu = None
     if not args.no_cpu:
         log("CPU baseline leg")
@@ -450,7 +480,8 @@ def main():
         "step_achieved_GBs": round(step_bytes / (ms32 * 1e-3) / 1e9, 1),
         "wall_ms_per_step": round(wall32, 4),
         "fp64": ({"value": round(res64["value"], 2), "ms_per_step": round(res64["ms_per_step"], 4),
-                  "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["phases_ms"].items()}}
+                  "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["phases_ms"].items()},
+                  "roofline": roofline64}
                  if res64 else None),
         "fft_shape": {"fp32": shape32, "fp64": shape64, "tuned": not args.no_tune,
                       "slots": ["z_fwd", "y_fwd", "x_green", "y_inv", "z_inv"]},
