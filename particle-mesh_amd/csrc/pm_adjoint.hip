@@ -140,9 +140,11 @@ __device__ __forceinline__ double vec3_scale(unsigned long long maxbits) {
     return ldexp(1.0, 40 - e);
 }
 
-// Particles in flight per thread.  Full reverse sweep of 20 steps at 512^3
-// in an interleaved A/B: 260.5 ms (2), 265.7 (4), 279.5 (8); 268.2 with 2
-// and the positions gathered by index instead of the staged records.
+// Particles in flight per thread.  Measured while the reverse sweep still
+// used this deposit (it now deposits the divergence directly), full sweep of
+// 20 steps at 512^3 in an interleaved A/B: 260.5 ms (2), 265.7 (4), 279.5
+// (8); 268.2 with 2 and the positions gathered by index instead of the
+// staged records.
 #ifndef PM_VEC3_BATCH
 #define PM_VEC3_BATCH 2
 #endif
