@@ -143,27 +143,21 @@ __device__ __forceinline__ double vec3_scale(unsigned long long maxbits) {
 // Particles in flight per thread.  Measured while the reverse sweep still
 // used this deposit (it now deposits the divergence directly), full sweep of
 // 20 steps at 512^3 in an interleaved A/B: 260.5 ms (2), 265.7 (4), 279.5
-// (8); 268.2 with 2 and the positions gathered by index instead of the
-// staged records.
+// (8).
 #ifndef PM_VEC3_BATCH
 #define PM_VEC3_BATCH 2
 #endif
 template <typename Rl, int TS>
 __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
-    const void* __restrict__ stage, const double* __restrict__ ux, const double* __restrict__ uy,
-    const double* __restrict__ uz, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off,
-    const uint32_t* __restrict__ cnt, int ng, const unsigned long long* __restrict__ umax, Rl* __restrict__ U0,
-    Rl* __restrict__ U1, Rl* __restrict__ U2, Rl* __restrict__ shell, size_t shell_stride,
-    const uint32_t* __restrict__ gate, int run_if) {
-    // tile t's particles: idx[off[t] .. off[t] + cnt[t]) with counts (the
-    // buckets of a staging gather, whose records at the same slots hold the
-    // in-tile offsets), else idx[off[t] .. off[t+1]) (a binning; offsets
-    // formed from the positions in the same fixed point, so both give the
-    // same weights); with a gate the kernel runs only if (*gate != 0) == run_if
-    if (gate && ((*gate != 0u) != (run_if != 0))) return;
+    const double* __restrict__ ux, const double* __restrict__ uy, const double* __restrict__ uz,
+    const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off, int ng,
+    const unsigned long long* __restrict__ umax, Rl* __restrict__ U0, Rl* __restrict__ U1, Rl* __restrict__ U2,
+    Rl* __restrict__ shell, size_t shell_stride) {
+    // tile t's particles: idx[off[t] .. off[t+1]) of the binning, their
+    // in-tile offsets formed in the staging records' fixed point (the same
+    // weights as the other tile deposits)
     using St = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
-    const St* __restrict__ sp = static_cast<const St*>(stage);
     constexpr int E = TS + 1, E3 = E * E * E;
     constexpr int NT = kVec3Threads;
     __shared__ unsigned long long acc[3][E3];
@@ -178,7 +172,7 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
     for (int k = 0; k < sw.seg; ++k) {
         const int tz = sw.sg * sw.seg + k, oz = tz * TS;
         const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;
-        const uint32_t beg = off[t], end = cnt ? beg + cnt[t] : off[t + 1];
+        const uint32_t beg = off[t], end = off[t + 1];
         constexpr int B = PM_VEC3_BATCH;  // particles in flight per thread
         for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += NT * B) {
             uint32_t p[B];
@@ -188,12 +182,11 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
             for (int j = 0; j < B; ++j) {
                 const uint32_t q = q0 + NT * j;
                 p[j] = q < end ? idx[q] : 0xFFFFFFFFu;
-                if (sp && q < end) r[j] = sp[q];
             }
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 if (p[j] == 0xFFFFFFFFu) continue;
-                if (!sp) r[j] = tile_record_of<St>(px[p[j]] - ox, py[p[j]] - oy, pz[p[j]] - oz);
+                r[j] = tile_record_of<St>(px[p[j]] - ox, py[p[j]] - oy, pz[p[j]] - oz);
                 u[j][0] = ux[p[j]] * sc[0];
                 u[j][1] = uy[p[j]] * sc[1];
                 u[j][2] = uz[p[j]] * sc[2];
@@ -563,48 +556,33 @@ int launch_adj_deposit(int prec, const double* px, const double* py, const doubl
 
 int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
                         const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s,
-                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st, bool umax_ready,
-                        const uint32_t* bucket_idx) {
+                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
     const size_t sh = sorted_shell_elems(ng);
     const size_t rs = prec == PM_FP32 ? sizeof(float) : sizeof(double);
-    if (!umax_ready) {  // else the previous reverse-step gather left max |u_a|
-        if (np > 0) {
-            const unsigned nb = (unsigned)std::min<int64_t>((np + 255) / 256, 4096);
-            hipLaunchKernelGGL(k_absmax3, dim3(nb), dim3(256), 0, st, ux, uy, uz, np, umax);
-        }
-        hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
+    if (np > 0) {
+        const unsigned nb = (unsigned)std::min<int64_t>((np + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_absmax3, dim3(nb), dim3(256), 0, st, ux, uy, uz, np, umax);
     }
-    // with buckets: the bucket pass, or (overflow) the binning pass
-#define PM_VEC3_PASS(RL, TSV, REC, IDX, OFF, CNT, GATE, RUN)                                                       \
+    hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
+#define PM_VEC3(RL, TSV)                                                                                          \
     hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles / walk_seg(nt)), dim3(kVec3Threads), 0, st, px, \
-                       py, pz, REC, ux, uy, uz, IDX, OFF, CNT, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2],          \
-                       (RL*)shell3, sh, GATE, RUN)
-#define PM_VEC3(RL, TSV)                                                                                    \
-    if (bucket_idx) {                                                                                       \
-        PM_VEC3_PASS(RL, TSV, s.stage, bucket_idx, s.coff, s.fill, s.spill, 0);                             \
-        PM_VEC3_PASS(RL, TSV, nullptr, s.idx, s.off, nullptr, s.spill, 1);                                  \
-    } else {                                                                                                \
-        PM_VEC3_PASS(RL, TSV, nullptr, s.idx, s.off, nullptr, nullptr, 0);                                  \
-    }                                                                                                       \
-    for (int d = 0; d < 3; ++d) launch_fold_shells(prec, ts, ng, (const char*)shell3 + d * sh * rs, U[d], st);
+                       py, pz, ux, uy, uz, s.idx, s.off, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3, sh)
     if (prec == PM_FP32) {
-        if (ts == 16) {
-            PM_VEC3(float, 16)
-        } else {
-            PM_VEC3(float, 8)
-        }
+        if (ts == 16)
+            PM_VEC3(float, 16);
+        else
+            PM_VEC3(float, 8);
     } else {
-        if (ts == 16) {
-            PM_VEC3(double, 16)
-        } else {
-            PM_VEC3(double, 8)
-        }
+        if (ts == 16)
+            PM_VEC3(double, 16);
+        else
+            PM_VEC3(double, 8);
     }
 #undef PM_VEC3
-#undef PM_VEC3_PASS
+    for (int d = 0; d < 3; ++d) launch_fold_shells(prec, ts, ng, (const char*)shell3 + d * sh * rs, U[d], st);
     return check_launch("deposit_vec3");
 }
 
