@@ -280,3 +280,210 @@ def test_cosmo_zero_step_is_identity():
     for a, b in zip(pc, pos):
         assert np.array_equal(a, b)
     assert max(np.abs(v).max() for v in p) == 0.0
+
+
+# ------------------------------------------------ Zel'dovich initial conditions
+# The reference has no IC generator (utils.cpp:22-36 draws uniform particles),
+# so pyoracle.zeldovich_ic is pinned by what the mathematics of a Zel'dovich
+# displacement fixes: x = q + psi with psi = -grad laplacian^-1 delta, i.e.
+# psi_hat = i k delta_hat / |k|^2 and delta_hat = w_hat / |k| (P_delta ~ k^-2),
+# so psi is curl-free, -div psi = delta, and one Fourier mode of the noise
+# displaces the lattice along that mode's wave vector.
+
+def _lattice_displacement(pos, nrow, ng):
+    """Displacement fields (nrow^3 arrays) of the ICs about q = (i + 1/2) ng / nrow."""
+    idx = (np.arange(nrow) + 0.5) * (ng / nrow)
+    Q = np.meshgrid(idx, idx, idx, indexing="ij")
+    out = []
+    for x, q in zip(pos, Q):
+        d = x.reshape(nrow, nrow, nrow) - q
+        out.append(d - ng * np.round(d / ng))  # nearest periodic image
+    return out
+
+
+def _kvec(nrow):
+    f = np.fft.fftfreq(nrow, 1.0 / nrow)
+    kx = f[:, None, None] * np.ones((1, nrow, nrow // 2 + 1))
+    ky = f[None, :, None] * np.ones((nrow, 1, nrow // 2 + 1))
+    kz = np.arange(nrow // 2 + 1, dtype=np.float64)[None, None, :] * np.ones((nrow, nrow, 1))
+    return kx, ky, kz
+
+
+@pytest.mark.parametrize("m,nrow,ng", [((1, 0, 0), 16, 16), ((1, 2, 0), 16, 32), ((2, -1, 3), 32, 32)])
+def test_zeldovich_single_mode_is_analytic_sine(m, nrow, ng):
+    """Noise cos(2 pi m.i / nrow) has w_hat = nrow^3/2 at +-m, so
+    psi_a = -(m_a / |m|^3) sin(2 pi m.i / nrow): a plane sine wave along m
+    that moves particles towards the density peak at the wave's crest.  After
+    the rescale the rms of |s psi| is rms_cells, so s psi = -sqrt(2) rms m_hat
+    sin(theta), and v = vel_factor * s psi."""
+    i = np.arange(nrow)
+    I, J, K = np.meshgrid(i, i, i, indexing="ij")
+    th = 2 * np.pi * (m[0] * I + m[1] * J + m[2] * K) / nrow
+    rms, vf = 0.4, 2.5
+    pos, vel = pyoracle.zeldovich_ic(np.cos(th), ng, rms, vf)
+    disp = _lattice_displacement(pos, nrow, ng)
+    mhat = np.array(m, dtype=np.float64) / np.linalg.norm(m)
+    for a in range(3):
+        exact = -np.sqrt(2.0) * rms * mhat[a] * np.sin(th)
+        assert np.abs(disp[a] - exact).max() < 1e-12, a
+        assert np.abs(vel[a].reshape(nrow, nrow, nrow) - vf * exact).max() < 1e-12, a
+
+
+def test_zeldovich_amplitude_falls_as_inverse_square_of_k():
+    """Two orthogonal modes of equal noise amplitude: m1 = (1,0,0) displaces
+    along x only, m2 = (0,3,0) along y only, with amplitudes in the ratio
+    |m2|^2 / |m1|^2 = 9 (psi ~ k / |k|^3 ~ 1/|k|^2 per unit noise)."""
+    nrow, ng = 16, 16
+    i = np.arange(nrow)
+    I, J, K = np.meshgrid(i, i, i, indexing="ij")
+    t1, t2 = 2 * np.pi * I / nrow, 2 * np.pi * 3 * J / nrow
+    pos, _ = pyoracle.zeldovich_ic(np.cos(t1) + np.cos(t2), ng, 0.5, 1.0)
+    dx, dy, dz = _lattice_displacement(pos, nrow, ng)
+    a1 = -np.sum(dx * np.sin(t1)) / np.sum(np.sin(t1) ** 2)
+    a2 = -np.sum(dy * np.sin(t2)) / np.sum(np.sin(t2) ** 2)
+    assert a1 > 0 and a2 > 0
+    assert abs(a1 / a2 - 9.0) < 1e-10
+    assert np.abs(dz).max() < 1e-13
+    assert np.abs(dx + a1 * np.sin(t1)).max() < 1e-12 and np.abs(dy + a2 * np.sin(t2)).max() < 1e-12
+
+
+@pytest.mark.parametrize("nrow,ng", [(16, 16), (16, 64)])
+def test_zeldovich_divergence_is_minus_delta_and_curl_free(nrow, ng):
+    """With the spectral derivative i k_a (zero at the Nyquist k_a, whose
+    derivative has no real-valued representation; the generator removes psi_a
+    there): -i k.psi_hat = s w_hat / |k| with one positive s on every mode
+    without a Nyquist component, k x psi_hat = 0, psi_hat_a = 0 where k_a is
+    Nyquist, and psi_hat(0) = 0 (no net displacement)."""
+    rng = np.random.default_rng(23)
+    w = rng.standard_normal((nrow, nrow, nrow))
+    pos, _ = pyoracle.zeldovich_ic(w, ng, 0.8, 1.0)
+    disp = _lattice_displacement(pos, nrow, ng)
+    ph = [np.fft.rfftn(d) for d in disp]
+    kx, ky, kz = _kvec(nrow)
+    nyq = nrow // 2
+    ks = []
+    for k in (kx, ky, kz):
+        ks.append(np.where(np.abs(k) == nyq, 0.0, k))
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    clean = (k2 > 0) & (np.abs(kx) != nyq) & (np.abs(ky) != nyq) & (kz != nyq)
+    div = -1j * (ks[0] * ph[0] + ks[1] * ph[1] + ks[2] * ph[2])
+    target = np.fft.rfftn(w) / np.sqrt(np.where(k2 > 0, k2, 1.0))
+    s = np.real(np.vdot(target[clean], div[clean]) / np.vdot(target[clean], target[clean]))
+    assert s > 0
+    scale = np.abs(div[clean]).max()
+    assert np.abs(div[clean] - s * target[clean]).max() < 1e-11 * scale
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        curl = ks[a] * ph[b] - ks[b] * ph[a]
+        assert np.abs(curl).max() < 1e-11 * scale
+    for a, k in enumerate((kx, ky, kz)):
+        assert np.abs(ph[a][np.abs(k) == nyq]).max() < 1e-11 * scale
+    assert max(abs(p[0, 0, 0]) for p in ph) < 1e-11 * scale * nrow ** 3
+    # rms normalisation and the velocity factor
+    assert abs(np.sqrt(np.mean(disp[0] ** 2 + disp[1] ** 2 + disp[2] ** 2)) - 0.8) < 1e-12
+
+
+def test_zeldovich_density_spectrum_slope_is_minus_two():
+    """On a 64^3 draw the shell-averaged power of delta = -div psi falls as
+    k^-2 (fitted slope within 0.05 over 2 <= |k| <= 24) and the displacement
+    power as k^-4."""
+    nrow = 64
+    w = np.random.default_rng(41).standard_normal((nrow, nrow, nrow))
+    pos, _ = pyoracle.zeldovich_ic(w, nrow, 1.0, 1.0)
+    disp = _lattice_displacement(pos, nrow, nrow)
+    ph = [np.fft.rfftn(d) for d in disp]
+    kx, ky, kz = _kvec(nrow)
+    kk = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    delta = -1j * (kx * ph[0] + ky * ph[1] + kz * ph[2])
+    pd = np.abs(delta) ** 2
+    pp = np.abs(ph[0]) ** 2 + np.abs(ph[1]) ** 2 + np.abs(ph[2]) ** 2
+    edges = np.arange(2, 25)
+    cen, mean_d, mean_p = [], [], []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        sel = (kk >= lo) & (kk < hi)
+        cen.append(np.mean(kk[sel]))
+        mean_d.append(np.mean(pd[sel]))
+        mean_p.append(np.mean(pp[sel]))
+    slope_d = np.polyfit(np.log(cen), np.log(mean_d), 1)[0]
+    slope_p = np.polyfit(np.log(cen), np.log(mean_p), 1)[0]
+    assert abs(slope_d + 2.0) < 0.05, slope_d
+    assert abs(slope_p + 4.0) < 0.05, slope_p
+
+
+# ------------------------------------------------ kick-drift-kick schedule
+# orc_run_kdk (merged kicks dt/2, dt, ..., dt, dt/2 with one force
+# evaluation per step) has no reference counterpart (the reference kicks
+# once and never drifts, utils.cpp:141-170, :145-146), so it is pinned by
+# the linear growth of a single Zel'dovich mode, whose exact solution is known,
+# and by the time reversibility of the leapfrog.
+
+def _mode_growth_run(ng, nsteps, dt, amps, modes):
+    """Plane-sine displacements psi_a = A_a sin(2 pi m_a q_a / ng) of the
+    one-particle-per-cell lattice q = i + 1/2, with the growing-mode velocity
+    v = sqrt(4 pi) psi of the continuous static background."""
+    idx = np.arange(ng) + 0.5
+    Q = [v.ravel() for v in np.meshgrid(idx, idx, idx, indexing="ij")]
+    psi0 = [A * np.sin(2 * np.pi * m * q / ng) for A, m, q in zip(amps, modes, Q)]
+    pos = [q + p for q, p in zip(Q, psi0)]
+    vel = [np.sqrt(4 * np.pi) * p for p in psi0]
+    pyoracle.run_kdk(pos, vel, ng, nsteps, dt, pyoracle.GREEN_CONT_K2, 1.0)
+    disp = [x - q for x, q in zip(pos, Q)]
+    return psi0, disp, vel
+
+
+def _mode_growth_exact(psi0, ng, modes, t):
+    """A displacement mode m_a of the lattice in the PM scheme (CIC deposit
+    onto nodes, Green's function -4 pi G rho_bar / k^2 in cell units, centred
+    difference, CIC interpolation) feels the linear force omega^2 psi with
+    omega^2 = 4 pi sin^2(theta) / theta^2, theta = 2 pi m / ng: the deposit
+    of the displaced cell-centred lattice gives delta_j = psi(j-1/2) -
+    psi(j+1/2), the centred difference multiplies by sin(theta)/theta relative
+    to the derivative and the interpolation to cell centres by cos(theta/2).
+    With x(0) = psi0 and v(0) = sqrt(4 pi) psi0 the solution of
+    psi'' = omega^2 psi is psi0 (cosh wt + (sqrt(4 pi)/w) sinh wt)."""
+    out_x, out_v = [], []
+    for p, m in zip(psi0, modes):
+        th = 2 * np.pi * m / ng
+        w = np.sqrt(4 * np.pi) * np.sin(th) / th
+        c = np.sqrt(4 * np.pi) / w
+        out_x.append(p * (np.cosh(w * t) + c * np.sinh(w * t)))
+        out_v.append(p * w * (np.sinh(w * t) + c * np.cosh(w * t)))
+    return out_x, out_v
+
+
+def test_kdk_linear_mode_growth_converges_at_second_order():
+    """Three small plane-sine modes (x: m=1, y: m=2, z: m=3) grow for t = 0.5
+    (about 1.75 e-folds).  Against the exact linear solution the KDK error in
+    positions and velocities must fall by 4 when dt halves (second order).  A
+    wrong end-kick weight (a full dt first or last, or a missing last half
+    kick) leaves a first-order error and a ratio near 2."""
+    ng, T = 32, 0.5
+    amps, modes = (1e-5, 0.7e-5, 0.5e-5), (1, 2, 3)
+    errs_x, errs_v = [], []
+    for n in (10, 20, 40):
+        psi0, disp, vel = _mode_growth_run(ng, n, T / n, amps, modes)
+        ex, ev = _mode_growth_exact(psi0, ng, modes, T)
+        errs_x.append(max(np.abs(d - e).max() for d, e in zip(disp, ex)) / max(np.abs(e).max() for e in ex))
+        errs_v.append(max(np.abs(d - e).max() for d, e in zip(vel, ev)) / max(np.abs(e).max() for e in ev))
+    for errs in (errs_x, errs_v):
+        assert errs[0] < 5e-3, errs
+        r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+        assert 3.6 < r1 < 4.4 and 3.6 < r2 < 4.4, (errs, r1, r2)
+
+
+def test_kdk_is_time_reversible():
+    """Leapfrog is symmetric: n steps, negate v, n steps, negate v returns to
+    the initial state (here to rounding, after shell-crossing dynamics)."""
+    ng, npart = 16, 3000
+    rng = np.random.default_rng(29)
+    pos = [rng.uniform(0, ng, npart) for _ in range(3)]
+    vel = [rng.standard_normal(npart) * 2.0 for _ in range(3)]
+    p0, v0 = [v.copy() for v in pos], [v.copy() for v in vel]
+    pyoracle.run_kdk(pos, vel, ng, 12, 0.02, pyoracle.GREEN_CONT_K2, 1.0)
+    moved = max(np.abs(((a - b + ng / 2) % ng) - ng / 2).max() for a, b in zip(pos, p0))
+    assert moved > 0.3
+    vel = [-v for v in vel]
+    pyoracle.run_kdk(pos, vel, ng, 12, 0.02, pyoracle.GREEN_CONT_K2, 1.0)
+    for a, b in zip(pos, p0):
+        assert np.abs(((a - b + ng / 2) % ng) - ng / 2).max() < 1e-10
+    for a, b in zip(vel, v0):
+        assert np.abs(-a - b).max() < 1e-10
