@@ -31,19 +31,21 @@ static int hip_ok(hipError_t e, const char* what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// One range-error counter per device, allocated on first use (under a
-// lock: plans on one device may be driven from several host threads).
+// One range-error counter per device: a zero-initialised device variable of
+// this module, so that resolving it allocates nothing and cannot run out of
+// memory.  Its address is looked up once per device (under a lock: plans on
+// one device may be driven from several host threads).
+static __device__ unsigned long long range_count_dev;
 static unsigned long long* g_range[64] = {};
 static std::mutex g_range_mu;
 
 unsigned long long* range_counter() {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
     std::lock_guard<std::mutex> lock(g_range_mu);
     if (!g_range[dev]) {
         void* p = nullptr;
-        if (hipMalloc(&p, sizeof(unsigned long long)) != hipSuccess) return nullptr;
-        if (hipMemset(p, 0, sizeof(unsigned long long)) != hipSuccess) return nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(range_count_dev)) != hipSuccess) return nullptr;
         g_range[dev] = (unsigned long long*)p;
     }
     return g_range[dev];
@@ -230,7 +232,9 @@ extern "C" int pm_plan_create(pm_plan_t** out, int ng, int64_t np, int precision
     wisdom_find(ng, precision, &p->fft_shape);  // the cached shape, else the default
     const size_t rs = precision == PM_FP32 ? sizeof(float) : sizeof(double);
     const size_t nc = (size_t)ng * ng * ng;
-    int rc = tables_create(&p->tab, ng, precision, p->stream);
+    p->err = range_counter();
+    int rc = p->err ? PM_OK : fail(PM_ERR_HIP, "pm_plan_create: range-error counter not found");
+    if (!rc) rc = tables_create(&p->tab, ng, precision, p->stream);
     if (!rc) rc = hip_ok(hipMalloc(&p->rho, rs * nc), "pm_plan_create: hipMalloc rho");
     if (!rc) rc = hip_ok(hipMalloc(&p->phi, rs * nc), "pm_plan_create: hipMalloc phi");
     if (!rc) rc = hip_ok(hipMalloc(&p->specA, 2 * rs * spec_elems(ng)), "pm_plan_create: hipMalloc spec");
@@ -327,13 +331,13 @@ static int deposit_impl(pm_plan* p, const double* x, const double* y, const doub
         int rc = ensure_sorted(p);
         if (rc) return rc;
         invalidate_reuse(p);
-        return launch_deposit_sorted(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
+        return launch_deposit_sorted(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, p->err,
                                      p->stream);
     }
     if (variant != PM_DEPOSIT_ATOMIC) return fail(PM_ERR_ARG, "pm_deposit: unknown variant");
     int rc = launch_fill(p->prec, p->rho, nc, 0.0, p->stream);
     if (rc) return rc;
-    return launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, range_counter(), p->stream);
+    return launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->err, p->stream);
 }
 
 // potential = IFFT[G * scale / N^3 * FFT[in - sub1]], in -> out (5 passes)
@@ -376,7 +380,7 @@ extern "C" int pm_interp_kick_drift(pm_plan_t* p, double* x, double* y, double* 
                                     double* vz, double kick, double drift) {
     if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_interp_kick_drift: null argument");
     invalidate_reuse(p);
-    return launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 0, range_counter(),
+    return launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, 0, p->err,
                          p->stream);
 }
 
@@ -384,7 +388,7 @@ extern "C" int pm_interp_force(pm_plan_t* p, const double* x, const double* y, c
                                double* gy, double* gz) {
     if (!p || !x || !y || !z || !gx || !gy || !gz) return fail(PM_ERR_ARG, "pm_interp_force: null argument");
     return launch_interp(p->prec, (double*)x, (double*)y, (double*)z, gx, gy, gz, p->np, p->ng, p->phi, 0.0, 0.0,
-                         1, range_counter(), p->stream);
+                         1, p->err, p->stream);
 }
 
 extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
@@ -404,10 +408,10 @@ extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx
         if (staged && reuse && p->staged_valid && p->staged_x == x) {
             // positions staged in tile order by the previous step's
             // interpolation (caller's guarantee that x is unchanged since)
-            rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
+            rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, p->err,
                                        p->stream);
         } else {
-            rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
+            rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, p->err, p->stream);
             if (!rc) rc = launch_deposit_binned(p->prec, x, y, z, p->ng, mass, p->rho, p->sorted, p->stream);
             if (!rc && staged) rc = launch_cap_scan(p->ng, p->sorted, true, p->stream);
         }
@@ -423,7 +427,7 @@ extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx
     // gathers in L2); with the sorted deposit the drifted positions are
     // staged in tile order for the next step's deposit
     rc = launch_interp(p->prec, x, y, z, vx, vy, vz, p->np, p->ng, p->phi, kick, drift, staged ? 2 : 0,
-                       range_counter(), p->stream, staged ? &p->sorted : nullptr);
+                       p->err, p->stream, staged ? &p->sorted : nullptr);
     if (staged) {
         p->staged_valid = rc == PM_OK;
         p->staged_x = x;
@@ -543,13 +547,13 @@ static int forward_field(pm_plan* p, const double* x, const double* y, const dou
     const size_t nc = (size_t)p->ng * p->ng * p->ng;
     const double mass = (double)nc / (double)p->np;
     invalidate_reuse(p);
-    int rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
+    int rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, p->err, p->stream);
     if (rc) return rc;
     if (variant == PM_DEPOSIT_SORTED) {
         rc = launch_deposit_binned(p->prec, x, y, z, p->ng, mass, p->rho, p->sorted, p->stream);
     } else if (variant == PM_DEPOSIT_ATOMIC) {
         rc = launch_fill(p->prec, p->rho, nc, 0.0, p->stream);
-        if (!rc) rc = launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, range_counter(), p->stream);
+        if (!rc) rc = launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->err, p->stream);
     } else {
         rc = fail(PM_ERR_ARG, "adjoint: unknown deposit variant");
     }
@@ -574,7 +578,7 @@ extern "C" int pm_adjoint_deposit(pm_plan_t* p, const double* x, const double* y
     if (!p || !x || !y || !z || !mesh_bar || !xbar || !ybar || !zbar)
         return fail(PM_ERR_ARG, "pm_adjoint_deposit: null argument");
     const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
-    return launch_adj_deposit(p->prec, x, y, z, p->np, p->ng, mesh_bar, mass, xbar, ybar, zbar, range_counter(),
+    return launch_adj_deposit(p->prec, x, y, z, p->np, p->ng, mesh_bar, mass, xbar, ybar, zbar, p->err,
                               p->stream);
 }
 
@@ -583,7 +587,7 @@ extern "C" int pm_interp_transpose(pm_plan_t* p, const double* x, const double* 
     if (!p || !x || !y || !z || !ux || !uy || !uz) return fail(PM_ERR_ARG, "pm_interp_transpose: null argument");
     int rc = ensure_adjoint(p);
     invalidate_reuse(p);
-    if (!rc) rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
+    if (!rc) rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, p->err, p->stream);
     if (!rc)
         rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
                                  p->adjMax, p->stream);
@@ -623,7 +627,7 @@ extern "C" int pm_adjoint_force(pm_plan_t* p, const double* x, const double* y, 
         rc = launch_adj_gather(p->prec, 1, const_cast<double*>(x), const_cast<double*>(y), const_cast<double*>(z),
                                nullptr, nullptr, nullptr, const_cast<double*>(ux), const_cast<double*>(uy),
                                const_cast<double*>(uz), xbar, ybar, zbar, p->np, p->ng, p->phi, p->adjLam, mass, 0.0,
-                               1.0, range_counter(), p->stream);  // mode 1 writes only xbar
+                               1.0, p->err, p->stream);  // mode 1 writes only xbar
     p->timing_slots = slots;
     return rc;
 }
@@ -652,10 +656,10 @@ static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx,
     //    overflow flag of the buckets
     int rc;
     if (from_stage) {
-        rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream, p->sorted.spill);
+        rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, p->err, p->stream, p->sorted.spill);
     } else {
         invalidate_reuse(p);
-        rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, range_counter(), p->stream);
+        rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, p->err, p->stream);
     }
     // 2. the divergence deposit mu = sum_a D_a^T interp^T(u_a) on the second
     //    stream, beside the forward deposit and solve: it reads x, u and the
@@ -669,12 +673,12 @@ static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx,
     // 3. rho and phi at x
     if (!rc) {
         if (from_stage)
-            rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, range_counter(),
+            rc = launch_deposit_staged(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->sorted, p->err,
                                        p->stream, false, false);
         else if (variant == PM_DEPOSIT_SORTED)
             rc = launch_deposit_binned(p->prec, x, y, z, p->ng, mass, p->rho, p->sorted, p->stream);
         else if (!(rc = launch_fill(p->prec, p->rho, nc, 0.0, p->stream)))
-            rc = launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, range_counter(), p->stream);
+            rc = launch_deposit_atomic(p->prec, x, y, z, p->np, p->ng, mass, p->rho, p->err, p->stream);
     }
     if (!rc) rc = solve_into(p, p->rho, p->phi, true);
     // 4. join; lambda = S mu
@@ -694,7 +698,7 @@ static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx,
     }
     if (!rc)
         rc = launch_adj_gather(p->prec, gmode, x, y, z, vx, vy, vz, ux, uy, uz, xbar, ybar, zbar, p->np, p->ng, p->phi,
-                               p->adjLam, mass, kick, kick, range_counter(), p->stream, next_drift, p->adjMax,
+                               p->adjLam, mass, kick, kick, p->err, p->stream, next_drift, p->adjMax,
                                stage_next ? &stg : nullptr);
     return rc;
 }

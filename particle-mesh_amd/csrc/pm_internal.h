@@ -221,6 +221,8 @@ int launch_interp(int prec, double* px, double* py, double* pz, double* vx, doub
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_launch(const char* what);
+// The current device's range-error counter (a device variable; nullptr only
+// if the library's code object cannot be found on the device).
 unsigned long long* range_counter();
 
 inline bool valid_ng(int ng) { return ng >= 8 && ng <= 1024 && (ng & (ng - 1)) == 0; }
@@ -235,6 +237,7 @@ struct pm_plan {
     int green_mode = PM_GREEN_REF_KH;
     double green_scale = 1.0;
     hipStream_t stream = nullptr;
+    unsigned long long* err = nullptr;  // the device's range-error counter (range_counter())
     void* rho = nullptr;     // ng^3 reals
     void* phi = nullptr;     // ng^3 reals
     void* specA = nullptr;   // (ng/2+1) ng^2 complex

@@ -174,9 +174,11 @@ extern "C" int pm_deposit_particles(const double* posx, const double* posy, cons
     if (np < 0 || !buf || nx < 2 || ny < 2 || nz < 2 || (np > 0 && (!posx || !posy || !posz)))
         return fail(PM_ERR_ARG, "pm_deposit_particles: bad arguments");
     if (np == 0) return PM_OK;
+    unsigned long long* err = range_counter();
+    if (!err) return fail(PM_ERR_HIP, "pm_deposit_particles: range-error counter not found");
     const unsigned nb = (unsigned)((np + 255) / 256);
     hipLaunchKernelGGL(k_deposit_cube, dim3(nb), dim3(256), 0, (hipStream_t)stream, posx, posy, posz, np, buf,
-                       nx, ny, nz, x0, y0, z0, mass, range_counter());
+                       nx, ny, nz, x0, y0, z0, mass, err);
     return check_launch("pm_deposit_particles");
 }
 
@@ -230,8 +232,10 @@ extern "C" int pm_update_particles(const double* posx, const double* posy, const
         return fail(PM_ERR_ARG, "pm_update_particles: bad arguments");
     if (np == 0) return PM_OK;
     const double fa_da = pm_function_a(a) * 0.001;  // Da (utils_inline.h:17)
+    unsigned long long* err = range_counter();
+    if (!err) return fail(PM_ERR_HIP, "pm_update_particles: range-error counter not found");
     const unsigned nb = (unsigned)((np + 255) / 256);
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, (hipStream_t)stream, posx, posy, posz, d_dx, d_dy, d_dz,
-                       px, py, pz, fa_da, np, nx, ny, nz, x0, y0, z0, range_counter());
+                       px, py, pz, fa_da, np, nx, ny, nz, x0, y0, z0, err);
     return check_launch("pm_update_particles");
 }
