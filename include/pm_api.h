@@ -301,6 +301,25 @@ int pm_wisdom_export(const char* path);
 int pm_wisdom_lookup(int ng, int precision, int variants[PM_FFT_SLOTS], int* found);
 void pm_wisdom_forget(void);
 
+/* ---- 5. deposit without device allocation -------------------------------
+ * No counterpart in the reference: for callers that cannot call hipMalloc
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  The same sorted CIC deposit as pm_deposit(plan, ...,
+ * PM_DEPOSIT_SORTED) -- rho (ng^3 reals of `precision`) = mass * CIC of the
+ * np particles, mass = ng^3/np, periodic, bit for bit the plan's density --
+ * without a plan: its binning buffers live in a fixed device array of
+ * PM_NOALLOC_SCRATCH_MB MiB that the library holds on every device it runs
+ * on until the process ends.  The call needs about 12 np + 0.14 ng^3 * (4 or
+ * 8) bytes of it (e.g. up to 3.4 M particles on 256^3 FP32, 2.6 M on 256^3
+ * FP64, 4.1 M on 128^3) and returns PM_ERR_ARG, touching nothing, when that
+ * exceeds the array.  Out-of-box particles are skipped and counted
+ * (pm_range_errors).  Asynchronous on `stream`.  Every call on one device
+ * shares that device's array: calls that overlap on one device (several
+ * streams or host threads at once) race and give wrong results. */
+#define PM_NOALLOC_SCRATCH_MB 48
+int pm_deposit_sorted_noalloc(const double* posx, const double* posy, const double* posz, int64_t np, int ng,
+                              int precision, void* rho, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,6 +104,7 @@ SIGNATURES = {
     "pm_wisdom_export": (_c_int, [ctypes.c_char_p]),
     "pm_wisdom_lookup": (_c_int, [_c_int, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
     "pm_wisdom_forget": (None, []),
+    "pm_deposit_sorted_noalloc": (_c_int, [_c_vp, _c_vp, _c_vp, _c_i64, _c_int, _c_int, _c_vp, _c_vp]),
 }
 
 _lib = None
