@@ -220,7 +220,13 @@ int pm_run_cosmo(pm_plan_t* plan, double* posx, double* posy, double* posz, doub
  *   lambda  = S sum_a D_a^T U_a,  U_a = interp^T(u_a),
  * S the (symmetric) Green's-function solve and D_a the centred difference
  * (D_a^T = -D_a).  The adjoint entry points allocate their work meshes
- * (3 + 1 meshes and the binning) on first use. */
+ * (3 + 1 meshes and the binning) on first use.
+ * The deposits of u (interpolation transpose, divergence deposit) sum in
+ * signed 64-bit fixed point at a power-of-two scale set by max |u_a|: exact
+ * and order-free, valid while no mesh node collects more than 2^22
+ * particles (an overflow there wraps silently).  An inf or NaN component of
+ * u has no such scale: it is counted in pm_range_errors() and the adjoint
+ * meshes are then meaningless. */
 
 /* Transposed CIC (adjoint of the deposit w.r.t. positions):
  * xbar += m * grad_x sum_c W(x, c) mesh_bar[c]; mesh_bar is ng^3 in the

@@ -325,6 +325,14 @@ __global__ __launch_bounds__(kDivThreads) void k_deposit_div_tiles(
     walk_halo_seam_out<Rl, TS, H, NT>(acc, inv, sw, shell);
 }
 
+// Max of two magnitudes (sign bit clear) by their bit patterns, which order
+// like the values for finite ones and put +inf above them and NaN above +inf:
+// unlike fmax, a NaN is kept, so a non-finite cotangent reaches the max.
+__device__ __forceinline__ double mag_max(double a, double b) {
+    return __longlong_as_double(max(__double_as_longlong(a), __double_as_longlong(b)));
+}
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ULL;
+
 // Workgroup (NT threads) max of m[0..3) into one of kMaxSlots slots per
 // component (umax[3 + d * kMaxSlots + slot]): bit patterns of non-negative
 // doubles order like the values, so an integer atomic max works, and the
@@ -334,20 +342,23 @@ __device__ __forceinline__ void wg_max_to_slots(double (&m)[3], unsigned long lo
     __shared__ double wm[3][NT / 64];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        for (int o = 32; o > 0; o >>= 1) m[d] = fmax(m[d], __shfl_xor(m[d], o));
+        for (int o = 32; o > 0; o >>= 1) m[d] = mag_max(m[d], __shfl_xor(m[d], o));
         if (__lane_id() == 0) wm[d][threadIdx.x >> 6] = m[d];
     }
     __syncthreads();
     if (threadIdx.x < 3) {
         double v = wm[threadIdx.x][0];
-        for (int w = 1; w < NT / 64; ++w) v = fmax(v, wm[threadIdx.x][w]);
+        for (int w = 1; w < NT / 64; ++w) v = mag_max(v, wm[threadIdx.x][w]);
         atomicMax(&umax[3 + threadIdx.x * kMaxSlots + (blockIdx.x & (kMaxSlots - 1))],
                   (unsigned long long)__double_as_longlong(v));
     }
 }
 
-// umax[d] = max over the slots of component d; the slots are cleared
-__global__ __launch_bounds__(kMaxSlots) void k_max_slots(unsigned long long* __restrict__ umax) {
+// umax[d] = max over the slots of component d; the slots are cleared.  A
+// non-finite max (an inf or NaN cotangent, which no fixed-point scale can
+// hold) is counted in the range-error counter.
+__global__ __launch_bounds__(kMaxSlots) void k_max_slots(unsigned long long* __restrict__ umax,
+                                                         unsigned long long* __restrict__ err) {
     __shared__ unsigned long long r[3][kMaxSlots];
     for (int d = 0; d < 3; ++d) {
         r[d][threadIdx.x] = umax[3 + d * kMaxSlots + threadIdx.x];
@@ -359,7 +370,10 @@ __global__ __launch_bounds__(kMaxSlots) void k_max_slots(unsigned long long* __r
             for (int d = 0; d < 3; ++d) r[d][threadIdx.x] = max(r[d][threadIdx.x], r[d][threadIdx.x + h]);
         __syncthreads();
     }
-    if (threadIdx.x < 3) umax[threadIdx.x] = r[threadIdx.x][0];
+    if (threadIdx.x < 3) {
+        umax[threadIdx.x] = r[threadIdx.x][0];
+        if (r[threadIdx.x][0] >= kInfBits) atomicAdd(err, 1ULL);
+    }
 }
 
 // max |u_a| over all particles into the slots
@@ -368,9 +382,9 @@ __global__ __launch_bounds__(256) void k_absmax3(const double* __restrict__ ux, 
                                                  unsigned long long* __restrict__ umax) {
     double m[3] = {0.0, 0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < np; i += (int64_t)gridDim.x * 256) {
-        m[0] = fmax(m[0], fabs(ux[i]));
-        m[1] = fmax(m[1], fabs(uy[i]));
-        m[2] = fmax(m[2], fabs(uz[i]));
+        m[0] = mag_max(m[0], fabs(ux[i]));
+        m[1] = mag_max(m[1], fabs(uy[i]));
+        m[2] = mag_max(m[2], fabs(uz[i]));
     }
     wg_max_to_slots<256>(m, umax);
 }
@@ -556,7 +570,8 @@ int launch_adj_deposit(int prec, const double* px, const double* py, const doubl
 
 int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
                         const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s,
-                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st) {
+                        void* const U[3], void* shell3, unsigned long long* umax, unsigned long long* err,
+                        hipStream_t st) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
@@ -566,7 +581,7 @@ int launch_deposit_vec3(int prec, const double* px, const double* py, const doub
         const unsigned nb = (unsigned)std::min<int64_t>((np + 255) / 256, 4096);
         hipLaunchKernelGGL(k_absmax3, dim3(nb), dim3(256), 0, st, ux, uy, uz, np, umax);
     }
-    hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
+    hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax, err);
 #define PM_VEC3(RL, TSV)                                                                                          \
     hipLaunchKernelGGL((k_deposit_vec3_tiles<RL, TSV>), dim3(ntiles / walk_seg(nt)), dim3(kVec3Threads), 0, st, px, \
                        py, pz, ux, uy, uz, s.idx, s.off, ng, umax, (RL*)U[0], (RL*)U[1], (RL*)U[2], (RL*)shell3, sh)
@@ -588,8 +603,8 @@ int launch_deposit_vec3(int prec, const double* px, const double* py, const doub
 
 int launch_deposit_div(int prec, const double* px, const double* py, const double* pz, const double* ux,
                        const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s, void* mu,
-                       void* shell, unsigned long long* umax, hipStream_t st, bool umax_ready,
-                       const uint32_t* bucket_idx) {
+                       void* shell, unsigned long long* umax, unsigned long long* err, hipStream_t st,
+                       bool umax_ready, const uint32_t* bucket_idx) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
@@ -598,7 +613,7 @@ int launch_deposit_div(int prec, const double* px, const double* py, const doubl
             const unsigned nb = (unsigned)std::min<int64_t>((np + 255) / 256, 4096);
             hipLaunchKernelGGL(k_absmax3, dim3(nb), dim3(256), 0, st, ux, uy, uz, np, umax);
         }
-        hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
+        hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax, err);
     }
     // with buckets: the bucket pass, or (overflow) the binning pass
 #define PM_DIV_PASS(RL, TSV, REC, IDX, OFF, CNT, GATE, RUN)                                                      \
@@ -679,7 +694,7 @@ int launch_adj_gather(int prec, int mode, double* px, double* py, double* pz, do
             PM_GATHER(double, 2, true, kAdjThreads64);
     }
 #undef PM_GATHER
-    if (mode == 2) hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax);
+    if (mode == 2) hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax, err);
     return check_launch("adjoint_gather");
 }
 

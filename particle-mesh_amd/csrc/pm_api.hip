@@ -568,7 +568,7 @@ static int adjoint_field(pm_plan* p, const double* x, const double* y, const dou
                          const double* uy, const double* uz, bool umax_ready = false,
                          const uint32_t* bucket_idx = nullptr) {
     int rc = launch_deposit_div(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU[0], p->adjShell,
-                                p->adjMax, p->stream, umax_ready, bucket_idx);
+                                p->adjMax, p->err, p->stream, umax_ready, bucket_idx);
     if (!rc) rc = solve_into(p, p->adjU[0], p->adjLam, false);
     return rc;
 }
@@ -590,7 +590,7 @@ extern "C" int pm_interp_transpose(pm_plan_t* p, const double* x, const double* 
     if (!rc) rc = launch_bin(x, y, z, p->np, p->ng, p->sorted, p->err, p->stream);
     if (!rc)
         rc = launch_deposit_vec3(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU, p->adjShell,
-                                 p->adjMax, p->stream);
+                                 p->adjMax, p->err, p->stream);
     return rc;
 }
 
@@ -668,7 +668,7 @@ static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx,
     if (!rc) rc = hip_ok(hipStreamWaitEvent(p->adjStream, p->adjFork, 0), "adjoint: fork");
     if (!rc)
         rc = launch_deposit_div(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU[0], p->adjShell,
-                                p->adjMax, p->adjStream, umax_ready, from_stage ? p->adjIdx : nullptr);
+                                p->adjMax, p->err, p->adjStream, umax_ready, from_stage ? p->adjIdx : nullptr);
     if (!rc) rc = hip_ok(hipEventRecord(p->adjJoin, p->adjStream), "adjoint: join");
     // 3. rho and phi at x
     if (!rc) {

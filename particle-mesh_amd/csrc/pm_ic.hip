@@ -34,8 +34,12 @@ __global__ __launch_bounds__(256) void k_ic_component(const double2* __restrict_
     }
 }
 
+// Sum of squares of the displacement: one partial per workgroup (grid-stride
+// over a fixed grid, tree in LDS), then one workgroup adds the partials in a
+// fixed order, so the normalisation -- and the ICs -- are bitwise
+// reproducible from run to run.
 __global__ __launch_bounds__(256) void k_sumsq(const double* __restrict__ a, const double* __restrict__ b,
-                                               const double* __restrict__ c, size_t n, double* __restrict__ acc) {
+                                               const double* __restrict__ c, size_t n, double* __restrict__ part) {
     __shared__ double red[256];
     double s = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -46,7 +50,21 @@ __global__ __launch_bounds__(256) void k_sumsq(const double* __restrict__ a, con
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(acc, red[0]);
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int nparts,
+                                                   double* __restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 // psx/psy/psz alias vx/vy/vz (the displacement is staged there): each
@@ -102,29 +120,32 @@ extern "C" int pm_ic_zeldovich(int nrow, int ng, const double* noise, double rms
     alloc(&specA, nspec * sizeof(double2));
     alloc(&specB, nspec * sizeof(double2));
     alloc(&specC, nspec * sizeof(double2));
-    alloc(&acc, sizeof(double));
+    const unsigned nb = (unsigned)std::min<size_t>((nreal + 255) / 256, 16384);
+    alloc(&acc, sizeof(double) * (nb + 1));  // nb partial sums, then the total
     // psi components are staged in the caller's velocity arrays (nreal each)
     double* psi[3] = {vx, vy, vz};
     if (!rc) {
         const GreenArgs ga{PM_GREEN_REF_KH, 1.0, tab.cos_tab};
-        fft_pass(PM_FP64, N, kPassZFwd, st, noise, specA, tab.twM, tab.twN, ga, 1.0, false);
-        fft_pass(PM_FP64, N, kPassYFwd, st, specA, specB, tab.twM, tab.twN, ga, 1.0, false);
-        fft_pass(PM_FP64, N, kPassXFwd, st, nullptr, specB, tab.twM, tab.twN, ga, 1.0, false);
-        const unsigned nb = (unsigned)std::min<size_t>((nspec + 255) / 256, 16384);
-        for (int a = 0; a < 3 && !rc; ++a) {
-            hipLaunchKernelGGL(k_ic_component, dim3(nb), dim3(256), 0, st, (const double2*)specB, (double2*)specC, N, a);
-            fft_pass(PM_FP64, N, kPassXInv, st, nullptr, specC, tab.twM, tab.twN, ga, 1.0, false);
-            fft_pass(PM_FP64, N, kPassYInv, st, specC, specA, tab.twM, tab.twN, ga, 1.0, false);
-            fft_pass(PM_FP64, N, kPassZInv, st, specA, psi[a], tab.twM, tab.twN, ga, 1.0, false);
+        int bad = fft_pass(PM_FP64, N, kPassZFwd, st, noise, specA, tab.twM, tab.twN, ga, 1.0, false);
+        bad |= fft_pass(PM_FP64, N, kPassYFwd, st, specA, specB, tab.twM, tab.twN, ga, 1.0, false);
+        bad |= fft_pass(PM_FP64, N, kPassXFwd, st, nullptr, specB, tab.twM, tab.twN, ga, 1.0, false);
+        const unsigned nbs = (unsigned)std::min<size_t>((nspec + 255) / 256, 16384);
+        for (int a = 0; a < 3 && !rc && !bad; ++a) {
+            hipLaunchKernelGGL(k_ic_component, dim3(nbs), dim3(256), 0, st, (const double2*)specB, (double2*)specC, N, a);
+            bad |= fft_pass(PM_FP64, N, kPassXInv, st, nullptr, specC, tab.twM, tab.twN, ga, 1.0, false);
+            bad |= fft_pass(PM_FP64, N, kPassYInv, st, specC, specA, tab.twM, tab.twN, ga, 1.0, false);
+            bad |= fft_pass(PM_FP64, N, kPassZInv, st, specA, psi[a], tab.twM, tab.twN, ga, 1.0, false);
             rc = check_launch("pm_ic_zeldovich: psi");
         }
+        if (bad && !rc) rc = fail(PM_ERR_ARG, "pm_ic_zeldovich: unsupported FFT pass");
     }
-    if (!rc) rc = launch_fill(PM_FP64, acc, 1, 0.0, st);
     if (!rc) {
-        const unsigned nb = (unsigned)std::min<size_t>((nreal + 255) / 256, 16384);
-        hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, st, psi[0], psi[1], psi[2], nreal, (double*)acc);
+        double* part = (double*)acc;
+        hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, st, psi[0], psi[1], psi[2], nreal, part);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, (const double*)part, (int)nb, part + nb);
         hipLaunchKernelGGL(k_ic_place, dim3(nb), dim3(256), 0, st, psi[0], psi[1], psi[2], N, (double)ng / N,
-                           (double)ng, (const double*)acc, rms_cells, vel_factor, posx, posy, posz, vx, vy, vz);
+                           (double)ng, (const double*)(part + nb), rms_cells, vel_factor, posx, posy, posz, vx, vy,
+                           vz);
         rc = check_launch("pm_ic_zeldovich: place");
     }
     if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = fail(PM_ERR_HIP, "pm_ic_zeldovich: sync");

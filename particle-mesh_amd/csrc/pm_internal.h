@@ -166,10 +166,12 @@ int launch_adj_deposit(int prec, const double* px, const double* py, const doubl
                        const void* mbar, double scale, double* bx, double* by, double* bz, unsigned long long* err,
                        hipStream_t st);
 // The interpolation transpose U_a = interp^T(u_a) over the tile binning of
-// x in s (pm_interp_transpose), with its own max |u_a| and shell folds.
+// x in s (pm_interp_transpose), with its own max |u_a| and shell folds.  A
+// non-finite cotangent component is counted in err (the range-error counter).
 int launch_deposit_vec3(int prec, const double* px, const double* py, const double* pz, const double* ux,
                         const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s,
-                        void* const U[3], void* shell3, unsigned long long* umax, hipStream_t st);
+                        void* const U[3], void* shell3, unsigned long long* umax, unsigned long long* err,
+                        hipStream_t st);
 int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st);
 // mu = sum_a D_a^T interp^T(u_a) in one deposit (the mesh that
 // launch_deposit_vec3 + launch_div3 form), folded; shell holds
@@ -177,11 +179,11 @@ int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st);
 // left by a mode-2 gather; bucket_idx: the particles come from the index
 // buckets that a staging mode-2 gather filled (coff / fill of s), or, if a
 // bucket overflowed (s.spill), from the binning that the staged deposit's
-// fallback made
+// fallback made.  A non-finite cotangent component is counted in err.
 int launch_deposit_div(int prec, const double* px, const double* py, const double* pz, const double* ux,
                        const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s, void* mu,
-                       void* shell, unsigned long long* umax, hipStream_t st, bool umax_ready = false,
-                       const uint32_t* bucket_idx = nullptr);
+                       void* shell, unsigned long long* umax, unsigned long long* err, hipStream_t st,
+                       bool umax_ready = false, const uint32_t* bucket_idx = nullptr);
 // mode 1: xbar += VJP; 0: also undo the kick; 2: also the next reverse
 // step's undrift (next_drift) with max |u| into umax[0..3) (umax holds
 // 3 + 3 * kMaxSlots entries, the slots zero between calls)

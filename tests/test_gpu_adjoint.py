@@ -40,17 +40,28 @@ def cic_scatter(pos, w, ng):
     return out
 
 
+def _seam_positions(rng, ng, npart):
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    for p in ph:  # on the periodic seams
+        p[:64] = 0.0
+        p[64:128] = np.nextafter(float(ng), 0.0)
+    return ph
+
+
+@pytest.mark.parametrize("ng,npart", [(8, 3000), (32, 20000), (256, 300000)])
 @pytest.mark.parametrize("prec,tol", [(pmhip.PM_FP64, 5e-11), (pmhip.PM_FP32, 2e-6)])
-def test_interp_transpose_vs_numpy(dev, prec, tol):
+def test_interp_transpose_vs_numpy(dev, prec, tol, ng, npart):
     """The LDS accumulation is exact in units of 2^-40 of the largest
     cotangent component, so FP64 agrees to ~1e-11 relative; FP32 to its
-    final rounding."""
+    final rounding.  ng = 8 is one 8^3 tile folded onto itself, 256 has two
+    walk segments per column (the z-seam fold of each of the three meshes)."""
     import torch
 
-    ng, npart = 32, 20000
-    rng = np.random.default_rng(1)
-    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
-    uh = [rng.standard_normal(npart) for _ in range(3)]
+    print(f"case interp transpose ng={ng} np={npart} prec={prec}", flush=True)
+    pmhip.range_errors()  # reset the counter
+    rng = np.random.default_rng(1 + ng)
+    ph = _seam_positions(rng, ng, npart)
+    uh = [rng.standard_normal(npart) * s for s in (1.0, 3.0, 0.25)]
     with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
         plan.interp_transpose(to_dev(ph, dev), to_dev(uh, dev))
         torch.cuda.synchronize()
@@ -58,15 +69,20 @@ def test_interp_transpose_vs_numpy(dev, prec, tol):
             ref = cic_scatter(ph, uh[a], ng)
             got = host(plan.adjoint_mesh(a)).astype(np.float64)
             err = float(np.max(np.abs(got - ref)) / np.abs(ref).max())
-            assert err < tol, err
+            assert err < tol, (a, err)
+    assert pmhip.range_errors() == 0
 
 
-def test_adjoint_solve_vs_oracle(dev):
+@pytest.mark.parametrize("ng,npart", [(8, 3000), (32, 8000), (256, 300000)])
+def test_adjoint_solve_vs_oracle(dev, ng, npart):
+    """lambda = S sum_a D_a^T U_a from the three interpolation-transpose
+    meshes (k_div3 + the solve) against the oracle's solve of the numpy
+    scatter's divergence."""
     import torch
 
-    ng, npart = 32, 8000
-    rng = np.random.default_rng(2)
-    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    print(f"case adjoint solve ng={ng} np={npart}", flush=True)
+    rng = np.random.default_rng(2 + ng)
+    ph = _seam_positions(rng, ng, npart)
     uh = [rng.standard_normal(npart) for _ in range(3)]
     with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
         plan.interp_transpose(to_dev(ph, dev), to_dev(uh, dev))
@@ -78,6 +94,32 @@ def test_adjoint_solve_vs_oracle(dev):
         lam = pyoracle.solve(mu, pyoracle.GREEN_CONT_K2, 1.0)
         err = float(np.max(np.abs(host(plan.adjoint_mesh(3)) - lam)) / np.abs(lam).max())
         assert err < 1e-9, err
+
+
+@pytest.mark.parametrize("bad", [float("nan"), float("inf")])
+def test_non_finite_cotangent_is_reported(dev, bad):
+    """No fixed-point scale holds an inf or NaN cotangent: the deposits of u
+    count it in the range-error counter instead of failing silently."""
+    import torch
+
+    ng, npart = 16, 4000
+    rng = np.random.default_rng(9)
+    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
+    uh = [rng.standard_normal(npart) for _ in range(3)]
+    uh[1][17] = bad
+    pmhip.range_errors()  # reset the counter
+    xbar = to_dev([np.zeros(npart)] * 3, dev)
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.interp_transpose(to_dev(ph, dev), to_dev(uh, dev))
+        torch.cuda.synchronize()
+        assert pmhip.range_errors() >= 1
+        plan.adjoint_force(to_dev(ph, dev), to_dev(uh, dev), xbar, pmhip.PM_DEPOSIT_SORTED)
+        torch.cuda.synchronize()
+        assert pmhip.range_errors() >= 1
+        uh[1][17] = 0.5
+        plan.interp_transpose(to_dev(ph, dev), to_dev(uh, dev))
+        torch.cuda.synchronize()
+        assert pmhip.range_errors() == 0
 
 
 @pytest.mark.parametrize("ng,npart", [(8, 3000), (32, 20000), (256, 300000)])
@@ -95,10 +137,7 @@ def test_divergence_deposit_vs_numpy(dev, prec, tol, ng, npart):
     print(f"case divergence deposit ng={ng} np={npart} prec={prec}", flush=True)
     pmhip.range_errors()  # reset the counter
     rng = np.random.default_rng(ng + npart)
-    ph = [rng.uniform(0, ng, npart) for _ in range(3)]
-    for p in ph:  # on the periodic seams
-        p[:64] = 0.0
-        p[64:128] = np.nextafter(float(ng), 0.0)
+    ph = _seam_positions(rng, ng, npart)
     uh = [rng.standard_normal(npart) * s for s in (1.0, 3.0, 0.25)]
     xbar = to_dev([np.zeros(npart)] * 3, dev)
     with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:

@@ -4,11 +4,18 @@ config 2: 256^3 Zel'dovich-lattice particles on a 256^3 mesh, 0.3-cell rms,
           20 KDK steps of dt = 0.005; the FP64 mesh path against the FP64
           oracle on the same inputs, and the FP32 mesh path against both
           (final-position error budget 1e-4 cells rms).
+config 3: 512^3 particles on a 512^3 mesh, clustered Zel'dovich ICs at 1.5-cell
+          rms, 20 KDK steps of dt = 0.005; the FP64 and FP32 mesh paths against
+          the FP64 oracle, and the sorted deposit against the atomic one.
 config 4: 128^3 particles on a 128^3 mesh, 10 KDK steps; the gradient of
           ||rho_final - rho_target||^2 with respect to the initial
           displacements through the adjoint kernels with per-step
           checkpoints, checked by central differences on 64 coordinates and
           against the float64 autograd gradient of tests/torch_pm.py.
+config 5: 512^3 particles on a 1024^3 mesh, 1.0-cell rms, 30 KDK steps, then
+          the reverse sweep (reverse-time leapfrog, no stored trajectory): the
+          reversal drift, and the gradient of a linear loss on the final
+          positions against central differences of full forward runs.
 """
 import math
 
@@ -145,3 +152,132 @@ def test_config4_gradient_128(dev):
     plan64.close()
     print(f"central differences: worst error / tolerance {worst:.3f}", flush=True)
     assert worst < 1.0
+
+
+def _oracle_threads():
+    # the host cores this GPU's process may use (16 on one GPU's share)
+    pyoracle.set_threads(min(16, pyoracle.max_threads()))
+
+
+def test_config3_512_fp64_fp32_vs_oracle_and_sorted_vs_atomic(dev):
+    """FP64 mesh path vs the FP64 oracle after 20 steps: max position error
+    below 1e-8 cells (the sorted deposit rounds each weight to 2^-32 of the
+    mass before its exact sum; the oracle sums in FP64).  FP32 mesh path vs
+    the oracle: rms below 1e-4 cells (north-star budget), max below 1e-3.
+    Sorted vs atomic FP32 density after the first step: 1e-5 of the peak."""
+    import torch
+
+    print("case config3 512^3 20 steps", flush=True)
+    ng = nrow = 512
+    nsteps, dt = 20, 0.005
+    npart = nrow ** 3
+    pos0, vel0 = _zeldovich(nrow, ng, 1.5, 3, dev)
+    pmhip.range_errors()
+    final = {}
+    for prec in (pmhip.PM_FP64, pmhip.PM_FP32):
+        pos, vel = [p.clone() for p in pos0], [v.clone() for v in vel0]
+        with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+            plan.run_kdk(pos, vel, nsteps, dt, pmhip.PM_DEPOSIT_SORTED)
+            torch.cuda.synchronize()
+        final[prec] = [p.cpu() for p in pos]
+        del pos, vel
+    assert pmhip.range_errors() == 0
+    # sorted vs atomic deposit on the positions after one step
+    pos1, vel1 = [p.clone() for p in pos0], [v.clone() for v in vel0]
+    with pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.run_kdk(pos1, vel1, 1, dt, pmhip.PM_DEPOSIT_SORTED)
+        plan.deposit(pos1, pmhip.PM_DEPOSIT_SORTED)
+        rs = plan.density().clone()
+        plan.deposit(pos1, pmhip.PM_DEPOSIT_ATOMIC)
+        ra = plan.density().clone()
+        torch.cuda.synchronize()
+    da = float((rs - ra).abs().max()) / float(rs.abs().max())
+    print(f"sorted vs atomic density after one step: max {da:.3e} of the peak "
+          f"(peak {float(rs.max()):.1f}, min {float(rs.min()):.3f})", flush=True)
+    assert da < 1e-5
+    del pos1, vel1, rs, ra
+    # the FP64 oracle on the same inputs
+    ph = [p.cpu().numpy().copy() for p in pos0]
+    vh = [v.cpu().numpy().copy() for v in vel0]
+    del pos0, vel0
+    _oracle_threads()
+    pyoracle.run_kdk(ph, vh, ng, nsteps, dt, pyoracle.GREEN_CONT_K2, 1.0)
+    ref = [torch.from_numpy(p) for p in ph]
+    rms64, max64 = _periodic_rms_max(final[pmhip.PM_FP64], ref, ng)
+    rms32, max32 = _periodic_rms_max(final[pmhip.PM_FP32], ref, ng)
+    print(f"FP64 vs oracle rms {rms64:.3e} max {max64:.3e}; FP32 vs oracle rms {rms32:.3e} max {max32:.3e}",
+          flush=True)
+    assert max64 < 1e-8
+    assert rms32 < 1e-4
+    assert max32 < 1e-3
+
+
+def test_config5_1024_mesh_reversal_and_gradient(dev):
+    """Forward 30 KDK steps of 512^3 particles on a 1024^3 mesh, then
+    pm_run_adjoint with the cotangent w of L = sum w . x_final.
+    FP32 mesh (sorted deposit, the benchmark path): the recomputed initial
+    positions are within 2e-6 cells rms and 5e-5 max of the true ones.
+    FP64 mesh (atomic deposit, smooth in x for differencing): reversal within
+    1e-9 cells max, and dL/dx_0 at 8 coordinates against central
+    differences (eps = 1e-4 cells) of two full forward runs each, to 1e-4
+    relative plus 1e-6 of the largest gradient component."""
+    import torch
+
+    print("case config5 512^3 on 1024^3 30 steps + adjoint", flush=True)
+    ng, nrow = 1024, 512
+    nsteps, dt = 30, 0.005
+    npart = nrow ** 3
+    pos0, vel0 = _zeldovich(nrow, ng, 1.0, 7, dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(8)
+    w = [torch.randn(npart, dtype=torch.float64, device=dev, generator=g) for _ in range(3)]
+    pmhip.range_errors()
+
+    def forward(plan, pos, vel, variant):
+        plan.run_kdk(pos, vel, nsteps, dt, variant)
+
+    for prec, variant in ((pmhip.PM_FP32, pmhip.PM_DEPOSIT_SORTED), (pmhip.PM_FP64, pmhip.PM_DEPOSIT_ATOMIC)):
+        with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+            pos, vel = [p.clone() for p in pos0], [v.clone() for v in vel0]
+            forward(plan, pos, vel, variant)
+            xfinal = [p.clone() for p in pos]
+            xbar = [t.clone() for t in w]
+            vbar = [torch.zeros_like(t) for t in w]
+            plan.run_adjoint(pos, vel, xbar, vbar, nsteps, dt, variant)
+            torch.cuda.synchronize()
+            rms, mx = _periodic_rms_max(pos, pos0, ng)
+            print(f"prec={prec}: reversal drift rms {rms:.3e} max {mx:.3e} cells", flush=True)
+            if prec == pmhip.PM_FP32:
+                assert rms < 2e-6 and mx < 5e-5
+                continue
+            assert mx < 1e-9
+            gmax = max(float(t.abs().max()) for t in xbar)
+            rng = np.random.default_rng(9)
+            eps, worst, checked = 1e-4, 0.0, 0
+            while checked < 8:
+                a, i = int(rng.integers(3)), int(rng.integers(npart))
+                # keep away from the periodic wrap of the loss and of the box
+                x0, xf = float(pos0[a][i]), float(xfinal[a][i])
+                if min(xf, ng - xf, x0, ng - x0) < 0.1:
+                    continue
+                runs = []
+                for sgn in (1.0, -1.0):
+                    p, v = [t.clone() for t in pos0], [t.clone() for t in vel0]
+                    p[a][i] += sgn * eps
+                    forward(plan, p, v, variant)
+                    runs.append(p)
+                d = 0.0
+                for b in range(3):
+                    dx = runs[0][b] - runs[1][b]
+                    dx = dx - ng * torch.round(dx / ng)
+                    d += float((w[b] * dx).sum())
+                fd = d / (2 * eps)
+                an = float(xbar[a][i])
+                err = abs(fd - an) / (1e-4 * max(1.0, abs(an)) + 1e-6 * gmax)
+                print(f"  dL/dx0[{a}][{i}]: adjoint {an:.8f} central difference {fd:.8f}", flush=True)
+                worst = max(worst, err)
+                checked += 1
+                del runs, p, v
+            print(f"central differences: worst error / tolerance {worst:.3f}", flush=True)
+            assert worst < 1.0
+    assert pmhip.range_errors() == 0

@@ -265,6 +265,21 @@ def test_zeldovich_ic_vs_numpy(dev, nrow, ng):
     assert abs(disp - 0.7) < 1e-9
 
 
+def test_zeldovich_ic_bitwise_reproducible(dev):
+    """The rms normalisation is summed in a fixed order (per-workgroup
+    partials, then one ordered reduction): the same noise gives the same
+    particles bit for bit, also at a size with thousands of partials."""
+    import torch
+
+    nrow, ng = 128, 128
+    print(f"case zeldovich reproducible nrow={nrow}", flush=True)
+    noise = torch.tensor(np.random.default_rng(5).standard_normal(nrow ** 3), device=dev)
+    a = pmhip.zeldovich_ic(nrow, ng, noise, 1.5, 3.0)
+    b = pmhip.zeldovich_ic(nrow, ng, noise, 1.5, 3.0)
+    for u, v in zip(a[0] + a[1], b[0] + b[1]):
+        assert torch.equal(u, v)
+
+
 # --------------------------------------------------------- sorted deposit
 def _plan_density(dev, ng, ph, prec, variant):
     import torch
