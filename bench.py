@@ -44,15 +44,15 @@ def parse():
     ap.add_argument("--no-alt-deposit", action="store_true", help="skip the leg with the other deposit variant")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
-    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-vs-FP64 accuracy leg")
-    ap.add_argument("--accuracy-steps", type=int, default=20)
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the GPU side of the accuracy leg")
+    ap.add_argument("--accuracy-steps", type=int, default=20,
+                    help="KDK steps of the accuracy leg and of the timed CPU baseline")
     ap.add_argument("--no-large-adjoint", action="store_true", help="skip the 1024^3-mesh adjoint leg")
     ap.add_argument("--large-nrow", type=int, default=512)
     ap.add_argument("--large-ng", type=int, default=1024)
     ap.add_argument("--large-steps", type=int, default=30)
     ap.add_argument("--no-tune", action="store_true", help="use the default FFT pass shapes (no pm_plan_tune)")
-    ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = nproc (honours OMP_NUM_THREADS)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (PMC byte-counter calibration)")
     return ap.parse_args()
@@ -237,10 +237,10 @@ def run_adjoint_leg(pos0, vel0, args, ng, npart, variant):
             "mode": "reverse-time leapfrog recomputation, FP32 mesh, FP64 positions"}
 
 
-def run_accuracy_leg(pos0, vel0, args, ng, npart, variant, nsteps=20):
-    """FP32 mesh path against the FP64 mesh path on the same ICs: density,
-    potential and force of the initial state (relative RMS), and the
-    final-position error after nsteps KDK steps (cells, periodic)."""
+def gpu_fields(pos0, vel0, args, ng, npart, variant, nsteps):
+    """Both GPU mesh paths on the same ICs: density contrast, potential and
+    per-particle force of the initial state, and the positions after nsteps
+    KDK steps (pm_run_kdk)."""
     import torch
     import pmhip
 
@@ -248,7 +248,7 @@ def run_accuracy_leg(pos0, vel0, args, ng, npart, variant, nsteps=20):
     for prec in (pmhip.PM_FP64, pmhip.PM_FP32):
         plan = pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0)
         plan.deposit(pos0, variant)
-        rho = plan.density().double().clone()
+        delta = plan.density().double() - 1.0
         plan.solve()
         phi = plan.potential().double().clone()
         g = [torch.empty_like(p) for p in pos0]
@@ -258,25 +258,119 @@ def run_accuracy_leg(pos0, vel0, args, ng, npart, variant, nsteps=20):
         plan.run_kdk(pos, vel, nsteps, args.dt, variant)
         torch.cuda.synchronize()
         plan.close()
-        fields[prec] = (rho, phi, g, pos)
-    r64, p64, g64, x64 = fields[pmhip.PM_FP64]
-    r32, p32, g32, x32 = fields[pmhip.PM_FP32]
+        del vel
+        fields[prec] = (delta, phi, g, pos)
+    return fields
+
+
+def host_threads():
+    """The host cores this process may use, as `nproc` counts them (it
+    honours OMP_NUM_THREADS and the affinity mask), and the machine's total."""
+    import subprocess
+
+    def count(cmd, fallback):
+        try:
+            return int(subprocess.run(cmd, capture_output=True, text=True, check=True).stdout.strip())
+        except (OSError, ValueError, subprocess.CalledProcessError):
+            return fallback
+
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return count(["nproc"], avail), count(["nproc", "--all"], os.cpu_count() or 1)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# The oracle's one-box step against the reference binary on the same 8 cores
+# of the build container (Intel Xeon @ 2.70 GHz, init_particles input,
+# OMP_NUM_THREADS=8): oracle/calibrate.py gives 1.451 s/step at 256^3 and
+# 12.673 s/step at 512^3; the reference (ffte1, P=1 x 8 threads, 20 steps
+# after 10 warm-up) took 1.229 and 10.94 s/step on that container.
+CALIBRATION = {"host": "build container, 8 cores, Intel Xeon @ 2.70 GHz", "threads": 8,
+               "oracle_s_per_step": {"256": 1.451, "512": 12.673},
+               "reference_s_per_step": {"256": 1.229, "512": 10.94},
+               "oracle_over_reference": {"256": round(1.451 / 1.229, 3), "512": round(12.673 / 10.94, 3)},
+               "how": "python oracle/calibrate.py --ng N --threads 8 --steps 3; reference: ffte1 build, "
+                      "mpirun -n 1, OMP_NUM_THREADS=8, main.cpp's 20 timed steps"}
+
+
+def oracle_leg(pos_dev, vel_dev, ng, args, nsteps):
+    """The FP64 oracle (C/OpenMP restatement of the reference step) on the same
+    ICs with OMP_NUM_THREADS = nproc.  Untimed warm-up: the initial state's
+    density, potential and force (it allocates the private grids).  Timed:
+    nsteps KDK steps (nsteps + 1 force evaluations, as pm_run_kdk) -- the CPU
+    baseline -- whose final positions are the accuracy reference."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+
+    nproc, nproc_all = host_threads()
+    threads = args.cpu_threads or nproc
+    pyoracle.set_threads(threads)
+    pos = [p.cpu().numpy().copy() for p in pos_dev]
+    vel = [v.cpu().numpy().copy() for v in vel_dev]
+    npart = pos[0].size
+    t0 = time.perf_counter()
+    delta = pyoracle.deposit_periodic(pos, ng)
+    phi = pyoracle.solve(delta, pyoracle.GREEN_CONT_K2, 1.0)
+    g = pyoracle.interp_force(pos, phi)
+    t_warm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pyoracle.run_kdk(pos, vel, ng, nsteps, args.dt, pyoracle.GREEN_CONT_K2, 1.0)
+    t = time.perf_counter() - t0
+    per_eval = t / (nsteps + 1)
+    cpu = {"value": npart / per_eval / 1e6, "unit": "Mparticle-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{nsteps} KDK steps ({nsteps + 1} force evaluations, each deposit + FFT solve + "
+                     f"interpolation + kick + drift) of the FP64 oracle from the same {round(npart ** (1 / 3))}^3 "
+                     f"ICs after one untimed force evaluation: {t:.2f} s, {per_eval:.3f} s per evaluation, "
+                     f"{threads} OpenMP threads",
+           "nproc": nproc, "nproc_all": nproc_all, "cpu_model": cpu_model(), "warmup_s": round(t_warm, 3),
+           "s_per_step": round(per_eval, 4), "calibration": CALIBRATION}
+    return cpu, (delta.reshape(ng, ng, ng), phi, g, pos)
+
+
+def accuracy_report(fields, ref, ng, nsteps, dt, dev):
+    """Errors of both GPU mesh paths against the FP64 oracle (relative rms of
+    the density contrast, potential and force of the initial state; final
+    positions in cells after nsteps steps), and of the FP32 path against the
+    GPU FP64 path."""
+    import torch
+    import pmhip
 
     def rel(a, b):
         return float(torch.sqrt(((a - b) ** 2).mean() / (b ** 2).mean()))
 
-    gnum = sum(float(((a - b) ** 2).mean()) for a, b in zip(g32, g64))
-    gden = sum(float((b ** 2).mean()) for b in g64)
-    d2 = None
-    for a, b in zip(x32, x64):
-        d = a - b
-        d = d - ng * torch.round(d / ng)
-        d2 = d * d if d2 is None else d2 + d * d
-    del fields
-    return {"steps": nsteps, "dt": args.dt, "density_rel_rms": rel(r32, r64), "potential_rel_rms": rel(p32, p64),
-            "force_rel_rms": (gnum / gden) ** 0.5, "final_position_rms_cells": float(torch.sqrt(d2.mean())),
-            "final_position_max_cells": float(torch.sqrt(d2.max())), "target_rms_cells": 1e-4,
-            "against": "FP64 mesh/FFT/force path on the GPU (the same step as the FP64 oracle)"}
+    def errors(f, r):
+        d, p, g, x = f
+        rd, rp, rg, rx = r
+        gnum = sum(float(((a - b) ** 2).mean()) for a, b in zip(g, rg))
+        gden = sum(float((b ** 2).mean()) for b in rg)
+        d2 = None
+        for a, b in zip(x, rx):
+            e = a - b
+            e = e - ng * torch.round(e / ng)
+            d2 = e * e if d2 is None else d2 + e * e
+        return {"density_rel_rms": rel(d, rd), "potential_rel_rms": rel(p, rp), "force_rel_rms": (gnum / gden) ** 0.5,
+                "final_position_rms_cells": float(torch.sqrt(d2.mean())),
+                "final_position_max_cells": float(torch.sqrt(d2.max()))}
+
+    rd, rp, rg, rx = ref
+    oref = (torch.from_numpy(rd).to(dev), torch.from_numpy(rp).to(dev), [torch.from_numpy(v).to(dev) for v in rg],
+            [torch.from_numpy(v).to(dev) for v in rx])
+    out = {"against": "FP64 oracle (C/OpenMP restatement of the reference step) on the same ICs", "steps": nsteps,
+           "dt": dt, "target_rms_cells": 1e-4,
+           "fp32": errors(fields[pmhip.PM_FP32], oref), "fp64": errors(fields[pmhip.PM_FP64], oref),
+           "fp32_vs_gpu_fp64": errors(fields[pmhip.PM_FP32], fields[pmhip.PM_FP64])}
+    del oref
+    return out
 
 
 def run_large_adjoint_leg(args, dev):
@@ -330,42 +424,6 @@ def run_large_adjoint_leg(args, dev):
     return out
 
 
-def cpu_baseline(pos_dev, vel_dev, ng, args):
-    """The FP64 oracle (C/OpenMP restatement of the reference step) on the
-    same ICs, args.cpu_steps steps, timed on the host cores."""
-    import numpy as np
-
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import pyoracle
-
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    pyoracle.set_threads(threads)
-    pos = [p.cpu().numpy().copy() for p in pos_dev]
-    vel = [v.cpu().numpy().copy() for v in vel_dev]
-    npart = pos[0].size
-    mass = ng ** 3 / npart
-    delta = np.zeros((ng, ng, ng))
-    # one untimed pass so that the oracle's work buffers are allocated
-    delta[:] = pyoracle.deposit_periodic(pos, ng, mass)
-    t0 = time.perf_counter()
-    for s in range(args.cpu_steps):
-        delta = pyoracle.deposit_periodic(pos, ng, mass)
-        phi = pyoracle.solve(delta, pyoracle.GREEN_CONT_K2, 1.0)
-        pyoracle.interp_kick_drift(pos, vel, phi, args.dt, args.dt)
-    t = (time.perf_counter() - t0) / args.cpu_steps
-    model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return {"value": npart / t / 1e6, "unit": "Mparticle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{args.cpu_steps} step(s) of the FP64 oracle (deposit, FFT solve, interpolation+kick+drift) "
-                      f"on the same {round(npart ** (1 / 3))}^3 ICs, {t:.2f} s/step, {threads} OpenMP threads, {model}"}
-
-
 def main():
     args = parse()
     if args.gpus != 1:
@@ -401,11 +459,10 @@ def main():
         log(f"FP64: {ms64:.3f} ms/step, phases {ph64}")
         res64 = {"value": npart / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "phases_ms": ph64}
 
-    acc = None
+    fields = None
     if not args.no_accuracy:
-        log(f"accuracy leg (FP32 vs FP64, {args.accuracy_steps} KDK steps)")
-        acc = run_accuracy_leg(pos0, vel0, args, ng, npart, variant, args.accuracy_steps)
-        log(f"accuracy: {acc}")
+        log(f"accuracy leg: both GPU mesh paths, initial fields and {args.accuracy_steps} KDK steps")
+        fields = gpu_fields(pos0, vel0, args, ng, npart, variant, args.accuracy_steps)
 
     adj = None
     if not args.no_adjoint:
@@ -439,6 +496,11 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes": nbytes[dom], "avg_ms": round(ph32[dom], 4),
                 "traffic_source": f"profiles/pmc_traffic_{ng}.txt" if traffic else None}
+    # the same kernel priced without its staging record (the step's compulsory
+    # bytes only: b N_c + 96 N_p for the interpolation, 24 N_p + b N_c for the deposit)
+    unstaged = algorithmic_bytes(ng, npart, pmhip.PM_FP32, False)[dom]
+    roofline["algorithmic_bytes_unstaged"] = unstaged
+    roofline["frac_unstaged"] = round(unstaged / (ph32[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     step_bytes = sum(nbytes.values())
     roofline64 = None
     if res64:
@@ -452,11 +514,19 @@ def main():
                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
                       "traffic": tr64, "algorithmic_bytes": nb64[dom64], "avg_ms": round(ph64[dom64], 4),
                       "traffic_source": f"profiles/pmc_traffic_{ng}_fp64_adjoint.txt" if tr64 else None}
-    cpu = None
+        un64 = algorithmic_bytes(ng, npart, pmhip.PM_FP64, False)[dom64]
+        roofline64["algorithmic_bytes_unstaged"] = un64
+        roofline64["frac_unstaged"] = round(un64 / (ph64[dom64] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    cpu, acc = None, None
     if not args.no_cpu:
-        log("CPU baseline leg")
-        cpu = cpu_baseline(pos0, vel0, ng, args)
-        log(f"CPU: {cpu['value']:.2f} Mparticle-steps/s")
+        log(f"CPU baseline leg: the FP64 oracle, {args.accuracy_steps} KDK steps on the host cores")
+        cpu, ref = oracle_leg(pos0, vel0, ng, args, args.accuracy_steps)
+        log(f"CPU: {cpu['value']:.2f} Mparticle-steps/s ({cpu['cores']} threads)")
+        if fields is not None:
+            acc = accuracy_report(fields, ref, ng, args.accuracy_steps, args.dt, dev)
+            log(f"accuracy: {acc}")
+        del ref
+    fields = None
     value = npart / (ms32 * 1e-3) / 1e6
     out = {
         "metric": "Mparticle-steps/s at 512^3 (FP32 vs FP64 PM, 1 GPU)",
@@ -486,7 +556,7 @@ def main():
         "fft_shape": {"fp32": shape32, "fp64": shape64, "tuned": not args.no_tune,
                       "slots": ["z_fwd", "y_fwd", "x_green", "y_inv", "z_inv"]},
         "alt_deposit": alt,
-        "accuracy_vs_fp64": acc,
+        "accuracy": acc,
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "adjoint": adj,
         "large_adjoint": large,

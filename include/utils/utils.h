@@ -3,6 +3,8 @@
 // kernels of the C ABI (include/pm_api.h).  They run on the default stream
 // and throw std::runtime_error when a kernel launch fails.
 #pragma once
+#include <mpi.h>
+
 #include <cstddef>
 
 constexpr double Omega_0 = 1.0;
@@ -13,9 +15,9 @@ constexpr double Da = 0.001, Start_a = 0.9;
 
 // Wall-clock seconds since the previous call (utils.cpp:3-10).
 double getTime(double* now_time);
-// Sum of phase times after the warm-up (utils.cpp:12-20); one rank, so no
-// MPI reduction.
-void report_max_time(double local_sec, const int i, const int worm_up, double& sum_time);
+// Sum of phase times after the warm-up (utils.cpp:12-20): the maximum over
+// the ranks of comm, added on its rank 0.
+void report_max_time(double local_sec, MPI_Comm comm, const int i, const int worm_up, double& sum_time);
 // utils.cpp:22-36 on the host (mt19937_64 seeds 3r, 3r+1, 3r+2), copied to
 // the device arrays.
 void init_particles(double* posx, double* posy, double* posz, size_t Np_local, int x0, int y0, int z0, int nx_loc,

@@ -17,10 +17,22 @@ void check(int rc, const char* what) {
 void require(bool ok, const char* msg) {
     if (!ok) throw std::invalid_argument(msg);
 }
+// The transposes exchange between the ranks of comm; one GPU owns the whole
+// box, so comm must be a single rank.
+void require_one_rank(MPI_Comm comm, const char* who) {
+    int init = 0;
+    MPI_Initialized(&init);
+    if (!init) throw std::logic_error(std::string(who) + ": MPI is not initialised");
+    int size = 0;
+    if (comm == MPI_COMM_NULL || MPI_Comm_size(comm, &size) != MPI_SUCCESS || size != 1)
+        throw std::invalid_argument(std::string(who) + ": one GPU owns the whole box; comm must hold one rank");
+}
 }  // namespace
 
 TransposeSlab::TransposeSlab(int Ng, int nx, int ny, int nz, int x0, int dimsx, const std::vector<ptrdiff_t>& all_i0,
-                             const std::vector<ptrdiff_t>& all_n0) {
+                             const std::vector<ptrdiff_t>& all_n0, MPI_Comm comm)
+    : comm_(comm) {
+    require_one_rank(comm, "TransposeSlab");
     require(Ng >= 2 && nx == Ng + 1 && ny == Ng + 1 && nz == Ng + 1 && x0 == 0 && dimsx == 1,
             "TransposeSlab: one box expects nx = ny = nz = Ng + 1, x0 = 0, dimsx = 1");
     require(all_i0.size() == 1 && all_n0.size() == 1 && all_i0[0] == 0 && all_n0[0] == Ng,
@@ -68,8 +80,9 @@ void ReorderSlabBack::execute(const double* slab_in, double* cube_out) const {
 }
 
 TransposeSlabBack::TransposeSlabBack(int Ng, int color, int nx_loc, int ny1, int nz1, std::array<int, 3> dims,
-                                     ptrdiff_t local_0_start, ptrdiff_t local_n0)
-    : ng_(Ng) {
+                                     ptrdiff_t local_0_start, ptrdiff_t local_n0, MPI_Comm comm)
+    : ng_(Ng), comm_(comm) {
+    require_one_rank(comm, "TransposeSlabBack");
     require(Ng >= 2 && color == 1 && nx_loc == Ng && ny1 == Ng + 3 && nz1 == Ng + 3 && local_0_start == 0 &&
                 local_n0 == Ng,
             "TransposeSlabBack: one box expects color 1, nx_loc = Ng, ny1 = nz1 = Ng + 3");

@@ -21,6 +21,16 @@ std::string wisdom_file(int n) {
     return dir && *dir ? std::string(dir) + "/" + name : name;
 }
 
+// The one-rank world the one-GPU build runs in.
+int world_size() {
+    int init = 0;
+    MPI_Initialized(&init);
+    if (!init) throw std::logic_error("FFT: MPI is not initialised (construct MPIEnv first, as main.cpp:10 does)");
+    int size = 0;
+    MPI_Comm_size(MPI_COMM_WORLD, &size);
+    return size;
+}
+
 bool readable(const std::string& path) {
     std::FILE* fp = std::fopen(path.c_str(), "r");
     if (fp) std::fclose(fp);
@@ -31,14 +41,26 @@ bool readable(const std::string& path) {
 // fftw/fft.cpp:28-49: import the wisdom file if it is there; otherwise
 // measure (here: time every FFT pass shape on buf, which is overwritten as
 // FFTW_EXHAUSTIVE planning overwrites its arrays) and export at destruction.
-FFT::FFT(int n, double* buf) : n_(n), real_(buf), h_(nullptr), filename_(wisdom_file(n)), need_export_(false) {
+FFT::FFT(int n, double* buf)
+    : n_(n), real_(buf), h_(nullptr), comm_(MPI_COMM_NULL), filename_(wisdom_file(n)), need_export_(false) {
     if (!buf) throw std::invalid_argument("FFT: null buffer");
+    // ffte1/fft.cpp:4-8: ranks below n/2 form the FFT communicator; one GPU
+    // holds the whole box, so the world is that one rank
+    if (world_size() != 1) throw std::invalid_argument("FFT: one GPU owns the whole box; run one MPI rank");
+    int rank = 0;
+    MPI_Comm_rank(MPI_COMM_WORLD, &rank);
+    MPI_Comm_split(MPI_COMM_WORLD, 1, rank, &comm_);
     const bool loaded = readable(filename_) && pm_wisdom_import(filename_.c_str()) == PM_OK;
-    check(pm_fft_create(&h_, n, PM_FP64, nullptr), "FFT::FFT");
+    const int rc = pm_fft_create(&h_, n, PM_FP64, nullptr);
+    if (rc != PM_OK) {
+        MPI_Comm_free(&comm_);
+        check(rc, "FFT::FFT");
+    }
     if (!loaded) {
         const int rc = pm_fft_tune(h_, buf, 5, nullptr);
         if (rc != PM_OK) {
             pm_fft_destroy(h_);
+            MPI_Comm_free(&comm_);
             check(rc, "FFT::FFT tune");
         }
         need_export_ = true;
@@ -49,8 +71,10 @@ FFT::FFT(int n, double* buf) : n_(n), real_(buf), h_(nullptr), filename_(wisdom_
 FFT::~FFT() {
     if (need_export_) (void)pm_wisdom_export(filename_.c_str());
     pm_fft_destroy(h_);
+    MPI_Comm_free(&comm_);  // ffte1/fft.cpp:39-44
 }
 
+MPI_Comm FFT::comm() const { return comm_; }
 int FFT::color() const { return 1; }
 
 // complex elements of the half spectrum (ffte1/fft.cpp:13-19 at one rank)

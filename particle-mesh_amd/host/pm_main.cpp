@@ -1,9 +1,10 @@
 // One-box PM step driver on one MI355X: the reference's src/core/main.cpp
 // (main.cpp:9-152) with its 9-phase loop, rebuilt on the drop-in classes
 // (FFT, TransposeSlab, ReorderSlab, ReorderSlabBack, TransposeSlabBack and
-// the utils functions) over device buffers.  One rank owns the whole box,
-// so there is no MPI; every phase is timed as wall clock around a device
-// synchronisation, as report_max_time times the reference's phases.
+// the utils functions) over device buffers.  It runs as one MPI rank that
+// owns the whole box (MPIEnv, the communicators of main.cpp:46,61-62 and
+// report_max_time as in the reference; more ranks are rejected); every phase
+// is timed as wall clock around a device synchronisation.
 //
 //   pm_main Ng Nrow [--warmup W] [--loop L] [--dump FILE]
 //
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "fft.h"
+#include "mpi_env.h"
 #include "reorder_slab.h"
 #include "reorder_slab_back.h"
 #include "transpose_slab.h"
@@ -39,7 +41,23 @@ void sync() {
 }
 }  // namespace
 
-int main(int argc, char** argv) try {
+int run(int argc, char** argv);
+
+int main(int argc, char** argv) {
+    MPIEnv mpi(argc, argv);  // main.cpp:10
+    try {
+        return run(argc, argv);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "pm_main: %s\n", e.what());
+        return 1;
+    }
+}
+
+int run(int argc, char** argv) {
+    int world_rank = 0, world_size = 0;
+    MPI_Comm_rank(MPI_COMM_WORLD, &world_rank);
+    MPI_Comm_size(MPI_COMM_WORLD, &world_size);
+    if (world_size != 1) throw std::invalid_argument("one GPU owns the whole box: run one MPI rank");
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s Ng Nrow [--warmup W] [--loop L] [--dump FILE]\n", argv[0]);
         return 2;
@@ -73,11 +91,12 @@ int main(int argc, char** argv) try {
     double* real_buf3 = dalloc(ncell_local);
 
     FFT fft(Ng, slab);
+    MPI_Comm fft_comm = fft.comm();
     const int color = fft.color();
     const ptrdiff_t local_n0 = fft.local_n0(), local_0_start = fft.local_0_start();
     std::vector<ptrdiff_t> all_i0{local_0_start}, all_n0{local_n0};
-    TransposeSlab trans(Ng, nx, ny, nz, 0, dims3[0], all_i0, all_n0);
-    TransposeSlabBack trans_back(Ng, color, nx_loc, ny1, nz1, dims3, local_0_start, local_n0);
+    TransposeSlab trans(Ng, nx, ny, nz, 0, dims3[0], all_i0, all_n0, MPI_COMM_WORLD);
+    TransposeSlabBack trans_back(Ng, color, nx_loc, ny1, nz1, dims3, local_0_start, local_n0, MPI_COMM_WORLD);
     auto reorder = std::make_unique<ReorderSlab>(Ng, ny, nz, local_n0, stride, trans.recvcounts(), trans.rdispls(),
                                                  dims3);
     auto reorder_back = std::make_unique<ReorderSlabBack>(Ng, ny_loc, nz_loc, stride, dims3, local_n0);
@@ -88,7 +107,7 @@ int main(int argc, char** argv) try {
     double* px = dalloc(Np);
     double* py = dalloc(Np);
     double* pz = dalloc(Np);
-    init_particles(posx, posy, posz, Np, 0, 0, 0, nx_loc, ny_loc, nz_loc, 0);
+    init_particles(posx, posy, posz, Np, 0, 0, 0, nx_loc, ny_loc, nz_loc, world_rank);
     if (hipMemset(px, 0, Np * sizeof(double)) != hipSuccess || hipMemset(py, 0, Np * sizeof(double)) != hipSuccess ||
         hipMemset(pz, 0, Np * sizeof(double)) != hipSuccess)
         throw std::runtime_error("hipMemset failed");
@@ -105,39 +124,40 @@ int main(int argc, char** argv) try {
         deposit_particles(posx, posy, posz, Np, real_buf, ncell_local, nx, ny, nz, 0, 0, 0, mass);
         reduce_private_to_shared(real_buf, ncell_local, 1);
         sync();
-        report_max_time(getTime(&now_time), i, worm_up, exe_time[0]);
+        report_max_time(getTime(&now_time), MPI_COMM_WORLD, i, worm_up, exe_time[0]);
 
         trans.execute(real_buf, recv_buf);
         sync();
-        report_max_time(getTime(&now_time), i, worm_up, exe_time[1]);
+        report_max_time(getTime(&now_time), MPI_COMM_WORLD, i, worm_up, exe_time[1]);
 
         if (color == 1) {
             reorder->execute(recv_buf, slab);
             sync();
-            report_max_time(getTime(&now_time), i, worm_up, exe_time[2]);
+            report_max_time(getTime(&now_time), fft_comm, i, worm_up, exe_time[2]);
             fft.forward();
             sync();
-            report_max_time(getTime(&now_time), i, worm_up, exe_time[3]);
+            report_max_time(getTime(&now_time), fft_comm, i, worm_up, exe_time[3]);
             fft.apply_green(a);
             sync();
-            report_max_time(getTime(&now_time), i, worm_up, exe_time[4]);
+            report_max_time(getTime(&now_time), fft_comm, i, worm_up, exe_time[4]);
             fft.backward();
             sync();
-            report_max_time(getTime(&now_time), i, worm_up, exe_time[5]);
+            report_max_time(getTime(&now_time), fft_comm, i, worm_up, exe_time[5]);
             reorder_back->execute(slab, yz_buf);
             sync();
-            report_max_time(getTime(&now_time), i, worm_up, exe_time[6]);
+            report_max_time(getTime(&now_time), fft_comm, i, worm_up, exe_time[6]);
         }
         trans_back.execute(yz_buf, halo);
         sync();
-        report_max_time(getTime(&now_time), i, worm_up, exe_time[7]);
+        report_max_time(getTime(&now_time), MPI_COMM_WORLD, i, worm_up, exe_time[7]);
 
         centered_derivatives(halo, real_buf1, real_buf2, real_buf3, nx1, ny1, nz1);
         update_particles(posx, posy, posz, real_buf1, real_buf2, real_buf3, px, py, pz, a, Np, ny, nz, 0, 0, 0);
         sync();
-        report_max_time(getTime(&now_time), i, worm_up, exe_time[8]);
+        report_max_time(getTime(&now_time), MPI_COMM_WORLD, i, worm_up, exe_time[8]);
     }
-    for (double t : exe_time) std::cout << (loop > 0 ? t / loop : 0.0) << std::endl;
+    if (world_rank == 0)
+        for (double t : exe_time) std::cout << (loop > 0 ? t / loop : 0.0) << std::endl;
 
     if (!dump.empty()) {
         std::vector<double> h(3 * Np);
@@ -153,7 +173,4 @@ int main(int argc, char** argv) try {
                       py, pz})
         (void)hipFree(p);
     return 0;
-} catch (const std::exception& e) {
-    std::fprintf(stderr, "pm_main: %s\n", e.what());
-    return 1;
 }

@@ -26,9 +26,14 @@ double getTime(double* now_time) {
     return *now_time - told;
 }
 
-void report_max_time(double local_sec, const int i, const int worm_up, double& sum_time) {
+void report_max_time(double local_sec, MPI_Comm comm, const int i, const int worm_up, double& sum_time) {
     if (i < worm_up) return;
-    sum_time += local_sec;
+    double global_max = 0.0;
+    MPI_Reduce(&local_sec, &global_max, 1, MPI_DOUBLE, MPI_MAX, 0, comm);
+    int rank = 0;
+    MPI_Comm_rank(comm, &rank);
+    if (rank == 0) sum_time += global_max;
+    MPI_Barrier(comm);
 }
 
 void init_particles(double* posx, double* posy, double* posz, size_t Np_local, int x0, int y0, int z0, int nx_loc,

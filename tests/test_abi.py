@@ -123,3 +123,34 @@ def test_snapshot_header_parsed_by_library_and_numpy(tmp_path):
         assert e.value.code == pmhip.PM_ERR_ARG
     with pytest.raises(pmhip.PMError):
         pmhip.snapshot_info(tmp_path / "missing.bin")
+
+
+def test_driver_rejects_more_than_one_rank():
+    """The drop-in driver keeps the reference's MPI set-up (MPIEnv,
+    MPI_COMM_WORLD, FFT::comm()) and runs as one rank owning the whole box:
+    under two ranks it stops before touching the GPU."""
+    exe = ROOT / "particle-mesh_amd" / "bin" / "pm_main"
+    mpirun = Path("/opt/conda/bin/mpirun")
+    if not exe.exists() or not mpirun.exists():
+        pytest.skip("driver or mpirun not present")
+    r = subprocess.run([str(mpirun), "-n", "2", str(exe), "16", "16"], capture_output=True, text=True, timeout=60,
+                       cwd="/tmp")
+    assert r.returncode != 0
+    assert "one MPI rank" in r.stderr
+
+
+def test_drop_in_headers_keep_the_reference_communicators():
+    """The signatures a reference caller binds: FFT::comm(), the MPI_Comm of
+    TransposeSlab / TransposeSlabBack (transpose_slab.h:6,
+    transpose_slab_back.h:10) and report_max_time (utils.h:4)."""
+    inc = ROOT / "include"
+    fft = (inc / "fft" / "hip" / "fft.h").read_text()
+    assert re.search(r"MPI_Comm\s+comm\(\)\s+const;", fft)
+    ts = (inc / "decomp" / "slab" / "transpose_slab.h").read_text()
+    assert re.search(r"const std::vector<ptrdiff_t>& all_n0, MPI_Comm comm\);", ts)
+    tsb = (inc / "decomp" / "slab" / "transpose_slab_back.h").read_text()
+    assert "MPI_Comm comm = MPI_COMM_WORLD" in tsb
+    ut = (inc / "utils" / "utils.h").read_text()
+    assert re.search(r"report_max_time\(double local_sec, MPI_Comm comm, const int i, const int worm_up, "
+                     r"double& sum_time\);", ut)
+    assert (inc / "utils" / "mpi_env.h").exists()
