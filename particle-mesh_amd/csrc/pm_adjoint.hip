@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kVec3Threads) void k_deposit_vec3_tiles(
                     const int n = n0 + ((m >> 2) & 1) * E * E + ((m >> 1) & 1) * E + (m & 1);
 #pragma unroll
                     for (int a = 0; a < 3; ++a)
-                        atomicAdd(&acc[a][n], (unsigned long long)__double2ll_rn(w[m] * u[j][a]));
+                        atomicAdd(&acc[a][n], (unsigned long long)fix_rne(w[m] * u[j][a]));
                 }
             }
         }
@@ -242,15 +242,19 @@ constexpr int kDivThreads = PM_DIV_THREADS;
 template <typename Rl, int TS>
 __global__ __launch_bounds__(kDivThreads) void k_deposit_div_tiles(
     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
-    const void* __restrict__ stage, const double* __restrict__ ux, const double* __restrict__ uy,
-    const double* __restrict__ uz, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ off,
+    const void* __restrict__ stage, const void* __restrict__ ust, const double* __restrict__ ux,
+    const double* __restrict__ uy, const double* __restrict__ uz, const uint32_t* __restrict__ idx,
+    const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ cnt, int ng, const unsigned long long* __restrict__ umax, Rl* __restrict__ mu,
     Rl* __restrict__ shell, const uint32_t* __restrict__ gate, int run_if) {
-    // particles of tile t as in k_deposit_vec3_tiles (buckets with records,
-    // or a binning with the records formed from the positions)
+    // particles of tile t: buckets with records and staged cotangents
+    // (stage, ust), or a binning (idx) with the records formed from the
+    // positions and u read by index, rounded to the mesh precision as the
+    // staging rounds it
     if (gate && ((*gate != 0u) != (run_if != 0))) return;
     using St = typename std::conditional<sizeof(Rl) == 4, StageQ, StageD>::type;
     const St* __restrict__ sp = static_cast<const St*>(stage);
+    const Real3<Rl>* __restrict__ su = static_cast<const Real3<Rl>*>(ust);
     constexpr int H = kDivHalo, E = TS + H, E3 = E * E * E, NT = kDivThreads;
     __shared__ unsigned long long acc[E3];
     const WalkSeg sw = walk_segment(ng, TS);
@@ -265,32 +269,44 @@ __global__ __launch_bounds__(kDivThreads) void k_deposit_div_tiles(
         const uint32_t beg = off[t], end = cnt ? beg + cnt[t] : off[t + 1];
         constexpr int B = PM_DIV_BATCH;  // particles in flight per thread
         for (uint32_t q0 = beg + threadIdx.x; q0 < end; q0 += NT * B) {
-            uint32_t p[B];
+            bool in[B];
             St r[B];
             double u[B][3];
 #pragma unroll
             for (int j = 0; j < B; ++j) {
                 const uint32_t q = q0 + NT * j;
-                p[j] = q < end ? idx[q] : 0xFFFFFFFFu;
-                if (sp && q < end) r[j] = sp[q];
+                in[j] = q < end;
+                if (!in[j]) continue;
+                if (sp) {
+                    r[j] = sp[q];
+                    const Real3<Rl> w = su[q];
+                    u[j][0] = (double)w.x;
+                    u[j][1] = (double)w.y;
+                    u[j][2] = (double)w.z;
+                } else {
+                    const uint32_t p = idx[q];
+                    r[j] = tile_record_of<St>(px[p] - ox, py[p] - oy, pz[p] - oz);
+                    u[j][0] = (double)(Rl)ux[p];
+                    u[j][1] = (double)(Rl)uy[p];
+                    u[j][2] = (double)(Rl)uz[p];
+                }
             }
 #pragma unroll
             for (int j = 0; j < B; ++j) {
-                if (p[j] == 0xFFFFFFFFu) continue;
-                if (!sp) r[j] = tile_record_of<St>(px[p[j]] - ox, py[p[j]] - oy, pz[p[j]] - oz);
-                u[j][0] = ux[p[j]] * hs;
-                u[j][1] = uy[p[j]] * hs;
-                u[j][2] = uz[p[j]] * hs;
+                if (!in[j]) continue;
+                u[j][0] *= hs;
+                u[j][1] *= hs;
+                u[j][2] *= hs;
             }
 #pragma unroll
             for (int j = 0; j < B; ++j) {
-                if (p[j] == 0xFFFFFFFFu) continue;
+                if (!in[j]) continue;
                 const TileOffs o = tile_offs(r[j]);
                 const double wx[2] = {1.0 - o.fx, o.fx}, wy[2] = {1.0 - o.fy, o.fy}, wz[2] = {1.0 - o.fz, o.fz};
                 // node (dx, dy, dz) relative to the cell's lower corner
                 unsigned long long* const n0 = acc + ((o.a + 1) * E + (o.b + 1)) * E + (o.c + 1);
                 auto add = [&](int dx, int dy, int dz, double v) {
-                    atomicAdd(n0 + (dx * E + dy) * E + dz, (unsigned long long)__double2ll_rn(v));
+                    atomicAdd(n0 + (dx * E + dy) * E + dz, (unsigned long long)fix_rne(v));
                 };
 #pragma unroll
                 for (int e = 0; e < 2; ++e)
@@ -444,6 +460,7 @@ __global__ __launch_bounds__(NT) void k_adj_gather(double* __restrict__ px, doub
     __shared__ uint32_t hkey[HS], hcnt[HS], hbase[HS];
     uint32_t kk = kNo;  // STAGE: tile of the undrifted position
     Rec rec{};
+    double nu_st[3] = {0.0, 0.0, 0.0};  // STAGE: its cotangent u
     const int64_t i = (int64_t)band_remap(blockIdx.x, gridDim.x, bm) * NT + threadIdx.x;
     const double box = (double)N;
     // three load phases, as in k_interp: the particle's streams, then the
@@ -516,7 +533,10 @@ __global__ __launch_bounds__(NT) void k_adj_gather(double* __restrict__ px, doub
                 stream_store(uy + i, nu[1]);
                 stream_store(uz + i, nu[2]);
 #pragma unroll
-                for (int d = 0; d < 3; ++d) m[d] = fabs(nu[d]);
+                for (int d = 0; d < 3; ++d) {
+                    m[d] = fabs(nu[d]);
+                    nu_st[d] = nu[d];
+                }
             }
         }
     }
@@ -524,7 +544,7 @@ __global__ __launch_bounds__(NT) void k_adj_gather(double* __restrict__ px, doub
         const int64_t pos = stage_pass<HS, NT>(kk, hkey, hcnt, hbase, stg.fill, stg.coff, stg.spill);
         if (pos >= 0) {
             static_cast<Rec*>(stg.rec)[pos] = rec;
-            stg.idx[pos] = (uint32_t)i;
+            static_cast<Real3<Rl>*>(stg.u)[pos] = Real3<Rl>{(Rl)nu_st[0], (Rl)nu_st[1], (Rl)nu_st[2]};
         }
     }
     if constexpr (MODE == 2) wg_max_to_slots<NT>(m, umax);
@@ -604,7 +624,7 @@ int launch_deposit_vec3(int prec, const double* px, const double* py, const doub
 int launch_deposit_div(int prec, const double* px, const double* py, const double* pz, const double* ux,
                        const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s, void* mu,
                        void* shell, unsigned long long* umax, unsigned long long* err, hipStream_t st,
-                       bool umax_ready, const uint32_t* bucket_idx) {
+                       bool umax_ready, const void* bucket_u) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
@@ -616,15 +636,15 @@ int launch_deposit_div(int prec, const double* px, const double* py, const doubl
         hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(kMaxSlots), 0, st, umax, err);
     }
     // with buckets: the bucket pass, or (overflow) the binning pass
-#define PM_DIV_PASS(RL, TSV, REC, IDX, OFF, CNT, GATE, RUN)                                                      \
+#define PM_DIV_PASS(RL, TSV, REC, UST, IDX, OFF, CNT, GATE, RUN)                                                 \
     hipLaunchKernelGGL((k_deposit_div_tiles<RL, TSV>), dim3(ntiles / walk_seg(nt)), dim3(kDivThreads), 0, st, px, \
-                       py, pz, REC, ux, uy, uz, IDX, OFF, CNT, ng, umax, (RL*)mu, (RL*)shell, GATE, RUN)
+                       py, pz, REC, UST, ux, uy, uz, IDX, OFF, CNT, ng, umax, (RL*)mu, (RL*)shell, GATE, RUN)
 #define PM_DIV(RL, TSV)                                                                                     \
-    if (bucket_idx) {                                                                                       \
-        PM_DIV_PASS(RL, TSV, s.stage, bucket_idx, s.coff, s.fill, s.spill, 0);                              \
-        PM_DIV_PASS(RL, TSV, nullptr, s.idx, s.off, nullptr, s.spill, 1);                                   \
+    if (bucket_u) {                                                                                         \
+        PM_DIV_PASS(RL, TSV, s.stage, bucket_u, nullptr, s.coff, s.fill, s.spill, 0);                       \
+        PM_DIV_PASS(RL, TSV, nullptr, nullptr, s.idx, s.off, nullptr, s.spill, 1);                          \
     } else {                                                                                                \
-        PM_DIV_PASS(RL, TSV, nullptr, s.idx, s.off, nullptr, nullptr, 0);                                   \
+        PM_DIV_PASS(RL, TSV, nullptr, nullptr, s.idx, s.off, nullptr, nullptr, 0);                          \
     }
     if (prec == PM_FP32) {
         if (ts == 16) {
@@ -665,7 +685,7 @@ int launch_adj_gather(int prec, int mode, double* px, double* py, double* pz, do
                       const AdjStage* stage) {
     if (np <= 0) return PM_OK;
     if (mode == 2 && !umax) return fail(PM_ERR_ARG, "adjoint gather: no max buffer");
-    if (stage && (mode != 2 || !stage->rec || !stage->idx)) return fail(PM_ERR_ARG, "adjoint gather: bad staging");
+    if (stage && (mode != 2 || !stage->rec || !stage->u)) return fail(PM_ERR_ARG, "adjoint gather: bad staging");
     const int nt = prec == PM_FP32 ? kAdjThreads32 : kAdjThreads64;
     const unsigned nb = (unsigned)((np + nt - 1) / nt);
     const AdjStage stg = stage ? *stage : AdjStage();

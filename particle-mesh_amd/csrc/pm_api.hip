@@ -200,7 +200,7 @@ extern "C" int pm_plan_destroy(pm_plan_t* p) {
                       p->sorted.fill,  p->sorted.coff,   p->sorted.spill, p->sorted.stage};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    void* abufs[7] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell, p->adjMax, p->adjIdx};
+    void* abufs[7] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell, p->adjMax, p->adjUst};
     for (void* b : abufs)
         if (b) (void)hipFree(b);
     for (auto& e : p->ev)
@@ -517,9 +517,9 @@ static int ensure_adjoint(pm_plan* p) {
     const size_t nmax = 3 + 3 * (size_t)kMaxSlots;
     if (!rc) rc = hip_ok(hipMalloc((void**)&p->adjMax, nmax * sizeof(unsigned long long)), "adjoint: hipMalloc");
     if (!rc) rc = launch_fill(PM_FP64, p->adjMax, nmax, 0.0, p->stream);  // 0.0 has all bits zero
-    // index buckets for the staging of the reverse sweep (beside the staging records)
+    // cotangent buckets for the staging of the reverse sweep (beside the staging records)
     if (!rc && p->sorted.stage)
-        rc = hip_ok(hipMalloc((void**)&p->adjIdx, 4 * p->sorted.cap_total), "adjoint: hipMalloc");
+        rc = hip_ok(hipMalloc(&p->adjUst, 3 * rs * p->sorted.cap_total), "adjoint: hipMalloc");
     if (!rc) rc = hip_ok(hipStreamCreateWithFlags(&p->adjStream, hipStreamNonBlocking), "adjoint: stream");
     if (!rc) rc = hip_ok(hipEventCreateWithFlags(&p->adjFork, hipEventDisableTiming), "adjoint: event");
     if (!rc) rc = hip_ok(hipEventCreateWithFlags(&p->adjJoin, hipEventDisableTiming), "adjoint: event");
@@ -528,13 +528,13 @@ static int ensure_adjoint(pm_plan* p) {
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (p->adjMax) (void)hipFree(p->adjMax);
-        if (p->adjIdx) (void)hipFree(p->adjIdx);
+        if (p->adjUst) (void)hipFree(p->adjUst);
         if (p->adjFork) (void)hipEventDestroy(p->adjFork);
         if (p->adjJoin) (void)hipEventDestroy(p->adjJoin);
         if (p->adjStream) (void)hipStreamDestroy(p->adjStream);
         p->adjU[0] = p->adjU[1] = p->adjU[2] = p->adjShell = p->adjLam = nullptr;
         p->adjMax = nullptr;
-        p->adjIdx = nullptr;
+        p->adjUst = nullptr;
         p->adjStream = nullptr;
         p->adjFork = p->adjJoin = nullptr;
     }
@@ -565,10 +565,9 @@ static int forward_field(pm_plan* p, const double* x, const double* y, const dou
 // (binning of x already in p->sorted): the divergence mu is deposited
 // directly into adjoint mesh 0.
 static int adjoint_field(pm_plan* p, const double* x, const double* y, const double* z, const double* ux,
-                         const double* uy, const double* uz, bool umax_ready = false,
-                         const uint32_t* bucket_idx = nullptr) {
+                         const double* uy, const double* uz) {
     int rc = launch_deposit_div(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU[0], p->adjShell,
-                                p->adjMax, p->err, p->stream, umax_ready, bucket_idx);
+                                p->adjMax, p->err, p->stream);
     if (!rc) rc = solve_into(p, p->adjU[0], p->adjLam, false);
     return rc;
 }
@@ -639,8 +638,8 @@ extern "C" int pm_adjoint_force(pm_plan_t* p, const double* x, const double* y, 
 //
 // Staging (pm_run_adjoint with the sorted deposit): from_stage = the previous
 // reverse step's gather staged x in the tile buckets (in-tile records and
-// particle indices), so the forward deposit reads the records and the
-// divergence deposit records and indices, with the full binning behind the
+// cotangents), so the forward deposit reads the records and the
+// divergence deposit records and cotangents, with the full binning behind the
 // overflow flag;
 // stage_next = this step's gather stages the undrifted positions, into
 // buckets sized from this step's counts.
@@ -668,7 +667,7 @@ static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx,
     if (!rc) rc = hip_ok(hipStreamWaitEvent(p->adjStream, p->adjFork, 0), "adjoint: fork");
     if (!rc)
         rc = launch_deposit_div(p->prec, x, y, z, ux, uy, uz, p->np, p->ng, p->sorted, p->adjU[0], p->adjShell,
-                                p->adjMax, p->err, p->adjStream, umax_ready, from_stage ? p->adjIdx : nullptr);
+                                p->adjMax, p->err, p->adjStream, umax_ready, from_stage ? p->adjUst : nullptr);
     if (!rc) rc = hip_ok(hipEventRecord(p->adjJoin, p->adjStream), "adjoint: join");
     // 3. rho and phi at x
     if (!rc) {
@@ -694,7 +693,7 @@ static int adjoint_core(pm_plan* p, double* x, double* y, double* z, double* vx,
         stg.coff = p->sorted.coff;
         stg.spill = p->sorted.spill;
         stg.rec = p->sorted.stage;
-        stg.idx = p->adjIdx;
+        stg.u = p->adjUst;
     }
     if (!rc)
         rc = launch_adj_gather(p->prec, gmode, x, y, z, vx, vy, vz, ux, uy, uz, xbar, ybar, zbar, p->np, p->ng, p->phi,
@@ -736,7 +735,7 @@ extern "C" int pm_run_adjoint(pm_plan_t* p, double* x, double* y, double* z, dou
     // other step's undrift is done by the gather of the step after it.
     // With the sorted deposit every gather but the last stages the positions
     // it undrifts for the next reverse step (no rebinning from scratch).
-    const bool stage = variant == PM_DEPOSIT_SORTED && p->adjIdx && p->sorted.stage;
+    const bool stage = variant == PM_DEPOSIT_SORTED && p->adjUst && p->sorted.stage;
     for (int s = nsteps; s >= 0 && !rc; --s) {
         const double kick = (s == 0 || s == nsteps) ? 0.5 * dt : dt;
         rc = adjoint_core(p, x, y, z, vx, vy, vz, xbar, ybar, zbar, vxbar, vybar, vzbar, kick, variant, s > 0 ? 2 : 0,

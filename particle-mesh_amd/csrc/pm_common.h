@@ -67,6 +67,19 @@ __device__ __forceinline__ void stream_store(double* p, double v) {
         *p = v;
 }
 
+// Nearest integer (ties to even) of a double with |v| < 2^51, as a signed
+// 64-bit integer: v + 1.5 * 2^52 lands in [2^52, 2^53), where the doubles are
+// the integers, so that sum's bit pattern minus that of 1.5 * 2^52 is the
+// rounded value -- the same result as __double2ll_rn, in two instructions
+// instead of its conversion sequence (the fixed-point deposits convert 8 to
+// 32 weights per particle).  The add is kept apart from the product that
+// usually feeds it, so v is rounded to a double first, as for __double2ll_rn.
+__device__ __forceinline__ long long fix_rne(double v) {
+#pragma clang fp contract(off)
+    const double t = v + 6755399441055744.0;  // 1.5 * 2^52
+    return __double_as_longlong(t) - 0x4338000000000000LL;
+}
+
 template <typename R> struct vec2;
 template <> struct vec2<float> { using type = float2; };
 template <> struct vec2<double> { using type = double2; };

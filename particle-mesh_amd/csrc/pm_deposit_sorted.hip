@@ -234,6 +234,16 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const uint32_t* __restrict_
 
 // 4. one workgroup per segment of the column walk, particles by index
 // (binning of positions).
+// Fixed-point unit of the tile deposits: 2^-q of a density unit, q = 32
+// unless a particle's mass m is so large (m >= 2^19, fewer than ng^3 / 2^19
+// particles) that m 2^32 would reach 2^51, the range of fix_rne; a tile's
+// sum stays below ng^3 2^q <= 2^62 either way.
+__device__ __forceinline__ int fix_bits(double mass) {
+    int q = 32;
+    while (q > 0 && ldexp(mass, q) >= 2251799813685248.0) --q;  // 2^51
+    return q;
+}
+
 template <typename Rl, int TS>
 __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict__ px, const double* __restrict__ py,
                                                        const double* __restrict__ pz,
@@ -246,12 +256,13 @@ __global__ __launch_bounds__(256) void k_deposit_tiles(const double* __restrict_
     constexpr int E = TS + 1, E3 = E * E * E;
     __shared__ unsigned long long acc[E3];
     const WalkSeg sw = walk_segment(ng, TS);
-    const double scale[1] = {1.0 / 4294967296.0};
+    const int qb = fix_bits(mass);
+    const double scale[1] = {ldexp(1.0, -qb)};
     Rl* const out[1] = {rho};
     Rl* const sh[1] = {shell};
     for (int i = threadIdx.x; i < E3; i += 256) acc[i] = 0ULL;
     __syncthreads();
-    const double fix = 4294967296.0 * mass;  // 2^32 * m
+    const double fix = ldexp(mass, qb);  // 2^q * m
     const int ox = sw.tx * TS, oy = sw.ty * TS;
     for (int k = 0; k < sw.seg; ++k) {
         const int tz = sw.sg * sw.seg + k, oz = tz * TS;
@@ -307,12 +318,13 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
     constexpr int E = TS + 1, E3 = E * E * E;
     __shared__ unsigned long long acc[E3];
     const WalkSeg sw = walk_segment(ng, TS);
-    const double scale[1] = {1.0 / 4294967296.0};
+    const int qb = fix_bits(mass);
+    const double scale[1] = {ldexp(1.0, -qb)};
     Rl* const out[1] = {rho};
     Rl* const sh[1] = {shell};
     for (int i = threadIdx.x; i < E3; i += 256) acc[i] = 0ULL;
     __syncthreads();
-    const double fix = 4294967296.0 * mass;
+    const double fix = ldexp(mass, qb);
     for (int k = 0; k < sw.seg; ++k) {
         const int tz = sw.sg * sw.seg + k;
         const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;

@@ -176,28 +176,35 @@ int launch_div3(int prec, void* const U[3], int ng, void* mu, hipStream_t st);
 // mu = sum_a D_a^T interp^T(u_a) in one deposit (the mesh that
 // launch_deposit_vec3 + launch_div3 form), folded; shell holds
 // halo_shell_elems(ng) reals.  umax_ready: umax already holds max |u_a|,
-// left by a mode-2 gather; bucket_idx: the particles come from the index
-// buckets that a staging mode-2 gather filled (coff / fill of s), or, if a
+// left by a mode-2 gather; bucket_u: the particles come from the buckets
+// that a staging mode-2 gather filled (coff / fill of s), or, if a
 // bucket overflowed (s.spill), from the binning that the staged deposit's
 // fallback made.  A non-finite cotangent component is counted in err.
+// bucket_u: the cotangents staged beside the records (Real3 of the mesh
+// precision); the binned pass reads u by particle index and rounds it to the
+// mesh precision the same way, so both passes deposit the same values.
 int launch_deposit_div(int prec, const double* px, const double* py, const double* pz, const double* ux,
                        const double* uy, const double* uz, int64_t np, int ng, const SortedScratch& s, void* mu,
                        void* shell, unsigned long long* umax, unsigned long long* err, hipStream_t st,
-                       bool umax_ready = false, const uint32_t* bucket_idx = nullptr);
+                       bool umax_ready = false, const void* bucket_u = nullptr);
 // mode 1: xbar += VJP; 0: also undo the kick; 2: also the next reverse
 // step's undrift (next_drift) with max |u| into umax[0..3) (umax holds
 // 3 + 3 * kMaxSlots entries, the slots zero between calls)
 constexpr int kMaxSlots = 256;
 // Staging of a mode-2 gather: the undrifted positions go to the tile
-// buckets of s (coff, fill, spill) as in-tile records (s.stage) and as
-// particle indices (idx), at the same slots, for the next reverse step.
+// buckets of s (coff, fill, spill) as in-tile records (s.stage) and their
+// cotangents u, rounded to the mesh precision, at the same slots (u), for
+// the next reverse step's divergence deposit.
+template <typename Rl> struct Real3 {
+    Rl x, y, z;
+};
 struct AdjStage {
     int ts = 0;
     uint32_t* fill = nullptr;
     const uint32_t* coff = nullptr;
     uint32_t* spill = nullptr;
     void* rec = nullptr;
-    uint32_t* idx = nullptr;
+    void* u = nullptr;  // the next step's cotangent u, 3 mesh reals per slot (Real3)
 };
 int launch_adj_gather(int prec, int mode, double* px, double* py, double* pz, double* vx, double* vy, double* vz,
                       double* ux, double* uy, double* uz, double* bx, double* by, double* bz, int64_t np, int ng,
@@ -256,7 +263,7 @@ struct pm_plan {
     void* adjLam = nullptr;    // lambda = S D^T U
     void* adjShell = nullptr;  // shells of the vec3 deposit (3) or the divergence deposit
     unsigned long long* adjMax = nullptr;  // max |u_a| (fixed-point scales of the adjoint deposits) + slots
-    uint32_t* adjIdx = nullptr;  // index buckets of the reverse sweep (cap_total), beside sorted.stage
+    void* adjUst = nullptr;  // staged cotangents of the reverse sweep (cap_total Real3), beside sorted.stage
     // Second stream of an adjoint step (default priority): the divergence
     // deposit runs there beside the forward deposit and solve; fork / join events.
     hipStream_t adjStream = nullptr;

@@ -187,14 +187,14 @@ __device__ __forceinline__ void tile_cic_add(unsigned long long* acc, int a, int
     const double sxy00 = sx0 * sy0, sxy10 = fx * sy0, sxy01 = sx0 * fy, sxy11 = fx * fy;
     const double m0 = fix * (1.0 - fz), m1 = fix * fz;
     unsigned long long* n0 = acc + (a * E + b) * E + c;
-    atomicAdd(n0, (unsigned long long)__double2ll_rn(sxy00 * m0));
-    atomicAdd(n0 + E * E, (unsigned long long)__double2ll_rn(sxy10 * m0));
-    atomicAdd(n0 + E, (unsigned long long)__double2ll_rn(sxy01 * m0));
-    atomicAdd(n0 + E * E + E, (unsigned long long)__double2ll_rn(sxy11 * m0));
-    atomicAdd(n0 + 1, (unsigned long long)__double2ll_rn(sxy00 * m1));
-    atomicAdd(n0 + E * E + 1, (unsigned long long)__double2ll_rn(sxy10 * m1));
-    atomicAdd(n0 + E + 1, (unsigned long long)__double2ll_rn(sxy01 * m1));
-    atomicAdd(n0 + E * E + E + 1, (unsigned long long)__double2ll_rn(sxy11 * m1));
+    atomicAdd(n0, (unsigned long long)fix_rne(sxy00 * m0));
+    atomicAdd(n0 + E * E, (unsigned long long)fix_rne(sxy10 * m0));
+    atomicAdd(n0 + E, (unsigned long long)fix_rne(sxy01 * m0));
+    atomicAdd(n0 + E * E + E, (unsigned long long)fix_rne(sxy11 * m0));
+    atomicAdd(n0 + 1, (unsigned long long)fix_rne(sxy00 * m1));
+    atomicAdd(n0 + E * E + 1, (unsigned long long)fix_rne(sxy10 * m1));
+    atomicAdd(n0 + E + 1, (unsigned long long)fix_rne(sxy01 * m1));
+    atomicAdd(n0 + E * E + E + 1, (unsigned long long)fix_rne(sxy11 * m1));
 }
 
 // Slot reservation in the staging buckets (called by every thread of the
@@ -211,6 +211,14 @@ __device__ __forceinline__ int64_t stage_reserve(uint32_t kk, uint32_t* hkey, ui
                                                  uint32_t* __restrict__ fill, const uint32_t* __restrict__ coff,
                                                  uint32_t* __restrict__ spill) {
     constexpr uint32_t kNo = 0xFFFFFFFFu;
+    // the bucket bounds are loaded first, so that they arrive during the
+    // barriers and the fill atomics (interpolation at 512^3 FP32 in an
+    // interleaved A/B: 3.24 -> 3.17 ms)
+    uint32_t c0 = 0u, c1 = 0u;
+    if (kk != kNo) {
+        c0 = coff[kk];
+        c1 = coff[kk + 1];
+    }
     const uint32_t lane = __lane_id();
     const uint32_t prev = __shfl_up(kk, 1);
     const bool head = kk != kNo && (lane == 0 || prev != kk);
@@ -242,8 +250,7 @@ __device__ __forceinline__ int64_t stage_reserve(uint32_t kk, uint32_t* hkey, ui
     const uint32_t base = __shfl(gb, hl);
     if (kk == kNo) return -1;
     const uint32_t slot = base + (lane - (uint32_t)hl);
-    const uint32_t c0 = coff[kk];
-    if (slot < coff[kk + 1] - c0) return (int64_t)c0 + slot;
+    if (slot < c1 - c0) return (int64_t)c0 + slot;
     *spill = 1u;
     return -1;
 }
