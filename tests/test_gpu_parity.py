@@ -421,6 +421,37 @@ def test_step_interp_matches_standalone_interp(dev):
 
 
 @pytest.mark.parametrize("prec", [pmhip.PM_FP32, pmhip.PM_FP64])
+@pytest.mark.parametrize("ng,nrow", [(8, 8), (64, 32), (256, 128)])
+def test_step_field_bitwise_equals_deposit_and_solve(dev, prec, ng, nrow):
+    """pm_step's density and potential must equal, bit for bit, those of
+    pm_deposit followed by pm_solve on the same positions: on the first step
+    (binned) and on the second (deposited from the records that the first
+    step's interpolation staged), and again after a re-solve of the step's
+    density.  ng = 8 is one tile folded onto itself."""
+    import torch
+
+    npart = nrow ** 3
+    noise = np.random.default_rng(ng + nrow).standard_normal(npart)
+    pos, vel = pmhip.zeldovich_ic(nrow, ng, torch.tensor(noise, device=dev), 1.5, 3.0)
+    dt = 0.02
+    with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as pa, \
+            pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0) as pb:
+        for k in range(2):
+            x_before = [t.clone() for t in pos]
+            pa.step(pos, vel, dt, dt, pmhip.PM_DEPOSIT_SORTED | (pmhip.PM_STEP_REUSE_BINS if k else 0))
+            pb.deposit(x_before, pmhip.PM_DEPOSIT_SORTED)
+            pb.solve()
+            torch.cuda.synchronize()
+            assert torch.equal(pa.potential(), pb.potential()), f"step {k}: potential"
+            rho_a = pa.density().clone()
+            torch.cuda.synchronize()
+            assert torch.equal(rho_a, pb.density()), f"step {k}: density"
+            pa.solve()
+            torch.cuda.synchronize()
+            assert torch.equal(pa.potential(), pb.potential()), f"step {k}: re-solve of the folded density"
+
+
+@pytest.mark.parametrize("prec", [pmhip.PM_FP32, pmhip.PM_FP64])
 @pytest.mark.parametrize("ng,nrow,shuffle", [(32, 32, False), (128, 64, True)])
 def test_reused_binning_bitwise_equals_fresh_binning(dev, prec, ng, nrow, shuffle):
     """pm_run_kdk bins every step after the first from the keys and tile
