@@ -98,11 +98,11 @@ PHASE_KERNELS = {
 }
 
 
-def measured_traffic(kernel: str, ng: int, npart: int):
+def measured_traffic(kernel: str, ng: int, npart: int, table: str = ""):
     """HBM bytes per dispatch of a kernel from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what
     the counter reads for a plain copy of known size); None if absent."""
-    path = ROOT / "profiles" / f"pmc_traffic_{ng}.txt"
+    path = ROOT / "profiles" / f"pmc_traffic_{ng}{table}.txt"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -112,6 +112,27 @@ def measured_traffic(kernel: str, ng: int, npart: int):
         if len(cols) >= 6 and cols[0] == kernel and cols[1] == str(npart):
             return int(float(cols[5]))
     return None
+
+
+# kernels of the FP32 deposit phase of each variant and the PMC summary that
+# holds their HBM bytes (the sorted step's steady state: staged records)
+DEPOSIT_KERNELS = {
+    "sorted": ("", ["void pm::k_deposit_staged<float, 16>", "void pm::k_fold_shells<float, 16>"]),
+    "atomic": ("_atomic", ["void pm::k_fill<float>", "void pm::k_deposit_atomic<float>"]),
+}
+
+
+def deposit_traffic(name: str, ng: int, npart: int, phase_ms: float):
+    """HBM bytes of the FP32 deposit phase's kernels (committed PMC summary)
+    and the phase's HBM rate at the measured phase time; None if absent."""
+    table, kernels = DEPOSIT_KERNELS[name]
+    per = {k: measured_traffic(k, ng, npart, table) for k in kernels}
+    if not phase_ms or any(v is None for v in per.values()):
+        return None
+    total = sum(per.values())
+    return {"kernels_hbm_bytes": per, "hbm_bytes": total, "phase_ms": round(phase_ms, 4),
+            "hbm_GBs": round(total / (phase_ms * 1e-3) / 1e9, 1),
+            "source": f"profiles/pmc_traffic_{ng}{table}.txt"}
 
 
 # the FP64 step's dominant kernel, and the PMC summary that holds its traffic
@@ -450,7 +471,8 @@ def main():
         log(f"FP32-mesh leg ({alt_name} deposit)")
         ms_a, _, ph_a, _, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart, alt_variant)
         alt = {"deposit": alt_name, "value": round(npart / (ms_a * 1e-3) / 1e6, 2), "ms_per_step": round(ms_a, 4),
-               "deposit_ms": round(ph_a["deposit"], 4)}
+               "deposit_ms": round(ph_a["deposit"], 4),
+               "deposit_hbm": deposit_traffic(alt_name, ng, npart, ph_a["deposit"])}
         log(f"FP32 {alt_name}: {ms_a:.3f} ms/step")
     res64 = None
     if not args.no_fp64:
@@ -546,6 +568,7 @@ def main():
                                f"CONT_K2 Green",
                    "ng": ng, "np": npart, "dt": args.dt, "rms_cells": args.rms},
         "phases_ms": {k: (round(v, 4) if v else v) for k, v in ph32.items()},
+        "deposit_hbm": deposit_traffic(args.deposit, ng, npart, ph32.get("deposit")),
         "step_algorithmic_GB": round(step_bytes / 1e9, 3),
         "step_achieved_GBs": round(step_bytes / (ms32 * 1e-3) / 1e9, 1),
         "wall_ms_per_step": round(wall32, 4),
