@@ -117,7 +117,8 @@ def measured_traffic(kernel: str, ng: int, npart: int, table: str = ""):
 # kernels of the FP32 deposit phase of each variant and the PMC summary that
 # holds their HBM bytes (the sorted step's steady state: staged records)
 DEPOSIT_KERNELS = {
-    "sorted": ("", ["void pm::k_deposit_staged<float, 16>", "void pm::k_fold_shells<float, 16>"]),
+    "sorted": ("", ["void pm::k_deposit_staged<float, 16>", "void pm::k_fold_faces<float, 16, 1, 0>",
+                    "void pm::k_fold_seams<float, 16, 1, 0>"]),
     "atomic": ("_atomic", ["void pm::k_fill<float>", "void pm::k_deposit_atomic<float>"]),
 }
 

@@ -16,8 +16,9 @@ namespace pm {
 // integer addition.  Written out per tile: the TS^3 owned nodes to the mesh
 // and, for c < TS, the upper x and y faces to the tile's xy shell; per
 // segment: the upper z plane of its last tile to the segment's z seam.
-// k_fold_shells then adds the lower neighbours' xy shells to each tile's
-// lower x and y faces, and the seams to the first plane of each segment.
+// The fold (k_fold_faces, k_fold_seams) then adds the lower neighbours' xy
+// shells to each tile's lower x and y faces, and the seams to the first plane
+// of each segment.
 #ifndef PM_WALK_SEG
 #define PM_WALK_SEG 8
 #endif
