@@ -168,7 +168,7 @@ int pm_ic_zeldovich(int nrow, int ng, const double* noise, double rms_cells, dou
 typedef struct pm_plan pm_plan_t;
 
 /* Create a plan for an ng^3 periodic mesh (ng a power of two in [8, 1024])
- * and np particles.  green_mode selects the Green's function; green_scale
+ * and np particles (1 <= np < 2^32).  green_mode selects the Green's function; green_scale
  * multiplies it (1/a for the reference's apply_green(a); 1 for CONT_K2).
  * The plan owns its meshes and FFT work space (about 4.2 ng^3 reals). */
 int pm_plan_create(pm_plan_t** out, int ng, int64_t np, int precision, int green_mode,

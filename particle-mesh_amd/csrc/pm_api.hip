@@ -218,6 +218,8 @@ extern "C" int pm_plan_create(pm_plan_t** out, int ng, int64_t np, int precision
     *out = nullptr;
     if (!valid_ng(ng)) return fail(PM_ERR_ARG, "pm_plan_create: ng must be a power of two in [8, 1024]");
     if (np <= 0) return fail(PM_ERR_ARG, "pm_plan_create: np must be positive");
+    // tile keys, slots and the tile-ordered index list are 32-bit
+    if (np > (int64_t)0xFFFFFFFFll) return fail(PM_ERR_ARG, "pm_plan_create: np must be below 2^32");
     if (precision != PM_FP32 && precision != PM_FP64) return fail(PM_ERR_ARG, "pm_plan_create: bad precision");
     if (green_mode != PM_GREEN_REF_KH && green_mode != PM_GREEN_CONT_K2)
         return fail(PM_ERR_ARG, "pm_plan_create: bad green mode");

@@ -51,6 +51,8 @@ def test_plan_rejects_bad_modes():
     lib = pmhip.lib()
     h = ctypes.c_void_p()
     assert lib.pm_plan_create(ctypes.byref(h), 16, 0, pmhip.PM_FP32, 0, 1.0, None) == pmhip.PM_ERR_ARG
+    assert lib.pm_plan_create(ctypes.byref(h), 16, 1 << 32, pmhip.PM_FP32, 0, 1.0, None) == pmhip.PM_ERR_ARG
+    assert "2^32" in pmhip.last_error()
     assert lib.pm_plan_create(ctypes.byref(h), 16, 10, 7, 0, 1.0, None) == pmhip.PM_ERR_ARG
     assert lib.pm_plan_create(ctypes.byref(h), 16, 10, 0, 9, 1.0, None) == pmhip.PM_ERR_ARG
     assert lib.pm_plan_create(None, 16, 10, 0, 0, 1.0, None) == pmhip.PM_ERR_ARG
