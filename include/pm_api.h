@@ -198,7 +198,8 @@ int pm_interp_force(pm_plan_t* plan, const double* posx, const double* posy, con
 int pm_step(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
             double* vz, double kick, double drift, int variant);
 /* nsteps kick-drift-kick steps of size dt with merged kicks (nsteps + 1
- * force evaluations: kick dt/2 first, dt between, dt/2 last). */
+ * force evaluations: kick dt/2 first, dt between, dt/2 last; nsteps = 0
+ * leaves the particles as they are). */
 int pm_run_kdk(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
                double* vz, int nsteps, double dt, int variant);
 /* Cosmological kick-drift-kick in the expansion factor a, from a0 in nsteps
@@ -256,7 +257,8 @@ int pm_adjoint_step(pm_plan_t* plan, double* posx, double* posy, double* posz, d
                     double* vz, double* xbar, double* ybar, double* zbar, double* vxbar, double* vybar,
                     double* vzbar, double kick, double drift, int variant);
 /* Reverse sweep of pm_run_kdk(nsteps, dt): reverse-time leapfrog
- * recomputation, O(1) memory in the number of steps. */
+ * recomputation, O(1) memory in the number of steps (nsteps = 0: nothing
+ * changes). */
 int pm_run_adjoint(pm_plan_t* plan, double* posx, double* posy, double* posz, double* vx, double* vy,
                    double* vz, double* xbar, double* ybar, double* zbar, double* vxbar, double* vybar,
                    double* vzbar, int nsteps, double dt, int variant);

@@ -606,7 +606,8 @@ void orc_interp_kick_drift(int64_t np, double* posx, double* posy, double* posz,
 
 int orc_run_kdk(int64_t np, double* posx, double* posy, double* posz, double* vx, double* vy,
                 double* vz, int ng, int nsteps, double dt, int green_mode, double green_scale) {
-    if (!is_pow2(ng) || np <= 0) return -1;
+    if (!is_pow2(ng) || np <= 0 || nsteps < 0) return -1;
+    if (nsteps == 0) return 0; /* no step: no kick either */
     const size_t nc = (size_t)ng * ng * ng;
     double* delta = (double*)malloc(sizeof(double) * nc);
     double* phi = (double*)malloc(sizeof(double) * nc);

@@ -443,6 +443,7 @@ extern "C" int pm_step(pm_plan_t* p, double* x, double* y, double* z, double* vx
 extern "C" int pm_run_kdk(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
                           int nsteps, double dt, int variant) {
     if (!p || nsteps < 0) return fail(PM_ERR_ARG, "pm_run_kdk: bad arguments");
+    if (nsteps == 0) return PM_OK;  // no step: no kick either
     variant &= ~PM_STEP_REUSE_BINS;
     for (int s = 0; s <= nsteps; ++s) {
         const double kick = (s == 0 || s == nsteps) ? 0.5 * dt : dt;
@@ -727,6 +728,7 @@ extern "C" int pm_run_adjoint(pm_plan_t* p, double* x, double* y, double* z, dou
     if (!p || nsteps < 0 || !x || !y || !z || !vx || !vy || !vz || !xbar || !ybar || !zbar || !vxbar || !vybar ||
         !vzbar)
         return fail(PM_ERR_ARG, "pm_run_adjoint: bad arguments");
+    if (nsteps == 0) return PM_OK;  // the adjoint of zero steps (the identity)
     int rc = ensure_adjoint(p);
     if (rc) return rc;
     const int slots = p->timing_slots;

@@ -27,7 +27,9 @@ class _Evolve(torch.autograd.Function):
         x = x0.detach().clone().contiguous()
         v = v0.detach().clone().contiguous()
         states = []
-        if checkpoint:
+        if nsteps == 0:
+            pass  # zero steps: the identity
+        elif checkpoint:
             for s in range(nsteps + 1):
                 states.append((x.clone(), v.clone()))
                 kick = 0.5 * dt if (s == 0 or s == nsteps) else dt
@@ -47,7 +49,9 @@ class _Evolve(torch.autograd.Function):
         x, v = xf.clone(), vf.clone()
         xb = (xbar if xbar is not None else torch.zeros_like(xf)).detach().clone().contiguous()
         vb = (vbar if vbar is not None else torch.zeros_like(vf)).detach().clone().contiguous()
-        if ctx.checkpoint:
+        if nsteps == 0:
+            pass
+        elif ctx.checkpoint:
             for s in range(nsteps, -1, -1):
                 kick = 0.5 * dt if (s == 0 or s == nsteps) else dt
                 drift = 0.0 if s == nsteps else dt

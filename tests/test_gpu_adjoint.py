@@ -309,6 +309,34 @@ def test_fp32_adjoint_close_to_fp64(dev):
     assert err < 1e-3, err
 
 
+@pytest.mark.parametrize("checkpoint", [False, True])
+def test_zero_steps_are_the_identity(dev, checkpoint):
+    """pm_run_kdk and pm_run_adjoint of zero steps change nothing (as the
+    oracle's run_kdk), and the autograd harness passes the cotangents of a
+    zero-step evolution through unchanged."""
+    import torch
+
+    ng, nrow = 16, 16
+    noise = torch.tensor(np.random.default_rng(8).standard_normal(nrow ** 3), device=dev)
+    pos, vel = pmhip.zeldovich_ic(nrow, ng, noise, 1.0, 3.0)
+    p0, v0 = [t.clone() for t in pos], [t.clone() for t in vel]
+    xb = [torch.full_like(t, 0.25) for t in pos]
+    vb = [torch.full_like(t, -0.5) for t in pos]
+    with pmhip.Plan(ng, nrow ** 3, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.run_kdk(pos, vel, 0, 0.05, pmhip.PM_DEPOSIT_SORTED)
+        plan.run_adjoint(pos, vel, xb, vb, 0, 0.05, pmhip.PM_DEPOSIT_SORTED)
+        torch.cuda.synchronize()
+        for a, b in zip(pos + vel, p0 + v0):
+            assert torch.equal(a, b)
+        assert all(bool((t == 0.25).all()) for t in xb) and all(bool((t == -0.5).all()) for t in vb)
+        x0 = torch.stack(p0).requires_grad_(True)
+        v0s = torch.stack(v0).requires_grad_(True)
+        x, v = pag.evolve(x0, v0s, plan, 0, 0.05, checkpoint=checkpoint)
+        assert torch.equal(x.detach(), x0.detach()) and torch.equal(v.detach(), v0s.detach())
+        (x.sum() * 2.0 + v.sum() * 3.0).backward()
+        assert bool((x0.grad == 2.0).all()) and bool((v0s.grad == 3.0).all())
+
+
 @pytest.mark.parametrize("ng,nrow", [(32, 32), (256, 64)])
 @pytest.mark.parametrize("prec", [pmhip.PM_FP64, pmhip.PM_FP32])
 def test_run_adjoint_equals_step_by_step(dev, prec, ng, nrow):

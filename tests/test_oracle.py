@@ -470,6 +470,23 @@ def test_kdk_linear_mode_growth_converges_at_second_order():
         assert 3.6 < r1 < 4.4 and 3.6 < r2 < 4.4, (errs, r1, r2)
 
 
+def test_kdk_zero_steps_is_the_identity():
+    """Zero KDK steps apply no kick (the merged first and last half kicks of
+    one step would otherwise coincide in a lone half kick); one step moves
+    the state."""
+    ng, npart = 16, 512
+    rng = np.random.default_rng(4)
+    pos = [rng.uniform(0, ng, npart) for _ in range(3)]
+    vel = [rng.normal(0, 0.1, npart) for _ in range(3)]
+    p0 = [a.copy() for a in pos]
+    v0 = [a.copy() for a in vel]
+    pyoracle.run_kdk(pos, vel, ng, 0, 0.05, pyoracle.GREEN_CONT_K2, 1.0)
+    for a, b in zip(pos + vel, p0 + v0):
+        assert np.array_equal(a, b)
+    pyoracle.run_kdk(pos, vel, ng, 1, 0.05, pyoracle.GREEN_CONT_K2, 1.0)
+    assert max(float(np.abs(a - b).max()) for a, b in zip(vel, v0)) > 0
+
+
 def test_kdk_is_time_reversible():
     """Leapfrog is symmetric: n steps, negate v, n steps, negate v returns to
     the initial state (here to rounding, after shell-crossing dynamics)."""

@@ -77,7 +77,10 @@ def force(x, ng, mass, green):
 
 
 def run_kdk(x, v, ng, nsteps, dt, mass, green):
-    """Merged-kick KDK of pm_run_kdk (nsteps + 1 force evaluations)."""
+    """Merged-kick KDK of pm_run_kdk (nsteps + 1 force evaluations; none
+    for zero steps)."""
+    if nsteps == 0:
+        return x, v
     for s in range(nsteps + 1):
         kick = 0.5 * dt if (s == 0 or s == nsteps) else dt
         drift = 0.0 if s == nsteps else dt
