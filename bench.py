@@ -8,10 +8,13 @@ KDK steps of dt = 0.005.  A step = one force evaluation (zero + CIC deposit,
 5-pass R2C/Green/C2R FFT solve, interpolation of the centred-difference
 force) + kick + drift, all on the GPU with inputs resident in HBM.
 
-Prints one JSON line: value = FP32-mesh Mparticle-steps/s (FP64 positions
-and velocities); the FP64-mesh path, per-phase times, the roofline of the
-dominant kernel and the CPU baseline (the FP64 oracle on the host cores, on
-a bounded sample of the same ICs) are reported beside it.
+Prints one JSON line: value = FP32-mesh Mparticle-steps/s with the
+split-precision particle state (positions as integer cell + 2^-23-cell
+fixed-point fraction, FP32 velocities, resident on the GPU: pm_split_step);
+the FP32 mesh with FP64 positions and velocities (pm_step), the FP64-mesh
+path, per-phase times, the roofline of the dominant kernel and the CPU
+baseline (the FP64 oracle on the host cores, on a bounded sample of the same
+ICs) are reported beside it.
 """
 from __future__ import annotations
 
@@ -62,15 +65,18 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(ng: int, npart: int, prec: int, staged: bool = True):
+def algorithmic_bytes(ng: int, npart: int, prec: int, staged: bool = True, split: bool = False):
     """Compulsory HBM bytes per launch of each phase of a steady-state step
-    (b = mesh real bytes; positions and velocities FP64; spectrum
-    (ng/2+1) ng^2 complex).  With the sorted deposit the interpolation
-    stages every drifted position in tile order as a record of its in-tile
-    offsets (12 B: 3 x u32 for an FP32 mesh; 16 B: 3 x 42 bits for an FP64
-    mesh) and the next deposit reads it back: written once and read once."""
+    (b = mesh real bytes; positions and velocities FP64, 48 B per particle,
+    or the split state, 24 B: u32 cell + fraction and FP32 velocity per
+    axis; spectrum (ng/2+1) ng^2 complex).  With the sorted deposit the
+    interpolation stages every drifted position in tile order as a record of
+    its in-tile offsets (12 B: 3 x u32 for an FP32 mesh; 16 B: 3 x 42 bits
+    for an FP64 mesh) and the next deposit reads it back: written once and
+    read once."""
     b = 4 if prec == 0 else 8
     st = (12 if prec == 0 else 16) if staged else 0
+    state = 24 if split else 48
     nc = ng ** 3
     nk = (ng // 2 + 1) * ng * ng
     return {
@@ -80,7 +86,7 @@ def algorithmic_bytes(ng: int, npart: int, prec: int, staged: bool = True):
         "fft_x_green": 4 * b * nk,
         "fft_y_inv": 4 * b * nk,
         "fft_z_inv": 2 * b * nk + b * nc,
-        "interp_kick_drift": b * nc + (96 + st) * npart,     # read phi; read+write x, v; write staging
+        "interp_kick_drift": b * nc + (2 * state + st) * npart,  # read phi; read+write x, v; write staging
     }
 
 
@@ -96,6 +102,8 @@ PHASE_KERNELS = {
     "fft_z_inv": "void pm::k_zinv<float, 512,",
     "interp_kick_drift": "void pm::k_interp<float, 2, 512, 1>",
 }
+# the same for the split-state step (pm_split_step)
+SPLIT_PHASE_KERNELS = dict(PHASE_KERNELS, interp_kick_drift="void pm::k_split_interp<true>")
 
 
 def measured_traffic(kernel: str, ng: int, npart: int, table: str = ""):
@@ -168,9 +176,11 @@ def make_ics(nrow, ng, rms, seed, dev):
     return pos, vel
 
 
-def run_gpu(prec, pos0, vel0, args, ng, npart, variant):
+def run_gpu(prec, pos0, vel0, args, ng, npart, variant, split=False):
     """Warm up, then time exactly args.steps steps; returns (ms_per_step,
-    phase ms per launch)."""
+    phase ms per launch).  split: the plan's split-precision state
+    (pm_split_load before the warm-up, pm_split_step, pm_split_store after
+    the timed steps into the returned arrays)."""
     import torch
     import pmhip
 
@@ -184,13 +194,18 @@ def run_gpu(prec, pos0, vel0, args, ng, npart, variant):
     shape = plan.fft_shape
     dt = args.dt
     step = 0
+    if split:
+        plan.split_load(pos, vel)
 
     def one():
         nonlocal step
-        # after the first step the positions are the ones this plan wrote:
-        # let the deposit bin from the keys the interpolation left
-        plan.step(pos, vel, 0.5 * dt if step == 0 else dt, dt,
-                  variant | (pmhip.PM_STEP_REUSE_BINS if step > 0 else 0))
+        kick = 0.5 * dt if step == 0 else dt
+        if split:
+            plan.split_step(kick, dt)
+        else:
+            # after the first step the positions are the ones this plan wrote:
+            # let the deposit bin from the keys the interpolation left
+            plan.step(pos, vel, kick, dt, variant | (pmhip.PM_STEP_REUSE_BINS if step > 0 else 0))
         step += 1
 
     for _ in range(args.warmup):
@@ -210,6 +225,8 @@ def run_gpu(prec, pos0, vel0, args, ng, npart, variant):
     wall = time.perf_counter() - t0
     ev_ms = e0.elapsed_time(e1)
     ms, cnt = plan.phase_ms()
+    if split:
+        plan.split_store(pos, vel)
     nerr = pmhip.range_errors()
     plan.close()
     phases = {n: (m / c if c else None) for n, m, c in zip(pmhip.PHASE_NAMES, ms, cnt)}
@@ -262,11 +279,22 @@ def run_adjoint_leg(pos0, vel0, args, ng, npart, variant):
 def gpu_fields(pos0, vel0, args, ng, npart, variant, nsteps):
     """Both GPU mesh paths on the same ICs: density contrast, potential and
     per-particle force of the initial state, and the positions after nsteps
-    KDK steps (pm_run_kdk)."""
+    KDK steps (pm_run_kdk); and the positions after nsteps steps of the
+    split-precision state (pm_split_run_kdk, key "split")."""
     import torch
     import pmhip
 
     fields = {}
+    plan = pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0)
+    plan.split_load(pos0, vel0)
+    plan.split_run_kdk(nsteps, args.dt)
+    pos = [torch.empty_like(p) for p in pos0]
+    vel = [torch.empty_like(v) for v in vel0]
+    plan.split_store(pos, vel)
+    torch.cuda.synchronize()
+    plan.close()
+    del vel
+    fields["split"] = pos
     for prec in (pmhip.PM_FP64, pmhip.PM_FP32):
         plan = pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0)
         plan.deposit(pos0, variant)
@@ -362,33 +390,37 @@ def oracle_leg(pos_dev, vel_dev, ng, args, nsteps):
 def accuracy_report(fields, ref, ng, nsteps, dt, dev):
     """Errors of both GPU mesh paths against the FP64 oracle (relative rms of
     the density contrast, potential and force of the initial state; final
-    positions in cells after nsteps steps), and of the FP32 path against the
-    GPU FP64 path."""
+    positions in cells after nsteps steps), of the split state's final
+    positions, and of the FP32 path against the GPU FP64 path."""
     import torch
     import pmhip
 
     def rel(a, b):
         return float(torch.sqrt(((a - b) ** 2).mean() / (b ** 2).mean()))
 
-    def errors(f, r):
-        d, p, g, x = f
-        rd, rp, rg, rx = r
-        gnum = sum(float(((a - b) ** 2).mean()) for a, b in zip(g, rg))
-        gden = sum(float((b ** 2).mean()) for b in rg)
+    def pos_errors(x, rx):
         d2 = None
         for a, b in zip(x, rx):
             e = a - b
             e = e - ng * torch.round(e / ng)
             d2 = e * e if d2 is None else d2 + e * e
-        return {"density_rel_rms": rel(d, rd), "potential_rel_rms": rel(p, rp), "force_rel_rms": (gnum / gden) ** 0.5,
-                "final_position_rms_cells": float(torch.sqrt(d2.mean())),
+        return {"final_position_rms_cells": float(torch.sqrt(d2.mean())),
                 "final_position_max_cells": float(torch.sqrt(d2.max()))}
+
+    def errors(f, r):
+        d, p, g, x = f
+        rd, rp, rg, rx = r
+        gnum = sum(float(((a - b) ** 2).mean()) for a, b in zip(g, rg))
+        gden = sum(float((b ** 2).mean()) for b in rg)
+        return {"density_rel_rms": rel(d, rd), "potential_rel_rms": rel(p, rp), "force_rel_rms": (gnum / gden) ** 0.5,
+                **pos_errors(x, rx)}
 
     rd, rp, rg, rx = ref
     oref = (torch.from_numpy(rd).to(dev), torch.from_numpy(rp).to(dev), [torch.from_numpy(v).to(dev) for v in rg],
             [torch.from_numpy(v).to(dev) for v in rx])
     out = {"against": "FP64 oracle (C/OpenMP restatement of the reference step) on the same ICs", "steps": nsteps,
            "dt": dt, "target_rms_cells": 1e-4,
+           "fp32_split": pos_errors(fields["split"], oref[3]),
            "fp32": errors(fields[pmhip.PM_FP32], oref), "fp64": errors(fields[pmhip.PM_FP64], oref),
            "fp32_vs_gpu_fp64": errors(fields[pmhip.PM_FP32], fields[pmhip.PM_FP64])}
     del oref
@@ -464,9 +496,18 @@ def main():
     variant = pmhip.PM_DEPOSIT_SORTED if args.deposit == "sorted" else pmhip.PM_DEPOSIT_ATOMIC
     alt_variant = pmhip.PM_DEPOSIT_ATOMIC if args.deposit == "sorted" else pmhip.PM_DEPOSIT_SORTED
     alt_name = "atomic" if args.deposit == "sorted" else "sorted"
-    log(f"FP32-mesh leg ({args.deposit} deposit)")
-    ms32, wall32, ph32, err32, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart, variant)
-    log(f"FP32: {ms32:.3f} ms/step, phases {ph32}")
+    split = variant == pmhip.PM_DEPOSIT_SORTED
+    if split:
+        # the headline: split-precision positions, as the 512^3 benchmark step
+        # names them (the sorted deposit; the atomic variant keeps FP64 positions)
+        log("FP32-mesh leg (split-precision state, sorted deposit)")
+        ms32, wall32, ph32, err32, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart, variant, split=True)
+        log(f"FP32 split: {ms32:.3f} ms/step, phases {ph32}")
+    log(f"FP32-mesh leg, FP64 positions ({args.deposit} deposit)")
+    msd, walld, phd, errd, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart, variant)
+    log(f"FP32 FP64-positions: {msd:.3f} ms/step, phases {phd}")
+    if not split:
+        ms32, wall32, ph32, err32 = msd, walld, dict(phd), errd
     alt = None
     if not args.no_alt_deposit:
         log(f"FP32-mesh leg ({alt_name} deposit)")
@@ -509,22 +550,33 @@ def main():
         torch.cuda.synchronize()
 
     shape32 = ph32.pop("fft_shape")
+    phd.pop("fft_shape", None)
     shape64 = res64["phases_ms"].pop("fft_shape") if res64 else None
-    nbytes = algorithmic_bytes(ng, npart, pmhip.PM_FP32, variant == pmhip.PM_DEPOSIT_SORTED)
-    dom = max((k for k in ph32 if ph32[k]), key=lambda k: ph32[k])
-    achieved = nbytes[dom] / (ph32[dom] * 1e-3) / 1e9
-    kname = PHASE_KERNELS.get(dom) if variant == pmhip.PM_DEPOSIT_SORTED else None
-    traffic = measured_traffic(kname, ng, npart) if kname else None
-    roofline = {"bound": "hbm", "kernel": kname or dom, "phase": dom, "achieved": round(achieved, 1),
+    sorted_dep = variant == pmhip.PM_DEPOSIT_SORTED
+
+    def fp32_roofline(ph, is_split):
+        """Roofline of the dominant kernel of an FP32-mesh step: its
+        algorithmic bytes (with the staging record; beside them without
+        it) over its average launch time, and its PMC traffic."""
+        nb = algorithmic_bytes(ng, npart, pmhip.PM_FP32, sorted_dep, is_split)
+        dom = max((k for k in ph if ph[k]), key=lambda k: ph[k])
+        achieved = nb[dom] / (ph[dom] * 1e-3) / 1e9
+        kname = (SPLIT_PHASE_KERNELS if is_split else PHASE_KERNELS).get(dom) if sorted_dep else None
+        traffic = measured_traffic(kname, ng, npart) if kname else None
+        roof = {"bound": "hbm", "kernel": kname or dom, "phase": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes": nbytes[dom], "avg_ms": round(ph32[dom], 4),
+                "algorithmic_bytes": nb[dom], "avg_ms": round(ph[dom], 4),
                 "traffic_source": f"profiles/pmc_traffic_{ng}.txt" if traffic else None}
-    # the same kernel priced without its staging record (the step's compulsory
-    # bytes only: b N_c + 96 N_p for the interpolation, 24 N_p + b N_c for the deposit)
-    unstaged = algorithmic_bytes(ng, npart, pmhip.PM_FP32, False)[dom]
-    roofline["algorithmic_bytes_unstaged"] = unstaged
-    roofline["frac_unstaged"] = round(unstaged / (ph32[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-    step_bytes = sum(nbytes.values())
+        # the same kernel priced without its staging record (the step's
+        # compulsory bytes only: b N_c + 2 x state N_p for the interpolation,
+        # 24 N_p + b N_c for the deposit)
+        unstaged = algorithmic_bytes(ng, npart, pmhip.PM_FP32, False, is_split)[dom]
+        roof["algorithmic_bytes_unstaged"] = unstaged
+        roof["frac_unstaged"] = round(unstaged / (ph[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        return roof, sum(nb.values())
+
+    roofline, step_bytes = fp32_roofline(ph32, split)
+    roofline_d, step_bytes_d = fp32_roofline(phd, False)
     roofline64 = None
     if res64:
         ph64 = res64["phases_ms"]
@@ -562,17 +614,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 mesh/FFT/force, f64 positions/velocities",
+        "dtype": ("f32 mesh/FFT/force/velocities, split positions (u32 cell + fraction, 2^-23 cell)" if split
+                  else "f32 mesh/FFT/force, f64 positions/velocities"),
         "data": "synthetic Zel'dovich lattice ICs generated on the GPU",
         "config": {"workload": f"PM KDK step, {nrow}^3 particles on a {ng}^3 periodic mesh, Zel'dovich k^-2 ICs "
                                f"{args.rms}-cell rms, dt={args.dt}, {args.deposit} CIC deposit, 5-pass FFT, "
-                               f"CONT_K2 Green",
+                               f"CONT_K2 Green" + (", split-precision positions" if split else ""),
                    "ng": ng, "np": npart, "dt": args.dt, "rms_cells": args.rms},
         "phases_ms": {k: (round(v, 4) if v else v) for k, v in ph32.items()},
         "deposit_hbm": deposit_traffic(args.deposit, ng, npart, ph32.get("deposit")),
         "step_algorithmic_GB": round(step_bytes / 1e9, 3),
         "step_achieved_GBs": round(step_bytes / (ms32 * 1e-3) / 1e9, 1),
         "wall_ms_per_step": round(wall32, 4),
+        "fp32_fp64_positions": ({"value": round(npart / (msd * 1e-3) / 1e6, 2), "ms_per_step": round(msd, 4),
+                                 "phases_ms": {k: (round(v, 4) if v else v) for k, v in phd.items()},
+                                 "step_algorithmic_GB": round(step_bytes_d / 1e9, 3),
+                                 "roofline": roofline_d, "range_errors": errd} if split else None),
         "fp64": ({"value": round(res64["value"], 2), "ms_per_step": round(res64["ms_per_step"], 4),
                   "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["phases_ms"].items()},
                   "roofline": roofline64}

@@ -200,6 +200,7 @@ extern "C" int pm_plan_destroy(pm_plan_t* p) {
                       p->sorted.fill,  p->sorted.coff,   p->sorted.spill, p->sorted.stage};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (p->split.q) (void)hipFree(p->split.q);  // one allocation: q, u, gate
     void* abufs[7] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell, p->adjMax, p->adjUst};
     for (void* b : abufs)
         if (b) (void)hipFree(b);
@@ -323,8 +324,12 @@ static int ensure_sorted(pm_plan* p) {
     return rc;
 }
 
-// Other users of the binning buffers or of x end the step-to-step reuse.
-static void invalidate_reuse(pm_plan* p) { p->staged_valid = false; }
+// Other users of the binning buffers or of x end the step-to-step reuse (of
+// the FP64 positions' staging and of the split state's).
+static void invalidate_reuse(pm_plan* p) {
+    p->staged_valid = false;
+    p->split_staged = false;
+}
 
 static int deposit_impl(pm_plan* p, const double* x, const double* y, const double* z, int variant) {
     const size_t nc = (size_t)p->ng * p->ng * p->ng;
@@ -465,6 +470,82 @@ extern "C" int pm_run_cosmo(pm_plan_t* p, double* x, double* y, double* z, doubl
         const double kick = w * da * pm_function_a(a) / a;
         const double drift = s == nsteps ? 0.0 : da * pm_function_a(ah) / (ah * ah);
         const int rc = pm_step(p, x, y, z, px, py, pz, kick, drift, s == 0 ? variant : variant | PM_STEP_REUSE_BINS);
+        if (rc) return rc;
+    }
+    return PM_OK;
+}
+
+// ------------------------------------------------- split-precision state
+extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, const double* z, const double* vx,
+                             const double* vy, const double* vz) {
+    if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_split_load: null argument");
+    if (p->prec != PM_FP32) return fail(PM_ERR_ARG, "pm_split_load: the split state needs an FP32-mesh plan");
+    int rc = ensure_sorted(p);
+    if (rc) return rc;
+    if (!p->sorted.stage) return fail(PM_ERR_ARG, "pm_split_load: too many particles for the staging buckets");
+    SplitState& st = p->split;
+    if (!st.q) {
+        const size_t n = (size_t)p->np;
+        char* buf = nullptr;
+        rc = hip_ok(hipMalloc((void**)&buf, 24 * n + 64), "pm_split_load: hipMalloc");
+        if (rc) return rc;
+        st.q = reinterpret_cast<uint32_t*>(buf);
+        st.u = reinterpret_cast<float*>(buf + 12 * n);
+        st.gate = reinterpret_cast<uint32_t*>(buf + 24 * n);
+    }
+    invalidate_reuse(p);
+    rc = launch_split_load(x, y, z, vx, vy, vz, p->np, p->ng, st, p->err, p->stream);
+    if (rc) return rc;
+    p->split_loaded = true;
+    return PM_OK;
+}
+
+extern "C" int pm_split_store(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz) {
+    if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_split_store: null argument");
+    if (!p->split_loaded) return fail(PM_ERR_ARG, "pm_split_store: no particles loaded (pm_split_load)");
+    return launch_split_store(p->split, p->np, p->ng, x, y, z, vx, vy, vz, p->stream);
+}
+
+extern "C" int pm_split_step(pm_plan_t* p, double kick, double drift) {
+    if (!p) return fail(PM_ERR_ARG, "pm_split_step: null plan");
+    if (!p->split_loaded) return fail(PM_ERR_ARG, "pm_split_step: no particles loaded (pm_split_load)");
+    if (!std::isfinite(kick) || !std::isfinite(drift)) return fail(PM_ERR_ARG, "pm_split_step: bad kick or drift");
+    mark(p, 0);
+    SortedScratch& s = p->sorted;
+    const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
+    int rc = PM_OK;
+    // buckets filled by another path (or none yet): stage the positions afresh
+    if (!p->split_staged) {
+        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, p->stream, s.spill, (size_t)1, 1u);
+        rc = check_launch("pm_split_step: restage flag");
+    }
+    // restaging behind the overflow flag, then the staged deposit (no
+    // fallback of its own) and the next capacities
+    if (!rc) rc = launch_split_restage(p->split, p->np, p->ng, s, p->stream);
+    if (!rc) rc = launch_deposit_staged(p->prec, nullptr, nullptr, nullptr, p->np, p->ng, mass, p->rho, s, p->err,
+                                        p->stream);
+    p->staged_valid = false;
+    p->split_staged = false;
+    if (rc) return rc;
+    mark(p, 1);
+    rc = solve_impl(p);
+    if (rc) return rc;
+    rc = launch_split_interp(p->split, p->np, p->ng, static_cast<const float*>(p->phi), kick, drift, s, p->err,
+                             p->stream);
+    if (rc) return rc;
+    p->split_staged = true;
+    mark(p, 7);
+    next_slot(p);
+    return PM_OK;
+}
+
+extern "C" int pm_split_run_kdk(pm_plan_t* p, int nsteps, double dt) {
+    if (!p || nsteps < 0) return fail(PM_ERR_ARG, "pm_split_run_kdk: bad arguments");
+    if (nsteps == 0) return PM_OK;  // no step: no kick either
+    for (int s = 0; s <= nsteps; ++s) {
+        const double kick = (s == 0 || s == nsteps) ? 0.5 * dt : dt;
+        const double drift = s == nsteps ? 0.0 : dt;
+        const int rc = pm_split_step(p, kick, drift);
         if (rc) return rc;
     }
     return PM_OK;

@@ -66,11 +66,11 @@ __device__ __forceinline__ bool in_box3(double x, double y, double z, double box
 #ifndef PM_STREAM_NT
 #define PM_STREAM_NT 3
 #endif
-__device__ __forceinline__ double stream_load(const double* p) {
+template <typename T> __device__ __forceinline__ T stream_load(const T* p) {
     if constexpr (PM_STREAM_NT & 1) return __builtin_nontemporal_load(p);
     return *p;
 }
-__device__ __forceinline__ void stream_store(double* p, double v) {
+template <typename T> __device__ __forceinline__ void stream_store(T* p, T v) {
     if constexpr (PM_STREAM_NT & 2)
         __builtin_nontemporal_store(v, p);
     else

@@ -350,17 +350,29 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
 }
 
 // Capacities of the staging buckets for the next step: count + count/8 + 32
-// per tile (counts from the binning offsets on a first step, else from the
-// fill counters of the staged step), exclusive scan into coff[0..n]; the
-// fill counters and the overflow flag are cleared for the next staging.
+// per tile (counts from the binning offsets on a first step, from the tile
+// counts of a split-state restaging (which clears them), else from the fill
+// counters of the staged step), exclusive scan into coff[0..n]; the fill
+// counters and the overflow flag are cleared for the next staging.  A
+// non-null gate makes the kernel a no-op unless *gate != 0.
 __global__ __launch_bounds__(kScanThreads) void k_cap_scan(uint32_t* __restrict__ fill,
                                                            const uint32_t* __restrict__ off,
+                                                           uint32_t* __restrict__ cnt,
                                                            uint32_t* __restrict__ coff, uint32_t* __restrict__ spill,
-                                                           int n, int from_off) {
+                                                           int n, int from, const uint32_t* __restrict__ gate) {
+    if (gate && *gate == 0u) return;
     scan_workgroup(
         n,
         [&](int i) {
-            const uint32_t c = from_off ? off[i + 1] - off[i] : fill[i];
+            uint32_t c;
+            if (from == kCapFromOff) {
+                c = off[i + 1] - off[i];
+            } else if (from == kCapFromCnt) {
+                c = cnt[i];
+                cnt[i] = 0u;
+            } else {
+                c = fill[i];
+            }
             fill[i] = 0u;
             return c + (c >> 3) + 32u;
         },
@@ -566,10 +578,14 @@ int launch_deposit_binned(int prec, const double* px, const double* py, const do
 }
 
 int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st) {
+    return launch_cap_scan_from(ng, s, from_off ? kCapFromOff : kCapFromFill, nullptr, st);
+}
+
+int launch_cap_scan_from(int ng, SortedScratch& s, int from, const uint32_t* gate, hipStream_t st) {
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
-    hipLaunchKernelGGL(k_cap_scan, dim3(1), dim3(kScanThreads), 0, st, s.fill, s.off, s.coff, s.spill, nt * nt * nt,
-                       from_off ? 1 : 0);
+    hipLaunchKernelGGL(k_cap_scan, dim3(1), dim3(kScanThreads), 0, st, s.fill, s.off, s.cnt, s.coff, s.spill,
+                       nt * nt * nt, from, gate);
     return check_launch("cap_scan");
 }
 
@@ -579,10 +595,14 @@ int launch_deposit_staged(int prec, const double* px, const double* py, const do
     const int ts = sorted_tile(ng);
     const int nt = ng / ts, seg = walk_seg(nt);
     const uint32_t ntiles = (uint32_t)nt * nt * nt;
-    // fallback, run only if some tile overflowed its staging capacity
-    int rc = bin ? launch_bin(px, py, pz, np, ng, s, err, st, s.spill) : PM_OK;
-    if (!rc) rc = launch_deposit_binned(prec, px, py, pz, ng, mass, rho, s, st, s.spill, false);
-    if (rc) return rc;
+    // fallback, run only if some tile overflowed its staging capacity (none
+    // for the split state, px == nullptr: its restaging ran before)
+    int rc = PM_OK;
+    if (px) {
+        rc = bin ? launch_bin(px, py, pz, np, ng, s, err, st, s.spill) : PM_OK;
+        if (!rc) rc = launch_deposit_binned(prec, px, py, pz, ng, mass, rho, s, st, s.spill, false);
+        if (rc) return rc;
+    }
 #define PM_DS(RL, TSV)                                                                                               \
     hipLaunchKernelGGL((k_deposit_staged<RL, TSV>), dim3(ntiles / seg), dim3(256), 0, st, s.stage, s.coff, s.fill,    \
                        s.spill, ng, mass, (RL*)rho, (RL*)s.shell)

@@ -151,8 +151,12 @@ int launch_deposit_binned(int prec, const double* px, const double* py, const do
 // staging capacities for the next step (from the binning offsets, or from
 // the fill counts of a staged step)
 int launch_cap_scan(int ng, SortedScratch& s, bool from_off, hipStream_t st);
-// deposit from the staging arrays (binned fallback if a bucket overflowed),
-// then the next capacities
+// the same from the fill counters, the binning offsets or the tile counts
+// (s.cnt, cleared on the way), behind an optional gate
+enum CapFrom { kCapFromFill = 0, kCapFromOff = 1, kCapFromCnt = 2 };
+int launch_cap_scan_from(int ng, SortedScratch& s, int from, const uint32_t* gate, hipStream_t st);
+// deposit from the staging arrays (binned fallback from px, py, pz if a
+// bucket overflowed; px == nullptr: no fallback), then the next capacities
 // (rescan = false leaves the buckets and their fill counts for a second
 // reader, the reverse sweep's divergence deposit; launch_cap_scan follows it.
 // bin = false: the fallback binning was launched already)
@@ -215,6 +219,27 @@ int launch_undrift(double* px, double* py, double* pz, const double* vx, const d
                    const double* bx, const double* by, const double* bz, double* ubx, double* uby, double* ubz,
                    int64_t np, int ng, double drift, hipStream_t st);
 
+// Split-precision particle state (pm_split.hip): positions as u32 fixed point
+// over the box (q = x * 2^(32 - log2 ng): cell and fraction), FP32
+// velocities, each as three planes of np (q[a np + i], u[a np + i]); gate:
+// the restaging flag.
+struct SplitState {
+    uint32_t* q = nullptr;
+    float* u = nullptr;
+    uint32_t* gate = nullptr;
+};
+int launch_split_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
+                      const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err, hipStream_t s);
+int launch_split_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
+                       double* vy, double* vz, hipStream_t s);
+// the records of the current positions into freshly sized buckets, if the
+// overflow flag (s.spill) is set; clears it
+int launch_split_restage(const SplitState& st, int64_t np, int ng, SortedScratch& s, hipStream_t hs);
+// interpolation + kick + drift of the split state, staging the records of
+// the drifted positions (FP32 mesh)
+int launch_split_interp(SplitState& st, int64_t np, int ng, const float* phi, double kick, double drift,
+                        SortedScratch& s, unsigned long long* err, hipStream_t hs);
+
 // Particle kernels (pm_particles.hip).
 int launch_fill(int prec, void* p, size_t n, double v, hipStream_t st);
 int launch_deposit_atomic(int prec, const double* px, const double* py, const double* pz,
@@ -258,6 +283,9 @@ struct pm_plan {
     // order (sorted.stage) for the next deposit (PM_STEP_REUSE_BINS)
     bool staged_valid = false;
     const double* staged_x = nullptr;
+    pm::SplitState split;  // allocated by the first pm_split_load
+    bool split_loaded = false;  // split holds the particles of a pm_split_load
+    bool split_staged = false;  // the buckets hold the records of split's positions
     void* adjU[3] = {};        // adjoint meshes U_a (interpolation transpose); the
                                // fused paths put the divergence in adjU[0]
     void* adjLam = nullptr;    // lambda = S D^T U

@@ -78,6 +78,10 @@ SIGNATURES = {
     "pm_solve": (_c_int, [_c_vp]),
     "pm_interp_kick_drift": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl]),
     "pm_interp_force": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "pm_split_load": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "pm_split_store": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "pm_split_step": (_c_int, [_c_vp, _c_dbl, _c_dbl]),
+    "pm_split_run_kdk": (_c_int, [_c_vp, _c_int, _c_dbl]),
     "pm_step": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl, _c_int]),
     "pm_run_kdk": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_dbl, _c_int]),
     "pm_run_cosmo": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl, _c_int, _c_int]),
@@ -242,6 +246,21 @@ class Plan:
     def run_cosmo(self, pos, mom, a0: float, da: float, nsteps: int, variant: int = PM_DEPOSIT_SORTED):
         """Cosmological KDK in the expansion factor (pm_run_cosmo)."""
         check(lib().pm_run_cosmo(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in mom), a0, da, nsteps, variant))
+
+    # ------------------------------------------- split-precision state
+    def split_load(self, pos, vel):
+        """Copy FP64 particles into the plan's split state (pm_split_load)."""
+        check(lib().pm_split_load(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel)))
+
+    def split_store(self, pos, vel):
+        """Write the plan's split state back into FP64 arrays (pm_split_store)."""
+        check(lib().pm_split_store(self.h, *(dptr(t) for t in pos), *(dptr(t) for t in vel)))
+
+    def split_step(self, kick: float, drift: float):
+        check(lib().pm_split_step(self.h, kick, drift))
+
+    def split_run_kdk(self, nsteps: int, dt: float):
+        check(lib().pm_split_run_kdk(self.h, nsteps, dt))
 
     # ---------------------------------------------------------- adjoint
     def adjoint_deposit(self, pos, mesh_bar, xbar):

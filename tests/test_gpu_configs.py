@@ -6,7 +6,8 @@ config 2: 256^3 Zel'dovich-lattice particles on a 256^3 mesh, 0.3-cell rms,
           (final-position error budget 1e-4 cells rms).
 config 3: 512^3 particles on a 512^3 mesh, clustered Zel'dovich ICs at 1.5-cell
           rms, 20 KDK steps of dt = 0.005; the FP64 and FP32 mesh paths against
-          the FP64 oracle, and the sorted deposit against the atomic one.
+          the FP64 oracle, the split-precision state (the benchmark step)
+          against it too, and the sorted deposit against the atomic one.
 config 4: 128^3 particles on a 128^3 mesh, 10 KDK steps; the gradient of
           ||rho_final - rho_target||^2 with respect to the initial
           displacements through the adjoint kernels with per-step
@@ -162,8 +163,9 @@ def _oracle_threads():
 def test_config3_512_fp64_fp32_vs_oracle_and_sorted_vs_atomic(dev):
     """FP64 mesh path vs the FP64 oracle after 20 steps: max position error
     below 1e-8 cells (the sorted deposit rounds each weight to 2^-32 of the
-    mass before its exact sum; the oracle sums in FP64).  FP32 mesh path vs
-    the oracle: rms below 1e-4 cells (north-star budget), max below 1e-3.
+    mass before its exact sum; the oracle sums in FP64).  FP32 mesh path,
+    with FP64 positions and with the split state, vs the oracle: rms below
+    1e-4 cells (the error budget), max below 1e-3.
     Sorted vs atomic FP32 density after the first step: 1e-5 of the peak."""
     import torch
 
@@ -181,6 +183,16 @@ def test_config3_512_fp64_fp32_vs_oracle_and_sorted_vs_atomic(dev):
             torch.cuda.synchronize()
         final[prec] = [p.cpu() for p in pos]
         del pos, vel
+    # split-precision state (integer cell + 2^-23-cell fraction, FP32 velocities)
+    with pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.split_load(pos0, vel0)
+        plan.split_run_kdk(nsteps, dt)
+        pos = [torch.empty_like(p) for p in pos0]
+        vel = [torch.empty_like(v) for v in vel0]
+        plan.split_store(pos, vel)
+        torch.cuda.synchronize()
+    final["split"] = [p.cpu() for p in pos]
+    del pos, vel
     assert pmhip.range_errors() == 0
     # sorted vs atomic deposit on the positions after one step
     pos1, vel1 = [p.clone() for p in pos0], [v.clone() for v in vel0]
@@ -205,11 +217,14 @@ def test_config3_512_fp64_fp32_vs_oracle_and_sorted_vs_atomic(dev):
     ref = [torch.from_numpy(p) for p in ph]
     rms64, max64 = _periodic_rms_max(final[pmhip.PM_FP64], ref, ng)
     rms32, max32 = _periodic_rms_max(final[pmhip.PM_FP32], ref, ng)
-    print(f"FP64 vs oracle rms {rms64:.3e} max {max64:.3e}; FP32 vs oracle rms {rms32:.3e} max {max32:.3e}",
-          flush=True)
+    rmss, maxs = _periodic_rms_max(final["split"], ref, ng)
+    print(f"FP64 vs oracle rms {rms64:.3e} max {max64:.3e}; FP32 vs oracle rms {rms32:.3e} max {max32:.3e}; "
+          f"split state vs oracle rms {rmss:.3e} max {maxs:.3e}", flush=True)
     assert max64 < 1e-8
     assert rms32 < 1e-4
     assert max32 < 1e-3
+    assert rmss < 1e-4
+    assert maxs < 1e-3
 
 
 def test_config5_1024_mesh_reversal_and_gradient(dev):
