@@ -200,7 +200,7 @@ extern "C" int pm_plan_destroy(pm_plan_t* p) {
                       p->sorted.fill,  p->sorted.coff,   p->sorted.spill, p->sorted.stage};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
-    if (p->split.q) (void)hipFree(p->split.q);  // one allocation: q, u, gate
+    if (p->split_buf) (void)hipFree(p->split_buf);
     void* abufs[7] = {p->adjU[0], p->adjU[1], p->adjU[2], p->adjLam, p->adjShell, p->adjMax, p->adjUst};
     for (void* b : abufs)
         if (b) (void)hipFree(b);
@@ -324,12 +324,8 @@ static int ensure_sorted(pm_plan* p) {
     return rc;
 }
 
-// Other users of the binning buffers or of x end the step-to-step reuse (of
-// the FP64 positions' staging and of the split state's).
-static void invalidate_reuse(pm_plan* p) {
-    p->staged_valid = false;
-    p->split_staged = false;
-}
+// Other users of the binning buffers or of x end the step-to-step reuse.
+static void invalidate_reuse(pm_plan* p) { p->staged_valid = false; }
 
 static int deposit_impl(pm_plan* p, const double* x, const double* y, const double* z, int variant) {
     const size_t nc = (size_t)p->ng * p->ng * p->ng;
@@ -480,21 +476,40 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
                              const double* vy, const double* vz) {
     if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_split_load: null argument");
     if (p->prec != PM_FP32) return fail(PM_ERR_ARG, "pm_split_load: the split state needs an FP32-mesh plan");
-    int rc = ensure_sorted(p);
+    int rc = ensure_sorted(p);  // the shell buffers of the deposit's column walk
     if (rc) return rc;
-    if (!p->sorted.stage) return fail(PM_ERR_ARG, "pm_split_load: too many particles for the staging buckets");
     SplitState& st = p->split;
-    if (!st.q) {
-        const size_t n = (size_t)p->np;
-        char* buf = nullptr;
-        rc = hip_ok(hipMalloc((void**)&buf, 24 * n + 64), "pm_split_load: hipMalloc");
+    if (!p->split_buf) {
+        // bucket slots: count + count/8 + 32 per tile, as the staging buckets
+        const int ts = sorted_tile(p->ng);
+        const size_t nt = (size_t)(p->ng / ts), ntiles = nt * nt * nt, n = (size_t)p->np;
+        const size_t cap = n + n / 8 + 32 * ntiles;
+        if (cap >= 0xFFFFFFF0ull) return fail(PM_ERR_ARG, "pm_split_load: too many particles for 32-bit slots");
+        // two sets of slot planes (q, u, id: 28 B a slot), the pending list
+        // (28 B a particle), boff, bfill, bnow, ctl
+        const size_t bytes = 2 * 28 * cap + 28 * n + 4 * (3 * ntiles + 1) + 64;
+        char* b = nullptr;
+        rc = hip_ok(hipMalloc((void**)&b, bytes), "pm_split_load: hipMalloc");
         if (rc) return rc;
-        st.q = reinterpret_cast<uint32_t*>(buf);
-        st.u = reinterpret_cast<float*>(buf + 12 * n);
-        st.gate = reinterpret_cast<uint32_t*>(buf + 24 * n);
+        p->split_buf = b;
+        st.cap = (int64_t)cap;
+        st.q = reinterpret_cast<uint32_t*>(b);
+        st.u = reinterpret_cast<float*>(b + 12 * cap);
+        st.id = reinterpret_cast<uint32_t*>(b + 24 * cap);
+        st.q2 = reinterpret_cast<uint32_t*>(b + 28 * cap);
+        st.u2 = reinterpret_cast<float*>(b + 40 * cap);
+        st.id2 = reinterpret_cast<uint32_t*>(b + 52 * cap);
+        char* c = b + 56 * cap;
+        st.pq = reinterpret_cast<uint32_t*>(c);
+        st.pu = reinterpret_cast<float*>(c + 12 * n);
+        st.pid = reinterpret_cast<uint32_t*>(c + 24 * n);
+        c += 28 * n;
+        st.boff = reinterpret_cast<uint32_t*>(c);
+        st.bfill = st.boff + ntiles + 1;
+        st.bnow = st.bfill + ntiles;
+        st.ctl = st.bnow + ntiles;
     }
-    invalidate_reuse(p);
-    rc = launch_split_load(x, y, z, vx, vy, vz, p->np, p->ng, st, p->err, p->stream);
+    rc = launch_state_load(x, y, z, vx, vy, vz, p->np, p->ng, st, p->err, p->stream);
     if (rc) return rc;
     p->split_loaded = true;
     return PM_OK;
@@ -503,7 +518,7 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
 extern "C" int pm_split_store(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz) {
     if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_split_store: null argument");
     if (!p->split_loaded) return fail(PM_ERR_ARG, "pm_split_store: no particles loaded (pm_split_load)");
-    return launch_split_store(p->split, p->np, p->ng, x, y, z, vx, vy, vz, p->stream);
+    return launch_state_store(p->split, p->np, p->ng, x, y, z, vx, vy, vz, p->stream);
 }
 
 extern "C" int pm_split_step(pm_plan_t* p, double kick, double drift) {
@@ -511,29 +526,18 @@ extern "C" int pm_split_step(pm_plan_t* p, double kick, double drift) {
     if (!p->split_loaded) return fail(PM_ERR_ARG, "pm_split_step: no particles loaded (pm_split_load)");
     if (!std::isfinite(kick) || !std::isfinite(drift)) return fail(PM_ERR_ARG, "pm_split_step: bad kick or drift");
     mark(p, 0);
-    SortedScratch& s = p->sorted;
     const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
-    int rc = PM_OK;
-    // buckets filled by another path (or none yet): stage the positions afresh
-    if (!p->split_staged) {
-        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, p->stream, s.spill, (size_t)1, 1u);
-        rc = check_launch("pm_split_step: restage flag");
-    }
-    // restaging behind the overflow flag, then the staged deposit (no
-    // fallback of its own) and the next capacities
-    if (!rc) rc = launch_split_restage(p->split, p->np, p->ng, s, p->stream);
-    if (!rc) rc = launch_deposit_staged(p->prec, nullptr, nullptr, nullptr, p->np, p->ng, mass, p->rho, s, p->err,
-                                        p->stream);
-    p->staged_valid = false;
-    p->split_staged = false;
+    // buckets rebuilt first if a load or a full bucket asked for it, then
+    // the deposit straight from them
+    int rc = launch_state_compact(p->split, p->np, p->ng, p->sorted, p->stream);
+    if (!rc) rc = launch_deposit_state(p->ng, mass, p->split, p->rho, p->sorted, p->stream);
     if (rc) return rc;
     mark(p, 1);
     rc = solve_impl(p);
     if (rc) return rc;
-    rc = launch_split_interp(p->split, p->np, p->ng, static_cast<const float*>(p->phi), kick, drift, s, p->err,
+    rc = launch_state_interp(p->split, p->np, p->ng, static_cast<const float*>(p->phi), kick, drift, p->err,
                              p->stream);
     if (rc) return rc;
-    p->split_staged = true;
     mark(p, 7);
     next_slot(p);
     return PM_OK;

@@ -349,6 +349,60 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
     walk_seam_out<Rl, TS, 1, 256>(acc, scale, sw, sh);
 }
 
+// 4''. as 4, particles from the tile buckets of the split-precision state
+// (pm_split.hip: positions as u32 cell + fraction in three planes of cap
+// slots; tile t holds bfill[t] slots from boff[t], holes among them, whose
+// tile does not match): the records are formed from the positions exactly
+// (split_rec), so the weights are those of the staged deposit of the same
+// positions.  Also leaves each tile's used slot count in bnow, the slots
+// the step's interpolation takes.
+template <int TS>
+__global__ __launch_bounds__(256) void k_deposit_state(const uint32_t* __restrict__ q, int64_t cap,
+                                                       const uint32_t* __restrict__ boff,
+                                                       const uint32_t* __restrict__ bfill, uint32_t* __restrict__ bnow,
+                                                       int ng, double mass, float* __restrict__ rho,
+                                                       float* __restrict__ shell) {
+    constexpr int E = TS + 1, E3 = E * E * E;
+    __shared__ unsigned long long acc[E3];
+    const WalkSeg sw = walk_segment(ng, TS);
+    const int qb = fix_bits(mass);
+    const double scale[1] = {ldexp(1.0, -qb)};
+    float* const out[1] = {rho};
+    float* const sh[1] = {shell};
+    const int S = split_shift(ng), ksh = S + ilog2(TS);
+    const uint32_t nt = (uint32_t)sw.nt, tmask = TS - 1;
+    for (int i = threadIdx.x; i < E3; i += 256) acc[i] = 0ULL;
+    __syncthreads();
+    const double fix = ldexp(mass, qb);
+    for (int k = 0; k < sw.seg; ++k) {
+        const int tz = sw.sg * sw.seg + k;
+        const uint32_t t = ((uint32_t)sw.tx * sw.nt + sw.ty) * sw.nt + tz;
+        const uint32_t beg = boff[t], n = min(bfill[t], boff[t + 1] - beg), end = beg + n;
+        if (threadIdx.x == 0) bnow[t] = n;
+        for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += 256 * kDepBatch) {
+            uint32_t a[kDepBatch], b[kDepBatch], c[kDepBatch];
+#pragma unroll
+            for (int j = 0; j < kDepBatch; ++j) {
+                const uint32_t s = s0 + 256 * j;
+                const int64_t r = s < end ? s : beg;
+                a[j] = q[sel(cap, r, 0)];
+                b[j] = q[sel(cap, r, 1)];
+                c[j] = q[sel(cap, r, 2)];
+            }
+#pragma unroll
+            for (int j = 0; j < kDepBatch; ++j) {
+                if (s0 + 256 * j >= end || split_key(a[j], b[j], c[j], ksh, nt) != t) continue;  // a hole
+                const TileOffs o =
+                    tile_offs(StageQ{split_rec(a[j], S, tmask), split_rec(b[j], S, tmask), split_rec(c[j], S, tmask)});
+                tile_cic_add<TS>(acc, o.a, o.b, o.c, o.fx, o.fy, o.fz, fix);
+            }
+        }
+        __syncthreads();
+        walk_tile_out<float, TS, 1, 256>(acc, scale, t, sw.tx * TS, sw.ty * TS, tz * TS, ng, out, sh);
+    }
+    walk_seam_out<float, TS, 1, 256>(acc, scale, sw, sh);
+}
+
 // Capacities of the staging buckets for the next step: count + count/8 + 32
 // per tile (counts from the binning offsets on a first step, from the tile
 // counts of a split-state restaging (which clears them), else from the fill
@@ -622,6 +676,27 @@ int launch_deposit_staged(int prec, const double* px, const double* py, const do
     if (!rc) rc = launch_fold_shells(prec, ts, ng, s.shell, rho, st);
     if (!rc && rescan) rc = launch_cap_scan(ng, s, false, st);
     return rc;
+}
+
+int launch_cap_scan_raw(int n, uint32_t* fill, uint32_t* cnt, uint32_t* coff, uint32_t* flag, const uint32_t* gate,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_cap_scan, dim3(1), dim3(kScanThreads), 0, st, fill, nullptr, cnt, coff, flag, n, kCapFromCnt,
+                       gate);
+    return check_launch("cap_scan");
+}
+
+int launch_deposit_state(int ng, double mass, const SplitState& st, void* rho, SortedScratch& s, hipStream_t hs) {
+    const int ts = sorted_tile(ng);
+    const int nt = ng / ts, seg = walk_seg(nt);
+    const unsigned nsegs = (unsigned)(nt * nt * (nt / seg));
+    if (ts == 16)
+        hipLaunchKernelGGL(k_deposit_state<16>, dim3(nsegs), dim3(256), 0, hs, st.q, st.cap, st.boff, st.bfill, st.bnow,
+                           ng, mass, (float*)rho, (float*)s.shell);
+    else
+        hipLaunchKernelGGL(k_deposit_state<8>, dim3(nsegs), dim3(256), 0, hs, st.q, st.cap, st.boff, st.bfill, st.bnow,
+                           ng, mass, (float*)rho, (float*)s.shell);
+    const int rc = check_launch("deposit_state");
+    return rc ? rc : launch_fold_shells(PM_FP32, ts, ng, s.shell, rho, hs);
 }
 
 int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,

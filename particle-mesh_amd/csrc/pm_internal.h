@@ -219,26 +219,47 @@ int launch_undrift(double* px, double* py, double* pz, const double* vx, const d
                    const double* bx, const double* by, const double* bz, double* ubx, double* uby, double* ubz,
                    int64_t np, int ng, double drift, hipStream_t st);
 
-// Split-precision particle state (pm_split.hip): positions as u32 fixed point
-// over the box (q = x * 2^(32 - log2 ng): cell and fraction), FP32
-// velocities, each as three planes of np (q[a np + i], u[a np + i]); gate:
-// the restaging flag.
+// Split-precision particle state (pm_split.hip), resident in tile order:
+// positions as u32 fixed point over the box (q = x * 2^(32 - log2 ng): cell
+// and fraction) and FP32 velocities, each as three planes of cap slots
+// (q[a cap + s]), with the particle index of every slot (id, kNoId for a
+// free slot or a hole).  Tile t owns slots [boff[t], boff[t+1]) and uses
+// bfill[t] of them; bnow[t]: the slots the current step's interpolation
+// takes.  q2, u2, id2: the second set of planes for a compaction; pq, pu,
+// pid: the pending list (np entries: the particles of a load, and the
+// arrivals that found their bucket full); ctl[0]: pending count, ctl[1]:
+// compaction flag.
 struct SplitState {
+    int64_t cap = 0;
     uint32_t* q = nullptr;
     float* u = nullptr;
-    uint32_t* gate = nullptr;
+    uint32_t* id = nullptr;
+    uint32_t* q2 = nullptr;
+    float* u2 = nullptr;
+    uint32_t* id2 = nullptr;
+    uint32_t* pq = nullptr;
+    float* pu = nullptr;
+    uint32_t* pid = nullptr;
+    uint32_t* boff = nullptr;
+    uint32_t* bfill = nullptr;
+    uint32_t* bnow = nullptr;
+    uint32_t* ctl = nullptr;
 };
-int launch_split_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
+int launch_state_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
                       const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err, hipStream_t s);
-int launch_split_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
+int launch_state_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
                        double* vy, double* vz, hipStream_t s);
-// the records of the current positions into freshly sized buckets, if the
-// overflow flag (s.spill) is set; clears it
-int launch_split_restage(const SplitState& st, int64_t np, int ng, SortedScratch& s, hipStream_t hs);
-// interpolation + kick + drift of the split state, staging the records of
-// the drifted positions (FP32 mesh)
-int launch_split_interp(SplitState& st, int64_t np, int ng, const float* phi, double kick, double drift,
-                        SortedScratch& s, unsigned long long* err, hipStream_t hs);
+// rebuild the buckets from the occupied slots and the pending list, if the
+// compaction flag is set (clears it)
+int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, hipStream_t hs);
+// the sorted deposit of the buckets' particles (+ fold), leaving bnow
+int launch_deposit_state(int ng, double mass, const SplitState& st, void* rho, SortedScratch& s, hipStream_t hs);
+// interpolation + kick + drift tile by tile (FP32 mesh)
+int launch_state_interp(SplitState& st, int64_t np, int ng, const float* phi, double kick, double drift,
+                        unsigned long long* err, hipStream_t hs);
+// the capacity scan on explicit arrays, from the tile counts cnt (cleared)
+int launch_cap_scan_raw(int n, uint32_t* fill, uint32_t* cnt, uint32_t* coff, uint32_t* flag, const uint32_t* gate,
+                        hipStream_t st);
 
 // Particle kernels (pm_particles.hip).
 int launch_fill(int prec, void* p, size_t n, double v, hipStream_t st);
@@ -283,9 +304,9 @@ struct pm_plan {
     // order (sorted.stage) for the next deposit (PM_STEP_REUSE_BINS)
     bool staged_valid = false;
     const double* staged_x = nullptr;
-    pm::SplitState split;  // allocated by the first pm_split_load
+    pm::SplitState split;       // allocated by the first pm_split_load
+    void* split_buf = nullptr;  // its one allocation
     bool split_loaded = false;  // split holds the particles of a pm_split_load
-    bool split_staged = false;  // the buckets hold the records of split's positions
     void* adjU[3] = {};        // adjoint meshes U_a (interpolation transpose); the
                                // fused paths put the divergence in adjU[0]
     void* adjLam = nullptr;    // lambda = S D^T U

@@ -1,40 +1,45 @@
-// Split-precision particle state of the fused step with an FP32 mesh:
-// positions as an integer cell plus a fixed-point fraction in one u32 per
-// axis, velocities in FP32, resident in the plan across steps.
+// Split-precision particle state of the fused step with an FP32 mesh,
+// resident in the plan in tile order.
 //
-// A position x in [0, ng) is held as q = x * 2^S, S = 32 - log2(ng): the
-// cell is q >> S and the fraction the low S bits (2^-23 cells at ng = 512),
-// so the whole periodic box is the u32 range and a drift is an integer add
-// that wraps at the box edge by itself.  The state takes 24 bytes per
-// particle instead of the 48 of FP64 positions and velocities, which is what
-// the interpolation streams twice per step.  The deposit reads the same
-// in-tile records as the FP64 path (StageQ), formed from q exactly.
+// Positions: an integer cell plus a fixed-point fraction in one u32 per axis,
+// q = x * 2^S with S = 32 - log2(ng) (2^-23 cells at ng = 512): the whole
+// periodic box is the u32 range, so a drift is an integer add that wraps at
+// the box edge by itself.  Velocities in FP32.  24 bytes per particle
+// against the 48 of FP64 positions and velocities.
+//
+// The particles live in the buckets of the deposit's 16^3-cell tiles: tile t
+// owns slots [boff[t], boff[t+1]) of the position and velocity arrays (a
+// slot's three values side by side, sel()) and of the particle indices (ids,
+// for pm_split_store), of which bfill[t] are used.  Every kernel then works tile by tile on contiguous
+// slots:
+//   deposit  the column walk of the sorted deposit (pm_tiles.h), reading a
+//            tile's positions straight from its bucket;
+//   interp   one workgroup per tile stages the potential around the tile
+//            (19^3 nodes) in LDS and interpolates, kicks and drifts the
+//            tile's particles from it: no gathers from HBM, no records.  A
+//            particle that stays in its tile (all but ~0.2% per step) is
+//            written back in place; one that leaves is appended to its new
+//            tile's bucket (global atomic on bfill) and leaves a hole (its
+//            id becomes kNoId and its x the other half of the box, so that
+//            its tile no longer matches and the tile kernels skip it).
+//            The buckets' slack (count/8 + 32 slots) takes the arrivals;
+//   compact  when a bucket runs out of slack (the mover goes to the pending
+//            list and raises the flag), and after a load (every particle
+//            pending), the next step first rebuilds all buckets from the
+//            occupied slots and the pending list (count, capacity scan,
+//            placement into the second set of planes, copy back).  Every
+//            compaction kernel is a no-op unless the flag is set.
+// A tile processes only the slots it held when the step's deposit read it
+// (bnow), so an arrival is not moved twice in one step.  Results do not
+// depend on the slot order: the deposit sums exactly, the rest is per
+// particle.
 #include "pm_gather.h"
 #include "pm_internal.h"
 #include "pm_tiles.h"
 
 namespace pm {
 
-__host__ __device__ inline int split_shift(int ng) { return 32 - ilog2(ng); }
-
-// element a (axis) of particle i in a state array: three planes of np
-// (a particle's triple as one 12-byte access measured slower: 2.39 -> 2.43
-// ms for the 512^3 interpolation)
-__device__ __forceinline__ int64_t sidx(int64_t i, int a, int64_t np) { return (int64_t)a * np + i; }
-
-// tile of q: sh = S + log2(ts) = 32 - log2(tiles per axis), 32 for one tile
-__device__ __forceinline__ uint32_t split_key(uint32_t qx, uint32_t qy, uint32_t qz, int sh, uint32_t nt) {
-    const auto t = [sh](uint32_t q) { return (uint32_t)((uint64_t)q >> sh); };
-    return (t(qx) * nt + t(qy)) * nt + t(qz);
-}
-
-// In-tile record of one axis: cell in the tile << 28 | fraction at 2^-28
-// cells (truncated for S > 28), the StageQ layout of tile_record.
-__device__ __forceinline__ uint32_t split_rec(uint32_t q, int S, uint32_t tmask) {
-    const uint32_t cell = (q >> S) & tmask;
-    const uint32_t frac = q & ((1u << S) - 1u);
-    return (cell << 28) | (S >= 28 ? frac >> (S - 28) : frac << (28 - S));
-}
+constexpr uint32_t kNoId = 0xFFFFFFFFu;
 
 // FP64 position -> u32 fixed point, nearest (x * 2^S is exact).  A position
 // outside [0, ng) (or not finite) is counted as a range error and wrapped
@@ -48,210 +53,392 @@ __device__ __forceinline__ uint32_t split_pos(double x, double box, double scale
     return (uint32_t)(unsigned long long)llrint(x * scale);  // 2^32 wraps to 0: the box is periodic
 }
 
-__global__ __launch_bounds__(256) void k_split_load(const double* __restrict__ x, const double* __restrict__ y,
+// load: every particle into the pending list (the first step compacts it
+// into the buckets)
+__global__ __launch_bounds__(256) void k_state_load(const double* __restrict__ x, const double* __restrict__ y,
                                                     const double* __restrict__ z, const double* __restrict__ vx,
                                                     const double* __restrict__ vy, const double* __restrict__ vz,
-                                                    int64_t np, int ng, uint32_t* __restrict__ q,
-                                                    float* __restrict__ u, unsigned long long* __restrict__ err) {
+                                                    int64_t np, int ng, uint32_t* __restrict__ pq,
+                                                    float* __restrict__ pu, uint32_t* __restrict__ pid,
+                                                    unsigned long long* __restrict__ err) {
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (i >= np) return;
     const double box = (double)ng, scale = ldexp(1.0, split_shift(ng));
     bool bad = false;
-    q[sidx(i, 0, np)] = split_pos(x[i], box, scale, bad);
-    q[sidx(i, 1, np)] = split_pos(y[i], box, scale, bad);
-    q[sidx(i, 2, np)] = split_pos(z[i], box, scale, bad);
-    u[sidx(i, 0, np)] = (float)vx[i];
-    u[sidx(i, 1, np)] = (float)vy[i];
-    u[sidx(i, 2, np)] = (float)vz[i];
+    pq[sel(np, i, 0)] = split_pos(x[i], box, scale, bad);
+    pq[sel(np, i, 1)] = split_pos(y[i], box, scale, bad);
+    pq[sel(np, i, 2)] = split_pos(z[i], box, scale, bad);
+    pu[sel(np, i, 0)] = (float)vx[i];
+    pu[sel(np, i, 1)] = (float)vy[i];
+    pu[sel(np, i, 2)] = (float)vz[i];
+    pid[i] = (uint32_t)i;
     if (bad) atomicAdd(err, 1ULL);
 }
 
-__global__ __launch_bounds__(256) void k_split_store(const uint32_t* __restrict__ q, const float* __restrict__ u,
-                                                     int64_t np, int ng, double* __restrict__ x,
-                                                     double* __restrict__ y, double* __restrict__ z,
-                                                     double* __restrict__ vx, double* __restrict__ vy,
-                                                     double* __restrict__ vz) {
-    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (i >= np) return;
+// store: occupied bucket slots, then the pending list (non-empty only after
+// an overflow that no step has compacted yet)
+__global__ __launch_bounds__(256) void k_state_store(const uint32_t* __restrict__ q, const float* __restrict__ u,
+                                                     const uint32_t* __restrict__ id, int64_t cap,
+                                                     const uint32_t* __restrict__ pq, const float* __restrict__ pu,
+                                                     const uint32_t* __restrict__ pid, int64_t np,
+                                                     const uint32_t* __restrict__ ctl, int ng,
+                                                     double* __restrict__ x, double* __restrict__ y,
+                                                     double* __restrict__ z, double* __restrict__ vx,
+                                                     double* __restrict__ vy, double* __restrict__ vz) {
     const double inv = ldexp(1.0, -split_shift(ng));
-    x[i] = (double)q[sidx(i, 0, np)] * inv;
-    y[i] = (double)q[sidx(i, 1, np)] * inv;
-    z[i] = (double)q[sidx(i, 2, np)] * inv;
-    vx[i] = (double)u[sidx(i, 0, np)];
-    vy[i] = (double)u[sidx(i, 1, np)];
-    vz[i] = (double)u[sidx(i, 2, np)];
+    const int64_t npend = ctl[0];
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < cap + npend; k += (int64_t)gridDim.x * 256) {
+        const bool st = k < cap;
+        const int64_t s = st ? k : k - cap, stride = st ? cap : np;
+        const uint32_t* qq = st ? q : pq;
+        const float* uu = st ? u : pu;
+        const uint32_t i = st ? id[s] : pid[s];
+        if (i == kNoId) continue;
+        x[i] = (double)qq[sel(stride, s, 0)] * inv;
+        y[i] = (double)qq[sel(stride, s, 1)] * inv;
+        z[i] = (double)qq[sel(stride, s, 2)] * inv;
+        vx[i] = (double)uu[sel(stride, s, 0)];
+        vy[i] = (double)uu[sel(stride, s, 1)];
+        vz[i] = (double)uu[sel(stride, s, 2)];
+    }
 }
 
-// Restaging (the first step after a load, or a step after a bucket
-// overflow; every kernel a no-op unless *gate != 0).  The gate is copied
-// from the overflow flag first, because the capacity scan clears that flag.
-__global__ void k_split_gate(const uint32_t* __restrict__ spill, uint32_t* __restrict__ gate) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *gate = *spill;
+// ------------------------------------------------------------ compaction
+// ctl[0]: pending count, ctl[1]: compaction flag, ctl[2]: the flag that the
+// capacity scan clears (unused)
+template <typename F>
+__device__ __forceinline__ void for_each_entry(int64_t cap, const uint32_t* __restrict__ ctl, F f) {
+    const int64_t n = cap + (int64_t)ctl[0];
+    for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x; k0 < n; k0 += (int64_t)gridDim.x * blockDim.x)
+        f(k0 + threadIdx.x, n);
 }
 
-// tile counts: runs of equal tiles in consecutive lanes add once
-__global__ __launch_bounds__(256) void k_split_count(const uint32_t* __restrict__ q, int64_t np, int ng, int ts,
-                                                     uint32_t* __restrict__ cnt, const uint32_t* __restrict__ gate) {
-    constexpr uint32_t kNo = 0xFFFFFFFFu;
-    if (*gate == 0u) return;
+// tile counts of the occupied slots and the pending entries
+__global__ __launch_bounds__(256) void k_state_count(const uint32_t* __restrict__ q, const uint32_t* __restrict__ id,
+                                                     int64_t cap, const uint32_t* __restrict__ pq, int64_t np,
+                                                     const uint32_t* __restrict__ ctl, int ng, int ts,
+                                                     uint32_t* __restrict__ cnt) {
+    if (ctl[1] == 0u) return;
     const int lt = __builtin_ctz((unsigned)ts), sh = split_shift(ng) + lt;
     const uint32_t nt = (uint32_t)(ng >> lt);
     const uint32_t lane = __lane_id();
-    // grid-stride over blocks of 256 (a capped grid: the gated no-op launch
-    // of a normal step stays cheap)
-    for (int64_t i0 = (int64_t)blockIdx.x * 256; i0 < np; i0 += (int64_t)gridDim.x * 256) {
-        const int64_t i = i0 + threadIdx.x;
-        const uint32_t kk =
-            i < np ? split_key(q[sidx(i, 0, np)], q[sidx(i, 1, np)], q[sidx(i, 2, np)], sh, nt) : kNo;
+    for_each_entry(cap, ctl, [&](int64_t k, int64_t n) {
+        uint32_t kk = kNoId;
+        if (k < cap) {
+            if (id[k] != kNoId) kk = split_key(q[sel(cap, k, 0)], q[sel(cap, k, 1)], q[sel(cap, k, 2)], sh, nt);
+        } else if (k < n) {
+            const int64_t p = k - cap;
+            kk = split_key(pq[sel(np, p, 0)], pq[sel(np, p, 1)], pq[sel(np, p, 2)], sh, nt);
+        }
+        // runs of equal tiles in consecutive lanes add once
         const uint32_t prev = __shfl_up(kk, 1);
-        const bool head = kk != kNo && (lane == 0 || prev != kk);
-        const unsigned long long bounds = __ballot(head || kk == kNo);
+        const bool head = kk != kNoId && (lane == 0 || prev != kk);
+        const unsigned long long bounds = __ballot(head || kk == kNoId);
         if (head) {
             const unsigned long long above = lane == 63 ? 0ull : (bounds >> (lane + 1)) << (lane + 1);
             const uint32_t run = (above ? (uint32_t)__ffsll((long long)above) - 1u : 64u) - lane;
             atomicAdd(&cnt[kk], run);
         }
-    }
+    });
 }
 
-constexpr int kSplitThreads = 512, kSplitHash = 128;
-
-// records of the current positions into the buckets (after the capacity scan)
-__global__ __launch_bounds__(kSplitThreads) void k_split_stage(const uint32_t* __restrict__ q, int64_t np, int ng,
-                                                               int ts, uint32_t* __restrict__ fill,
-                                                               const uint32_t* __restrict__ coff,
-                                                               uint32_t* __restrict__ spill,
-                                                               StageQ* __restrict__ stage,
-                                                               const uint32_t* __restrict__ gate) {
-    constexpr uint32_t kNo = 0xFFFFFFFFu;
-    __shared__ uint32_t hkey[kSplitHash], hcnt[kSplitHash], hbase[kSplitHash];
-    if (*gate == 0u) return;
-    const int S = split_shift(ng), lt = __builtin_ctz((unsigned)ts);
-    const uint32_t nt = (uint32_t)(ng >> lt), tmask = (uint32_t)ts - 1u;
-    // grid-stride over blocks (capped grid, as in k_split_count); the loop
-    // bound is uniform in the workgroup, as the reservation's barriers need
-    for (int64_t i0 = (int64_t)blockIdx.x * kSplitThreads; i0 < np; i0 += (int64_t)gridDim.x * kSplitThreads) {
-        const int64_t i = i0 + threadIdx.x;
-        uint32_t kk = kNo;
-        StageQ rec{};
-        if (i < np) {
-            const uint32_t a = q[sidx(i, 0, np)], b = q[sidx(i, 1, np)], c = q[sidx(i, 2, np)];
-            kk = split_key(a, b, c, S + lt, nt);
-            rec = StageQ{split_rec(a, S, tmask), split_rec(b, S, tmask), split_rec(c, S, tmask)};
+// every entry into its tile's bucket of the second set of planes
+__global__ __launch_bounds__(256) void k_state_place(const uint32_t* __restrict__ q, const float* __restrict__ u,
+                                                     const uint32_t* __restrict__ id, int64_t cap,
+                                                     const uint32_t* __restrict__ pq, const float* __restrict__ pu,
+                                                     const uint32_t* __restrict__ pid, int64_t np,
+                                                     const uint32_t* __restrict__ ctl, int ng, int ts,
+                                                     const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill,
+                                                     uint32_t* __restrict__ q2, float* __restrict__ u2,
+                                                     uint32_t* __restrict__ id2) {
+    if (ctl[1] == 0u) return;
+    const int lt = __builtin_ctz((unsigned)ts), sh = split_shift(ng) + lt;
+    const uint32_t nt = (uint32_t)(ng >> lt);
+    const uint32_t lane = __lane_id();
+    for_each_entry(cap, ctl, [&](int64_t k, int64_t n) {
+        uint32_t kk = kNoId, a = 0u, b = 0u, c = 0u, i = kNoId;
+        float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+        if (k < cap) {
+            i = id[k];
+            if (i != kNoId) {
+                a = q[sel(cap, k, 0)];
+                b = q[sel(cap, k, 1)];
+                c = q[sel(cap, k, 2)];
+                v0 = u[sel(cap, k, 0)];
+                v1 = u[sel(cap, k, 1)];
+                v2 = u[sel(cap, k, 2)];
+            }
+        } else if (k < n) {
+            const int64_t p = k - cap;
+            i = pid[p];
+            a = pq[sel(np, p, 0)];
+            b = pq[sel(np, p, 1)];
+            c = pq[sel(np, p, 2)];
+            v0 = pu[sel(np, p, 0)];
+            v1 = pu[sel(np, p, 1)];
+            v2 = pu[sel(np, p, 2)];
         }
-        const int64_t pos = stage_pass<kSplitHash, kSplitThreads>(kk, hkey, hcnt, hbase, fill, coff, spill);
-        if (pos >= 0) stage[pos] = rec;
+        if (i != kNoId) kk = split_key(a, b, c, sh, nt);
+        const uint32_t prev = __shfl_up(kk, 1);
+        const bool head = kk != kNoId && (lane == 0 || prev != kk);
+        const unsigned long long bounds = __ballot(head || kk == kNoId);
+        uint32_t base = 0u;
+        if (head) {
+            const unsigned long long above = lane == 63 ? 0ull : (bounds >> (lane + 1)) << (lane + 1);
+            const uint32_t run = (above ? (uint32_t)__ffsll((long long)above) - 1u : 64u) - lane;
+            base = atomicAdd(&bfill[kk], run);
+        }
+        const unsigned long long upto = bounds & (lane == 63 ? ~0ull : (2ull << lane) - 1ull);
+        const int hl = upto ? 63 - __clzll((long long)upto) : 0;
+        const uint32_t slot = __shfl(base, hl) + (lane - (uint32_t)hl);
+        if (kk != kNoId) {  // capacities are the counts + slack: every entry fits
+            const int64_t s = (int64_t)boff[kk] + slot;
+            q2[sel(cap, s, 0)] = a;
+            q2[sel(cap, s, 1)] = b;
+            q2[sel(cap, s, 2)] = c;
+            u2[sel(cap, s, 0)] = v0;
+            u2[sel(cap, s, 1)] = v1;
+            u2[sel(cap, s, 2)] = v2;
+            id2[s] = i;
+        }
+    });
+}
+
+// the second set of planes back into the first: 7 planes of cap words
+// (q, u, id; unused slots carry kNoId ids)
+__global__ __launch_bounds__(256) void k_state_copy(const uint32_t* __restrict__ ctl, const uint32_t* __restrict__ src,
+                                                    uint32_t* __restrict__ dst, int64_t n) {
+    if (ctl[1] == 0u) return;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) dst[k] = src[k];
+}
+
+// ids of every slot to kNoId (behind the flag if ctl is given)
+__global__ __launch_bounds__(256) void k_state_clear_ids(const uint32_t* __restrict__ ctl, uint32_t* __restrict__ id,
+                                                         int64_t n) {
+    if (ctl && ctl[1] == 0u) return;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) id[k] = kNoId;
+}
+
+__global__ void k_state_done(uint32_t* __restrict__ ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && ctl[1] != 0u) {
+        ctl[0] = 0u;
+        ctl[1] = 0u;
     }
 }
 
-// update_particles (utils.cpp:141-170) on the split state: the force of
-// cic_force at the cell and fraction of q (pm_gather.h), the kick in FP32,
-// then the drift as the integer add of the displacement rounded to 2^-S
-// cells (wrapping at the box edge), and the drifted position's record
-// staged for the next deposit, as the FP64 staging interpolation does.  A
-// kick that leaves a velocity non-finite is counted as a range error and
-// that particle keeps its velocity and position.
-template <bool DRIFT>
-__global__ __launch_bounds__(kSplitThreads) void k_split_interp(
-    uint32_t* __restrict__ q, float* __restrict__ u, int64_t np, int ng, const float* __restrict__ phi, float kick,
-    float dscale, int ts, uint32_t* __restrict__ fill, const uint32_t* __restrict__ coff,
-    uint32_t* __restrict__ spill, StageQ* __restrict__ stage, unsigned long long* __restrict__ err, BandMap bm) {
-    constexpr uint32_t kNo = 0xFFFFFFFFu;
-    __shared__ uint32_t hkey[kSplitHash], hcnt[kSplitHash], hbase[kSplitHash];
-    const int64_t i = (int64_t)band_remap(blockIdx.x, gridDim.x, bm) * kSplitThreads + threadIdx.x;
-    const bool live = i < np;
-    const int64_t j = live ? i : 0;
-    const uint32_t a = stream_load(q + sidx(j, 0, np)), b = stream_load(q + sidx(j, 1, np)),
-                   c = stream_load(q + sidx(j, 2, np));
-    const float u0 = stream_load(u + sidx(j, 0, np)), u1 = stream_load(u + sidx(j, 1, np)),
-                u2 = stream_load(u + sidx(j, 2, np));
-    const int S = split_shift(ng), lt = __builtin_ctz((unsigned)ts);
+// ---------------------------------------------------------------- interp
+// update_particles (utils.cpp:141-170) on a tile's particles: the force of
+// cic_force (centered_derivatives + trilerp1, the arithmetic of pm_gather.h
+// on the same 32 potential values, here read from the tile's LDS copy), the
+// kick in FP32, then the drift as the integer add of the displacement
+// rounded to 2^-S cells.  A kick that leaves a velocity non-finite is
+// counted as a range error and leaves that particle as it was.
+constexpr int kTileThreads = 256, kTileBatch = 4;
+
+template <int TS, bool DRIFT>
+__global__ __launch_bounds__(kTileThreads) void k_state_interp(
+    uint32_t* __restrict__ q, float* __restrict__ u, uint32_t* __restrict__ id, int64_t cap,
+    const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill, const uint32_t* __restrict__ bnow,
+    uint32_t* __restrict__ pq, float* __restrict__ pu, uint32_t* __restrict__ pid, int64_t np,
+    uint32_t* __restrict__ ctl, int ng, const float* __restrict__ phi, float kick, float dscale,
+    unsigned long long* __restrict__ err) {
+    constexpr int W = TS + 3;  // nodes -1 .. TS+1 around the tile's cells
+    __shared__ float ph[W * W * W];
+    const int lt = ilog2(TS), S = split_shift(ng), sh = S + lt;
+    const uint32_t nt = (uint32_t)(ng >> lt);
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = (int)(t / (nt * nt)), ty = (int)((t / nt) % nt), tz = (int)(t % nt);
+    const int m = ng - 1;
+    // all of a thread's staging loads in flight at once, then the LDS writes
+    constexpr int NE = W * W * W, NIT = (NE + kTileThreads - 1) / kTileThreads;
+    float pv[NIT];
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+        const int e = threadIdx.x + j * kTileThreads;
+        if (e < NE) {
+            const int a = e / (W * W), r = e - a * W * W, b = r / W, c = r - b * W;
+            const int gx = (tx * TS - 1 + a) & m, gy = (ty * TS - 1 + b) & m, gz = (tz * TS - 1 + c) & m;
+            pv[j] = phi[((size_t)gx * ng + gy) * ng + gz];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+        const int e = threadIdx.x + j * kTileThreads;
+        if (e < NE) ph[e] = pv[j];
+    }
+    __syncthreads();
     const uint32_t fmask = (1u << S) - 1u;
     const float fs = ldexpf(1.0f, -S);
-    const CellIdx<float> cc = cell_at<float>((int)(a >> S), (int)(b >> S), (int)(c >> S), (float)(a & fmask) * fs,
-                                             (float)(b & fmask) * fs, (float)(c & fmask) * fs, ng);
-    float cl[2][2][4], xr[2][2][2], yr[2][2][2];
-    gather_runs_main(phi, ng, cc, cl, xr, yr);
-    // the shorter seam reload (the full one: 2.39 -> 2.48 ms at 512^3)
-    seam_reload_missing(phi, ng, cc, cl, xr, yr);
-    float f[3];
-    force_from_runs(cl, xr, yr, cc.tx, cc.ty, cc.tz, f);
-    float n0 = fmaf(f[0], kick, u0), n1 = fmaf(f[1], kick, u1), n2 = fmaf(f[2], kick, u2);
-    const bool fin = isfinite(n0) && isfinite(n1) && isfinite(n2);
-    if (!fin) {
-        n0 = u0;
-        n1 = u1;
-        n2 = u2;
-    }
-    uint32_t na = a, nb = b, nc = c;
-    if (DRIFT && fin) {
-        na = a + (uint32_t)__float2int_rn(n0 * dscale);
-        nb = b + (uint32_t)__float2int_rn(n1 * dscale);
-        nc = c + (uint32_t)__float2int_rn(n2 * dscale);
-    }
-    uint32_t kk = kNo;
-    StageQ rec{};
-    if (live) {
-        stream_store(u + sidx(i, 0, np), n0);
-        stream_store(u + sidx(i, 1, np), n1);
-        stream_store(u + sidx(i, 2, np), n2);
-        if (DRIFT) {
-            stream_store(q + sidx(i, 0, np), na);
-            stream_store(q + sidx(i, 1, np), nb);
-            stream_store(q + sidx(i, 2, np), nc);
+    const int64_t beg = boff[t], end = beg + bnow[t];
+    unsigned nerr = 0u;
+    for (int64_t s0 = beg + threadIdx.x; s0 < end; s0 += (int64_t)kTileThreads * kTileBatch) {
+        uint32_t qa[kTileBatch], qb[kTileBatch], qc[kTileBatch];
+        float ua[kTileBatch], ub[kTileBatch], uc[kTileBatch];
+#pragma unroll
+        for (int j = 0; j < kTileBatch; ++j) {
+            const int64_t s = s0 + (int64_t)kTileThreads * j;
+            const int64_t r = s < end ? s : beg;
+            qa[j] = stream_load(q + sel(cap, r, 0));
+            qb[j] = stream_load(q + sel(cap, r, 1));
+            qc[j] = stream_load(q + sel(cap, r, 2));
+            ua[j] = stream_load(u + sel(cap, r, 0));
+            ub[j] = stream_load(u + sel(cap, r, 1));
+            uc[j] = stream_load(u + sel(cap, r, 2));
         }
-        if (!fin) atomicAdd(err, 1ULL);
-        const uint32_t nt = (uint32_t)(ng >> lt), tmask = (uint32_t)ts - 1u;
-        kk = split_key(na, nb, nc, S + lt, nt);
-        rec = StageQ{split_rec(na, S, tmask), split_rec(nb, S, tmask), split_rec(nc, S, tmask)};
+#pragma unroll
+        for (int j = 0; j < kTileBatch; ++j) {
+            const int64_t s = s0 + (int64_t)kTileThreads * j;
+            const uint32_t a = qa[j], b = qb[j], c = qc[j];
+            if (s >= end || split_key(a, b, c, sh, nt) != t) continue;  // past the end, or a hole
+            const int lx = (int)(a >> S) - tx * TS, ly = (int)(b >> S) - ty * TS, lz = (int)(c >> S) - tz * TS;
+            const float fx = (float)(a & fmask) * fs, fy = (float)(b & fmask) * fs, fz = (float)(c & fmask) * fs;
+            // the 12 z-runs of gather_runs_main from LDS: node (ix+x, iy+y,
+            // iz+z) of the cell at local (lx, ly, lz) is L(lx+1+x, ly+1+y, lz+1+z)
+            float cl[2][2][4], xr[2][2][2], yr[2][2][2];
+            const auto L = [&](int x, int y, int z) { return ph[(x * W + y) * W + z]; };
+#pragma unroll
+            for (int aa = 0; aa < 2; ++aa)
+#pragma unroll
+                for (int bb = 0; bb < 2; ++bb) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) cl[aa][bb][k] = L(lx + 1 + aa, ly + 1 + bb, lz + k);
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        xr[aa][bb][k] = L(lx + 3 * aa, ly + 1 + bb, lz + 1 + k);
+                        yr[aa][bb][k] = L(lx + 1 + bb, ly + 3 * aa, lz + 1 + k);
+                    }
+                }
+            float f[3];
+            force_from_runs(cl, xr, yr, fx, fy, fz, f);
+            float n0 = fmaf(f[0], kick, ua[j]), n1 = fmaf(f[1], kick, ub[j]), n2 = fmaf(f[2], kick, uc[j]);
+            const bool fin = isfinite(n0) && isfinite(n1) && isfinite(n2);
+            if (!fin) {
+                n0 = ua[j];
+                n1 = ub[j];
+                n2 = uc[j];
+                ++nerr;
+            }
+            uint32_t na = a, nb = b, nc = c;
+            if (DRIFT && fin) {
+                na = a + (uint32_t)__float2int_rn(n0 * dscale);
+                nb = b + (uint32_t)__float2int_rn(n1 * dscale);
+                nc = c + (uint32_t)__float2int_rn(n2 * dscale);
+            }
+            const uint32_t t2 = DRIFT ? split_key(na, nb, nc, sh, nt) : t;
+            if (t2 == t) {
+                stream_store(u + sel(cap, s, 0), n0);
+                stream_store(u + sel(cap, s, 1), n1);
+                stream_store(u + sel(cap, s, 2), n2);
+                if (DRIFT) {
+                    stream_store(q + sel(cap, s, 0), na);
+                    stream_store(q + sel(cap, s, 1), nb);
+                    stream_store(q + sel(cap, s, 2), nc);
+                }
+                continue;
+            }
+            // a mover: appended to its new tile's bucket (or, if that is
+            // full, to the pending list, which makes the next step compact);
+            // its slot becomes a hole
+            const uint32_t i = id[s];
+            const uint32_t slot = atomicAdd(&bfill[t2], 1u);
+            const int64_t c2 = (int64_t)boff[t2 + 1] - boff[t2];
+            uint32_t *dq = q, *di = id;
+            float* du = u;
+            int64_t ds, stride = cap;
+            if ((int64_t)slot < c2) {
+                ds = (int64_t)boff[t2] + slot;
+            } else {
+                ds = atomicAdd(&ctl[0], 1u);  // pending entries never exceed np
+                dq = pq;
+                du = pu;
+                di = pid;
+                stride = np;
+                ctl[1] = 1u;
+            }
+            dq[sel(stride, ds, 0)] = na;
+            dq[sel(stride, ds, 1)] = nb;
+            dq[sel(stride, ds, 2)] = nc;
+            du[sel(stride, ds, 0)] = n0;
+            du[sel(stride, ds, 1)] = n1;
+            du[sel(stride, ds, 2)] = n2;
+            di[ds] = i;
+            q[sel(cap, s, 0)] = a ^ 0x80000000u;  // x in the other half of the box: another tile
+            id[s] = kNoId;
+        }
     }
-    const int64_t pos = stage_pass<kSplitHash, kSplitThreads>(kk, hkey, hcnt, hbase, fill, coff, spill);
-    if (pos >= 0) stage[pos] = rec;
+    if (nerr) atomicAdd(err, (unsigned long long)nerr);
 }
 
 // ================================================================ launchers
-int launch_split_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
-                      const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err,
-                      hipStream_t s) {
-    hipLaunchKernelGGL(k_split_load, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, x, y, z, vx, vy, vz, np,
-                       ng, st.q, st.u, err);
+static unsigned grid_for(int64_t n, unsigned cap_blocks) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap_blocks));
+}
+
+int launch_state_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
+                      const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err, hipStream_t s) {
+    hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 16384)), dim3(256), 0, s, nullptr, st.id, st.cap);
+    hipLaunchKernelGGL(k_state_load, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, x, y, z, vx, vy, vz, np, ng,
+                       st.pq, st.pu, st.pid, err);
+    // every particle pending; the next step compacts
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, st.ctl, (size_t)1, (uint32_t)np);
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, st.ctl + 1, (size_t)1, 1u);
     return check_launch("split_load");
 }
 
-int launch_split_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
+int launch_state_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
                        double* vy, double* vz, hipStream_t s) {
-    hipLaunchKernelGGL(k_split_store, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, st.q, st.u, np, ng, x, y,
-                       z, vx, vy, vz);
+    hipLaunchKernelGGL(k_state_store, dim3(grid_for(st.cap + np, 16384)), dim3(256), 0, s, st.q, st.u, st.id, st.cap,
+                       st.pq, st.pu, st.pid, np, st.ctl, ng, x, y, z, vx, vy, vz);
     return check_launch("split_store");
 }
 
-int launch_split_restage(const SplitState& st, int64_t np, int ng, SortedScratch& s, hipStream_t hs) {
+int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, hipStream_t hs) {
     const int ts = sorted_tile(ng);
-    hipLaunchKernelGGL(k_split_gate, dim3(1), dim3(64), 0, hs, s.spill, st.gate);
-    // the tile counts are zero here: cleared at allocation and by every
-    // count-consuming scan
-    hipLaunchKernelGGL(k_split_count, dim3((unsigned)std::min<int64_t>((np + 255) / 256, 2048)), dim3(256), 0, hs,
-                       st.q, np, ng, ts, s.cnt, st.gate);
+    const int nt = ng / ts;
+    const int64_t n = st.cap + np;
+    // the tile counts (s.cnt) are zero here: cleared at allocation and by
+    // every count-consuming scan
+    // capped grids (grid-stride loops): in a step without compaction every
+    // kernel here is a no-op, and a workgroup that only exits still costs
+    // its launch
+    hipLaunchKernelGGL(k_state_count, dim3(grid_for(n, 1024)), dim3(256), 0, hs, st.q, st.id, st.cap, st.pq, np,
+                       st.ctl, ng, ts, s.cnt);
     int rc = check_launch("split_count");
-    if (!rc) rc = launch_cap_scan_from(ng, s, kCapFromCnt, st.gate, hs);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_split_stage, dim3((unsigned)std::min<int64_t>((np + kSplitThreads - 1) / kSplitThreads, 2048)),
-                       dim3(kSplitThreads), 0, hs, st.q, np, ng, ts, s.fill, s.coff, s.spill,
-                       static_cast<StageQ*>(s.stage), st.gate);
-    return check_launch("split_stage");
+    // capacities (count + count/8 + 32) into boff, bfill cleared
+    rc = launch_cap_scan_raw(nt * nt * nt, st.bfill, s.cnt, st.boff, st.ctl + 2, st.ctl + 1, hs);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 1024)), dim3(256), 0, hs, st.ctl, st.id2, st.cap);
+    hipLaunchKernelGGL(k_state_place, dim3(grid_for(n, 1024)), dim3(256), 0, hs, st.q, st.u, st.id, st.cap, st.pq,
+                       st.pu, st.pid, np, st.ctl, ng, ts, st.boff, st.bfill, st.q2, st.u2, st.id2);
+    // q, u, id are consecutive, and so are q2, u2, id2: one copy of 7 planes
+    hipLaunchKernelGGL(k_state_copy, dim3(grid_for(7 * st.cap, 2048)), dim3(256), 0, hs, st.ctl, st.q2, st.q,
+                       7 * st.cap);
+    hipLaunchKernelGGL(k_state_done, dim3(1), dim3(64), 0, hs, st.ctl);
+    return check_launch("split_compact");
 }
 
-int launch_split_interp(SplitState& st, int64_t np, int ng, const float* phi, double kick, double drift,
-                        SortedScratch& s, unsigned long long* err, hipStream_t hs) {
-    const unsigned nb = (unsigned)((np + kSplitThreads - 1) / kSplitThreads);
-    const BandMap bm = band_map(np, nb, kSplitThreads);
+int launch_state_interp(SplitState& st, int64_t np, int ng, const float* phi, double kick, double drift,
+                        unsigned long long* err, hipStream_t hs) {
+    const int ts = sorted_tile(ng);
+    const unsigned ntiles = (unsigned)((ng / ts) * (ng / ts) * (ng / ts));
     const float dscale = (float)ldexp(drift, split_shift(ng));
-    if (drift != 0.0)
-        hipLaunchKernelGGL(k_split_interp<true>, dim3(nb), dim3(kSplitThreads), 0, hs, st.q, st.u, np, ng, phi, (float)kick, dscale, sorted_tile(ng), s.fill,
-                           s.coff, s.spill, static_cast<StageQ*>(s.stage), err, bm);
-    else
-        hipLaunchKernelGGL(k_split_interp<false>, dim3(nb), dim3(kSplitThreads), 0, hs, st.q, st.u, np, ng, phi, (float)kick, dscale, sorted_tile(ng), s.fill,
-                           s.coff, s.spill, static_cast<StageQ*>(s.stage), err, bm);
+#define PM_SI(TSV, DR)                                                                                              \
+    hipLaunchKernelGGL((k_state_interp<TSV, DR>), dim3(ntiles), dim3(kTileThreads), 0, hs, st.q, st.u, st.id, st.cap, \
+                       st.boff, st.bfill, st.bnow, st.pq, st.pu, st.pid, np, st.ctl, ng, phi, (float)kick, dscale,  \
+                       err)
+    if (ts == 16) {
+        if (drift != 0.0)
+            PM_SI(16, true);
+        else
+            PM_SI(16, false);
+    } else {
+        if (drift != 0.0)
+            PM_SI(8, true);
+        else
+            PM_SI(8, false);
+    }
+#undef PM_SI
     return check_launch("split_interp");
 }
 

@@ -296,6 +296,31 @@ template <typename Rec> __device__ __forceinline__ Rec tile_record(double x, dou
     return tile_record_of<Rec>(x - ((int)x & msk), y - ((int)y & msk), z - ((int)z & msk));
 }
 
+// Split-precision positions: one u32 per axis, q = x * 2^S with
+// S = 32 - log2(ng), so the cell is q >> S and the fraction the low S bits.
+__host__ __device__ inline int split_shift(int ng) { return 32 - ilog2(ng); }
+// tile of q: sh = S + log2(ts) = 32 - log2(tiles per axis), 32 for one tile
+__device__ __forceinline__ uint32_t split_key(uint32_t qx, uint32_t qy, uint32_t qz, int sh, uint32_t nt) {
+    const auto t = [sh](uint32_t q) { return (uint32_t)((uint64_t)q >> sh); };
+    return (t(qx) * nt + t(qy)) * nt + t(qz);
+}
+// Element a (axis) of slot s in an array of n slots of the split state: the
+// three values of a slot side by side (a tile's bucket is then one
+// contiguous stream per array), or three planes of n.
+#ifndef PM_STATE_TRIPLES
+#define PM_STATE_TRIPLES 1
+#endif
+__host__ __device__ inline int64_t sel(int64_t n, int64_t s, int a) {
+    return PM_STATE_TRIPLES ? 3 * s + a : (int64_t)a * n + s;
+}
+// In-tile record of one axis: cell in the tile << 28 | fraction at 2^-28
+// cells (truncated for S > 28), the StageQ layout of tile_record.
+__device__ __forceinline__ uint32_t split_rec(uint32_t q, int S, uint32_t tmask) {
+    const uint32_t cell = (q >> S) & tmask;
+    const uint32_t frac = q & ((1u << S) - 1u);
+    return (cell << 28) | (S >= 28 ? frac >> (S - 28) : frac << (28 - S));
+}
+
 // Cell in the tile and offsets in the cell of a record: the deposits take
 // every particle through a record (read from the staging, or formed from its
 // position), so that staged and binned paths give the same weights.
