@@ -480,25 +480,24 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
     if (rc) return rc;
     SplitState& st = p->split;
     if (!p->split_buf) {
-        // bucket slots: count + count/8 + 32 per tile, as the staging buckets
         const int ts = sorted_tile(p->ng);
         const size_t nt = (size_t)(p->ng / ts), ntiles = nt * nt * nt, n = (size_t)p->np;
-        const size_t cap = n + n / 8 + 32 * ntiles;
+        const size_t cap = split_capacity(n, ntiles);
         if (cap >= 0xFFFFFFF0ull) return fail(PM_ERR_ARG, "pm_split_load: too many particles for 32-bit slots");
-        // two sets of slot planes (q, u, id: 28 B a slot), the pending list
-        // (28 B a particle), boff, bfill, bnow, ctl
+        // two sets of slots (q, u, id: 28 B a slot), the pending list (28 B
+        // a particle), boff, bfill, bnow, ctl
         const size_t bytes = 2 * 28 * cap + 28 * n + 4 * (3 * ntiles + 1) + 64;
         char* b = nullptr;
         rc = hip_ok(hipMalloc((void**)&b, bytes), "pm_split_load: hipMalloc");
         if (rc) return rc;
         p->split_buf = b;
         st.cap = (int64_t)cap;
-        st.q = reinterpret_cast<uint32_t*>(b);
-        st.u = reinterpret_cast<float*>(b + 12 * cap);
-        st.id = reinterpret_cast<uint32_t*>(b + 24 * cap);
-        st.q2 = reinterpret_cast<uint32_t*>(b + 28 * cap);
-        st.u2 = reinterpret_cast<float*>(b + 40 * cap);
-        st.id2 = reinterpret_cast<uint32_t*>(b + 52 * cap);
+        for (int k = 0; k < 2; ++k) {
+            char* c = b + 28 * cap * k;
+            st.q[k] = reinterpret_cast<uint32_t*>(c);
+            st.u[k] = reinterpret_cast<float*>(c + 12 * cap);
+            st.id[k] = reinterpret_cast<uint32_t*>(c + 24 * cap);
+        }
         char* c = b + 56 * cap;
         st.pq = reinterpret_cast<uint32_t*>(c);
         st.pu = reinterpret_cast<float*>(c + 12 * n);
@@ -508,6 +507,10 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
         st.bfill = st.boff + ntiles + 1;
         st.bnow = st.bfill + ntiles;
         st.ctl = st.bnow + ntiles;
+        // pending count, flags and the current set start at zero
+        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, p->stream, st.ctl, (size_t)16, 0u);
+        rc = check_launch("pm_split_load: control words");
+        if (rc) return rc;
     }
     rc = launch_state_load(x, y, z, vx, vy, vz, p->np, p->ng, st, p->err, p->stream);
     if (rc) return rc;

@@ -357,13 +357,16 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
 // positions.  Also leaves each tile's used slot count in bnow, the slots
 // the step's interpolation takes.
 template <int TS>
-__global__ __launch_bounds__(256) void k_deposit_state(const uint32_t* __restrict__ q, int64_t cap,
+__global__ __launch_bounds__(256) void k_deposit_state(const uint32_t* __restrict__ q0,
+                                                       const uint32_t* __restrict__ q1,
+                                                       const uint32_t* __restrict__ ctl, int64_t cap,
                                                        const uint32_t* __restrict__ boff,
                                                        const uint32_t* __restrict__ bfill, uint32_t* __restrict__ bnow,
                                                        int ng, double mass, float* __restrict__ rho,
                                                        float* __restrict__ shell) {
     constexpr int E = TS + 1, E3 = E * E * E;
     __shared__ unsigned long long acc[E3];
+    const uint32_t* __restrict__ q = ctl[3] ? q1 : q0;  // the current set of slots
     const WalkSeg sw = walk_segment(ng, TS);
     const int qb = fix_bits(mass);
     const double scale[1] = {ldexp(1.0, -qb)};
@@ -404,11 +407,12 @@ __global__ __launch_bounds__(256) void k_deposit_state(const uint32_t* __restric
 }
 
 // Capacities of the staging buckets for the next step: count + count/8 + 32
-// per tile (counts from the binning offsets on a first step, from the tile
-// counts of a split-state restaging (which clears them), else from the fill
-// counters of the staged step), exclusive scan into coff[0..n]; the fill
-// counters and the overflow flag are cleared for the next staging.  A
-// non-null gate makes the kernel a no-op unless *gate != 0.
+// per tile (counts from the binning offsets on a first step, else from the
+// fill counters of the staged step), or of the split state's buckets from
+// its tile counts (count + count/4 + 64; the counts are cleared), exclusive
+// scan into coff[0..n]; the fill counters and the overflow flag are cleared
+// for the next staging.  A non-null gate makes the kernel a no-op unless
+// *gate != 0.
 __global__ __launch_bounds__(kScanThreads) void k_cap_scan(uint32_t* __restrict__ fill,
                                                            const uint32_t* __restrict__ off,
                                                            uint32_t* __restrict__ cnt,
@@ -424,6 +428,8 @@ __global__ __launch_bounds__(kScanThreads) void k_cap_scan(uint32_t* __restrict_
             } else if (from == kCapFromCnt) {
                 c = cnt[i];
                 cnt[i] = 0u;
+                fill[i] = 0u;
+                return c + (c >> 2) + 64u;  // split_capacity: its buckets keep their arrivals
             } else {
                 c = fill[i];
             }
@@ -690,11 +696,11 @@ int launch_deposit_state(int ng, double mass, const SplitState& st, void* rho, S
     const int nt = ng / ts, seg = walk_seg(nt);
     const unsigned nsegs = (unsigned)(nt * nt * (nt / seg));
     if (ts == 16)
-        hipLaunchKernelGGL(k_deposit_state<16>, dim3(nsegs), dim3(256), 0, hs, st.q, st.cap, st.boff, st.bfill, st.bnow,
-                           ng, mass, (float*)rho, (float*)s.shell);
+        hipLaunchKernelGGL(k_deposit_state<16>, dim3(nsegs), dim3(256), 0, hs, st.q[0], st.q[1], st.ctl, st.cap, st.boff,
+                           st.bfill, st.bnow, ng, mass, (float*)rho, (float*)s.shell);
     else
-        hipLaunchKernelGGL(k_deposit_state<8>, dim3(nsegs), dim3(256), 0, hs, st.q, st.cap, st.boff, st.bfill, st.bnow,
-                           ng, mass, (float*)rho, (float*)s.shell);
+        hipLaunchKernelGGL(k_deposit_state<8>, dim3(nsegs), dim3(256), 0, hs, st.q[0], st.q[1], st.ctl, st.cap, st.boff,
+                           st.bfill, st.bnow, ng, mass, (float*)rho, (float*)s.shell);
     const int rc = check_launch("deposit_state");
     return rc ? rc : launch_fold_shells(PM_FP32, ts, ng, s.shell, rho, hs);
 }

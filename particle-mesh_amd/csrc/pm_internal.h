@@ -221,22 +221,19 @@ int launch_undrift(double* px, double* py, double* pz, const double* vx, const d
 
 // Split-precision particle state (pm_split.hip), resident in tile order:
 // positions as u32 fixed point over the box (q = x * 2^(32 - log2 ng): cell
-// and fraction) and FP32 velocities, each as three planes of cap slots
-// (q[a cap + s]), with the particle index of every slot (id, kNoId for a
-// free slot or a hole).  Tile t owns slots [boff[t], boff[t+1]) and uses
-// bfill[t] of them; bnow[t]: the slots the current step's interpolation
-// takes.  q2, u2, id2: the second set of planes for a compaction; pq, pu,
-// pid: the pending list (np entries: the particles of a load, and the
-// arrivals that found their bucket full); ctl[0]: pending count, ctl[1]:
-// compaction flag.
+// and fraction) and FP32 velocities, a slot's three values side by side
+// (sel() of pm_tiles.h), with the particle index of every slot (id, kNoId
+// for a free slot or a hole).  Two sets of slots: ctl[3] names the current
+// one, a compaction rebuilds the buckets into the other and switches.  Tile
+// t owns slots [boff[t], boff[t+1]) and uses bfill[t] of them; bnow[t]: the
+// slots the current step's interpolation takes.  pq, pu, pid: the pending
+// list (np entries: the particles of a load, and the arrivals that found
+// their bucket full); ctl[0]: pending count, ctl[1]: compaction flag.
 struct SplitState {
     int64_t cap = 0;
-    uint32_t* q = nullptr;
-    float* u = nullptr;
-    uint32_t* id = nullptr;
-    uint32_t* q2 = nullptr;
-    float* u2 = nullptr;
-    uint32_t* id2 = nullptr;
+    uint32_t* q[2] = {};
+    float* u[2] = {};
+    uint32_t* id[2] = {};
     uint32_t* pq = nullptr;
     float* pu = nullptr;
     uint32_t* pid = nullptr;
@@ -245,6 +242,8 @@ struct SplitState {
     uint32_t* bnow = nullptr;
     uint32_t* ctl = nullptr;
 };
+// slots per tile bucket of the split state: count + count/4 + 64
+inline size_t split_capacity(size_t n, size_t ntiles) { return n + n / 4 + 64 * ntiles; }
 int launch_state_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
                       const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err, hipStream_t s);
 int launch_state_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,

@@ -22,13 +22,15 @@
 //            tile's bucket (global atomic on bfill) and leaves a hole (its
 //            id becomes kNoId and its x the other half of the box, so that
 //            its tile no longer matches and the tile kernels skip it).
-//            The buckets' slack (count/8 + 32 slots) takes the arrivals;
+//            The buckets' slack takes the arrivals;
 //   compact  when a bucket runs out of slack (the mover goes to the pending
 //            list and raises the flag), and after a load (every particle
 //            pending), the next step first rebuilds all buckets from the
-//            occupied slots and the pending list (count, capacity scan,
-//            placement into the second set of planes, copy back).  Every
-//            compaction kernel is a no-op unless the flag is set.
+//            occupied slots and the pending list into the other of two sets
+//            of slots (count, capacity scan, placement), which becomes the
+//            current one (ctl[3]).  Every compaction kernel is a no-op unless
+//            the flag is set.  At 512^3 a bucket's slack (count/4 + 64
+//            slots) lasts about 25 steps of the clustered benchmark ICs.
 // A tile processes only the slots it held when the step's deposit read it
 // (bnow), so an arrival is not moved twice in one step.  Results do not
 // depend on the slot order: the deposit sums exactly, the rest is per
@@ -40,6 +42,16 @@
 namespace pm {
 
 constexpr uint32_t kNoId = 0xFFFFFFFFu;
+
+// both sets of slots, passed by value; a kernel takes set ctl[3]
+struct StateSets {
+    uint32_t* q[2];
+    float* u[2];
+    uint32_t* id[2];
+};
+static StateSets sets(const SplitState& st) {
+    return StateSets{{st.q[0], st.q[1]}, {st.u[0], st.u[1]}, {st.id[0], st.id[1]}};
+}
 
 // FP64 position -> u32 fixed point, nearest (x * 2^S is exact).  A position
 // outside [0, ng) (or not finite) is counted as a range error and wrapped
@@ -77,14 +89,17 @@ __global__ __launch_bounds__(256) void k_state_load(const double* __restrict__ x
 
 // store: occupied bucket slots, then the pending list (non-empty only after
 // an overflow that no step has compacted yet)
-__global__ __launch_bounds__(256) void k_state_store(const uint32_t* __restrict__ q, const float* __restrict__ u,
-                                                     const uint32_t* __restrict__ id, int64_t cap,
+__global__ __launch_bounds__(256) void k_state_store(StateSets ss, int64_t cap,
                                                      const uint32_t* __restrict__ pq, const float* __restrict__ pu,
                                                      const uint32_t* __restrict__ pid, int64_t np,
                                                      const uint32_t* __restrict__ ctl, int ng,
                                                      double* __restrict__ x, double* __restrict__ y,
                                                      double* __restrict__ z, double* __restrict__ vx,
                                                      double* __restrict__ vy, double* __restrict__ vz) {
+    const int cs = (int)ctl[3];
+    const uint32_t* __restrict__ q = ss.q[cs];
+    const float* __restrict__ u = ss.u[cs];
+    const uint32_t* __restrict__ id = ss.id[cs];
     const double inv = ldexp(1.0, -split_shift(ng));
     const int64_t npend = ctl[0];
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < cap + npend; k += (int64_t)gridDim.x * 256) {
@@ -105,7 +120,7 @@ __global__ __launch_bounds__(256) void k_state_store(const uint32_t* __restrict_
 
 // ------------------------------------------------------------ compaction
 // ctl[0]: pending count, ctl[1]: compaction flag, ctl[2]: the flag that the
-// capacity scan clears (unused)
+// capacity scan clears (unused), ctl[3]: the current set of slots
 template <typename F>
 __device__ __forceinline__ void for_each_entry(int64_t cap, const uint32_t* __restrict__ ctl, F f) {
     const int64_t n = cap + (int64_t)ctl[0];
@@ -114,11 +129,12 @@ __device__ __forceinline__ void for_each_entry(int64_t cap, const uint32_t* __re
 }
 
 // tile counts of the occupied slots and the pending entries
-__global__ __launch_bounds__(256) void k_state_count(const uint32_t* __restrict__ q, const uint32_t* __restrict__ id,
-                                                     int64_t cap, const uint32_t* __restrict__ pq, int64_t np,
+__global__ __launch_bounds__(256) void k_state_count(StateSets ss, int64_t cap, const uint32_t* __restrict__ pq, int64_t np,
                                                      const uint32_t* __restrict__ ctl, int ng, int ts,
                                                      uint32_t* __restrict__ cnt) {
     if (ctl[1] == 0u) return;
+    const uint32_t* __restrict__ q = ss.q[ctl[3]];
+    const uint32_t* __restrict__ id = ss.id[ctl[3]];
     const int lt = __builtin_ctz((unsigned)ts), sh = split_shift(ng) + lt;
     const uint32_t nt = (uint32_t)(ng >> lt);
     const uint32_t lane = __lane_id();
@@ -142,16 +158,20 @@ __global__ __launch_bounds__(256) void k_state_count(const uint32_t* __restrict_
     });
 }
 
-// every entry into its tile's bucket of the second set of planes
-__global__ __launch_bounds__(256) void k_state_place(const uint32_t* __restrict__ q, const float* __restrict__ u,
-                                                     const uint32_t* __restrict__ id, int64_t cap,
+// every entry into its tile's bucket of the other set of slots
+__global__ __launch_bounds__(256) void k_state_place(StateSets ss, int64_t cap,
                                                      const uint32_t* __restrict__ pq, const float* __restrict__ pu,
                                                      const uint32_t* __restrict__ pid, int64_t np,
                                                      const uint32_t* __restrict__ ctl, int ng, int ts,
-                                                     const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill,
-                                                     uint32_t* __restrict__ q2, float* __restrict__ u2,
-                                                     uint32_t* __restrict__ id2) {
+                                                     const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill) {
     if (ctl[1] == 0u) return;
+    const int cs = (int)ctl[3];
+    const uint32_t* __restrict__ q = ss.q[cs];
+    const float* __restrict__ u = ss.u[cs];
+    const uint32_t* __restrict__ id = ss.id[cs];
+    uint32_t* __restrict__ q2 = ss.q[cs ^ 1];
+    float* __restrict__ u2 = ss.u[cs ^ 1];
+    uint32_t* __restrict__ id2 = ss.id[cs ^ 1];
     const int lt = __builtin_ctz((unsigned)ts), sh = split_shift(ng) + lt;
     const uint32_t nt = (uint32_t)(ng >> lt);
     const uint32_t lane = __lane_id();
@@ -204,18 +224,12 @@ __global__ __launch_bounds__(256) void k_state_place(const uint32_t* __restrict_
     });
 }
 
-// the second set of planes back into the first: 7 planes of cap words
-// (q, u, id; unused slots carry kNoId ids)
-__global__ __launch_bounds__(256) void k_state_copy(const uint32_t* __restrict__ ctl, const uint32_t* __restrict__ src,
-                                                    uint32_t* __restrict__ dst, int64_t n) {
-    if (ctl[1] == 0u) return;
-    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) dst[k] = src[k];
-}
-
-// ids of every slot to kNoId (behind the flag if ctl is given)
-__global__ __launch_bounds__(256) void k_state_clear_ids(const uint32_t* __restrict__ ctl, uint32_t* __restrict__ id,
+// ids of every slot of a set to kNoId: the current set (a load), or behind
+// the compaction flag the other set (before the placement)
+__global__ __launch_bounds__(256) void k_state_clear_ids(StateSets ss, const uint32_t* __restrict__ ctl, int other,
                                                          int64_t n) {
-    if (ctl && ctl[1] == 0u) return;
+    if (other && ctl[1] == 0u) return;
+    uint32_t* __restrict__ id = ss.id[ctl[3] ^ (uint32_t)other];
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) id[k] = kNoId;
 }
 
@@ -223,6 +237,7 @@ __global__ void k_state_done(uint32_t* __restrict__ ctl) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && ctl[1] != 0u) {
         ctl[0] = 0u;
         ctl[1] = 0u;
+        ctl[3] ^= 1u;  // the rebuilt set is the current one
     }
 }
 
@@ -233,17 +248,27 @@ __global__ void k_state_done(uint32_t* __restrict__ ctl) {
 // kick in FP32, then the drift as the integer add of the displacement
 // rounded to 2^-S cells.  A kick that leaves a velocity non-finite is
 // counted as a range error and leaves that particle as it was.
-constexpr int kTileThreads = 256, kTileBatch = 4;
+#ifndef PM_TILE_THREADS
+#define PM_TILE_THREADS 1024
+#endif
+#ifndef PM_TILE_BATCH
+#define PM_TILE_BATCH 2
+#endif
+constexpr int kTileThreads = PM_TILE_THREADS, kTileBatch = PM_TILE_BATCH;
 
 template <int TS, bool DRIFT>
 __global__ __launch_bounds__(kTileThreads) void k_state_interp(
-    uint32_t* __restrict__ q, float* __restrict__ u, uint32_t* __restrict__ id, int64_t cap,
-    const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill, const uint32_t* __restrict__ bnow,
+    StateSets ss, int64_t cap, const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill,
+    const uint32_t* __restrict__ bnow,
     uint32_t* __restrict__ pq, float* __restrict__ pu, uint32_t* __restrict__ pid, int64_t np,
     uint32_t* __restrict__ ctl, int ng, const float* __restrict__ phi, float kick, float dscale,
     unsigned long long* __restrict__ err) {
     constexpr int W = TS + 3;  // nodes -1 .. TS+1 around the tile's cells
     __shared__ float ph[W * W * W];
+    const int cs = (int)ctl[3];
+    uint32_t* __restrict__ q = ss.q[cs];
+    float* __restrict__ u = ss.u[cs];
+    uint32_t* __restrict__ id = ss.id[cs];
     const int lt = ilog2(TS), S = split_shift(ng), sh = S + lt;
     const uint32_t nt = (uint32_t)(ng >> lt);
     const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
@@ -376,7 +401,7 @@ static unsigned grid_for(int64_t n, unsigned cap_blocks) {
 
 int launch_state_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
                       const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err, hipStream_t s) {
-    hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 16384)), dim3(256), 0, s, nullptr, st.id, st.cap);
+    hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 16384)), dim3(256), 0, s, sets(st), st.ctl, 0, st.cap);
     hipLaunchKernelGGL(k_state_load, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, x, y, z, vx, vy, vz, np, ng,
                        st.pq, st.pu, st.pid, err);
     // every particle pending; the next step compacts
@@ -387,8 +412,8 @@ int launch_state_load(const double* x, const double* y, const double* z, const d
 
 int launch_state_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
                        double* vy, double* vz, hipStream_t s) {
-    hipLaunchKernelGGL(k_state_store, dim3(grid_for(st.cap + np, 16384)), dim3(256), 0, s, st.q, st.u, st.id, st.cap,
-                       st.pq, st.pu, st.pid, np, st.ctl, ng, x, y, z, vx, vy, vz);
+    hipLaunchKernelGGL(k_state_store, dim3(grid_for(st.cap + np, 16384)), dim3(256), 0, s, sets(st), st.cap, st.pq,
+                       st.pu, st.pid, np, st.ctl, ng, x, y, z, vx, vy, vz);
     return check_launch("split_store");
 }
 
@@ -401,19 +426,17 @@ int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, h
     // capped grids (grid-stride loops): in a step without compaction every
     // kernel here is a no-op, and a workgroup that only exits still costs
     // its launch
-    hipLaunchKernelGGL(k_state_count, dim3(grid_for(n, 1024)), dim3(256), 0, hs, st.q, st.id, st.cap, st.pq, np,
-                       st.ctl, ng, ts, s.cnt);
+    hipLaunchKernelGGL(k_state_count, dim3(grid_for(n, 1024)), dim3(256), 0, hs, sets(st), st.cap, st.pq, np, st.ctl,
+                       ng, ts, s.cnt);
     int rc = check_launch("split_count");
     if (rc) return rc;
     // capacities (count + count/8 + 32) into boff, bfill cleared
     rc = launch_cap_scan_raw(nt * nt * nt, st.bfill, s.cnt, st.boff, st.ctl + 2, st.ctl + 1, hs);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 1024)), dim3(256), 0, hs, st.ctl, st.id2, st.cap);
-    hipLaunchKernelGGL(k_state_place, dim3(grid_for(n, 1024)), dim3(256), 0, hs, st.q, st.u, st.id, st.cap, st.pq,
-                       st.pu, st.pid, np, st.ctl, ng, ts, st.boff, st.bfill, st.q2, st.u2, st.id2);
-    // q, u, id are consecutive, and so are q2, u2, id2: one copy of 7 planes
-    hipLaunchKernelGGL(k_state_copy, dim3(grid_for(7 * st.cap, 2048)), dim3(256), 0, hs, st.ctl, st.q2, st.q,
-                       7 * st.cap);
+    hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 1024)), dim3(256), 0, hs, sets(st), st.ctl, 1,
+                       st.cap);
+    hipLaunchKernelGGL(k_state_place, dim3(grid_for(n, 1024)), dim3(256), 0, hs, sets(st), st.cap, st.pq, st.pu,
+                       st.pid, np, st.ctl, ng, ts, st.boff, st.bfill);
     hipLaunchKernelGGL(k_state_done, dim3(1), dim3(64), 0, hs, st.ctl);
     return check_launch("split_compact");
 }
@@ -424,9 +447,8 @@ int launch_state_interp(SplitState& st, int64_t np, int ng, const float* phi, do
     const unsigned ntiles = (unsigned)((ng / ts) * (ng / ts) * (ng / ts));
     const float dscale = (float)ldexp(drift, split_shift(ng));
 #define PM_SI(TSV, DR)                                                                                              \
-    hipLaunchKernelGGL((k_state_interp<TSV, DR>), dim3(ntiles), dim3(kTileThreads), 0, hs, st.q, st.u, st.id, st.cap, \
-                       st.boff, st.bfill, st.bnow, st.pq, st.pu, st.pid, np, st.ctl, ng, phi, (float)kick, dscale,  \
-                       err)
+    hipLaunchKernelGGL((k_state_interp<TSV, DR>), dim3(ntiles), dim3(kTileThreads), 0, hs, sets(st), st.cap, st.boff, \
+                       st.bfill, st.bnow, st.pq, st.pu, st.pid, np, st.ctl, ng, phi, (float)kick, dscale, err)
     if (ts == 16) {
         if (drift != 0.0)
             PM_SI(16, true);
