@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--deposit", choices=["sorted", "atomic"], default="sorted")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-mesh leg")
+    ap.add_argument("--no-fp64-positions", action="store_true",
+                    help="skip the FP32-mesh leg with FP64 positions (the headline runs the split state)")
     ap.add_argument("--no-alt-deposit", action="store_true", help="skip the leg with the other deposit variant")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
@@ -73,14 +75,22 @@ def algorithmic_bytes(ng: int, npart: int, prec: int, staged: bool = True, split
     interpolation stages every drifted position in tile order as a record of
     its in-tile offsets (12 B: 3 x u32 for an FP32 mesh; 16 B: 3 x 42 bits
     for an FP64 mesh) and the next deposit reads it back: written once and
-    read once."""
+    read once.  The split state is kept in tile order instead: no record."""
     b = 4 if prec == 0 else 8
     st = (12 if prec == 0 else 16) if staged else 0
     state = 24 if split else 48
     nc = ng ** 3
     nk = (ng // 2 + 1) * ng * ng
+    if split:
+        # the split state lives in tile order: the deposit reads the 12 B of
+        # positions from the tile buckets, the interpolation reads and writes
+        # the 24 B of state in place (no record)
+        st = 0
+        dep = 12
+    else:
+        dep = st if staged else 24
     return {
-        "deposit": (st if staged else 24) * npart + b * nc,  # read positions; write the mesh once
+        "deposit": dep * npart + b * nc,                     # read positions; write the mesh once
         "fft_z_fwd": b * nc + 2 * b * nk,                    # real in, half spectrum out
         "fft_y_fwd": 4 * b * nk,                             # spectrum in + out
         "fft_x_green": 4 * b * nk,
@@ -103,7 +113,8 @@ PHASE_KERNELS = {
     "interp_kick_drift": "void pm::k_interp<float, 2, 512, 1>",
 }
 # the same for the split-state step (pm_split_step)
-SPLIT_PHASE_KERNELS = dict(PHASE_KERNELS, interp_kick_drift="void pm::k_split_interp<true>")
+SPLIT_PHASE_KERNELS = dict(PHASE_KERNELS, deposit="void pm::k_deposit_state<16>",
+                           interp_kick_drift="void pm::k_state_interp<16, true>")
 
 
 def measured_traffic(kernel: str, ng: int, npart: int, table: str = ""):
@@ -128,6 +139,8 @@ DEPOSIT_KERNELS = {
     "sorted": ("", ["void pm::k_deposit_staged<float, 16>", "void pm::k_fold_faces<float, 16, 1, 0>",
                     "void pm::k_fold_seams<float, 16, 1, 0>"]),
     "atomic": ("_atomic", ["void pm::k_fill<float>", "void pm::k_deposit_atomic<float>"]),
+    "split": ("", ["void pm::k_deposit_state<16>", "void pm::k_fold_faces<float, 16, 1, 0>",
+                   "void pm::k_fold_seams<float, 16, 1, 0>"]),
 }
 
 
@@ -503,9 +516,11 @@ def main():
         log("FP32-mesh leg (split-precision state, sorted deposit)")
         ms32, wall32, ph32, err32, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart, variant, split=True)
         log(f"FP32 split: {ms32:.3f} ms/step, phases {ph32}")
-    log(f"FP32-mesh leg, FP64 positions ({args.deposit} deposit)")
-    msd, walld, phd, errd, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart, variant)
-    log(f"FP32 FP64-positions: {msd:.3f} ms/step, phases {phd}")
+    msd = None
+    if not (split and args.no_fp64_positions):
+        log(f"FP32-mesh leg, FP64 positions ({args.deposit} deposit)")
+        msd, walld, phd, errd, _ = run_gpu(pmhip.PM_FP32, pos0, vel0, args, ng, npart, variant)
+        log(f"FP32 FP64-positions: {msd:.3f} ms/step, phases {phd}")
     if not split:
         ms32, wall32, ph32, err32 = msd, walld, dict(phd), errd
     alt = None
@@ -550,7 +565,8 @@ def main():
         torch.cuda.synchronize()
 
     shape32 = ph32.pop("fft_shape")
-    phd.pop("fft_shape", None)
+    if msd is not None:
+        phd.pop("fft_shape", None)
     shape64 = res64["phases_ms"].pop("fft_shape") if res64 else None
     sorted_dep = variant == pmhip.PM_DEPOSIT_SORTED
 
@@ -576,7 +592,7 @@ def main():
         return roof, sum(nb.values())
 
     roofline, step_bytes = fp32_roofline(ph32, split)
-    roofline_d, step_bytes_d = fp32_roofline(phd, False)
+    roofline_d, step_bytes_d = fp32_roofline(phd, False) if msd is not None else (None, None)
     roofline64 = None
     if res64:
         ph64 = res64["phases_ms"]
@@ -622,14 +638,15 @@ def main():
                                f"CONT_K2 Green" + (", split-precision positions" if split else ""),
                    "ng": ng, "np": npart, "dt": args.dt, "rms_cells": args.rms},
         "phases_ms": {k: (round(v, 4) if v else v) for k, v in ph32.items()},
-        "deposit_hbm": deposit_traffic(args.deposit, ng, npart, ph32.get("deposit")),
+        "deposit_hbm": deposit_traffic("split" if split else args.deposit, ng, npart, ph32.get("deposit")),
         "step_algorithmic_GB": round(step_bytes / 1e9, 3),
         "step_achieved_GBs": round(step_bytes / (ms32 * 1e-3) / 1e9, 1),
         "wall_ms_per_step": round(wall32, 4),
         "fp32_fp64_positions": ({"value": round(npart / (msd * 1e-3) / 1e6, 2), "ms_per_step": round(msd, 4),
                                  "phases_ms": {k: (round(v, 4) if v else v) for k, v in phd.items()},
                                  "step_algorithmic_GB": round(step_bytes_d / 1e9, 3),
-                                 "roofline": roofline_d, "range_errors": errd} if split else None),
+                                 "roofline": roofline_d, "range_errors": errd} if split and msd is not None
+                                else None),
         "fp64": ({"value": round(res64["value"], 2), "ms_per_step": round(res64["ms_per_step"], 4),
                   "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["phases_ms"].items()},
                   "roofline": roofline64}

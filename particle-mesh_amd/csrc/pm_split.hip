@@ -430,7 +430,7 @@ int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, h
                        ng, ts, s.cnt);
     int rc = check_launch("split_count");
     if (rc) return rc;
-    // capacities (count + count/8 + 32) into boff, bfill cleared
+    // capacities (count + count/4 + 64) into boff, bfill cleared
     rc = launch_cap_scan_raw(nt * nt * nt, st.bfill, s.cnt, st.boff, st.ctl + 2, st.ctl + 1, hs);
     if (rc) return rc;
     hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 1024)), dim3(256), 0, hs, sets(st), st.ctl, 1,
