@@ -139,8 +139,8 @@ DEPOSIT_KERNELS = {
     "sorted": ("", ["void pm::k_deposit_staged<float, 16>", "void pm::k_fold_faces<float, 16, 1, 0>",
                     "void pm::k_fold_seams<float, 16, 1, 0>"]),
     "atomic": ("_atomic", ["void pm::k_fill<float>", "void pm::k_deposit_atomic<float>"]),
-    "split": ("", ["void pm::k_deposit_state<16>", "void pm::k_fold_faces<float, 16, 1, 0>",
-                   "void pm::k_fold_seams<float, 16, 1, 0>"]),
+    "split": ("", ["void pm::k_state_arrive<16>", "void pm::k_deposit_state<16>",
+                   "void pm::k_fold_faces<float, 16, 1, 0>", "void pm::k_fold_seams<float, 16, 1, 0>"]),
 }
 
 

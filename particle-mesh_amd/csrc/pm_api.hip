@@ -484,9 +484,11 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
         const size_t nt = (size_t)(p->ng / ts), ntiles = nt * nt * nt, n = (size_t)p->np;
         const size_t cap = split_capacity(n, ntiles);
         if (cap >= 0xFFFFFFF0ull) return fail(PM_ERR_ARG, "pm_split_load: too many particles for 32-bit slots");
-        // two sets of slots (q, u, id: 28 B a slot), the pending list (28 B
-        // a particle), boff, bfill, bnow, ctl
-        const size_t bytes = 2 * 28 * cap + 28 * n + 4 * (3 * ntiles + 1) + 64;
+        // two sets of slots (q, u, id: 28 B a slot), the outboxes (28 B an
+        // entry), the pending list (28 B a particle), boff, bfill, bnow,
+        // ocnt, ctl
+        const size_t nout = ntiles * kOutbox;
+        const size_t bytes = 2 * 28 * cap + 28 * nout + 28 * n + 4 * (4 * ntiles + 1) + 64;
         char* b = nullptr;
         rc = hip_ok(hipMalloc((void**)&b, bytes), "pm_split_load: hipMalloc");
         if (rc) return rc;
@@ -499,6 +501,10 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
             st.id[k] = reinterpret_cast<uint32_t*>(c + 24 * cap);
         }
         char* c = b + 56 * cap;
+        st.oq = reinterpret_cast<uint32_t*>(c);
+        st.ou = reinterpret_cast<float*>(c + 12 * nout);
+        st.os = reinterpret_cast<uint32_t*>(c + 24 * nout);
+        c += 28 * nout;
         st.pq = reinterpret_cast<uint32_t*>(c);
         st.pu = reinterpret_cast<float*>(c + 12 * n);
         st.pid = reinterpret_cast<uint32_t*>(c + 24 * n);
@@ -506,7 +512,8 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
         st.boff = reinterpret_cast<uint32_t*>(c);
         st.bfill = st.boff + ntiles + 1;
         st.bnow = st.bfill + ntiles;
-        st.ctl = st.bnow + ntiles;
+        st.ocnt = st.bnow + ntiles;
+        st.ctl = st.ocnt + ntiles;
         // pending count, flags and the current set start at zero
         hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, p->stream, st.ctl, (size_t)16, 0u);
         rc = check_launch("pm_split_load: control words");
@@ -521,6 +528,9 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
 extern "C" int pm_split_store(pm_plan_t* p, double* x, double* y, double* z, double* vx, double* vy, double* vz) {
     if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_split_store: null argument");
     if (!p->split_loaded) return fail(PM_ERR_ARG, "pm_split_store: no particles loaded (pm_split_load)");
+    // the last step's leavers into their buckets first
+    const int rc = launch_state_arrive(p->split, p->np, p->ng, p->stream);
+    if (rc) return rc;
     return launch_state_store(p->split, p->np, p->ng, x, y, z, vx, vy, vz, p->stream);
 }
 
@@ -530,9 +540,10 @@ extern "C" int pm_split_step(pm_plan_t* p, double kick, double drift) {
     if (!std::isfinite(kick) || !std::isfinite(drift)) return fail(PM_ERR_ARG, "pm_split_step: bad kick or drift");
     mark(p, 0);
     const double mass = (double)p->ng * p->ng * p->ng / (double)p->np;
-    // buckets rebuilt first if a load or a full bucket asked for it, then
-    // the deposit straight from them
-    int rc = launch_state_compact(p->split, p->np, p->ng, p->sorted, p->stream);
+    // the last step's leavers into their buckets, the buckets rebuilt if a
+    // load or a full bucket asked for it, then the deposit straight from them
+    int rc = launch_state_arrive(p->split, p->np, p->ng, p->stream);
+    if (!rc) rc = launch_state_compact(p->split, p->np, p->ng, p->sorted, p->stream);
     if (!rc) rc = launch_deposit_state(p->ng, mass, p->split, p->rho, p->sorted, p->stream);
     if (rc) return rc;
     mark(p, 1);

@@ -241,9 +241,20 @@ struct SplitState {
     uint32_t* bfill = nullptr;
     uint32_t* bnow = nullptr;
     uint32_t* ctl = nullptr;
+    // per tile, the particles that left it in the last interpolation:
+    // kOutbox entries of position and velocity triples and the old slot,
+    // and their count
+    uint32_t* oq = nullptr;
+    float* ou = nullptr;
+    uint32_t* os = nullptr;
+    uint32_t* ocnt = nullptr;
 };
-// slots per tile bucket of the split state: count + count/4 + 64
-inline size_t split_capacity(size_t n, size_t ntiles) { return n + n / 4 + 64 * ntiles; }
+constexpr int kOutbox = 128;
+// slots per tile bucket of the split state: count + count/4 + 64, rounded up
+// to 32 slots (a bucket starts on a 128-byte line of every slot array, so a
+// wave's 64 slots are whole lines); the total is a multiple of 32 as well, so
+// that every array of both sets starts on a line
+inline size_t split_capacity(size_t n, size_t ntiles) { return (n + n / 4 + 96 * ntiles + 31) / 32 * 32; }
 int launch_state_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
                       const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err, hipStream_t s);
 int launch_state_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
@@ -254,6 +265,9 @@ int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, h
 // the sorted deposit of the buckets' particles (+ fold), leaving bnow
 int launch_deposit_state(int ng, double mass, const SplitState& st, void* rho, SortedScratch& s, hipStream_t hs);
 // interpolation + kick + drift tile by tile (FP32 mesh)
+// the leavers that the last interpolation put in the outboxes, into their
+// new buckets (a no-op if the outboxes are empty)
+int launch_state_arrive(SplitState& st, int64_t np, int ng, hipStream_t hs);
 int launch_state_interp(SplitState& st, int64_t np, int ng, const float* phi, double kick, double drift,
                         unsigned long long* err, hipStream_t hs);
 // the capacity scan on explicit arrays, from the tile counts cnt (cleared)

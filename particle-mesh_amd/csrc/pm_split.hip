@@ -17,12 +17,16 @@
 //   interp   one workgroup per tile stages the potential around the tile
 //            (19^3 nodes) in LDS and interpolates, kicks and drifts the
 //            tile's particles from it: no gathers from HBM, no records.  A
-//            particle that stays in its tile (all but ~0.2% per step) is
-//            written back in place; one that leaves is appended to its new
-//            tile's bucket (global atomic on bfill) and leaves a hole (its
-//            id becomes kNoId and its x the other half of the box, so that
-//            its tile no longer matches and the tile kernels skip it).
-//            The buckets' slack takes the arrivals;
+//            particle that stays in its tile (all but 0.25-0.4% per step at
+//            512^3) is written back in place; one that leaves goes to the
+//            tile's outbox and its slot becomes a hole (x in the other half
+//            of the box, so that its tile no longer matches and the tile
+//            kernels skip it).  Every slot of the tile's range is written,
+//            holes too, so that the stores cover whole lines (buckets start
+//            on 32-slot boundaries);
+//   arrive   appends the outboxes' particles to their new tiles' buckets
+//            (global atomic on bfill) and clears their old ids.  The
+//            buckets' slack takes the arrivals;
 //   compact  when a bucket runs out of slack (the mover goes to the pending
 //            list and raises the flag), and after a load (every particle
 //            pending), the next step first rebuilds all buckets from the
@@ -252,19 +256,66 @@ __global__ void k_state_done(uint32_t* __restrict__ ctl) {
 #define PM_TILE_THREADS 1024
 #endif
 #ifndef PM_TILE_BATCH
-#define PM_TILE_BATCH 2
+#define PM_TILE_BATCH 1
 #endif
 constexpr int kTileThreads = PM_TILE_THREADS, kTileBatch = PM_TILE_BATCH;
 
+// per tile, where its leavers go and the kernels that read them
+struct Outbox {
+    uint32_t* q;
+    float* u;
+    uint32_t* s;
+    uint32_t* cnt;
+};
+static Outbox outbox(const SplitState& st) { return Outbox{st.oq, st.ou, st.os, st.ocnt}; }
+
+// where a particle that leaves its tile is appended: the new tile's bucket,
+// or, if that is full, the pending list (which makes the next step compact)
+struct Dest {
+    const uint32_t* boff;
+    uint32_t* bfill;
+    uint32_t* pq;
+    float* pu;
+    uint32_t* pid;
+    int64_t np;
+    uint32_t* ctl;
+};
+__device__ __forceinline__ void state_append(const Dest& d, uint32_t* __restrict__ q, float* __restrict__ u,
+                                             uint32_t* __restrict__ id, int64_t cap, uint32_t t2, uint32_t na,
+                                             uint32_t nb, uint32_t nc, float n0, float n1, float n2, uint32_t i) {
+    const uint32_t slot = atomicAdd(&d.bfill[t2], 1u);
+    const int64_t c2 = (int64_t)d.boff[t2 + 1] - d.boff[t2];
+    uint32_t *dq = q, *di = id;
+    float* du = u;
+    int64_t ds, stride = cap;
+    if ((int64_t)slot < c2) {
+        ds = (int64_t)d.boff[t2] + slot;
+    } else {
+        ds = atomicAdd(&d.ctl[0], 1u);  // pending entries never exceed np
+        dq = d.pq;
+        du = d.pu;
+        di = d.pid;
+        stride = d.np;
+        d.ctl[1] = 1u;
+    }
+    dq[sel(stride, ds, 0)] = na;
+    dq[sel(stride, ds, 1)] = nb;
+    dq[sel(stride, ds, 2)] = nc;
+    du[sel(stride, ds, 0)] = n0;
+    du[sel(stride, ds, 1)] = n1;
+    du[sel(stride, ds, 2)] = n2;
+    di[ds] = i;
+}
+
+// eight waves per SIMD (at most 64 VGPRs): two 1024-thread workgroups per CU
 template <int TS, bool DRIFT>
-__global__ __launch_bounds__(kTileThreads) void k_state_interp(
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_state_interp(
     StateSets ss, int64_t cap, const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill,
-    const uint32_t* __restrict__ bnow,
-    uint32_t* __restrict__ pq, float* __restrict__ pu, uint32_t* __restrict__ pid, int64_t np,
-    uint32_t* __restrict__ ctl, int ng, const float* __restrict__ phi, float kick, float dscale,
-    unsigned long long* __restrict__ err) {
+    const uint32_t* __restrict__ bnow, Dest dst, Outbox ob, const uint32_t* __restrict__ ctl, int ng,
+    const float* __restrict__ phi, float kick, float dscale, unsigned long long* __restrict__ err) {
     constexpr int W = TS + 3;  // nodes -1 .. TS+1 around the tile's cells
     __shared__ float ph[W * W * W];
+    __shared__ uint32_t nmov;
     const int cs = (int)ctl[3];
     uint32_t* __restrict__ q = ss.q[cs];
     float* __restrict__ u = ss.u[cs];
@@ -274,31 +325,14 @@ __global__ __launch_bounds__(kTileThreads) void k_state_interp(
     const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = (int)(t / (nt * nt)), ty = (int)((t / nt) % nt), tz = (int)(t % nt);
     const int m = ng - 1;
-    // all of a thread's staging loads in flight at once, then the LDS writes
-    constexpr int NE = W * W * W, NIT = (NE + kTileThreads - 1) / kTileThreads;
-    float pv[NIT];
-#pragma unroll
-    for (int j = 0; j < NIT; ++j) {
-        const int e = threadIdx.x + j * kTileThreads;
-        if (e < NE) {
-            const int a = e / (W * W), r = e - a * W * W, b = r / W, c = r - b * W;
-            const int gx = (tx * TS - 1 + a) & m, gy = (ty * TS - 1 + b) & m, gz = (tz * TS - 1 + c) & m;
-            pv[j] = phi[((size_t)gx * ng + gy) * ng + gz];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < NIT; ++j) {
-        const int e = threadIdx.x + j * kTileThreads;
-        if (e < NE) ph[e] = pv[j];
-    }
-    __syncthreads();
-    const uint32_t fmask = (1u << S) - 1u;
-    const float fs = ldexpf(1.0f, -S);
     const int64_t beg = boff[t], end = beg + bnow[t];
-    unsigned nerr = 0u;
-    for (int64_t s0 = beg + threadIdx.x; s0 < end; s0 += (int64_t)kTileThreads * kTileBatch) {
-        uint32_t qa[kTileBatch], qb[kTileBatch], qc[kTileBatch];
-        float ua[kTileBatch], ub[kTileBatch], uc[kTileBatch];
+    if (threadIdx.x == 0) nmov = 0u;
+    constexpr int64_t kStep = (int64_t)kTileThreads * kTileBatch;
+    uint32_t qa[kTileBatch], qb[kTileBatch], qc[kTileBatch];
+    float ua[kTileBatch], ub[kTileBatch], uc[kTileBatch];
+    // a batch's slots past the end read the tile's first slot (every bucket
+    // has at least 64 slots) and are skipped
+    const auto load_batch = [&](int64_t s0) {
 #pragma unroll
         for (int j = 0; j < kTileBatch; ++j) {
             const int64_t s = s0 + (int64_t)kTileThreads * j;
@@ -310,88 +344,155 @@ __global__ __launch_bounds__(kTileThreads) void k_state_interp(
             ub[j] = stream_load(u + sel(cap, r, 1));
             uc[j] = stream_load(u + sel(cap, r, 2));
         }
+    };
+    // all of a thread's staging loads in flight at once, then the first
+    // batch of particles, then the LDS writes: the two latencies overlap
+    constexpr int NE = W * W * W, NIT = (NE + kTileThreads - 1) / kTileThreads;
+    float pv[NIT];
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+        const int e = threadIdx.x + j * kTileThreads;
+        if (e < NE) {
+            const int a = e / (W * W), r = e - a * W * W, b = r / W, c = r - b * W;
+            const int gx = (tx * TS - 1 + a) & m, gy = (ty * TS - 1 + b) & m, gz = (tz * TS - 1 + c) & m;
+            pv[j] = phi[((size_t)gx * ng + gy) * ng + gz];
+        }
+    }
+    int64_t s0 = beg + threadIdx.x;
+    load_batch(s0);
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+        const int e = threadIdx.x + j * kTileThreads;
+        if (e < NE) ph[e] = pv[j];
+    }
+    __syncthreads();
+    const uint32_t fmask = (1u << S) - 1u;
+    const float fs = ldexpf(1.0f, -S);
+    unsigned nerr = 0u;
+    while (s0 < end) {
 #pragma unroll
         for (int j = 0; j < kTileBatch; ++j) {
             const int64_t s = s0 + (int64_t)kTileThreads * j;
+            if (s >= end) continue;
             const uint32_t a = qa[j], b = qb[j], c = qc[j];
-            if (s >= end || split_key(a, b, c, sh, nt) != t) continue;  // past the end, or a hole
-            const int lx = (int)(a >> S) - tx * TS, ly = (int)(b >> S) - ty * TS, lz = (int)(c >> S) - tz * TS;
-            const float fx = (float)(a & fmask) * fs, fy = (float)(b & fmask) * fs, fz = (float)(c & fmask) * fs;
-            // the 12 z-runs of gather_runs_main from LDS: node (ix+x, iy+y,
-            // iz+z) of the cell at local (lx, ly, lz) is L(lx+1+x, ly+1+y, lz+1+z)
-            float cl[2][2][4], xr[2][2][2], yr[2][2][2];
-            const auto L = [&](int x, int y, int z) { return ph[(x * W + y) * W + z]; };
+            // what the slot gets back: a hole (another tile's key) keeps its
+            // values.  Every slot of the range is written, so that the
+            // stores cover whole 128-byte lines (a partial line costs the
+            // memory a read-modify-write).
+            uint32_t wa = a, wb = b, wc = c;
+            float w0 = ua[j], w1 = ub[j], w2 = uc[j];
+            if (split_key(a, b, c, sh, nt) == t) {
+                const int lx = (int)(a >> S) - tx * TS, ly = (int)(b >> S) - ty * TS, lz = (int)(c >> S) - tz * TS;
+                const float fx = (float)(a & fmask) * fs, fy = (float)(b & fmask) * fs, fz = (float)(c & fmask) * fs;
+                // the 12 z-runs of gather_runs_main from LDS: node (ix+x,
+                // iy+y, iz+z) of the cell at local (lx, ly, lz) is
+                // L(lx+1+x, ly+1+y, lz+1+z)
+                float cl[2][2][4], xr[2][2][2], yr[2][2][2];
+                const auto L = [&](int x, int y, int z) { return ph[(x * W + y) * W + z]; };
 #pragma unroll
-            for (int aa = 0; aa < 2; ++aa)
+                for (int aa = 0; aa < 2; ++aa)
 #pragma unroll
-                for (int bb = 0; bb < 2; ++bb) {
+                    for (int bb = 0; bb < 2; ++bb) {
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) cl[aa][bb][k] = L(lx + 1 + aa, ly + 1 + bb, lz + k);
+                        for (int k = 0; k < 4; ++k) cl[aa][bb][k] = L(lx + 1 + aa, ly + 1 + bb, lz + k);
 #pragma unroll
-                    for (int k = 0; k < 2; ++k) {
-                        xr[aa][bb][k] = L(lx + 3 * aa, ly + 1 + bb, lz + 1 + k);
-                        yr[aa][bb][k] = L(lx + 1 + bb, ly + 3 * aa, lz + 1 + k);
+                        for (int k = 0; k < 2; ++k) {
+                            xr[aa][bb][k] = L(lx + 3 * aa, ly + 1 + bb, lz + 1 + k);
+                            yr[aa][bb][k] = L(lx + 1 + bb, ly + 3 * aa, lz + 1 + k);
+                        }
+                    }
+                float f[3];
+                force_from_runs(cl, xr, yr, fx, fy, fz, f);
+                float n0 = fmaf(f[0], kick, ua[j]), n1 = fmaf(f[1], kick, ub[j]), n2 = fmaf(f[2], kick, uc[j]);
+                const bool fin = isfinite(n0) && isfinite(n1) && isfinite(n2);
+                if (!fin) {
+                    n0 = ua[j];
+                    n1 = ub[j];
+                    n2 = uc[j];
+                    ++nerr;
+                }
+                uint32_t na = a, nb = b, nc = c;
+                if (DRIFT && fin) {
+                    na = a + (uint32_t)__float2int_rn(n0 * dscale);
+                    nb = b + (uint32_t)__float2int_rn(n1 * dscale);
+                    nc = c + (uint32_t)__float2int_rn(n2 * dscale);
+                }
+                w0 = n0;
+                w1 = n1;
+                w2 = n2;
+                const uint32_t t2 = DRIFT ? split_key(na, nb, nc, sh, nt) : t;
+                if (t2 == t) {
+                    wa = na;
+                    wb = nb;
+                    wc = nc;
+                } else {
+                    // a mover: its slot becomes a hole (x in the other half of
+                    // the box: another tile); the particle goes into the
+                    // tile's outbox (an LDS counter, no global round trip in
+                    // the loop), which k_state_arrive appends to its new
+                    // bucket.  A full outbox: appended here.
+                    wa = a ^ 0x80000000u;
+                    const uint32_t k = atomicAdd(&nmov, 1u);
+                    if (k < (uint32_t)kOutbox) {
+                        const int64_t e = (int64_t)t * kOutbox + k, no = (int64_t)gridDim.x * kOutbox;
+                        ob.q[sel(no, e, 0)] = na;
+                        ob.q[sel(no, e, 1)] = nb;
+                        ob.q[sel(no, e, 2)] = nc;
+                        ob.u[sel(no, e, 0)] = n0;
+                        ob.u[sel(no, e, 1)] = n1;
+                        ob.u[sel(no, e, 2)] = n2;
+                        ob.s[e] = (uint32_t)s;
+                    } else {
+                        state_append(dst, q, u, id, cap, t2, na, nb, nc, n0, n1, n2, id[s]);
+                        id[s] = kNoId;
                     }
                 }
-            float f[3];
-            force_from_runs(cl, xr, yr, fx, fy, fz, f);
-            float n0 = fmaf(f[0], kick, ua[j]), n1 = fmaf(f[1], kick, ub[j]), n2 = fmaf(f[2], kick, uc[j]);
-            const bool fin = isfinite(n0) && isfinite(n1) && isfinite(n2);
-            if (!fin) {
-                n0 = ua[j];
-                n1 = ub[j];
-                n2 = uc[j];
-                ++nerr;
             }
-            uint32_t na = a, nb = b, nc = c;
-            if (DRIFT && fin) {
-                na = a + (uint32_t)__float2int_rn(n0 * dscale);
-                nb = b + (uint32_t)__float2int_rn(n1 * dscale);
-                nc = c + (uint32_t)__float2int_rn(n2 * dscale);
+            stream_store(u + sel(cap, s, 0), w0);
+            stream_store(u + sel(cap, s, 1), w1);
+            stream_store(u + sel(cap, s, 2), w2);
+            if (DRIFT) {
+                stream_store(q + sel(cap, s, 0), wa);
+                stream_store(q + sel(cap, s, 1), wb);
+                stream_store(q + sel(cap, s, 2), wc);
             }
-            const uint32_t t2 = DRIFT ? split_key(na, nb, nc, sh, nt) : t;
-            if (t2 == t) {
-                stream_store(u + sel(cap, s, 0), n0);
-                stream_store(u + sel(cap, s, 1), n1);
-                stream_store(u + sel(cap, s, 2), n2);
-                if (DRIFT) {
-                    stream_store(q + sel(cap, s, 0), na);
-                    stream_store(q + sel(cap, s, 1), nb);
-                    stream_store(q + sel(cap, s, 2), nc);
-                }
-                continue;
-            }
-            // a mover: appended to its new tile's bucket (or, if that is
-            // full, to the pending list, which makes the next step compact);
-            // its slot becomes a hole
-            const uint32_t i = id[s];
-            const uint32_t slot = atomicAdd(&bfill[t2], 1u);
-            const int64_t c2 = (int64_t)boff[t2 + 1] - boff[t2];
-            uint32_t *dq = q, *di = id;
-            float* du = u;
-            int64_t ds, stride = cap;
-            if ((int64_t)slot < c2) {
-                ds = (int64_t)boff[t2] + slot;
-            } else {
-                ds = atomicAdd(&ctl[0], 1u);  // pending entries never exceed np
-                dq = pq;
-                du = pu;
-                di = pid;
-                stride = np;
-                ctl[1] = 1u;
-            }
-            dq[sel(stride, ds, 0)] = na;
-            dq[sel(stride, ds, 1)] = nb;
-            dq[sel(stride, ds, 2)] = nc;
-            du[sel(stride, ds, 0)] = n0;
-            du[sel(stride, ds, 1)] = n1;
-            du[sel(stride, ds, 2)] = n2;
-            di[ds] = i;
-            q[sel(cap, s, 0)] = a ^ 0x80000000u;  // x in the other half of the box: another tile
-            id[s] = kNoId;
         }
+        s0 += kStep;
+        if (s0 < end) load_batch(s0);
     }
     if (nerr) atomicAdd(err, (unsigned long long)nerr);
+    if (DRIFT) {
+        __syncthreads();
+        if (threadIdx.x == 0) ob.cnt[t] = min(nmov, (uint32_t)kOutbox);
+    }
+}
+
+// The leavers of the last interpolation: one wave per tile appends its
+// outbox entries to their new buckets, clears their old slots' ids (the
+// interpolation made those slots holes) and empties the outbox, so that a
+// second call is a no-op.
+template <int TS>
+__global__ __launch_bounds__(256) void k_state_arrive(StateSets ss, int64_t cap, Dest dst, Outbox ob, int ng,
+                                                      uint32_t ntiles) {
+    const uint32_t t = blockIdx.x * 4 + threadIdx.x / 64;
+    if (t >= ntiles) return;
+    const int cs = (int)dst.ctl[3];
+    uint32_t* __restrict__ q = ss.q[cs];
+    float* __restrict__ u = ss.u[cs];
+    uint32_t* __restrict__ id = ss.id[cs];
+    const int lt = ilog2(TS), sh = split_shift(ng) + lt;
+    const uint32_t nt = (uint32_t)(ng >> lt);
+    const uint32_t n = ob.cnt[t];
+    const int64_t no = (int64_t)ntiles * kOutbox;
+    for (uint32_t k = threadIdx.x & 63u; k < n; k += 64u) {
+        const int64_t e = (int64_t)t * kOutbox + k;
+        const uint32_t na = ob.q[sel(no, e, 0)], nb = ob.q[sel(no, e, 1)], nc = ob.q[sel(no, e, 2)];
+        const float n0 = ob.u[sel(no, e, 0)], n1 = ob.u[sel(no, e, 1)], n2 = ob.u[sel(no, e, 2)];
+        const int64_t s = ob.s[e];
+        state_append(dst, q, u, id, cap, split_key(na, nb, nc, sh, nt), na, nb, nc, n0, n1, n2, id[s]);
+        id[s] = kNoId;
+    }
+    if ((threadIdx.x & 63u) == 0u && n) ob.cnt[t] = 0u;
 }
 
 // ================================================================ launchers
@@ -404,7 +505,11 @@ int launch_state_load(const double* x, const double* y, const double* z, const d
     hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 16384)), dim3(256), 0, s, sets(st), st.ctl, 0, st.cap);
     hipLaunchKernelGGL(k_state_load, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, x, y, z, vx, vy, vz, np, ng,
                        st.pq, st.pu, st.pid, err);
-    // every particle pending; the next step compacts
+    // every particle pending, no outbox entries; the next step compacts
+    const int ts = sorted_tile(ng);
+    const size_t ntiles = (size_t)(ng / ts) * (ng / ts) * (ng / ts);
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3((unsigned)std::min<size_t>((ntiles + 255) / 256, 1024)), dim3(256), 0, s,
+                       st.ocnt, ntiles, 0u);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, st.ctl, (size_t)1, (uint32_t)np);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, st.ctl + 1, (size_t)1, 1u);
     return check_launch("split_load");
@@ -430,7 +535,7 @@ int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, h
                        ng, ts, s.cnt);
     int rc = check_launch("split_count");
     if (rc) return rc;
-    // capacities (count + count/4 + 64) into boff, bfill cleared
+    // capacities (count + count/4 + 64, rounded up to 32) into boff, bfill cleared
     rc = launch_cap_scan_raw(nt * nt * nt, st.bfill, s.cnt, st.boff, st.ctl + 2, st.ctl + 1, hs);
     if (rc) return rc;
     hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 1024)), dim3(256), 0, hs, sets(st), st.ctl, 1,
@@ -446,9 +551,10 @@ int launch_state_interp(SplitState& st, int64_t np, int ng, const float* phi, do
     const int ts = sorted_tile(ng);
     const unsigned ntiles = (unsigned)((ng / ts) * (ng / ts) * (ng / ts));
     const float dscale = (float)ldexp(drift, split_shift(ng));
+    const Dest dst{st.boff, st.bfill, st.pq, st.pu, st.pid, np, st.ctl};
 #define PM_SI(TSV, DR)                                                                                              \
     hipLaunchKernelGGL((k_state_interp<TSV, DR>), dim3(ntiles), dim3(kTileThreads), 0, hs, sets(st), st.cap, st.boff, \
-                       st.bfill, st.bnow, st.pq, st.pu, st.pid, np, st.ctl, ng, phi, (float)kick, dscale, err)
+                       st.bfill, st.bnow, dst, outbox(st), st.ctl, ng, phi, (float)kick, dscale, err)
     if (ts == 16) {
         if (drift != 0.0)
             PM_SI(16, true);
@@ -462,6 +568,18 @@ int launch_state_interp(SplitState& st, int64_t np, int ng, const float* phi, do
     }
 #undef PM_SI
     return check_launch("split_interp");
+}
+
+int launch_state_arrive(SplitState& st, int64_t np, int ng, hipStream_t hs) {
+    const int ts = sorted_tile(ng);
+    const unsigned ntiles = (unsigned)((ng / ts) * (ng / ts) * (ng / ts));
+    const Dest dst{st.boff, st.bfill, st.pq, st.pu, st.pid, np, st.ctl};
+    const dim3 grid((ntiles + 3) / 4);
+    if (ts == 16)
+        hipLaunchKernelGGL(k_state_arrive<16>, grid, dim3(256), 0, hs, sets(st), st.cap, dst, outbox(st), ng, ntiles);
+    else
+        hipLaunchKernelGGL(k_state_arrive<8>, grid, dim3(256), 0, hs, sets(st), st.cap, dst, outbox(st), ng, ntiles);
+    return check_launch("split_arrive");
 }
 
 }  // namespace pm
