@@ -485,10 +485,10 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
         const size_t cap = split_capacity(n, ntiles);
         if (cap >= 0xFFFFFFF0ull) return fail(PM_ERR_ARG, "pm_split_load: too many particles for 32-bit slots");
         // two sets of slots (q, u, id: 28 B a slot), the outboxes (28 B an
-        // entry), the pending list (28 B a particle), boff, bfill, bnow,
-        // ocnt, ctl
+        // entry), the hole lists (4 B an entry), the pending list (28 B a
+        // particle), boff, bfill, bnow, ocnt, hcnt, huse, ctl
         const size_t nout = ntiles * kOutbox;
-        const size_t bytes = 2 * 28 * cap + 28 * nout + 28 * n + 4 * (4 * ntiles + 1) + 64;
+        const size_t bytes = 2 * 28 * cap + 32 * nout + 28 * n + 4 * (6 * ntiles + 1) + 64;
         char* b = nullptr;
         rc = hip_ok(hipMalloc((void**)&b, bytes), "pm_split_load: hipMalloc");
         if (rc) return rc;
@@ -503,8 +503,10 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
         char* c = b + 56 * cap;
         st.oq = reinterpret_cast<uint32_t*>(c);
         st.ou = reinterpret_cast<float*>(c + 12 * nout);
-        st.os = reinterpret_cast<uint32_t*>(c + 24 * nout);
+        st.oid = reinterpret_cast<uint32_t*>(c + 24 * nout);
         c += 28 * nout;
+        st.hole = reinterpret_cast<uint32_t*>(c);
+        c += 4 * nout;
         st.pq = reinterpret_cast<uint32_t*>(c);
         st.pu = reinterpret_cast<float*>(c + 12 * n);
         st.pid = reinterpret_cast<uint32_t*>(c + 24 * n);
@@ -513,7 +515,9 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
         st.bfill = st.boff + ntiles + 1;
         st.bnow = st.bfill + ntiles;
         st.ocnt = st.bnow + ntiles;
-        st.ctl = st.ocnt + ntiles;
+        st.hcnt = st.ocnt + ntiles;
+        st.huse = st.hcnt + ntiles;
+        st.ctl = st.huse + ntiles;
         // pending count, flags and the current set start at zero
         hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, p->stream, st.ctl, (size_t)16, 0u);
         rc = check_launch("pm_split_load: control words");

@@ -242,12 +242,18 @@ struct SplitState {
     uint32_t* bnow = nullptr;
     uint32_t* ctl = nullptr;
     // per tile, the particles that left it in the last interpolation:
-    // kOutbox entries of position and velocity triples and the old slot,
-    // and their count
+    // kOutbox entries of position and velocity triples and the particle
+    // index, and their count
     uint32_t* oq = nullptr;
     float* ou = nullptr;
-    uint32_t* os = nullptr;
+    uint32_t* oid = nullptr;
     uint32_t* ocnt = nullptr;
+    // per tile, up to kOutbox holes of its range (slots whose particle left)
+    // that the last interpolation listed, their count (hcnt, right after
+    // ocnt) and how many arrivals took one (huse, right after hcnt)
+    uint32_t* hole = nullptr;
+    uint32_t* hcnt = nullptr;
+    uint32_t* huse = nullptr;
 };
 constexpr int kOutbox = 128;
 // slots per tile bucket of the split state: count + count/4 + 64, rounded up

@@ -19,22 +19,25 @@
 //            tile's particles from it: no gathers from HBM, no records.  A
 //            particle that stays in its tile (all but 0.25-0.4% per step at
 //            512^3) is written back in place; one that leaves goes to the
-//            tile's outbox and its slot becomes a hole (x in the other half
-//            of the box, so that its tile no longer matches and the tile
-//            kernels skip it).  Every slot of the tile's range is written,
-//            holes too, so that the stores cover whole lines (buckets start
-//            on 32-slot boundaries);
-//   arrive   appends the outboxes' particles to their new tiles' buckets
-//            (global atomic on bfill) and clears their old ids.  The
-//            buckets' slack takes the arrivals;
+//            tile's outbox with its index and its slot becomes a hole (x in
+//            the other half of the box, so that its tile no longer matches
+//            and the tile kernels skip it).  Every slot of the tile's range
+//            is written, holes too, so that the stores cover whole lines
+//            (buckets start on 32-slot boundaries).  The tile's holes, old
+//            and new, are listed for the arrivals;
+//   arrive   moves the outboxes' particles to their new tiles: into a
+//            listed hole of the new tile while there is one, else appended
+//            to its bucket (global atomic on bfill).  The buckets' slack
+//            takes the arrivals that find no hole;
 //   compact  when a bucket runs out of slack (the mover goes to the pending
 //            list and raises the flag), and after a load (every particle
 //            pending), the next step first rebuilds all buckets from the
 //            occupied slots and the pending list into the other of two sets
 //            of slots (count, capacity scan, placement), which becomes the
 //            current one (ctl[3]).  Every compaction kernel is a no-op unless
-//            the flag is set.  At 512^3 a bucket's slack (count/4 + 64
-//            slots) lasts about 25 steps of the clustered benchmark ICs.
+//            the flag is set.  At 512^3, with the holes refilled, a
+//            bucket's slack (count/4 + 64 slots) lasts past 25 steps of the
+//            clustered benchmark ICs.
 // A tile processes only the slots it held when the step's deposit read it
 // (bnow), so an arrival is not moved twice in one step.  Results do not
 // depend on the slot order: the deposit sums exactly, the rest is per
@@ -260,14 +263,24 @@ __global__ void k_state_done(uint32_t* __restrict__ ctl) {
 #endif
 constexpr int kTileThreads = PM_TILE_THREADS, kTileBatch = PM_TILE_BATCH;
 
-// per tile, where its leavers go and the kernels that read them
+// per tile, where its leavers go and the kernels that read them: position,
+// velocity and particle index of each, and their count
 struct Outbox {
     uint32_t* q;
     float* u;
-    uint32_t* s;
+    uint32_t* i;
     uint32_t* cnt;
 };
-static Outbox outbox(const SplitState& st) { return Outbox{st.oq, st.ou, st.os, st.ocnt}; }
+static Outbox outbox(const SplitState& st) { return Outbox{st.oq, st.ou, st.oid, st.ocnt}; }
+// per tile, the holes of its range that the last interpolation saw (slots
+// whose particle left, in this step or before), their count, and how many
+// of them k_state_arrive has given to arrivals
+struct Holes {
+    uint32_t* s;
+    uint32_t* cnt;
+    uint32_t* used;
+};
+static Holes holes(const SplitState& st) { return Holes{st.hole, st.hcnt, st.huse}; }
 
 // where a particle that leaves its tile is appended: the new tile's bucket,
 // or, if that is full, the pending list (which makes the next step compact)
@@ -311,11 +324,11 @@ __device__ __forceinline__ void state_append(const Dest& d, uint32_t* __restrict
 template <int TS, bool DRIFT>
 __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_state_interp(
     StateSets ss, int64_t cap, const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill,
-    const uint32_t* __restrict__ bnow, Dest dst, Outbox ob, const uint32_t* __restrict__ ctl, int ng,
+    const uint32_t* __restrict__ bnow, Dest dst, Outbox ob, Holes ho, const uint32_t* __restrict__ ctl, int ng,
     const float* __restrict__ phi, float kick, float dscale, unsigned long long* __restrict__ err) {
     constexpr int W = TS + 3;  // nodes -1 .. TS+1 around the tile's cells
     __shared__ float ph[W * W * W];
-    __shared__ uint32_t nmov;
+    __shared__ uint32_t nmov, nhole;
     const int cs = (int)ctl[3];
     uint32_t* __restrict__ q = ss.q[cs];
     float* __restrict__ u = ss.u[cs];
@@ -326,7 +339,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     const int tx = (int)(t / (nt * nt)), ty = (int)((t / nt) % nt), tz = (int)(t % nt);
     const int m = ng - 1;
     const int64_t beg = boff[t], end = beg + bnow[t];
-    if (threadIdx.x == 0) nmov = 0u;
+    if (threadIdx.x == 0) {
+        nmov = 0u;
+        nhole = 0u;
+    }
     constexpr int64_t kStep = (int64_t)kTileThreads * kTileBatch;
     uint32_t qa[kTileBatch], qb[kTileBatch], qc[kTileBatch];
     float ua[kTileBatch], ub[kTileBatch], uc[kTileBatch];
@@ -381,7 +397,9 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
             // memory a read-modify-write).
             uint32_t wa = a, wb = b, wc = c;
             float w0 = ua[j], w1 = ub[j], w2 = uc[j];
+            bool hole = DRIFT;  // a hole already, unless its particle is the tile's
             if (split_key(a, b, c, sh, nt) == t) {
+                hole = false;
                 const int lx = (int)(a >> S) - tx * TS, ly = (int)(b >> S) - ty * TS, lz = (int)(c >> S) - tz * TS;
                 const float fx = (float)(a & fmask) * fs, fy = (float)(b & fmask) * fs, fz = (float)(c & fmask) * fs;
                 // the 12 z-runs of gather_runs_main from LDS: node (ix+x,
@@ -427,11 +445,14 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
                     wc = nc;
                 } else {
                     // a mover: its slot becomes a hole (x in the other half of
-                    // the box: another tile); the particle goes into the
-                    // tile's outbox (an LDS counter, no global round trip in
-                    // the loop), which k_state_arrive appends to its new
-                    // bucket.  A full outbox: appended here.
+                    // the box: another tile); the particle, with its index,
+                    // goes into the tile's outbox (an LDS counter, no global
+                    // round trip in the loop), which k_state_arrive moves to
+                    // its new tile.  A full outbox: appended here.
                     wa = a ^ 0x80000000u;
+                    hole = true;
+                    const uint32_t pid = id[s];
+                    id[s] = kNoId;
                     const uint32_t k = atomicAdd(&nmov, 1u);
                     if (k < (uint32_t)kOutbox) {
                         const int64_t e = (int64_t)t * kOutbox + k, no = (int64_t)gridDim.x * kOutbox;
@@ -441,12 +462,15 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
                         ob.u[sel(no, e, 0)] = n0;
                         ob.u[sel(no, e, 1)] = n1;
                         ob.u[sel(no, e, 2)] = n2;
-                        ob.s[e] = (uint32_t)s;
+                        ob.i[e] = pid;
                     } else {
-                        state_append(dst, q, u, id, cap, t2, na, nb, nc, n0, n1, n2, id[s]);
-                        id[s] = kNoId;
+                        state_append(dst, q, u, id, cap, t2, na, nb, nc, n0, n1, n2, pid);
                     }
                 }
+            }
+            if (hole) {  // listed for the arrivals of the next k_state_arrive
+                const uint32_t k = atomicAdd(&nhole, 1u);
+                if (k < (uint32_t)kOutbox) ho.s[(int64_t)t * kOutbox + k] = (uint32_t)s;
             }
             stream_store(u + sel(cap, s, 0), w0);
             stream_store(u + sel(cap, s, 1), w1);
@@ -463,17 +487,24 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     if (nerr) atomicAdd(err, (unsigned long long)nerr);
     if (DRIFT) {
         __syncthreads();
-        if (threadIdx.x == 0) ob.cnt[t] = min(nmov, (uint32_t)kOutbox);
+        if (threadIdx.x == 0) {
+            ob.cnt[t] = min(nmov, (uint32_t)kOutbox);
+            ho.cnt[t] = min(nhole, (uint32_t)kOutbox);
+            ho.used[t] = 0u;
+        }
     }
 }
 
-// The leavers of the last interpolation: one wave per tile appends its
-// outbox entries to their new buckets, clears their old slots' ids (the
-// interpolation made those slots holes) and empties the outbox, so that a
-// second call is a no-op.
+// The leavers of the last interpolation: one wave per tile moves its outbox
+// entries to their new tiles, into a hole of the new tile's range if the
+// last interpolation listed one that no arrival has taken yet (the buckets
+// then keep their length), else appended to the bucket; then empties the
+// outbox, so that a second call is a no-op.  The interpolation's own appends
+// (full outbox) never take holes: a hole in a range that the same
+// interpolation walks could be moved twice in one step.
 template <int TS>
-__global__ __launch_bounds__(256) void k_state_arrive(StateSets ss, int64_t cap, Dest dst, Outbox ob, int ng,
-                                                      uint32_t ntiles) {
+__global__ __launch_bounds__(256) void k_state_arrive(StateSets ss, int64_t cap, Dest dst, Outbox ob, Holes ho,
+                                                      int ng, uint32_t ntiles) {
     const uint32_t t = blockIdx.x * 4 + threadIdx.x / 64;
     if (t >= ntiles) return;
     const int cs = (int)dst.ctl[3];
@@ -488,9 +519,20 @@ __global__ __launch_bounds__(256) void k_state_arrive(StateSets ss, int64_t cap,
         const int64_t e = (int64_t)t * kOutbox + k;
         const uint32_t na = ob.q[sel(no, e, 0)], nb = ob.q[sel(no, e, 1)], nc = ob.q[sel(no, e, 2)];
         const float n0 = ob.u[sel(no, e, 0)], n1 = ob.u[sel(no, e, 1)], n2 = ob.u[sel(no, e, 2)];
-        const int64_t s = ob.s[e];
-        state_append(dst, q, u, id, cap, split_key(na, nb, nc, sh, nt), na, nb, nc, n0, n1, n2, id[s]);
-        id[s] = kNoId;
+        const uint32_t pid = ob.i[e], t2 = split_key(na, nb, nc, sh, nt);
+        const uint32_t h = atomicAdd(&ho.used[t2], 1u);
+        if (h < ho.cnt[t2]) {
+            const int64_t s = ho.s[(int64_t)t2 * kOutbox + h];
+            q[sel(cap, s, 0)] = na;
+            q[sel(cap, s, 1)] = nb;
+            q[sel(cap, s, 2)] = nc;
+            u[sel(cap, s, 0)] = n0;
+            u[sel(cap, s, 1)] = n1;
+            u[sel(cap, s, 2)] = n2;
+            id[s] = pid;
+        } else {
+            state_append(dst, q, u, id, cap, t2, na, nb, nc, n0, n1, n2, pid);
+        }
     }
     if ((threadIdx.x & 63u) == 0u && n) ob.cnt[t] = 0u;
 }
@@ -508,8 +550,9 @@ int launch_state_load(const double* x, const double* y, const double* z, const d
     // every particle pending, no outbox entries; the next step compacts
     const int ts = sorted_tile(ng);
     const size_t ntiles = (size_t)(ng / ts) * (ng / ts) * (ng / ts);
-    hipLaunchKernelGGL(k_fill<uint32_t>, dim3((unsigned)std::min<size_t>((ntiles + 255) / 256, 1024)), dim3(256), 0, s,
-                       st.ocnt, ntiles, 0u);
+    // outbox counts, hole counts and their use (contiguous): none
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3((unsigned)std::min<size_t>((3 * ntiles + 255) / 256, 1024)), dim3(256), 0,
+                       s, st.ocnt, 3 * ntiles, 0u);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, st.ctl, (size_t)1, (uint32_t)np);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, st.ctl + 1, (size_t)1, 1u);
     return check_launch("split_load");
@@ -554,7 +597,7 @@ int launch_state_interp(SplitState& st, int64_t np, int ng, const float* phi, do
     const Dest dst{st.boff, st.bfill, st.pq, st.pu, st.pid, np, st.ctl};
 #define PM_SI(TSV, DR)                                                                                              \
     hipLaunchKernelGGL((k_state_interp<TSV, DR>), dim3(ntiles), dim3(kTileThreads), 0, hs, sets(st), st.cap, st.boff, \
-                       st.bfill, st.bnow, dst, outbox(st), st.ctl, ng, phi, (float)kick, dscale, err)
+                       st.bfill, st.bnow, dst, outbox(st), holes(st), st.ctl, ng, phi, (float)kick, dscale, err)
     if (ts == 16) {
         if (drift != 0.0)
             PM_SI(16, true);
@@ -576,9 +619,11 @@ int launch_state_arrive(SplitState& st, int64_t np, int ng, hipStream_t hs) {
     const Dest dst{st.boff, st.bfill, st.pq, st.pu, st.pid, np, st.ctl};
     const dim3 grid((ntiles + 3) / 4);
     if (ts == 16)
-        hipLaunchKernelGGL(k_state_arrive<16>, grid, dim3(256), 0, hs, sets(st), st.cap, dst, outbox(st), ng, ntiles);
+        hipLaunchKernelGGL(k_state_arrive<16>, grid, dim3(256), 0, hs, sets(st), st.cap, dst, outbox(st), holes(st), ng,
+                           ntiles);
     else
-        hipLaunchKernelGGL(k_state_arrive<8>, grid, dim3(256), 0, hs, sets(st), st.cap, dst, outbox(st), ng, ntiles);
+        hipLaunchKernelGGL(k_state_arrive<8>, grid, dim3(256), 0, hs, sets(st), st.cap, dst, outbox(st), holes(st), ng,
+                           ntiles);
     return check_launch("split_arrive");
 }
 
