@@ -5,7 +5,9 @@
 //   spectrum A  C1[kz][x][y]          (M+1) * N * N complex   (after the z pass)
 //   spectrum B  C2[kz][ky][kx]        (M+1) * N * N complex   (after the y pass)
 // Every pass reads or writes whole lines on one side and 128-byte segments
-// of T consecutive lines on the other, staging a tile of T lines in LDS:
+// of T consecutive lines on the other, staging a tile of T lines in LDS
+// (the x pass, and the contiguous side of the z passes, reads or writes its
+// lines in the first or last Stockham stage instead):
 //   zfwd : R  -> C1  real FFT along z (M-point complex FFT + split)
 //   yfwd : C1 -> C2  complex FFT along y
 //   x    : C2 -> C2  FFT along x, Green's multiply, inverse FFT along x
@@ -128,24 +130,30 @@ template <int DIR, typename C> struct Dft<8, DIR, C> {
 // done; tw[m] = exp(-2 pi i m / L) for m < L/2, and the stage takes
 // exp(-2 pi i m / L) = -tw[m - L/2] above that (the tables are built with
 // exactly this symmetry, so only their first half is ever read).
-template <typename C, int L, int T, int R, int DIR, int NT>
-__device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict__ tw) {
+// The stage's inputs come from src(line, i) and its outputs go to
+// dst(line, i, v): element i of a line in LDS for every stage but the first
+// and the last of a pass, which may read or write global memory instead (a
+// stage reads b + r Q and writes b + r p of a line for the Q = L/R lanes b,
+// i.e. consecutive lanes take consecutive elements when p = 1 on input and
+// p = L/R on output: whole segments).  GIN: the input is not the LDS tile,
+// so no barrier is needed between the reads and the writes; GOUT: the
+// output is not the LDS tile, so none after the writes.
+template <typename C, int L, int T, int R, int DIR, int NT, bool GIN, bool GOUT, typename Src, typename Dst>
+__device__ __forceinline__ void stockham_stage_io(int p, const C* __restrict__ tw, const Src& src, const Dst& dst) {
     constexpr int Q = L / R;
     constexpr int NBF = T * Q;
     constexpr int PER = (NBF + NT - 1) / NT;
-    constexpr int LP = lds_pitch(L);
     C u[PER][R];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int g = threadIdx.x + q * NT;
         if (NBF % NT == 0 || g < NBF) {
             const int line = g / Q, b = g % Q;
-            const C* row = s + line * LP;
 #pragma unroll
-            for (int r = 0; r < R; ++r) u[q][r] = row[lpos(b + r * Q)];
+            for (int r = 0; r < R; ++r) u[q][r] = src(line, b + r * Q);
         }
     }
-    __syncthreads();
+    if constexpr (!GIN) __syncthreads();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int g = threadIdx.x + q * NT;
@@ -164,31 +172,59 @@ __device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict_
             }
             Dft<R, DIR, C>::run(u[q]);
             const int j = (b - k) * R + k;
-            C* row = s + line * LP;
 #pragma unroll
-            for (int r = 0; r < R; ++r) row[lpos(j + r * p)] = u[q][r];
+            for (int r = 0; r < R; ++r) dst(line, j + r * p, u[q][r]);
         }
     }
-    __syncthreads();
+    if constexpr (!GOUT) __syncthreads();
 }
-
-// Full length-L FFT of T lines in LDS: radix-8 stages, then one radix-4 or
-// radix-2 stage for the remaining factor.
+// a product rounded on its own (never contracted into an FMA with the add
+// that consumes it)
+template <typename Rl> __device__ __forceinline__ Rl rmul(Rl a, Rl b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+// element i of line `line` of the LDS tile s
+template <typename C, int L> struct LdsLine {
+    C* s;
+    __device__ __forceinline__ C operator()(int line, int i) const { return s[line * lds_pitch(L) + lpos(i)]; }
+    __device__ __forceinline__ void operator()(int line, int i, C v) const { s[line * lds_pitch(L) + lpos(i)] = v; }
+};
+template <typename C, int L, int T, int R, int DIR, int NT>
+__device__ __forceinline__ void stockham_stage(C* s, int p, const C* __restrict__ tw) {
+    const LdsLine<C, L> l{s};
+    stockham_stage_io<C, L, T, R, DIR, NT, false, false>(p, tw, l, l);
+}
+// Full length-L FFT of T lines: radix-8 stages, then one radix-4 or
+// radix-2 stage for the remaining factor; the first stage reads src, the
+// last writes dst (GIN / GOUT: global memory), the others the LDS tile s.
+template <typename C, int L, int T, int DIR, int NT, bool GIN, bool GOUT, int ST, typename Src, typename Dst>
+__device__ __forceinline__ void fft_stages_io(C* s, int p, const C* __restrict__ tw, const Src& src, const Dst& dst) {
+    constexpr int lg = ilog2(L), n8 = lg / 3, NST = n8 + (lg % 3 ? 1 : 0);
+    if constexpr (ST < NST) {
+        constexpr int R = ST < n8 ? 8 : (lg % 3 == 2 ? 4 : 2);
+        const LdsLine<C, L> l{s};
+        if constexpr (ST == 0 && ST == NST - 1)
+            stockham_stage_io<C, L, T, R, DIR, NT, GIN, GOUT>(p, tw, src, dst);
+        else if constexpr (ST == 0)
+            stockham_stage_io<C, L, T, R, DIR, NT, GIN, false>(p, tw, src, l);
+        else if constexpr (ST == NST - 1)
+            stockham_stage_io<C, L, T, R, DIR, NT, false, GOUT>(p, tw, l, dst);
+        else
+            stockham_stage_io<C, L, T, R, DIR, NT, false, false>(p, tw, l, l);
+        fft_stages_io<C, L, T, DIR, NT, GIN, GOUT, ST + 1>(s, p * R, tw, src, dst);
+    }
+}
+template <typename C, int L, int T, int DIR, int NT, bool GIN, bool GOUT, typename Src, typename Dst>
+__device__ __forceinline__ void fft_lines_io(C* s, const C* __restrict__ tw, const Src& src, const Dst& dst) {
+    fft_stages_io<C, L, T, DIR, NT, GIN, GOUT, 0>(s, 1, tw, src, dst);
+}
+// The same on the LDS tile alone.
 template <typename C, int L, int T, int DIR, int NT>
 __device__ __forceinline__ void fft_lines(C* s, const C* __restrict__ tw) {
-    constexpr int lg = ilog2(L);
-    int p = 1;
-    if constexpr (lg >= 3) {
-#pragma unroll
-        for (int st = 0; st < lg / 3; ++st) {
-            stockham_stage<C, L, T, 8, DIR, NT>(s, p, tw);
-            p *= 8;
-        }
-    }
-    if constexpr (lg % 3 == 2) stockham_stage<C, L, T, 4, DIR, NT>(s, p, tw);
-    if constexpr (lg % 3 == 1) stockham_stage<C, L, T, 2, DIR, NT>(s, p, tw);
+    const LdsLine<C, L> l{s};
+    fft_lines_io<C, L, T, DIR, NT, false, false>(s, tw, l, l);
 }
-
 // Twiddle table of a pass's L-point FFTs: for the passes in the mask of
 // its precision (bit = slot: 0 z fwd, 1 y fwd, 2 x, 3 y inv, 4 z inv) a
 // copy in LDS (the L/2 complex that the stages read), which the workgroup
@@ -257,16 +293,16 @@ __global__ __launch_bounds__(NT) void k_zfwd(const Rl* __restrict__ rho,
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int x = tile / YT, y0 = (tile % YT) * T;
     const C* src = reinterpret_cast<const C*>(rho + ((size_t)x * N + y0) * N);
-    for (int g = threadIdx.x; g < T * M; g += NT) {
-        C v = gload<Rl>(src + g);
+    // the first stage reads the lines (pairs of reals) from global memory
+    const auto gsrc = [src](int line, int i) {
+        C v = gload<Rl>(src + (size_t)line * M + i);
         if (SUB1) {
             v.x -= Rl(1);
             v.y -= Rl(1);
         }
-        s[(g / M) * LP + lpos(g % M)] = v;
-    }
-    __syncthreads();
-    fft_lines<C, M, T, -1, NT>(s, tw);
+        return v;
+    };
+    fft_lines_io<C, M, T, -1, NT, true, false>(s, tw, gsrc, LdsLine<C, M>{s});
     // X[k] = (Z[k] + conj Z[M-k])/2 - i W^k (Z[k] - conj Z[M-k])/2, Z[M] = Z[0]
     for (int g = threadIdx.x; g < T * (M + 1); g += NT) {
         const int kz = g / T, line = g % T;
@@ -293,6 +329,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cplx_t<Rl>* __restrict__ in,
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / XT, x0 = (tile % XT) * T;
     const C* src = in + ((size_t)kz * N + x0) * N;
+    // (a first stage reading global memory measured slower here: 0.184 -> 0.187 ms)
     for (int g = threadIdx.x; g < T * N; g += NT) s[(g / N) * LP + lpos(g % N)] = gload<Rl>(src + g);
     __syncthreads();
     fft_lines<C, N, T, -1, NT>(s, tw);
@@ -332,6 +369,25 @@ __global__ __launch_bounds__(NT) void k_x(cplx_t<Rl>* __restrict__ data,
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const int kz = tile / YT, ky0 = (tile % YT) * T;
     C* base = data + ((size_t)kz * N + ky0) * N;
+    if constexpr (MODE == (kXForward | kXGreen | kXInverse)) {
+        // the forward FFT's first stage reads the lines from global memory,
+        // Green's multiply rides on the inverse FFT's first reads, and its
+        // last stage writes the lines back: three LDS round trips and five
+        // barriers fewer than load, transform, multiply, transform, store
+        const auto gsrc = [base](int line, int i) { return gload<Rl>(base + (size_t)line * N + i); };
+        const auto gdst = [base](int line, int i, C v) { gstore<Rl>(base + (size_t)line * N + i, v); };
+        const LdsLine<C, N> l{s};
+        fft_lines_io<C, N, T, -1, NT, true, false>(s, tw, gsrc, l);
+        // (rounded products, never fused into the butterflies' adds: the
+        // values of a separate multiply pass, bit for bit)
+        const auto green = [&](int line, int kx) {
+            const C v = l(line, kx);
+            const Rl gv = (Rl)(green_value(ga.mode, N, kx, ky0 + line, kz, ga.cos_tab) * ga.scale);
+            return C{rmul(v.x, gv), rmul(v.y, gv)};
+        };
+        fft_lines_io<C, N, T, +1, NT, false, true>(s, tw, green, gdst);
+        return;
+    }
     for (int g = threadIdx.x; g < T * N; g += NT) s[(g / N) * LP + lpos(g % N)] = gload<Rl>(base + g);
     __syncthreads();
     if constexpr (MODE & kXForward) fft_lines<C, N, T, -1, NT>(s, tw);
@@ -367,6 +423,7 @@ __global__ __launch_bounds__(NT) void k_yinv(const cplx_t<Rl>* __restrict__ in,
         s[line * LP + lpos(ky)] = gload<Rl>(src + (size_t)ky * N + line);
     }
     __syncthreads();
+    // (a last stage writing global memory measured slower here: 0.186 -> 0.188 ms)
     fft_lines<C, N, T, +1, NT>(s, tw);
     C* dst = out + ((size_t)kz * N + x0) * N;
     for (int g = threadIdx.x; g < T * N; g += NT) gstore<Rl>(dst + g, s[(g / N) * LP + lpos(g % N)]);
@@ -380,8 +437,6 @@ __global__ __launch_bounds__(NT) void k_zinv(const cplx_t<Rl>* __restrict__ in,
                                                       const cplx_t<Rl>* __restrict__ twN, Rl norm) {
     using C = cplx_t<Rl>;
     constexpr int M = N / 2, LP = lds_pitch(M), YT = N / T;
-    constexpr int NPRE = T * M;
-    constexpr int PER = (NPRE + NT - 1) / NT;
     __shared__ C s[T * LP];
     __shared__ tw_lds_t<C, M, 4> tws;
     const C* tw = stage_twiddles<C, M, NT, 4>(tws, twM);
@@ -392,38 +447,26 @@ __global__ __launch_bounds__(NT) void k_zinv(const cplx_t<Rl>* __restrict__ in,
         s[line * LP + lpos(kz)] = gload<Rl>(in + ((size_t)kz * N + x) * N + y0 + line);
     }
     __syncthreads();
-    // Z[k] = (X[k] + conj X[M-k]) + i W^{-k} (X[k] - conj X[M-k]), k in [0, M);
-    // the imaginary parts of X[0] and X[M] are dropped (Hermitian input).
-    C z[PER];
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int g = threadIdx.x + q * NT;
-        if (NPRE % NT == 0 || g < NPRE) {
-            const int line = g / M, k = g % M;
-            const C* row = s + line * LP;
-            C xk = row[lpos(k)], xm = row[lpos(M - k)];
-            if (k == 0) {
-                xk.y = Rl(0);
-                xm.y = Rl(0);
-            }
-            const C a = C{xk.x + xm.x, xk.y - xm.y};
-            const C b = cmulc(C{xk.x - xm.x, xk.y + xm.y}, twN[k]);
-            z[q] = C{a.x - b.y, a.y + b.x};
+    // Z[k] = (X[k] + conj X[M-k]) + i W^{-k} (X[k] - conj X[M-k]), k in [0, M),
+    // formed as the first stage reads it (the imaginary parts of X[0] and
+    // X[M] are dropped: Hermitian input); the last stage writes the lines,
+    // normalised, to global memory
+    const auto zsrc = [&](int line, int k) {
+        const C* row = s + line * LP;
+        C xk = row[lpos(k)], xm = row[lpos(M - k)];
+        if (k == 0) {
+            xk.y = Rl(0);
+            xm.y = Rl(0);
         }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int g = threadIdx.x + q * NT;
-        if (NPRE % NT == 0 || g < NPRE) s[(g / M) * LP + lpos(g % M)] = z[q];
-    }
-    __syncthreads();
-    fft_lines<C, M, T, +1, NT>(s, tw);
+        const C a = C{xk.x + xm.x, xk.y - xm.y};
+        const C b = cmulc(C{xk.x - xm.x, xk.y + xm.y}, twN[k]);
+        return C{a.x - b.y, a.y + b.x};
+    };
     C* dst = reinterpret_cast<C*>(out + ((size_t)x * N + y0) * N);
-    for (int g = threadIdx.x; g < T * M; g += NT) {
-        const C v = s[(g / M) * LP + lpos(g % M)];
-        gstore<Rl>(dst + g, C{v.x * norm, v.y * norm});
-    }
+    const auto gdst = [dst, norm](int line, int i, C v) {
+        gstore<Rl>(dst + (size_t)line * M + i, C{rmul(v.x, norm), rmul(v.y, norm)});
+    };
+    fft_lines_io<C, M, T, +1, NT, false, true>(s, tw, zsrc, gdst);
 }
 
 // ---------------------------------------------------------- apply_green
