@@ -409,7 +409,7 @@ __global__ __launch_bounds__(256) void k_deposit_state(const uint32_t* __restric
 // Capacities of the staging buckets for the next step: count + count/8 + 32
 // per tile (counts from the binning offsets on a first step, else from the
 // fill counters of the staged step), or of the split state's buckets from
-// its tile counts (count + count/4 + 64 rounded up to 32; the counts are
+// its tile counts (count + count/2 + 64 rounded up to 32; the counts are
 // cleared), exclusive
 // scan into coff[0..n]; the fill counters and the overflow flag are cleared
 // for the next staging.  A non-null gate makes the kernel a no-op unless
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(kScanThreads) void k_cap_scan(uint32_t* __restrict_
                 c = cnt[i];
                 cnt[i] = 0u;
                 fill[i] = 0u;
-                return (c + (c >> 2) + 64u + 31u) & ~31u;  // split_capacity: arrivals, whole lines
+                return (c + (c >> 1) + 64u + 31u) & ~31u;  // split_capacity: arrivals, whole lines
             } else {
                 c = fill[i];
             }

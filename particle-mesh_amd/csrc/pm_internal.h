@@ -256,11 +256,14 @@ struct SplitState {
     uint32_t* huse = nullptr;
 };
 constexpr int kOutbox = 128;
-// slots per tile bucket of the split state: count + count/4 + 64, rounded up
+// slots per tile bucket of the split state: count + count/2 + 64, rounded up
 // to 32 slots (a bucket starts on a 128-byte line of every slot array, so a
 // wave's 64 slots are whole lines); the total is a multiple of 32 as well, so
-// that every array of both sets starts on a line
-inline size_t split_capacity(size_t n, size_t ntiles) { return (n + n / 4 + 96 * ntiles + 31) / 32 * 32; }
+// that every array of both sets starts on a line.  The slack takes the
+// arrivals that find no hole: tiles that accrete (collapsing regions of a
+// clustered run) grow by several percent a step, and one full bucket makes
+// the next step rebuild every bucket.
+inline size_t split_capacity(size_t n, size_t ntiles) { return (n + n / 2 + 96 * ntiles + 31) / 32 * 32; }
 int launch_state_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
                       const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err, hipStream_t s);
 int launch_state_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,

@@ -36,8 +36,9 @@
 //            of slots (count, capacity scan, placement), which becomes the
 //            current one (ctl[3]).  Every compaction kernel is a no-op unless
 //            the flag is set.  At 512^3, with the holes refilled, a
-//            bucket's slack (count/4 + 64 slots) lasts past 25 steps of the
-//            clustered benchmark ICs.
+//            bucket's slack (count/2 + 64 slots) lasts past 25 steps of the
+//            clustered benchmark ICs; in the strongly clustered regime (150
+//            steps) a rebuild still comes every few steps.
 // A tile processes only the slots it held when the step's deposit read it
 // (bnow), so an arrival is not moved twice in one step.  Results do not
 // depend on the slot order: the deposit sums exactly, the rest is per
@@ -578,7 +579,7 @@ int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, h
                        ng, ts, s.cnt);
     int rc = check_launch("split_count");
     if (rc) return rc;
-    // capacities (count + count/4 + 64, rounded up to 32) into boff, bfill cleared
+    // capacities (count + count/2 + 64, rounded up to 32) into boff, bfill cleared
     rc = launch_cap_scan_raw(nt * nt * nt, st.bfill, s.cnt, st.boff, st.ctl + 2, st.ctl + 1, hs);
     if (rc) return rc;
     hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 1024)), dim3(256), 0, hs, sets(st), st.ctl, 1,
