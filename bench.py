@@ -564,6 +564,11 @@ def main():
             torch.mul(a, 1.0, out=b)
         torch.cuda.synchronize()
 
+    # the device's measured streaming ceiling (1 GiB read + 1 GiB written, one
+    # 16-byte load and store a thread), beside the 8 TB/s nameplate
+    copy_gbs = pmhip.copy_rate(1 << 30, 20)
+    log(f"device copy rate: {copy_gbs:.0f} GB/s (read + write)")
+
     shape32 = ph32.pop("fft_shape")
     if msd is not None:
         phd.pop("fft_shape", None)
@@ -589,6 +594,8 @@ def main():
         unstaged = algorithmic_bytes(ng, npart, pmhip.PM_FP32, False, is_split)[dom]
         roof["algorithmic_bytes_unstaged"] = unstaged
         roof["frac_unstaged"] = round(unstaged / (ph[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        roof["copy_ceiling"] = round(copy_gbs, 1)
+        roof["frac_of_copy"] = round(achieved / copy_gbs, 4)
         return roof, sum(nb.values())
 
     roofline, step_bytes = fp32_roofline(ph32, split)
@@ -608,6 +615,8 @@ def main():
         un64 = algorithmic_bytes(ng, npart, pmhip.PM_FP64, False)[dom64]
         roofline64["algorithmic_bytes_unstaged"] = un64
         roofline64["frac_unstaged"] = round(un64 / (ph64[dom64] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        roofline64["copy_ceiling"] = round(copy_gbs, 1)
+        roofline64["frac_of_copy"] = round(ach64 / copy_gbs, 4)
     cpu, acc = None, None
     if not args.no_cpu:
         log(f"CPU baseline leg: the FP64 oracle, {args.accuracy_steps} KDK steps on the host cores")

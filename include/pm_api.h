@@ -332,6 +332,15 @@ int pm_wisdom_export(const char* path);
 /* *found = 1 and the variants if the cache holds (ng, precision), else 0. */
 int pm_wisdom_lookup(int ng, int precision, int variants[PM_FFT_SLOTS], int* found);
 void pm_wisdom_forget(void);
+/* The device's streaming copy rate, the ceiling that the HBM-bound kernels
+ * are measured against beside the 8 TB/s nameplate (no counterpart in the
+ * reference, which reports times only, main.cpp:84-90 / utils.cpp:12-20):
+ * `bytes` (a multiple of 16, at least 16 MiB) copied between two fresh
+ * device buffers by one 16-byte load and store per thread, `repeats` timed
+ * launches after two warm-ups on a private stream; *gbs receives bytes read
+ * plus bytes written per second, in GB/s.  Allocates and frees the two
+ * buffers. */
+int pm_copy_rate(size_t bytes, int repeats, double* gbs);
 
 /* ---- 5. deposit without device allocation -------------------------------
  * No counterpart in the reference: for callers that cannot call hipMalloc

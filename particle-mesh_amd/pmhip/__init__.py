@@ -108,6 +108,7 @@ SIGNATURES = {
     "pm_wisdom_export": (_c_int, [ctypes.c_char_p]),
     "pm_wisdom_lookup": (_c_int, [_c_int, _c_int, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)]),
     "pm_wisdom_forget": (None, []),
+    "pm_copy_rate": (_c_int, [ctypes.c_size_t, _c_int, ctypes.POINTER(ctypes.c_double)]),
     "pm_deposit_sorted_noalloc": (_c_int, [_c_vp, _c_vp, _c_vp, _c_i64, _c_int, _c_int, _c_vp, _c_vp]),
 }
 
@@ -466,3 +467,11 @@ def wisdom_lookup(ng: int, precision: int):
 
 def wisdom_forget() -> None:
     lib().pm_wisdom_forget()
+
+
+def copy_rate(nbytes: int = 1 << 30, repeats: int = 20) -> float:
+    """The device's streaming copy rate in GB/s (bytes read + written), the
+    measured ceiling of the HBM-bound kernels (pm_copy_rate)."""
+    gbs = ctypes.c_double()
+    check(lib().pm_copy_rate(nbytes, repeats, ctypes.byref(gbs)))
+    return gbs.value

@@ -89,6 +89,11 @@ def test_tuning_entry_points_reject_bad_arguments():
     pmhip.wisdom_forget()
     assert lib.pm_wisdom_lookup(512, pmhip.PM_FP32, None, ctypes.byref(found)) == pmhip.PM_OK
     assert found.value == 0
+    gbs = ctypes.c_double()
+    assert lib.pm_copy_rate(1 << 30, 5, None) == pmhip.PM_ERR_ARG
+    assert lib.pm_copy_rate(1 << 20, 5, ctypes.byref(gbs)) == pmhip.PM_ERR_ARG  # below 16 MiB
+    assert lib.pm_copy_rate((1 << 30) + 8, 5, ctypes.byref(gbs)) == pmhip.PM_ERR_ARG  # not whole 16-byte units
+    assert lib.pm_copy_rate(1 << 30, 0, ctypes.byref(gbs)) == pmhip.PM_ERR_ARG
 
 
 def _write_snapshot_numpy(path, n, ng, step, a, magic=b"PMSNAP01", version=1):

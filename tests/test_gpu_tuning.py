@@ -177,3 +177,12 @@ def test_pm_main_writes_then_reads_its_wisdom_file(dev, tmp_path):
         assert any(ln.split()[:2] == ["64", "fp64"] for ln in lines[1:])
         dumps.append(np.fromfile(out, dtype=np.float64))
     assert np.max(np.abs(dumps[0] - dumps[1])) < 1e-11 * np.abs(dumps[0]).max()
+
+
+def test_copy_rate_is_a_plausible_hbm_rate(dev):
+    """pm_copy_rate times a plain 1 GiB device copy: on an MI355X (8 TB/s
+    nameplate) read + write lands between 2 and 8 TB/s."""
+    print("case copy rate", flush=True)
+    gbs = pmhip.copy_rate(1 << 30, 10)
+    print(f"copy rate {gbs:.0f} GB/s", flush=True)
+    assert 2000.0 < gbs < 8000.0
