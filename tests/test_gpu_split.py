@@ -220,3 +220,45 @@ def test_split_kdk_vs_fp64_oracle(dev, ng, nrow):
     print(f"split vs oracle after {nsteps} steps: rms {rms:.3e} max {mx:.3e} cells", flush=True)
     assert rms < 1e-5
     assert mx < 1e-4
+
+
+def test_split_long_high_mobility_vs_fp64_oracle(dev):
+    """100 KDK steps in which up to a fifth of the particles change tile every
+    step (velocities up to 40 cells per unit time, dt = 0.01; gravity weakened
+    by a Green's scale of 1e-3 so that FP32 and FP64 trajectories stay
+    comparable): the outboxes, the hole lists, the arrivals into holes and onto
+    bucket ends and the rebuilds behind full buckets all run many times.
+    Every particle comes back exactly once (outputs start as NaN), and the
+    positions match the FP64 oracle of the same run."""
+    import torch
+
+    print("case split long high-mobility run ng=64 np=64^3 100 steps", flush=True)
+    ng, nrow, nsteps, dt, scale = 64, 64, 100, 0.01, 1e-3
+    npart = nrow ** 3
+    rng = np.random.default_rng(53)
+    pos0 = [torch.tensor(rng.uniform(0.0, ng, npart), device=dev) for _ in range(3)]
+    pos0 = _on_grid(pos0, ng)
+    vel0 = [torch.tensor(rng.uniform(-40.0, 40.0, npart), device=dev).float().double() for _ in range(3)]
+    with pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, scale) as plan:
+        plan.split_load(pos0, vel0)
+        plan.split_run_kdk(nsteps, dt)
+        p = [torch.full_like(t, float("nan")) for t in pos0]
+        v = [torch.full_like(t, float("nan")) for t in vel0]
+        plan.split_store(p, v)
+        torch.cuda.synchronize()
+    assert pmhip.range_errors() == 0
+    for t in p + v:
+        assert not bool(torch.isnan(t).any()), "a particle was lost"
+    ph = [t.cpu().numpy().copy() for t in pos0]
+    vh = [t.cpu().numpy().copy() for t in vel0]
+    pyoracle.set_threads(min(16, pyoracle.max_threads()))
+    pyoracle.run_kdk(ph, vh, ng, nsteps, dt, pyoracle.GREEN_CONT_K2, scale)
+    d2 = sum(_periodic_diff(a.cpu().numpy(), b, ng) ** 2 for a, b in zip(p, ph))
+    rms, mx = float(np.sqrt(d2.mean())), float(np.sqrt(d2.max()))
+    moved = float(np.sqrt(sum(_periodic_diff(a.cpu().numpy(), b.cpu().numpy(), ng) ** 2
+                              for a, b in zip(p, pos0)).mean()))
+    print(f"after {nsteps} steps: rms displacement {moved:.1f} cells; vs oracle rms {rms:.3e} max {mx:.3e} cells",
+          flush=True)
+    assert moved > 5.0  # the particles did cross many tiles
+    assert rms < 1e-4
+    assert mx < 1e-3
