@@ -78,7 +78,8 @@ def algorithmic_bytes(ng: int, npart: int, prec: int, staged: bool = True, split
     read once.  The split state is kept in tile order instead: no record."""
     b = 4 if prec == 0 else 8
     st = (12 if prec == 0 else 16) if staged else 0
-    state = 24 if split else 48
+    # the split state: 3 u32 + 3 FP32 with an FP32 mesh, 3 u64 + 3 FP64 with an FP64 mesh
+    state = (24 if prec == 0 else 48) if split else 48
     nc = ng ** 3
     nk = (ng // 2 + 1) * ng * ng
     if split:
@@ -86,7 +87,7 @@ def algorithmic_bytes(ng: int, npart: int, prec: int, staged: bool = True, split
         # positions from the tile buckets, the interpolation reads and writes
         # the 24 B of state in place (no record)
         st = 0
-        dep = 12
+        dep = 12 if prec == 0 else 24
     else:
         dep = st if staged else 24
     return {
@@ -115,6 +116,7 @@ PHASE_KERNELS = {
 # the same for the split-state step (pm_split_step)
 SPLIT_PHASE_KERNELS = dict(PHASE_KERNELS, deposit="void pm::k_deposit_state<16>",
                            interp_kick_drift="void pm::k_state_interp<16, true>")
+FP64_SPLIT_INTERP_KERNEL = "void pm::k_state_interp64<16, true>"
 
 
 def measured_traffic(kernel: str, ng: int, npart: int, table: str = ""):
@@ -308,6 +310,17 @@ def gpu_fields(pos0, vel0, args, ng, npart, variant, nsteps):
     plan.close()
     del vel
     fields["split"] = pos
+    # the same with an FP64 mesh (u64 positions, FP64 velocities)
+    plan = pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0)
+    plan.split_load(pos0, vel0)
+    plan.split_run_kdk(nsteps, args.dt)
+    pos = [torch.empty_like(p) for p in pos0]
+    vel = [torch.empty_like(v) for v in vel0]
+    plan.split_store(pos, vel)
+    torch.cuda.synchronize()
+    plan.close()
+    del vel
+    fields["split64"] = pos
     for prec in (pmhip.PM_FP64, pmhip.PM_FP32):
         plan = pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, 1.0)
         plan.deposit(pos0, variant)
@@ -434,6 +447,7 @@ def accuracy_report(fields, ref, ng, nsteps, dt, dev):
     out = {"against": "FP64 oracle (C/OpenMP restatement of the reference step) on the same ICs", "steps": nsteps,
            "dt": dt, "target_rms_cells": 1e-4,
            "fp32_split": pos_errors(fields["split"], oref[3]),
+           "fp64_split": pos_errors(fields["split64"], oref[3]),
            "fp32": errors(fields[pmhip.PM_FP32], oref), "fp64": errors(fields[pmhip.PM_FP64], oref),
            "fp32_vs_gpu_fp64": errors(fields[pmhip.PM_FP32], fields[pmhip.PM_FP64])}
     del oref
@@ -537,6 +551,11 @@ def main():
         ms64, wall64, ph64, err64, _ = run_gpu(pmhip.PM_FP64, pos0, vel0, args, ng, npart, variant)
         log(f"FP64: {ms64:.3f} ms/step, phases {ph64}")
         res64 = {"value": npart / (ms64 * 1e-3) / 1e6, "ms_per_step": ms64, "phases_ms": ph64}
+        log("FP64-mesh leg, split state (u64 positions, FP64 velocities)")
+        ms64s, _, ph64s, err64s, _ = run_gpu(pmhip.PM_FP64, pos0, vel0, args, ng, npart, variant, split=True)
+        log(f"FP64 split: {ms64s:.3f} ms/step, phases {ph64s}")
+        res64["split"] = {"value": npart / (ms64s * 1e-3) / 1e6, "ms_per_step": ms64s, "phases_ms": ph64s,
+                          "range_errors": err64s}
 
     fields = None
     if not args.no_accuracy:
@@ -573,6 +592,8 @@ def main():
     if msd is not None:
         phd.pop("fft_shape", None)
     shape64 = res64["phases_ms"].pop("fft_shape") if res64 else None
+    if res64:
+        res64["split"]["phases_ms"].pop("fft_shape", None)
     sorted_dep = variant == pmhip.PM_DEPOSIT_SORTED
 
     def fp32_roofline(ph, is_split):
@@ -617,6 +638,22 @@ def main():
         roofline64["frac_unstaged"] = round(un64 / (ph64[dom64] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         roofline64["copy_ceiling"] = round(copy_gbs, 1)
         roofline64["frac_of_copy"] = round(ach64 / copy_gbs, 4)
+        # the FP64 mesh with the split state: its interpolation streams 96 B
+        # a particle (u64 position and FP64 velocity triples read and
+        # written) and reads the 8 B potential
+        ph64s = res64["split"]["phases_ms"]
+        nb64s = algorithmic_bytes(ng, npart, pmhip.PM_FP64, sorted_dep, True)
+        dom64s = max((k for k in ph64s if ph64s[k]), key=lambda k: ph64s[k])
+        ach64s = nb64s[dom64s] / (ph64s[dom64s] * 1e-3) / 1e9
+        k64s = FP64_SPLIT_INTERP_KERNEL if dom64s == "interp_kick_drift" else None
+        tr64s = measured_traffic_fp64(k64s, ng) if k64s else None
+        res64["split"]["roofline"] = {
+            "bound": "hbm", "kernel": k64s or dom64s, "phase": dom64s, "achieved": round(ach64s, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach64s / HBM_PEAK_GBS, 4), "traffic": tr64s,
+            "algorithmic_bytes": nb64s[dom64s], "avg_ms": round(ph64s[dom64s], 4),
+            "traffic_source": f"profiles/pmc_traffic_{ng}_fp64_adjoint.txt" if tr64s else None,
+            "copy_ceiling": round(copy_gbs, 1), "frac_of_copy": round(ach64s / copy_gbs, 4)}
+        res64["split"]["step_algorithmic_GB"] = round(sum(nb64s.values()) / 1e9, 3)
     cpu, acc = None, None
     if not args.no_cpu:
         log(f"CPU baseline leg: the FP64 oracle, {args.accuracy_steps} KDK steps on the host cores")
@@ -660,6 +697,12 @@ def main():
                   "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["phases_ms"].items()},
                   "roofline": roofline64}
                  if res64 else None),
+        "fp64_split": ({"value": round(res64["split"]["value"], 2),
+                        "ms_per_step": round(res64["split"]["ms_per_step"], 4),
+                        "phases_ms": {k: (round(v, 4) if v else v) for k, v in res64["split"]["phases_ms"].items()},
+                        "step_algorithmic_GB": res64["split"]["step_algorithmic_GB"],
+                        "roofline": res64["split"]["roofline"], "range_errors": res64["split"]["range_errors"]}
+                       if res64 else None),
         "fft_shape": {"fp32": shape32, "fp64": shape64, "tuned": not args.no_tune,
                       "slots": ["z_fwd", "y_fwd", "x_green", "y_inv", "z_inv"]},
         "alt_deposit": alt,

@@ -214,20 +214,24 @@ int pm_run_kdk(pm_plan_t* plan, double* posx, double* posy, double* posz, double
 int pm_run_cosmo(pm_plan_t* plan, double* posx, double* posy, double* posz, double* px, double* py,
                  double* pz, double a0, double da, int nsteps, int variant);
 
-/* Split-precision particle state (FP32-mesh plans only).  No counterpart in
- * the reference, which keeps FP64 positions and momenta (update_particles,
- * utils.cpp:141-170): this is the split representation of the 512^3
- * benchmark step.  The plan holds the particles itself, each position as an
- * integer cell plus a fixed-point fraction (one u32 per axis, x * 2^(32 -
- * log2 ng): 2^-23 cells at ng = 512; a drift is an integer add that wraps at
- * the box edge) and each velocity in FP32, 24 bytes per particle against 48.
- * The deposit, solve and force are those of pm_step with PM_DEPOSIT_SORTED;
- * the kick adds the FP32 force in FP32, and the drift adds v * drift rounded
- * to the nearest 2^-S cells.
+/* Split-precision particle state.  No counterpart in the reference, which
+ * keeps FP64 positions and momenta (update_particles, utils.cpp:141-170):
+ * this is the split representation of the 512^3 benchmark step.  The plan
+ * holds the particles itself, in tile order, each position as an integer
+ * cell plus a fixed-point fraction (a drift is an integer add that wraps at
+ * the box edge) and each velocity in the mesh precision.  FP32-mesh plans:
+ * one u32 per axis, x * 2^(32 - log2 ng), 2^-23 cells at ng = 512, and FP32
+ * velocities, 24 bytes per particle against 48.  FP64-mesh plans: one u64 per
+ * axis, x * 2^(64 - log2 ng), 2^-55 cells at ng = 512 (finer than an FP64
+ * position), and FP64 velocities.  The deposit, solve and force are those of
+ * pm_step with PM_DEPOSIT_SORTED (bit for bit from positions on the 2^-S
+ * grid); the kick adds the force in the mesh precision, and the drift adds
+ * v * drift rounded to the nearest 2^-S cells.
  * pm_split_load: copy the caller's FP64 particles in (positions rounded to
- * the nearest 2^-S cells; one outside [0, ng) is counted as a range error
- * and wrapped into the box).  pm_split_store: write them back as FP64 (exact
- * for the positions).  pm_split_step: one force evaluation + kick + drift on
+ * the nearest 2^-S cells, exact with an FP64 mesh; one outside [0, ng) is
+ * counted as a range error and wrapped into the box).  pm_split_store: write
+ * them back as FP64 (exact for the positions of an FP32 mesh; rounded to
+ * FP64 with an FP64 mesh).  pm_split_step: one force evaluation + kick + drift on
  * the plan's particles.  pm_split_run_kdk: nsteps kick-drift-kick steps as
  * pm_run_kdk does.  A kick that leaves a velocity non-finite is counted as a
  * range error and leaves that particle as it was. */

@@ -475,7 +475,6 @@ extern "C" int pm_run_cosmo(pm_plan_t* p, double* x, double* y, double* z, doubl
 extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, const double* z, const double* vx,
                              const double* vy, const double* vz) {
     if (!p || !x || !y || !z || !vx || !vy || !vz) return fail(PM_ERR_ARG, "pm_split_load: null argument");
-    if (p->prec != PM_FP32) return fail(PM_ERR_ARG, "pm_split_load: the split state needs an FP32-mesh plan");
     int rc = ensure_sorted(p);  // the shell buffers of the deposit's column walk
     if (rc) return rc;
     SplitState& st = p->split;
@@ -484,33 +483,38 @@ extern "C" int pm_split_load(pm_plan_t* p, const double* x, const double* y, con
         const size_t nt = (size_t)(p->ng / ts), ntiles = nt * nt * nt, n = (size_t)p->np;
         const size_t cap = split_capacity(n, ntiles);
         if (cap >= 0xFFFFFFF0ull) return fail(PM_ERR_ARG, "pm_split_load: too many particles for 32-bit slots");
-        // two sets of slots (q, u, id: 28 B a slot), the outboxes (28 B an
-        // entry), the hole lists (4 B an entry), the pending list (28 B a
-        // particle), boff, bfill, bnow, ocnt, hcnt, huse, ctl
+        // two sets of slots (q, u, id: 3 words + 3 words + 4 B a slot: 28 B
+        // with an FP32 mesh, 52 B with an FP64 mesh), the outboxes (the same
+        // an entry), the hole lists (4 B an entry), the pending list (the
+        // same a particle), boff, bfill, bnow, ocnt, hcnt, huse, ctl.  The
+        // 8-byte words start on 8-byte boundaries: cap is a multiple of 32,
+        // nout of 128, and the pending words come first in their part.
+        const size_t w = p->prec == PM_FP64 ? 8 : 4, sb = 6 * w + 4;
         const size_t nout = ntiles * kOutbox;
-        const size_t bytes = 2 * 28 * cap + 32 * nout + 28 * n + 4 * (6 * ntiles + 1) + 64;
+        const size_t bytes = 2 * sb * cap + (sb + 4) * nout + sb * n + 4 * (6 * ntiles + 1) + 64;
         char* b = nullptr;
         rc = hip_ok(hipMalloc((void**)&b, bytes), "pm_split_load: hipMalloc");
         if (rc) return rc;
         p->split_buf = b;
+        st.prec = p->prec;
         st.cap = (int64_t)cap;
         for (int k = 0; k < 2; ++k) {
-            char* c = b + 28 * cap * k;
-            st.q[k] = reinterpret_cast<uint32_t*>(c);
-            st.u[k] = reinterpret_cast<float*>(c + 12 * cap);
-            st.id[k] = reinterpret_cast<uint32_t*>(c + 24 * cap);
+            char* c = b + sb * cap * k;
+            st.q[k] = c;
+            st.u[k] = c + 3 * w * cap;
+            st.id[k] = reinterpret_cast<uint32_t*>(c + 6 * w * cap);
         }
-        char* c = b + 56 * cap;
-        st.oq = reinterpret_cast<uint32_t*>(c);
-        st.ou = reinterpret_cast<float*>(c + 12 * nout);
-        st.oid = reinterpret_cast<uint32_t*>(c + 24 * nout);
-        c += 28 * nout;
+        char* c = b + 2 * sb * cap;
+        st.oq = c;
+        st.ou = c + 3 * w * nout;
+        st.oid = reinterpret_cast<uint32_t*>(c + 6 * w * nout);
+        c += sb * nout;
         st.hole = reinterpret_cast<uint32_t*>(c);
         c += 4 * nout;
-        st.pq = reinterpret_cast<uint32_t*>(c);
-        st.pu = reinterpret_cast<float*>(c + 12 * n);
-        st.pid = reinterpret_cast<uint32_t*>(c + 24 * n);
-        c += 28 * n;
+        st.pq = c;
+        st.pu = c + 3 * w * n;
+        st.pid = reinterpret_cast<uint32_t*>(c + 6 * w * n);
+        c += sb * n;
         st.boff = reinterpret_cast<uint32_t*>(c);
         st.bfill = st.boff + ntiles + 1;
         st.bnow = st.bfill + ntiles;
@@ -553,8 +557,7 @@ extern "C" int pm_split_step(pm_plan_t* p, double kick, double drift) {
     mark(p, 1);
     rc = solve_impl(p);
     if (rc) return rc;
-    rc = launch_state_interp(p->split, p->np, p->ng, static_cast<const float*>(p->phi), kick, drift, p->err,
-                             p->stream);
+    rc = launch_state_interp(p->split, p->np, p->ng, p->phi, kick, drift, p->err, p->stream);
     if (rc) return rc;
     mark(p, 7);
     next_slot(p);

@@ -350,29 +350,30 @@ __global__ __launch_bounds__(256) void k_deposit_staged(const void* __restrict__
 }
 
 // 4''. as 4, particles from the tile buckets of the split-precision state
-// (pm_split.hip: positions as u32 cell + fraction in three planes of cap
-// slots; tile t holds bfill[t] slots from boff[t], holes among them, whose
-// tile does not match): the records are formed from the positions exactly
-// (split_rec), so the weights are those of the staged deposit of the same
-// positions.  Also leaves each tile's used slot count in bnow, the slots
-// the step's interpolation takes.
-template <int TS>
-__global__ __launch_bounds__(256) void k_deposit_state(const uint32_t* __restrict__ q0,
-                                                       const uint32_t* __restrict__ q1,
-                                                       const uint32_t* __restrict__ ctl, int64_t cap,
-                                                       const uint32_t* __restrict__ boff,
-                                                       const uint32_t* __restrict__ bfill, uint32_t* __restrict__ bnow,
-                                                       int ng, double mass, float* __restrict__ rho,
-                                                       float* __restrict__ shell) {
+// (pm_split.hip: positions as fixed-point cell + fraction, u32 for an FP32
+// mesh, u64 for an FP64 mesh, in triples of cap slots; tile t holds bfill[t]
+// slots from boff[t], holes among them, whose tile does not match): the
+// in-tile offsets are formed from the positions exactly (split_offs: at the
+// staged records' resolution, 2^-28 / 2^-38 cells), so the weights are those
+// of the staged deposit of the same positions.  Also leaves each tile's used
+// slot count in bnow, the slots the step's interpolation takes.
+template <typename Rl, int TS>
+__device__ __forceinline__ void deposit_state(const typename SplitWords<Rl>::Q* __restrict__ q0,
+                                              const typename SplitWords<Rl>::Q* __restrict__ q1,
+                                              const uint32_t* __restrict__ ctl, int64_t cap,
+                                              const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bfill,
+                                              uint32_t* __restrict__ bnow, int ng, double mass, Rl* __restrict__ rho,
+                                              Rl* __restrict__ shell) {
+    using Q = typename SplitWords<Rl>::Q;
     constexpr int E = TS + 1, E3 = E * E * E;
     __shared__ unsigned long long acc[E3];
-    const uint32_t* __restrict__ q = ctl[3] ? q1 : q0;  // the current set of slots
+    const Q* __restrict__ q = ctl[3] ? q1 : q0;  // the current set of slots
     const WalkSeg sw = walk_segment(ng, TS);
     const int qb = fix_bits(mass);
     const double scale[1] = {ldexp(1.0, -qb)};
-    float* const out[1] = {rho};
-    float* const sh[1] = {shell};
-    const int S = split_shift(ng), ksh = S + ilog2(TS);
+    Rl* const out[1] = {rho};
+    Rl* const sh[1] = {shell};
+    const int S = qshift<Q>(ng), ksh = S + ilog2(TS);
     const uint32_t nt = (uint32_t)sw.nt, tmask = TS - 1;
     for (int i = threadIdx.x; i < E3; i += 256) acc[i] = 0ULL;
     __syncthreads();
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(256) void k_deposit_state(const uint32_t* __restric
         const uint32_t beg = boff[t], n = min(bfill[t], boff[t + 1] - beg), end = beg + n;
         if (threadIdx.x == 0) bnow[t] = n;
         for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += 256 * kDepBatch) {
-            uint32_t a[kDepBatch], b[kDepBatch], c[kDepBatch];
+            Q a[kDepBatch], b[kDepBatch], c[kDepBatch];
 #pragma unroll
             for (int j = 0; j < kDepBatch; ++j) {
                 const uint32_t s = s0 + 256 * j;
@@ -394,16 +395,34 @@ __global__ __launch_bounds__(256) void k_deposit_state(const uint32_t* __restric
             }
 #pragma unroll
             for (int j = 0; j < kDepBatch; ++j) {
-                if (s0 + 256 * j >= end || split_key(a[j], b[j], c[j], ksh, nt) != t) continue;  // a hole
-                const TileOffs o =
-                    tile_offs(StageQ{split_rec(a[j], S, tmask), split_rec(b[j], S, tmask), split_rec(c[j], S, tmask)});
+                if (s0 + 256 * j >= end || qkey<Q>(a[j], b[j], c[j], ksh, nt) != t) continue;  // a hole
+                const TileOffs o = split_offs<Q>(a[j], b[j], c[j], S, tmask);
                 tile_cic_add<TS>(acc, o.a, o.b, o.c, o.fx, o.fy, o.fz, fix);
             }
         }
         __syncthreads();
-        walk_tile_out<float, TS, 1, 256>(acc, scale, t, sw.tx * TS, sw.ty * TS, tz * TS, ng, out, sh);
+        walk_tile_out<Rl, TS, 1, 256>(acc, scale, t, sw.tx * TS, sw.ty * TS, tz * TS, ng, out, sh);
     }
-    walk_seam_out<float, TS, 1, 256>(acc, scale, sw, sh);
+    walk_seam_out<Rl, TS, 1, 256>(acc, scale, sw, sh);
+}
+template <int TS>
+__global__ __launch_bounds__(256) void k_deposit_state(const uint32_t* __restrict__ q0, const uint32_t* __restrict__ q1,
+                                                       const uint32_t* __restrict__ ctl, int64_t cap,
+                                                       const uint32_t* __restrict__ boff,
+                                                       const uint32_t* __restrict__ bfill, uint32_t* __restrict__ bnow,
+                                                       int ng, double mass, float* __restrict__ rho,
+                                                       float* __restrict__ shell) {
+    deposit_state<float, TS>(q0, q1, ctl, cap, boff, bfill, bnow, ng, mass, rho, shell);
+}
+template <int TS>
+__global__ __launch_bounds__(256) void k_deposit_state64(const uint64_t* __restrict__ q0,
+                                                         const uint64_t* __restrict__ q1,
+                                                         const uint32_t* __restrict__ ctl, int64_t cap,
+                                                         const uint32_t* __restrict__ boff,
+                                                         const uint32_t* __restrict__ bfill,
+                                                         uint32_t* __restrict__ bnow, int ng, double mass,
+                                                         double* __restrict__ rho, double* __restrict__ shell) {
+    deposit_state<double, TS>(q0, q1, ctl, cap, boff, bfill, bnow, ng, mass, rho, shell);
 }
 
 // Capacities of the staging buckets for the next step: count + count/8 + 32
@@ -696,14 +715,24 @@ int launch_deposit_state(int ng, double mass, const SplitState& st, void* rho, S
     const int ts = sorted_tile(ng);
     const int nt = ng / ts, seg = walk_seg(nt);
     const unsigned nsegs = (unsigned)(nt * nt * (nt / seg));
-    if (ts == 16)
-        hipLaunchKernelGGL(k_deposit_state<16>, dim3(nsegs), dim3(256), 0, hs, st.q[0], st.q[1], st.ctl, st.cap, st.boff,
-                           st.bfill, st.bnow, ng, mass, (float*)rho, (float*)s.shell);
-    else
-        hipLaunchKernelGGL(k_deposit_state<8>, dim3(nsegs), dim3(256), 0, hs, st.q[0], st.q[1], st.ctl, st.cap, st.boff,
-                           st.bfill, st.bnow, ng, mass, (float*)rho, (float*)s.shell);
+#define PM_DS(K, RL, TSV)                                                                                            \
+    hipLaunchKernelGGL((K<TSV>), dim3(nsegs), dim3(256), 0, hs, static_cast<const typename SplitWords<RL>::Q*>(st.q[0]), \
+                       static_cast<const typename SplitWords<RL>::Q*>(st.q[1]), st.ctl, st.cap, st.boff, st.bfill,   \
+                       st.bnow, ng, mass, (RL*)rho, (RL*)s.shell)
+    if (st.prec == PM_FP64) {
+        if (ts == 16)
+            PM_DS(k_deposit_state64, double, 16);
+        else
+            PM_DS(k_deposit_state64, double, 8);
+    } else {
+        if (ts == 16)
+            PM_DS(k_deposit_state, float, 16);
+        else
+            PM_DS(k_deposit_state, float, 8);
+    }
+#undef PM_DS
     const int rc = check_launch("deposit_state");
-    return rc ? rc : launch_fold_shells(PM_FP32, ts, ng, s.shell, rho, hs);
+    return rc ? rc : launch_fold_shells(st.prec, ts, ng, s.shell, rho, hs);
 }
 
 int launch_deposit_sorted(int prec, const double* px, const double* py, const double* pz, int64_t np, int ng,

@@ -220,8 +220,9 @@ int launch_undrift(double* px, double* py, double* pz, const double* vx, const d
                    int64_t np, int ng, double drift, hipStream_t st);
 
 // Split-precision particle state (pm_split.hip), resident in tile order:
-// positions as u32 fixed point over the box (q = x * 2^(32 - log2 ng): cell
-// and fraction) and FP32 velocities, a slot's three values side by side
+// positions as fixed point over the box (q = x * 2^(bits - log2 ng): cell and
+// fraction; u32 for an FP32 mesh, u64 for an FP64 mesh) and velocities in the
+// mesh precision, a slot's three values side by side
 // (sel() of pm_tiles.h), with the particle index of every slot (id, kNoId
 // for a free slot or a hole).  Two sets of slots: ctl[3] names the current
 // one, a compaction rebuilds the buckets into the other and switches.  Tile
@@ -230,12 +231,13 @@ int launch_undrift(double* px, double* py, double* pz, const double* vx, const d
 // list (np entries: the particles of a load, and the arrivals that found
 // their bucket full); ctl[0]: pending count, ctl[1]: compaction flag.
 struct SplitState {
+    int prec = PM_FP32;  // the mesh precision: u32 positions and FP32 velocities, or u64 and FP64
     int64_t cap = 0;
-    uint32_t* q[2] = {};
-    float* u[2] = {};
+    void* q[2] = {};
+    void* u[2] = {};
     uint32_t* id[2] = {};
-    uint32_t* pq = nullptr;
-    float* pu = nullptr;
+    void* pq = nullptr;
+    void* pu = nullptr;
     uint32_t* pid = nullptr;
     uint32_t* boff = nullptr;
     uint32_t* bfill = nullptr;
@@ -244,8 +246,8 @@ struct SplitState {
     // per tile, the particles that left it in the last interpolation:
     // kOutbox entries of position and velocity triples and the particle
     // index, and their count
-    uint32_t* oq = nullptr;
-    float* ou = nullptr;
+    void* oq = nullptr;
+    void* ou = nullptr;
     uint32_t* oid = nullptr;
     uint32_t* ocnt = nullptr;
     // per tile, up to kOutbox holes of its range (slots whose particle left)
@@ -277,7 +279,7 @@ int launch_deposit_state(int ng, double mass, const SplitState& st, void* rho, S
 // the leavers that the last interpolation put in the outboxes, into their
 // new buckets (a no-op if the outboxes are empty)
 int launch_state_arrive(SplitState& st, int64_t np, int ng, hipStream_t hs);
-int launch_state_interp(SplitState& st, int64_t np, int ng, const float* phi, double kick, double drift,
+int launch_state_interp(SplitState& st, int64_t np, int ng, const void* phi, double kick, double drift,
                         unsigned long long* err, hipStream_t hs);
 // the capacity scan on explicit arrays, from the tile counts cnt (cleared)
 int launch_cap_scan_raw(int n, uint32_t* fill, uint32_t* cnt, uint32_t* coff, uint32_t* flag, const uint32_t* gate,

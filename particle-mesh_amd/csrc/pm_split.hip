@@ -1,11 +1,13 @@
-// Split-precision particle state of the fused step with an FP32 mesh,
-// resident in the plan in tile order.
+// Split-precision particle state of the fused step, resident in the plan in
+// tile order.
 //
-// Positions: an integer cell plus a fixed-point fraction in one u32 per axis,
-// q = x * 2^S with S = 32 - log2(ng) (2^-23 cells at ng = 512): the whole
-// periodic box is the u32 range, so a drift is an integer add that wraps at
-// the box edge by itself.  Velocities in FP32.  24 bytes per particle
-// against the 48 of FP64 positions and velocities.
+// Positions: an integer cell plus a fixed-point fraction in one word per
+// axis, q = x * 2^S: with an FP32 mesh a u32 (S = 32 - log2(ng), 2^-23 cells
+// at ng = 512) and FP32 velocities, 24 bytes per particle against the 48 of
+// FP64 positions and velocities; with an FP64 mesh a u64 (S = 64 - log2(ng),
+// 2^-55 cells at 512, finer than an FP64 position) and FP64 velocities.  The
+// whole periodic box is the word's range, so a drift is an integer add that
+// wraps at the box edge by itself.
 //
 // The particles live in the buckets of the deposit's 16^3-cell tiles: tile t
 // owns slots [boff[t], boff[t+1]) of the position and velocity arrays (a
@@ -52,69 +54,83 @@ namespace pm {
 constexpr uint32_t kNoId = 0xFFFFFFFFu;
 
 // both sets of slots, passed by value; a kernel takes set ctl[3]
-struct StateSets {
-    uint32_t* q[2];
-    float* u[2];
+template <typename Rl> struct StateSets {
+    typename SplitWords<Rl>::Q* q[2];
+    typename SplitWords<Rl>::U* u[2];
     uint32_t* id[2];
 };
-static StateSets sets(const SplitState& st) {
-    return StateSets{{st.q[0], st.q[1]}, {st.u[0], st.u[1]}, {st.id[0], st.id[1]}};
+template <typename Rl> static StateSets<Rl> sets(const SplitState& st) {
+    using Q = typename SplitWords<Rl>::Q;
+    using U = typename SplitWords<Rl>::U;
+    return StateSets<Rl>{{static_cast<Q*>(st.q[0]), static_cast<Q*>(st.q[1])},
+                         {static_cast<U*>(st.u[0]), static_cast<U*>(st.u[1])},
+                         {st.id[0], st.id[1]}};
 }
 
-// FP64 position -> u32 fixed point, nearest (x * 2^S is exact).  A position
+// FP64 position -> fixed point, nearest (x * 2^S is exact).  A position
 // outside [0, ng) (or not finite) is counted as a range error and wrapped
 // into the box (0 if not finite).
-__device__ __forceinline__ uint32_t split_pos(double x, double box, double scale, bool& bad) {
+template <typename Q> __device__ __forceinline__ Q split_pos(double x, double box, double scale, bool& bad) {
     if (!(x >= 0.0 && x < box)) {
         bad = true;
         x = isfinite(x) ? x - floor(x / box) * box : 0.0;
         if (!(x >= 0.0 && x < box)) x = 0.0;
     }
-    return (uint32_t)(unsigned long long)llrint(x * scale);  // 2^32 wraps to 0: the box is periodic
+    if constexpr (sizeof(Q) == 4)
+        return (uint32_t)(unsigned long long)llrint(x * scale);  // 2^32 wraps to 0: the box is periodic
+    else
+        return (uint64_t)__double2ull_rn(x * scale);  // below 2^64 (x * 2^S exact, an integer above 2^52)
 }
 
 // load: every particle into the pending list (the first step compacts it
 // into the buckets)
+template <typename Rl>
 __global__ __launch_bounds__(256) void k_state_load(const double* __restrict__ x, const double* __restrict__ y,
                                                     const double* __restrict__ z, const double* __restrict__ vx,
                                                     const double* __restrict__ vy, const double* __restrict__ vz,
-                                                    int64_t np, int ng, uint32_t* __restrict__ pq,
-                                                    float* __restrict__ pu, uint32_t* __restrict__ pid,
-                                                    unsigned long long* __restrict__ err) {
+                                                    int64_t np, int ng, typename SplitWords<Rl>::Q* __restrict__ pq,
+                                                    typename SplitWords<Rl>::U* __restrict__ pu,
+                                                    uint32_t* __restrict__ pid, unsigned long long* __restrict__ err) {
+    using Q = typename SplitWords<Rl>::Q;
+    using U = typename SplitWords<Rl>::U;
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (i >= np) return;
-    const double box = (double)ng, scale = ldexp(1.0, split_shift(ng));
+    const double box = (double)ng, scale = ldexp(1.0, qshift<Q>(ng));
     bool bad = false;
-    pq[sel(np, i, 0)] = split_pos(x[i], box, scale, bad);
-    pq[sel(np, i, 1)] = split_pos(y[i], box, scale, bad);
-    pq[sel(np, i, 2)] = split_pos(z[i], box, scale, bad);
-    pu[sel(np, i, 0)] = (float)vx[i];
-    pu[sel(np, i, 1)] = (float)vy[i];
-    pu[sel(np, i, 2)] = (float)vz[i];
+    pq[sel(np, i, 0)] = split_pos<Q>(x[i], box, scale, bad);
+    pq[sel(np, i, 1)] = split_pos<Q>(y[i], box, scale, bad);
+    pq[sel(np, i, 2)] = split_pos<Q>(z[i], box, scale, bad);
+    pu[sel(np, i, 0)] = (U)vx[i];
+    pu[sel(np, i, 1)] = (U)vy[i];
+    pu[sel(np, i, 2)] = (U)vz[i];
     pid[i] = (uint32_t)i;
     if (bad) atomicAdd(err, 1ULL);
 }
 
 // store: occupied bucket slots, then the pending list (non-empty only after
 // an overflow that no step has compacted yet)
-__global__ __launch_bounds__(256) void k_state_store(StateSets ss, int64_t cap,
-                                                     const uint32_t* __restrict__ pq, const float* __restrict__ pu,
+template <typename Rl>
+__global__ __launch_bounds__(256) void k_state_store(StateSets<Rl> ss, int64_t cap,
+                                                     const typename SplitWords<Rl>::Q* __restrict__ pq,
+                                                     const typename SplitWords<Rl>::U* __restrict__ pu,
                                                      const uint32_t* __restrict__ pid, int64_t np,
                                                      const uint32_t* __restrict__ ctl, int ng,
                                                      double* __restrict__ x, double* __restrict__ y,
                                                      double* __restrict__ z, double* __restrict__ vx,
                                                      double* __restrict__ vy, double* __restrict__ vz) {
+    using Q = typename SplitWords<Rl>::Q;
+    using U = typename SplitWords<Rl>::U;
     const int cs = (int)ctl[3];
-    const uint32_t* __restrict__ q = ss.q[cs];
-    const float* __restrict__ u = ss.u[cs];
+    const Q* __restrict__ q = ss.q[cs];
+    const U* __restrict__ u = ss.u[cs];
     const uint32_t* __restrict__ id = ss.id[cs];
-    const double inv = ldexp(1.0, -split_shift(ng));
+    const double inv = ldexp(1.0, -qshift<Q>(ng));
     const int64_t npend = ctl[0];
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < cap + npend; k += (int64_t)gridDim.x * 256) {
         const bool st = k < cap;
         const int64_t s = st ? k : k - cap, stride = st ? cap : np;
-        const uint32_t* qq = st ? q : pq;
-        const float* uu = st ? u : pu;
+        const Q* qq = st ? q : pq;
+        const U* uu = st ? u : pu;
         const uint32_t i = st ? id[s] : pid[s];
         if (i == kNoId) continue;
         x[i] = (double)qq[sel(stride, s, 0)] * inv;
@@ -137,22 +153,25 @@ __device__ __forceinline__ void for_each_entry(int64_t cap, const uint32_t* __re
 }
 
 // tile counts of the occupied slots and the pending entries
-__global__ __launch_bounds__(256) void k_state_count(StateSets ss, int64_t cap, const uint32_t* __restrict__ pq, int64_t np,
+template <typename Rl>
+__global__ __launch_bounds__(256) void k_state_count(StateSets<Rl> ss, int64_t cap,
+                                                     const typename SplitWords<Rl>::Q* __restrict__ pq, int64_t np,
                                                      const uint32_t* __restrict__ ctl, int ng, int ts,
                                                      uint32_t* __restrict__ cnt) {
+    using Q = typename SplitWords<Rl>::Q;
     if (ctl[1] == 0u) return;
-    const uint32_t* __restrict__ q = ss.q[ctl[3]];
+    const Q* __restrict__ q = ss.q[ctl[3]];
     const uint32_t* __restrict__ id = ss.id[ctl[3]];
-    const int lt = __builtin_ctz((unsigned)ts), sh = split_shift(ng) + lt;
+    const int lt = __builtin_ctz((unsigned)ts), sh = qshift<Q>(ng) + lt;
     const uint32_t nt = (uint32_t)(ng >> lt);
     const uint32_t lane = __lane_id();
     for_each_entry(cap, ctl, [&](int64_t k, int64_t n) {
         uint32_t kk = kNoId;
         if (k < cap) {
-            if (id[k] != kNoId) kk = split_key(q[sel(cap, k, 0)], q[sel(cap, k, 1)], q[sel(cap, k, 2)], sh, nt);
+            if (id[k] != kNoId) kk = qkey<Q>(q[sel(cap, k, 0)], q[sel(cap, k, 1)], q[sel(cap, k, 2)], sh, nt);
         } else if (k < n) {
             const int64_t p = k - cap;
-            kk = split_key(pq[sel(np, p, 0)], pq[sel(np, p, 1)], pq[sel(np, p, 2)], sh, nt);
+            kk = qkey<Q>(pq[sel(np, p, 0)], pq[sel(np, p, 1)], pq[sel(np, p, 2)], sh, nt);
         }
         // runs of equal tiles in consecutive lanes add once
         const uint32_t prev = __shfl_up(kk, 1);
@@ -167,25 +186,30 @@ __global__ __launch_bounds__(256) void k_state_count(StateSets ss, int64_t cap, 
 }
 
 // every entry into its tile's bucket of the other set of slots
-__global__ __launch_bounds__(256) void k_state_place(StateSets ss, int64_t cap,
-                                                     const uint32_t* __restrict__ pq, const float* __restrict__ pu,
+template <typename Rl>
+__global__ __launch_bounds__(256) void k_state_place(StateSets<Rl> ss, int64_t cap,
+                                                     const typename SplitWords<Rl>::Q* __restrict__ pq,
+                                                     const typename SplitWords<Rl>::U* __restrict__ pu,
                                                      const uint32_t* __restrict__ pid, int64_t np,
                                                      const uint32_t* __restrict__ ctl, int ng, int ts,
                                                      const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill) {
+    using Q = typename SplitWords<Rl>::Q;
+    using U = typename SplitWords<Rl>::U;
     if (ctl[1] == 0u) return;
     const int cs = (int)ctl[3];
-    const uint32_t* __restrict__ q = ss.q[cs];
-    const float* __restrict__ u = ss.u[cs];
+    const Q* __restrict__ q = ss.q[cs];
+    const U* __restrict__ u = ss.u[cs];
     const uint32_t* __restrict__ id = ss.id[cs];
-    uint32_t* __restrict__ q2 = ss.q[cs ^ 1];
-    float* __restrict__ u2 = ss.u[cs ^ 1];
+    Q* __restrict__ q2 = ss.q[cs ^ 1];
+    U* __restrict__ u2 = ss.u[cs ^ 1];
     uint32_t* __restrict__ id2 = ss.id[cs ^ 1];
-    const int lt = __builtin_ctz((unsigned)ts), sh = split_shift(ng) + lt;
+    const int lt = __builtin_ctz((unsigned)ts), sh = qshift<Q>(ng) + lt;
     const uint32_t nt = (uint32_t)(ng >> lt);
     const uint32_t lane = __lane_id();
     for_each_entry(cap, ctl, [&](int64_t k, int64_t n) {
-        uint32_t kk = kNoId, a = 0u, b = 0u, c = 0u, i = kNoId;
-        float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+        uint32_t kk = kNoId, i = kNoId;
+        Q a = 0u, b = 0u, c = 0u;
+        U v0 = 0, v1 = 0, v2 = 0;
         if (k < cap) {
             i = id[k];
             if (i != kNoId) {
@@ -206,7 +230,7 @@ __global__ __launch_bounds__(256) void k_state_place(StateSets ss, int64_t cap,
             v1 = pu[sel(np, p, 1)];
             v2 = pu[sel(np, p, 2)];
         }
-        if (i != kNoId) kk = split_key(a, b, c, sh, nt);
+        if (i != kNoId) kk = qkey<Q>(a, b, c, sh, nt);
         const uint32_t prev = __shfl_up(kk, 1);
         const bool head = kk != kNoId && (lane == 0 || prev != kk);
         const unsigned long long bounds = __ballot(head || kk == kNoId);
@@ -234,7 +258,8 @@ __global__ __launch_bounds__(256) void k_state_place(StateSets ss, int64_t cap,
 
 // ids of every slot of a set to kNoId: the current set (a load), or behind
 // the compaction flag the other set (before the placement)
-__global__ __launch_bounds__(256) void k_state_clear_ids(StateSets ss, const uint32_t* __restrict__ ctl, int other,
+template <typename Rl>
+__global__ __launch_bounds__(256) void k_state_clear_ids(StateSets<Rl> ss, const uint32_t* __restrict__ ctl, int other,
                                                          int64_t n) {
     if (other && ctl[1] == 0u) return;
     uint32_t* __restrict__ id = ss.id[ctl[3] ^ (uint32_t)other];
@@ -253,26 +278,36 @@ __global__ void k_state_done(uint32_t* __restrict__ ctl) {
 // update_particles (utils.cpp:141-170) on a tile's particles: the force of
 // cic_force (centered_derivatives + trilerp1, the arithmetic of pm_gather.h
 // on the same 32 potential values, here read from the tile's LDS copy), the
-// kick in FP32, then the drift as the integer add of the displacement
-// rounded to 2^-S cells.  A kick that leaves a velocity non-finite is
-// counted as a range error and leaves that particle as it was.
+// kick in the mesh precision, then the drift as the integer add of the
+// displacement rounded to 2^-S cells.  A kick that leaves a velocity
+// non-finite is counted as a range error and leaves that particle as it was.
+// FP32: 1024 threads a tile, at most 64 VGPRs (eight waves per SIMD, two
+// workgroups per CU, 27 KB of LDS each); FP64: 512 threads a tile, at most
+// 128 VGPRs (four waves per SIMD, two workgroups per CU, 55 KB of LDS each).
 #ifndef PM_TILE_THREADS
 #define PM_TILE_THREADS 1024
+#endif
+#ifndef PM_TILE_THREADS64
+#define PM_TILE_THREADS64 512
 #endif
 #ifndef PM_TILE_BATCH
 #define PM_TILE_BATCH 1
 #endif
 constexpr int kTileThreads = PM_TILE_THREADS, kTileBatch = PM_TILE_BATCH;
+template <typename Rl> constexpr int tile_threads() { return sizeof(Rl) == 4 ? kTileThreads : PM_TILE_THREADS64; }
 
 // per tile, where its leavers go and the kernels that read them: position,
 // velocity and particle index of each, and their count
-struct Outbox {
-    uint32_t* q;
-    float* u;
+template <typename Rl> struct Outbox {
+    typename SplitWords<Rl>::Q* q;
+    typename SplitWords<Rl>::U* u;
     uint32_t* i;
     uint32_t* cnt;
 };
-static Outbox outbox(const SplitState& st) { return Outbox{st.oq, st.ou, st.oid, st.ocnt}; }
+template <typename Rl> static Outbox<Rl> outbox(const SplitState& st) {
+    return Outbox<Rl>{static_cast<typename SplitWords<Rl>::Q*>(st.oq), static_cast<typename SplitWords<Rl>::U*>(st.ou),
+                      st.oid, st.ocnt};
+}
 // per tile, the holes of its range that the last interpolation saw (slots
 // whose particle left, in this step or before), their count, and how many
 // of them k_state_arrive has given to arrivals
@@ -285,22 +320,28 @@ static Holes holes(const SplitState& st) { return Holes{st.hole, st.hcnt, st.hus
 
 // where a particle that leaves its tile is appended: the new tile's bucket,
 // or, if that is full, the pending list (which makes the next step compact)
-struct Dest {
+template <typename Rl> struct Dest {
     const uint32_t* boff;
     uint32_t* bfill;
-    uint32_t* pq;
-    float* pu;
+    typename SplitWords<Rl>::Q* pq;
+    typename SplitWords<Rl>::U* pu;
     uint32_t* pid;
     int64_t np;
     uint32_t* ctl;
 };
-__device__ __forceinline__ void state_append(const Dest& d, uint32_t* __restrict__ q, float* __restrict__ u,
-                                             uint32_t* __restrict__ id, int64_t cap, uint32_t t2, uint32_t na,
-                                             uint32_t nb, uint32_t nc, float n0, float n1, float n2, uint32_t i) {
+template <typename Rl> static Dest<Rl> dest(const SplitState& st, int64_t np) {
+    return Dest<Rl>{st.boff, st.bfill, static_cast<typename SplitWords<Rl>::Q*>(st.pq),
+                    static_cast<typename SplitWords<Rl>::U*>(st.pu), st.pid, np, st.ctl};
+}
+template <typename Rl, typename Q = typename SplitWords<Rl>::Q, typename U = typename SplitWords<Rl>::U>
+__device__ __forceinline__ void state_append(const Dest<Rl>& d, Q* __restrict__ q, U* __restrict__ u,
+                                             uint32_t* __restrict__ id, int64_t cap, uint32_t t2, Q na, Q nb, Q nc,
+                                             U n0, U n1, U n2, uint32_t i) {
     const uint32_t slot = atomicAdd(&d.bfill[t2], 1u);
     const int64_t c2 = (int64_t)d.boff[t2 + 1] - d.boff[t2];
-    uint32_t *dq = q, *di = id;
-    float* du = u;
+    Q* dq = q;
+    uint32_t* di = id;
+    U* du = u;
     int64_t ds, stride = cap;
     if ((int64_t)slot < c2) {
         ds = (int64_t)d.boff[t2] + slot;
@@ -321,20 +362,26 @@ __device__ __forceinline__ void state_append(const Dest& d, uint32_t* __restrict
     di[ds] = i;
 }
 
-// eight waves per SIMD (at most 64 VGPRs): two 1024-thread workgroups per CU
-template <int TS, bool DRIFT>
-__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_state_interp(
-    StateSets ss, int64_t cap, const uint32_t* __restrict__ boff, uint32_t* __restrict__ bfill,
-    const uint32_t* __restrict__ bnow, Dest dst, Outbox ob, Holes ho, const uint32_t* __restrict__ ctl, int ng,
-    const float* __restrict__ phi, float kick, float dscale, unsigned long long* __restrict__ err) {
+// (the sets by reference: indexed by the current set in the kernel's
+// arguments, their pointers stay global; a by-value copy indexed at run time
+// would leave flat loads and stores)
+template <typename Rl, int TS, bool DRIFT>
+__device__ __forceinline__ void state_interp(const StateSets<Rl>& ss, int64_t cap, const uint32_t* __restrict__ boff,
+                                             const uint32_t* __restrict__ bnow, const Dest<Rl>& dst,
+                                             const Outbox<Rl>& ob, const Holes& ho,
+                                             const uint32_t* __restrict__ ctl, int ng, const Rl* __restrict__ phi,
+                                             Rl kick, Rl dscale, unsigned long long* __restrict__ err) {
+    using Q = typename SplitWords<Rl>::Q;
+    using U = Rl;
+    constexpr int NT = tile_threads<Rl>();
     constexpr int W = TS + 3;  // nodes -1 .. TS+1 around the tile's cells
-    __shared__ float ph[W * W * W];
+    __shared__ U ph[W * W * W];
     __shared__ uint32_t nmov, nhole;
     const int cs = (int)ctl[3];
-    uint32_t* __restrict__ q = ss.q[cs];
-    float* __restrict__ u = ss.u[cs];
+    Q* __restrict__ q = ss.q[cs];
+    U* __restrict__ u = ss.u[cs];
     uint32_t* __restrict__ id = ss.id[cs];
-    const int lt = ilog2(TS), S = split_shift(ng), sh = S + lt;
+    const int lt = ilog2(TS), S = qshift<Q>(ng), sh = S + lt;
     const uint32_t nt = (uint32_t)(ng >> lt);
     const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = (int)(t / (nt * nt)), ty = (int)((t / nt) % nt), tz = (int)(t % nt);
@@ -344,15 +391,15 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
         nmov = 0u;
         nhole = 0u;
     }
-    constexpr int64_t kStep = (int64_t)kTileThreads * kTileBatch;
-    uint32_t qa[kTileBatch], qb[kTileBatch], qc[kTileBatch];
-    float ua[kTileBatch], ub[kTileBatch], uc[kTileBatch];
+    constexpr int64_t kStep = (int64_t)NT * kTileBatch;
+    Q qa[kTileBatch], qb[kTileBatch], qc[kTileBatch];
+    U ua[kTileBatch], ub[kTileBatch], uc[kTileBatch];
     // a batch's slots past the end read the tile's first slot (every bucket
     // has at least 64 slots) and are skipped
     const auto load_batch = [&](int64_t s0) {
 #pragma unroll
         for (int j = 0; j < kTileBatch; ++j) {
-            const int64_t s = s0 + (int64_t)kTileThreads * j;
+            const int64_t s = s0 + (int64_t)NT * j;
             const int64_t r = s < end ? s : beg;
             qa[j] = stream_load(q + sel(cap, r, 0));
             qb[j] = stream_load(q + sel(cap, r, 1));
@@ -364,11 +411,11 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     };
     // all of a thread's staging loads in flight at once, then the first
     // batch of particles, then the LDS writes: the two latencies overlap
-    constexpr int NE = W * W * W, NIT = (NE + kTileThreads - 1) / kTileThreads;
-    float pv[NIT];
+    constexpr int NE = W * W * W, NIT = (NE + NT - 1) / NT;
+    U pv[NIT];
 #pragma unroll
     for (int j = 0; j < NIT; ++j) {
-        const int e = threadIdx.x + j * kTileThreads;
+        const int e = threadIdx.x + j * NT;
         if (e < NE) {
             const int a = e / (W * W), r = e - a * W * W, b = r / W, c = r - b * W;
             const int gx = (tx * TS - 1 + a) & m, gy = (ty * TS - 1 + b) & m, gz = (tz * TS - 1 + c) & m;
@@ -379,34 +426,39 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
     load_batch(s0);
 #pragma unroll
     for (int j = 0; j < NIT; ++j) {
-        const int e = threadIdx.x + j * kTileThreads;
+        const int e = threadIdx.x + j * NT;
         if (e < NE) ph[e] = pv[j];
     }
     __syncthreads();
-    const uint32_t fmask = (1u << S) - 1u;
-    const float fs = ldexpf(1.0f, -S);
+    const Q fmask = ((Q)1 << S) - 1u;
+    U fs;
+    if constexpr (sizeof(U) == 4)
+        fs = ldexpf(1.0f, -S);
+    else
+        fs = ldexp(1.0, -S);
+    constexpr Q kHole = (Q)1 << (8 * sizeof(Q) - 1);  // x in the other half of the box
     unsigned nerr = 0u;
     while (s0 < end) {
 #pragma unroll
         for (int j = 0; j < kTileBatch; ++j) {
-            const int64_t s = s0 + (int64_t)kTileThreads * j;
+            const int64_t s = s0 + (int64_t)NT * j;
             if (s >= end) continue;
-            const uint32_t a = qa[j], b = qb[j], c = qc[j];
+            const Q a = qa[j], b = qb[j], c = qc[j];
             // what the slot gets back: a hole (another tile's key) keeps its
             // values.  Every slot of the range is written, so that the
             // stores cover whole 128-byte lines (a partial line costs the
             // memory a read-modify-write).
-            uint32_t wa = a, wb = b, wc = c;
-            float w0 = ua[j], w1 = ub[j], w2 = uc[j];
+            Q wa = a, wb = b, wc = c;
+            U w0 = ua[j], w1 = ub[j], w2 = uc[j];
             bool hole = DRIFT;  // a hole already, unless its particle is the tile's
-            if (split_key(a, b, c, sh, nt) == t) {
+            if (qkey<Q>(a, b, c, sh, nt) == t) {
                 hole = false;
                 const int lx = (int)(a >> S) - tx * TS, ly = (int)(b >> S) - ty * TS, lz = (int)(c >> S) - tz * TS;
-                const float fx = (float)(a & fmask) * fs, fy = (float)(b & fmask) * fs, fz = (float)(c & fmask) * fs;
+                const U fx = (U)(a & fmask) * fs, fy = (U)(b & fmask) * fs, fz = (U)(c & fmask) * fs;
                 // the 12 z-runs of gather_runs_main from LDS: node (ix+x,
                 // iy+y, iz+z) of the cell at local (lx, ly, lz) is
                 // L(lx+1+x, ly+1+y, lz+1+z)
-                float cl[2][2][4], xr[2][2][2], yr[2][2][2];
+                U cl[2][2][4], xr[2][2][2], yr[2][2][2];
                 const auto L = [&](int x, int y, int z) { return ph[(x * W + y) * W + z]; };
 #pragma unroll
                 for (int aa = 0; aa < 2; ++aa)
@@ -420,9 +472,9 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
                             yr[aa][bb][k] = L(lx + 1 + bb, ly + 3 * aa, lz + 1 + k);
                         }
                     }
-                float f[3];
+                U f[3];
                 force_from_runs(cl, xr, yr, fx, fy, fz, f);
-                float n0 = fmaf(f[0], kick, ua[j]), n1 = fmaf(f[1], kick, ub[j]), n2 = fmaf(f[2], kick, uc[j]);
+                U n0 = fma(f[0], kick, ua[j]), n1 = fma(f[1], kick, ub[j]), n2 = fma(f[2], kick, uc[j]);
                 const bool fin = isfinite(n0) && isfinite(n1) && isfinite(n2);
                 if (!fin) {
                     n0 = ua[j];
@@ -430,16 +482,22 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
                     n2 = uc[j];
                     ++nerr;
                 }
-                uint32_t na = a, nb = b, nc = c;
+                Q na = a, nb = b, nc = c;
                 if (DRIFT && fin) {
-                    na = a + (uint32_t)__float2int_rn(n0 * dscale);
-                    nb = b + (uint32_t)__float2int_rn(n1 * dscale);
-                    nc = c + (uint32_t)__float2int_rn(n2 * dscale);
+                    if constexpr (sizeof(Q) == 4) {
+                        na = a + (uint32_t)__float2int_rn(n0 * dscale);
+                        nb = b + (uint32_t)__float2int_rn(n1 * dscale);
+                        nc = c + (uint32_t)__float2int_rn(n2 * dscale);
+                    } else {
+                        na = a + (uint64_t)__double2ll_rn(n0 * dscale);
+                        nb = b + (uint64_t)__double2ll_rn(n1 * dscale);
+                        nc = c + (uint64_t)__double2ll_rn(n2 * dscale);
+                    }
                 }
                 w0 = n0;
                 w1 = n1;
                 w2 = n2;
-                const uint32_t t2 = DRIFT ? split_key(na, nb, nc, sh, nt) : t;
+                const uint32_t t2 = DRIFT ? qkey<Q>(na, nb, nc, sh, nt) : t;
                 if (t2 == t) {
                     wa = na;
                     wb = nb;
@@ -450,7 +508,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
                     // goes into the tile's outbox (an LDS counter, no global
                     // round trip in the loop), which k_state_arrive moves to
                     // its new tile.  A full outbox: appended here.
-                    wa = a ^ 0x80000000u;
+                    wa = a ^ kHole;
                     hole = true;
                     const uint32_t pid = id[s];
                     id[s] = kNoId;
@@ -465,7 +523,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
                         ob.u[sel(no, e, 2)] = n2;
                         ob.i[e] = pid;
                     } else {
-                        state_append(dst, q, u, id, cap, t2, na, nb, nc, n0, n1, n2, pid);
+                        state_append<Rl>(dst, q, u, id, cap, t2, na, nb, nc, n0, n1, n2, pid);
                     }
                 }
             }
@@ -495,6 +553,20 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
         }
     }
 }
+template <int TS, bool DRIFT>
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_state_interp(
+    StateSets<float> ss, int64_t cap, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bnow,
+    Dest<float> dst, Outbox<float> ob, Holes ho, const uint32_t* __restrict__ ctl, int ng,
+    const float* __restrict__ phi, float kick, float dscale, unsigned long long* __restrict__ err) {
+    state_interp<float, TS, DRIFT>(ss, cap, boff, bnow, dst, ob, ho, ctl, ng, phi, kick, dscale, err);
+}
+template <int TS, bool DRIFT>
+__global__ __launch_bounds__(PM_TILE_THREADS64) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_state_interp64(
+    StateSets<double> ss, int64_t cap, const uint32_t* __restrict__ boff, const uint32_t* __restrict__ bnow,
+    Dest<double> dst, Outbox<double> ob, Holes ho, const uint32_t* __restrict__ ctl, int ng,
+    const double* __restrict__ phi, double kick, double dscale, unsigned long long* __restrict__ err) {
+    state_interp<double, TS, DRIFT>(ss, cap, boff, bnow, dst, ob, ho, ctl, ng, phi, kick, dscale, err);
+}
 
 // The leavers of the last interpolation: one wave per tile moves its outbox
 // entries to their new tiles, into a hole of the new tile's range if the
@@ -503,24 +575,26 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8,
 // outbox, so that a second call is a no-op.  The interpolation's own appends
 // (full outbox) never take holes: a hole in a range that the same
 // interpolation walks could be moved twice in one step.
-template <int TS>
-__global__ __launch_bounds__(256) void k_state_arrive(StateSets ss, int64_t cap, Dest dst, Outbox ob, Holes ho,
-                                                      int ng, uint32_t ntiles) {
+template <typename Rl, int TS>
+__global__ __launch_bounds__(256) void k_state_arrive(StateSets<Rl> ss, int64_t cap, Dest<Rl> dst, Outbox<Rl> ob,
+                                                      Holes ho, int ng, uint32_t ntiles) {
+    using Q = typename SplitWords<Rl>::Q;
+    using U = typename SplitWords<Rl>::U;
     const uint32_t t = blockIdx.x * 4 + threadIdx.x / 64;
     if (t >= ntiles) return;
     const int cs = (int)dst.ctl[3];
-    uint32_t* __restrict__ q = ss.q[cs];
-    float* __restrict__ u = ss.u[cs];
+    Q* __restrict__ q = ss.q[cs];
+    U* __restrict__ u = ss.u[cs];
     uint32_t* __restrict__ id = ss.id[cs];
-    const int lt = ilog2(TS), sh = split_shift(ng) + lt;
+    const int lt = ilog2(TS), sh = qshift<Q>(ng) + lt;
     const uint32_t nt = (uint32_t)(ng >> lt);
     const uint32_t n = ob.cnt[t];
     const int64_t no = (int64_t)ntiles * kOutbox;
     for (uint32_t k = threadIdx.x & 63u; k < n; k += 64u) {
         const int64_t e = (int64_t)t * kOutbox + k;
-        const uint32_t na = ob.q[sel(no, e, 0)], nb = ob.q[sel(no, e, 1)], nc = ob.q[sel(no, e, 2)];
-        const float n0 = ob.u[sel(no, e, 0)], n1 = ob.u[sel(no, e, 1)], n2 = ob.u[sel(no, e, 2)];
-        const uint32_t pid = ob.i[e], t2 = split_key(na, nb, nc, sh, nt);
+        const Q na = ob.q[sel(no, e, 0)], nb = ob.q[sel(no, e, 1)], nc = ob.q[sel(no, e, 2)];
+        const U n0 = ob.u[sel(no, e, 0)], n1 = ob.u[sel(no, e, 1)], n2 = ob.u[sel(no, e, 2)];
+        const uint32_t pid = ob.i[e], t2 = qkey<Q>(na, nb, nc, sh, nt);
         const uint32_t h = atomicAdd(&ho.used[t2], 1u);
         if (h < ho.cnt[t2]) {
             const int64_t s = ho.s[(int64_t)t2 * kOutbox + h];
@@ -532,7 +606,7 @@ __global__ __launch_bounds__(256) void k_state_arrive(StateSets ss, int64_t cap,
             u[sel(cap, s, 2)] = n2;
             id[s] = pid;
         } else {
-            state_append(dst, q, u, id, cap, t2, na, nb, nc, n0, n1, n2, pid);
+            state_append<Rl>(dst, q, u, id, cap, t2, na, nb, nc, n0, n1, n2, pid);
         }
     }
     if ((threadIdx.x & 63u) == 0u && n) ob.cnt[t] = 0u;
@@ -543,11 +617,15 @@ static unsigned grid_for(int64_t n, unsigned cap_blocks) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap_blocks));
 }
 
-int launch_state_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
+template <typename Rl>
+static int state_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
                       const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err, hipStream_t s) {
-    hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 16384)), dim3(256), 0, s, sets(st), st.ctl, 0, st.cap);
-    hipLaunchKernelGGL(k_state_load, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, x, y, z, vx, vy, vz, np, ng,
-                       st.pq, st.pu, st.pid, err);
+    using Q = typename SplitWords<Rl>::Q;
+    using U = typename SplitWords<Rl>::U;
+    hipLaunchKernelGGL(k_state_clear_ids<Rl>, dim3(grid_for(st.cap, 16384)), dim3(256), 0, s, sets<Rl>(st), st.ctl, 0,
+                       st.cap);
+    hipLaunchKernelGGL(k_state_load<Rl>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, x, y, z, vx, vy, vz, np,
+                       ng, static_cast<Q*>(st.pq), static_cast<U*>(st.pu), st.pid, err);
     // every particle pending, no outbox entries; the next step compacts
     const int ts = sorted_tile(ng);
     const size_t ntiles = (size_t)(ng / ts) * (ng / ts) * (ng / ts);
@@ -558,15 +636,31 @@ int launch_state_load(const double* x, const double* y, const double* z, const d
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, s, st.ctl + 1, (size_t)1, 1u);
     return check_launch("split_load");
 }
-
-int launch_state_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
-                       double* vy, double* vz, hipStream_t s) {
-    hipLaunchKernelGGL(k_state_store, dim3(grid_for(st.cap + np, 16384)), dim3(256), 0, s, sets(st), st.cap, st.pq,
-                       st.pu, st.pid, np, st.ctl, ng, x, y, z, vx, vy, vz);
-    return check_launch("split_store");
+int launch_state_load(const double* x, const double* y, const double* z, const double* vx, const double* vy,
+                      const double* vz, int64_t np, int ng, SplitState& st, unsigned long long* err, hipStream_t s) {
+    return st.prec == PM_FP64 ? state_load<double>(x, y, z, vx, vy, vz, np, ng, st, err, s)
+                              : state_load<float>(x, y, z, vx, vy, vz, np, ng, st, err, s);
 }
 
-int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, hipStream_t hs) {
+template <typename Rl>
+static int state_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
+                       double* vy, double* vz, hipStream_t s) {
+    using Q = typename SplitWords<Rl>::Q;
+    using U = typename SplitWords<Rl>::U;
+    hipLaunchKernelGGL(k_state_store<Rl>, dim3(grid_for(st.cap + np, 16384)), dim3(256), 0, s, sets<Rl>(st), st.cap,
+                       static_cast<const Q*>(st.pq), static_cast<const U*>(st.pu), st.pid, np, st.ctl, ng, x, y, z, vx,
+                       vy, vz);
+    return check_launch("split_store");
+}
+int launch_state_store(const SplitState& st, int64_t np, int ng, double* x, double* y, double* z, double* vx,
+                       double* vy, double* vz, hipStream_t s) {
+    return st.prec == PM_FP64 ? state_store<double>(st, np, ng, x, y, z, vx, vy, vz, s)
+                              : state_store<float>(st, np, ng, x, y, z, vx, vy, vz, s);
+}
+
+template <typename Rl> static int state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, hipStream_t hs) {
+    using Q = typename SplitWords<Rl>::Q;
+    using U = typename SplitWords<Rl>::U;
     const int ts = sorted_tile(ng);
     const int nt = ng / ts;
     const int64_t n = st.cap + np;
@@ -575,57 +669,69 @@ int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, h
     // capped grids (grid-stride loops): in a step without compaction every
     // kernel here is a no-op, and a workgroup that only exits still costs
     // its launch
-    hipLaunchKernelGGL(k_state_count, dim3(grid_for(n, 1024)), dim3(256), 0, hs, sets(st), st.cap, st.pq, np, st.ctl,
-                       ng, ts, s.cnt);
+    hipLaunchKernelGGL(k_state_count<Rl>, dim3(grid_for(n, 1024)), dim3(256), 0, hs, sets<Rl>(st), st.cap,
+                       static_cast<const Q*>(st.pq), np, st.ctl, ng, ts, s.cnt);
     int rc = check_launch("split_count");
     if (rc) return rc;
     // capacities (count + count/2 + 64, rounded up to 32) into boff, bfill cleared
     rc = launch_cap_scan_raw(nt * nt * nt, st.bfill, s.cnt, st.boff, st.ctl + 2, st.ctl + 1, hs);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_state_clear_ids, dim3(grid_for(st.cap, 1024)), dim3(256), 0, hs, sets(st), st.ctl, 1,
+    hipLaunchKernelGGL(k_state_clear_ids<Rl>, dim3(grid_for(st.cap, 1024)), dim3(256), 0, hs, sets<Rl>(st), st.ctl, 1,
                        st.cap);
-    hipLaunchKernelGGL(k_state_place, dim3(grid_for(n, 1024)), dim3(256), 0, hs, sets(st), st.cap, st.pq, st.pu,
-                       st.pid, np, st.ctl, ng, ts, st.boff, st.bfill);
+    hipLaunchKernelGGL(k_state_place<Rl>, dim3(grid_for(n, 1024)), dim3(256), 0, hs, sets<Rl>(st), st.cap,
+                       static_cast<const Q*>(st.pq), static_cast<const U*>(st.pu), st.pid, np, st.ctl, ng, ts, st.boff,
+                       st.bfill);
     hipLaunchKernelGGL(k_state_done, dim3(1), dim3(64), 0, hs, st.ctl);
     return check_launch("split_compact");
 }
+int launch_state_compact(SplitState& st, int64_t np, int ng, SortedScratch& s, hipStream_t hs) {
+    return st.prec == PM_FP64 ? state_compact<double>(st, np, ng, s, hs) : state_compact<float>(st, np, ng, s, hs);
+}
 
-int launch_state_interp(SplitState& st, int64_t np, int ng, const float* phi, double kick, double drift,
+int launch_state_interp(SplitState& st, int64_t np, int ng, const void* phi, double kick, double drift,
                         unsigned long long* err, hipStream_t hs) {
     const int ts = sorted_tile(ng);
     const unsigned ntiles = (unsigned)((ng / ts) * (ng / ts) * (ng / ts));
-    const float dscale = (float)ldexp(drift, split_shift(ng));
-    const Dest dst{st.boff, st.bfill, st.pq, st.pu, st.pid, np, st.ctl};
-#define PM_SI(TSV, DR)                                                                                              \
-    hipLaunchKernelGGL((k_state_interp<TSV, DR>), dim3(ntiles), dim3(kTileThreads), 0, hs, sets(st), st.cap, st.boff, \
-                       st.bfill, st.bnow, dst, outbox(st), holes(st), st.ctl, ng, phi, (float)kick, dscale, err)
-    if (ts == 16) {
-        if (drift != 0.0)
-            PM_SI(16, true);
-        else
-            PM_SI(16, false);
-    } else {
-        if (drift != 0.0)
-            PM_SI(8, true);
-        else
-            PM_SI(8, false);
+#define PM_SI(K, RL, TSV, DR, NT)                                                                                    \
+    hipLaunchKernelGGL((K<TSV, DR>), dim3(ntiles), dim3(NT), 0, hs, sets<RL>(st), st.cap, st.boff, st.bnow,          \
+                       dest<RL>(st, np), outbox<RL>(st), holes(st), st.ctl, ng, static_cast<const RL*>(phi), (RL)kick, \
+                       (RL)ldexp(drift, qshift<typename SplitWords<RL>::Q>(ng)), err)
+#define PM_SI_ALL(K, RL, NT)            \
+    if (ts == 16) {                     \
+        if (drift != 0.0)               \
+            PM_SI(K, RL, 16, true, NT);  \
+        else                            \
+            PM_SI(K, RL, 16, false, NT); \
+    } else {                            \
+        if (drift != 0.0)               \
+            PM_SI(K, RL, 8, true, NT);   \
+        else                            \
+            PM_SI(K, RL, 8, false, NT);  \
     }
+    if (st.prec == PM_FP64) {
+        PM_SI_ALL(k_state_interp64, double, PM_TILE_THREADS64)
+    } else {
+        PM_SI_ALL(k_state_interp, float, kTileThreads)
+    }
+#undef PM_SI_ALL
 #undef PM_SI
     return check_launch("split_interp");
 }
 
-int launch_state_arrive(SplitState& st, int64_t np, int ng, hipStream_t hs) {
+template <typename Rl> static int state_arrive(SplitState& st, int64_t np, int ng, hipStream_t hs) {
     const int ts = sorted_tile(ng);
     const unsigned ntiles = (unsigned)((ng / ts) * (ng / ts) * (ng / ts));
-    const Dest dst{st.boff, st.bfill, st.pq, st.pu, st.pid, np, st.ctl};
     const dim3 grid((ntiles + 3) / 4);
     if (ts == 16)
-        hipLaunchKernelGGL(k_state_arrive<16>, grid, dim3(256), 0, hs, sets(st), st.cap, dst, outbox(st), holes(st), ng,
-                           ntiles);
+        hipLaunchKernelGGL((k_state_arrive<Rl, 16>), grid, dim3(256), 0, hs, sets<Rl>(st), st.cap, dest<Rl>(st, np),
+                           outbox<Rl>(st), holes(st), ng, ntiles);
     else
-        hipLaunchKernelGGL(k_state_arrive<8>, grid, dim3(256), 0, hs, sets(st), st.cap, dst, outbox(st), holes(st), ng,
-                           ntiles);
+        hipLaunchKernelGGL((k_state_arrive<Rl, 8>), grid, dim3(256), 0, hs, sets<Rl>(st), st.cap, dest<Rl>(st, np),
+                           outbox<Rl>(st), holes(st), ng, ntiles);
     return check_launch("split_arrive");
+}
+int launch_state_arrive(SplitState& st, int64_t np, int ng, hipStream_t hs) {
+    return st.prec == PM_FP64 ? state_arrive<double>(st, np, ng, hs) : state_arrive<float>(st, np, ng, hs);
 }
 
 }  // namespace pm

@@ -321,6 +321,28 @@ __device__ __forceinline__ uint32_t split_rec(uint32_t q, int S, uint32_t tmask)
     return (cell << 28) | (S >= 28 ? frac >> (S - 28) : frac << (28 - S));
 }
 
+// The split state by mesh precision: positions as Q fixed point over the
+// box (q = x 2^S, S = bits of Q - log2 ng: 2^-23 cells at 512^3 in u32 for
+// the FP32 mesh, 2^-55 cells in u64 for the FP64 mesh, finer than an FP64
+// position), velocities as U.
+template <typename Rl> struct SplitWords;
+template <> struct SplitWords<float> {
+    using Q = uint32_t;
+    using U = float;
+};
+template <> struct SplitWords<double> {
+    using Q = uint64_t;
+    using U = double;
+};
+template <typename Q> __host__ __device__ inline int qshift(int ng) { return 8 * (int)sizeof(Q) - ilog2(ng); }
+// tile of q: sh = S + log2(ts) = bits of Q - log2(tiles per axis), all of Q
+// for one tile
+template <typename Q> __device__ __forceinline__ uint32_t qkey(Q qx, Q qy, Q qz, int sh, uint32_t nt) {
+    if constexpr (sizeof(Q) == 4) return split_key(qx, qy, qz, sh, nt);  // a 64-bit shift: 32 gives 0
+    const auto t = [sh](Q q) { return sh >= 64 ? 0u : (uint32_t)(q >> sh); };
+    return (t(qx) * nt + t(qy)) * nt + t(qz);
+}
+
 // Cell in the tile and offsets in the cell of a record: the deposits take
 // every particle through a record (read from the staging, or formed from its
 // position), so that staged and binned paths give the same weights.
@@ -338,6 +360,21 @@ __device__ __forceinline__ TileOffs tile_offs(const StageD& r) {
     unpack_stage_d(r, qx, qy, qz);
     return TileOffs{(int)(qx >> 38), (int)(qy >> 38), (int)(qz >> 38), (double)(qx & kQMaskD) / kQScaleD,
                     (double)(qy & kQMaskD) / kQScaleD, (double)(qz & kQMaskD) / kQScaleD};
+}
+
+// Cell in the tile and offsets of a split position triple, as the tile
+// deposits form them: the FP32 mesh through the StageQ record (2^-28 cells),
+// the FP64 mesh with the fraction truncated to 2^-38 cells (the StageD
+// resolution).
+template <typename Q>
+__device__ __forceinline__ TileOffs split_offs(Q qa, Q qb, Q qc, int S, uint32_t tmask) {
+    if constexpr (sizeof(Q) == 4) {
+        return tile_offs(StageQ{split_rec(qa, S, tmask), split_rec(qb, S, tmask), split_rec(qc, S, tmask)});
+    } else {
+        const auto cell = [&](Q q) { return (int)((q >> S) & tmask); };
+        const auto frac = [&](Q q) { return (double)((q & ((1ull << S) - 1ull)) >> (S - 38)) / kQScaleD; };
+        return TileOffs{cell(qa), cell(qb), cell(qc), frac(qa), frac(qb), frac(qc)};
+    }
 }
 
 }  // namespace pm

@@ -80,13 +80,10 @@ def test_split_load_store_round_trip(dev):
     assert out_p[0][5].item() == ng - 1.25 and out_p[1][6].item() == 0.0 and out_p[2][7].item() == 3.5
 
 
-def test_split_rejects_fp64_plan_and_unloaded_state(dev):
+def test_split_rejects_unloaded_state(dev):
     import torch
 
     pos = [torch.zeros(64, dtype=torch.float64, device=dev) for _ in range(3)]
-    with pmhip.Plan(16, 64, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
-        with pytest.raises(pmhip.PMError, match="FP32"):
-            plan.split_load(pos, pos)
     with pmhip.Plan(16, 64, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
         with pytest.raises(pmhip.PMError, match="no particles loaded"):
             plan.split_step(0.1, 0.1)
@@ -222,7 +219,8 @@ def test_split_kdk_vs_fp64_oracle(dev, ng, nrow):
     assert mx < 1e-4
 
 
-def test_split_long_high_mobility_vs_fp64_oracle(dev):
+@pytest.mark.parametrize("prec,tol_rms,tol_max", [(pmhip.PM_FP32, 1e-4, 1e-3), (pmhip.PM_FP64, 1e-9, 1e-8)])
+def test_split_long_high_mobility_vs_fp64_oracle(dev, prec, tol_rms, tol_max):
     """100 KDK steps in which up to a fifth of the particles change tile every
     step (velocities up to 40 cells per unit time, dt = 0.01; gravity weakened
     by a Green's scale of 1e-3 so that FP32 and FP64 trajectories stay
@@ -232,14 +230,16 @@ def test_split_long_high_mobility_vs_fp64_oracle(dev):
     positions match the FP64 oracle of the same run."""
     import torch
 
-    print("case split long high-mobility run ng=64 np=64^3 100 steps", flush=True)
+    print(f"case split long high-mobility run ng=64 np=64^3 100 steps prec={prec}", flush=True)
     ng, nrow, nsteps, dt, scale = 64, 64, 100, 0.01, 1e-3
     npart = nrow ** 3
     rng = np.random.default_rng(53)
     pos0 = [torch.tensor(rng.uniform(0.0, ng, npart), device=dev) for _ in range(3)]
     pos0 = _on_grid(pos0, ng)
-    vel0 = [torch.tensor(rng.uniform(-40.0, 40.0, npart), device=dev).float().double() for _ in range(3)]
-    with pmhip.Plan(ng, npart, pmhip.PM_FP32, pmhip.PM_GREEN_CONT_K2, scale) as plan:
+    vel0 = [torch.tensor(rng.uniform(-40.0, 40.0, npart), device=dev) for _ in range(3)]
+    if prec == pmhip.PM_FP32:
+        vel0 = [v.float().double() for v in vel0]
+    with pmhip.Plan(ng, npart, prec, pmhip.PM_GREEN_CONT_K2, scale) as plan:
         plan.split_load(pos0, vel0)
         plan.split_run_kdk(nsteps, dt)
         p = [torch.full_like(t, float("nan")) for t in pos0]
@@ -260,5 +260,95 @@ def test_split_long_high_mobility_vs_fp64_oracle(dev):
     print(f"after {nsteps} steps: rms displacement {moved:.1f} cells; vs oracle rms {rms:.3e} max {mx:.3e} cells",
           flush=True)
     assert moved > 5.0  # the particles did cross many tiles
-    assert rms < 1e-4
-    assert mx < 1e-3
+    assert rms < tol_rms
+    assert mx < tol_max
+
+
+# ------------------------------------------------------------ FP64 mesh
+def _shift64(ng):
+    return 64 - int(round(math.log2(ng)))
+
+
+def test_split64_load_store_round_trip(dev):
+    """With an FP64 mesh the split state holds u64 positions at 2^-(64 - log2 ng)
+    cells and FP64 velocities: any FP64 position in the box and any velocity
+    come back bit for bit."""
+    import torch
+
+    print("case split64 round trip ng=64", flush=True)
+    ng, npart = 64, 20000
+    rng = np.random.default_rng(61)
+    pos = [torch.tensor(rng.uniform(0.0, ng, npart), device=dev) for _ in range(3)]
+    vel = [torch.tensor(rng.normal(0.0, 3.0, npart), device=dev) for _ in range(3)]
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.split_load(pos, vel)
+        p = [torch.full_like(t, float("nan")) for t in pos]
+        v = [torch.full_like(t, float("nan")) for t in vel]
+        plan.split_store(p, v)
+        torch.cuda.synchronize()
+    for x, y in zip(p + v, pos + vel):
+        assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("ng,nrow", [(16, 16), (64, 64), (256, 128)])
+def test_split64_step_matches_fp64_position_step(dev, ng, nrow):
+    """One split step with an FP64 mesh deposits and solves exactly as pm_step
+    with FP64 positions (density and potential bit for bit: the in-tile
+    offsets are truncated to 2^-38 cells in both), and kicks and drifts to
+    the rounding of the positions (2^-(64 - log2 ng) cells against an FP64
+    add)."""
+    import torch
+
+    print(f"case split64 step vs pm_step ng={ng} nrow={nrow}", flush=True)
+    npart = nrow ** 3
+    pos0, vel0 = _ics(nrow, ng, 1.5, 43, dev, vel_factor=3.0)
+    kick, drift = 0.02, 0.05
+    pmhip.range_errors()
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as a, \
+            pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as b:
+        pa, va = [p.clone() for p in pos0], [v.clone() for v in vel0]
+        a.step(pa, va, kick, drift, pmhip.PM_DEPOSIT_SORTED)
+        b.split_load(pos0, vel0)
+        b.split_step(kick, drift)
+        pb, vb = [torch.empty_like(p) for p in pos0], [torch.empty_like(v) for v in vel0]
+        b.split_store(pb, vb)
+        torch.cuda.synchronize()
+        assert torch.equal(a.density(), b.density())
+        assert torch.equal(a.potential(), b.potential())
+    assert pmhip.range_errors() == 0
+    vmax = max(float(v.abs().max()) for v in va)
+    dv = max(float((x - y).abs().max()) for x, y in zip(va, vb))
+    dx = max(float(_periodic_diff(x, y, ng).abs().max()) for x, y in zip(pa, pb))
+    print(f"max |dv| {dv:.3e} (max |v| {vmax:.2f}), max |dx| {dx:.3e} cells", flush=True)
+    assert dv <= 1e-14 * vmax
+    assert dx <= 2.0 ** -(_shift64(ng) + 1) + 2.0 ** -52 * ng + drift * dv
+
+
+@pytest.mark.parametrize("ng,nrow", [(64, 64), (128, 128)])
+def test_split64_kdk_vs_fp64_oracle(dev, ng, nrow):
+    """10 KDK steps of clustered ICs with an FP64 mesh: the split state against
+    the FP64 oracle on the same inputs, final positions within 1e-10 cells
+    rms and 1e-9 max."""
+    import torch
+
+    print(f"case split64 kdk vs oracle ng={ng}", flush=True)
+    nsteps, dt = 10, 0.005
+    npart = nrow ** 3
+    pos0, vel0 = _ics(nrow, ng, 1.5, 49, dev)
+    with pmhip.Plan(ng, npart, pmhip.PM_FP64, pmhip.PM_GREEN_CONT_K2, 1.0) as plan:
+        plan.split_load(pos0, vel0)
+        plan.split_run_kdk(nsteps, dt)
+        p = [torch.empty_like(t) for t in pos0]
+        v = [torch.empty_like(t) for t in vel0]
+        plan.split_store(p, v)
+        torch.cuda.synchronize()
+    assert pmhip.range_errors() == 0
+    ph = [t.cpu().numpy().copy() for t in pos0]
+    vh = [t.cpu().numpy().copy() for t in vel0]
+    pyoracle.set_threads(min(16, pyoracle.max_threads()))
+    pyoracle.run_kdk(ph, vh, ng, nsteps, dt, pyoracle.GREEN_CONT_K2, 1.0)
+    d2 = sum(_periodic_diff(a.cpu().numpy(), b, ng) ** 2 for a, b in zip(p, ph))
+    rms, mx = float(np.sqrt(d2.mean())), float(np.sqrt(d2.max()))
+    print(f"split64 vs oracle after {nsteps} steps: rms {rms:.3e} max {mx:.3e} cells", flush=True)
+    assert rms < 1e-10
+    assert mx < 1e-9
