@@ -26,5 +26,5 @@ public:
 
 private:
     size_t bytes_;
-    MPI_Comm comm_;
+    [[maybe_unused]] MPI_Comm comm_;  // the caller's communicator, of one rank (checked at construction)
 };

@@ -19,5 +19,5 @@ public:
 
 private:
     int ng_;
-    MPI_Comm comm_;
+    [[maybe_unused]] MPI_Comm comm_;  // the caller's communicator, of one rank (checked at construction)
 };
